@@ -1,0 +1,242 @@
+/*
+ * tencirpauli_amd.h -- C ABI of the MI355X (gfx950) matrix-free Pauli-sum MVP.
+ *
+ * Drop-in boundary for TenCirPauli's native MVP path.  Each entry point names
+ * the reference interface it replaces (paths relative to the reference
+ * repository; core/ = crates/tencir-pauli-core/src/, native/ =
+ * crates/tencirpauli-native/src/, py/ = python/tencirpauli/).
+ *
+ * Conventions kept from the reference:
+ *   - Pauli codes 0=I 1=X 2=Y 3=Z; packed words put qubit q in word q/64,
+ *     bit q%64, and Y = (x=1, z=1)                       (core/word.rs:53-110)
+ *   - qubit 0 is the most significant bit of a computational index
+ *                                                      (core/pauli_action.rs:65)
+ *   - operators are canonical: duplicates summed in IEEE-bit-pattern order,
+ *     sorted by code tuple from qubit 0, exact zeros dropped
+ *                                                  (core/operator.rs:117-155)
+ *   - apply overwrites its output, never accumulates     (core/tests.rs:7-23)
+ *   - max_bytes is advisory below 1 GiB                (py/_native_mvp.py:19-58)
+ *   - complex numbers are interleaved (re, im)
+ *
+ * All functions return a tcp_status; on failure tcp_last_error() returns the
+ * message of the calling thread's last error.  Plans are immutable after
+ * creation: concurrent applies on distinct output buffers are allowed.
+ * Device pointers and hipStream_t handles are passed as void*.
+ */
+#ifndef TENCIRPAULI_AMD_H
+#define TENCIRPAULI_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One code per PauliError variant used on this path (core/error.rs:5-74);
+ * native/convert.rs:35-42 maps MEMORY_LIMIT to MemoryError, OVERFLOW to
+ * OverflowError and the rest to ValueError. */
+typedef enum tcp_status {
+    TCP_OK = 0,
+    TCP_ERR_INVALID_STRUCTURE_LENGTH = 1, /* InvalidStructureLength{expected, actual} */
+    TCP_ERR_INVALID_CODE = 2,             /* InvalidCode{code, index} */
+    TCP_ERR_NON_FINITE_COEFFICIENT = 3,   /* NonFiniteCoefficient{index} */
+    TCP_ERR_MEMORY_LIMIT = 4,             /* MemoryLimit{requested, limit} */
+    TCP_ERR_OVERFLOW = 5,                 /* Overflow{context} */
+    TCP_ERR_INVALID_SECTOR = 6,           /* InvalidSector{context} */
+    TCP_ERR_SECTOR_LEAKAGE = 7,           /* SectorLeakage{source_index, expected, actual} */
+    TCP_ERR_INVALID_INDEX = 8,            /* InvalidIndex{context} */
+    TCP_ERR_INVALID_ARGUMENT = 9,         /* null pointer, bad dtype/storage, overlap */
+    TCP_ERR_INCOMPATIBLE_QUBIT_COUNTS = 10, /* IncompatibleQubitCounts{left, right} */
+    TCP_ERR_INVALID_WORD_LENGTH = 11,     /* InvalidWordLength{expected, actual} */
+    TCP_ERR_HIP = 12,                     /* HIP runtime failure (no reference counterpart) */
+    TCP_ERR_UNSUPPORTED = 13              /* valid input outside the GPU kernels' limits */
+} tcp_status;
+
+typedef enum tcp_dtype {
+    TCP_C128 = 0, /* complex128 amplitudes, FP64 coefficients, FP64 accumulation */
+    TCP_C64 = 1   /* complex64 amplitudes, coefficients rounded to FP32, FP64 accumulation */
+} tcp_dtype;
+
+typedef enum tcp_storage {
+    TCP_STORAGE_LAZY = 0, /* "lazy" (py/pauli.py:985-1036) */
+    TCP_STORAGE_EAGER = 1 /* "eager": same GPU kernels, transition count precomputed */
+} tcp_storage;
+
+typedef struct tcp_operator tcp_operator;
+typedef struct tcp_mvp_plan tcp_mvp_plan;
+typedef struct tcp_sector_plan tcp_sector_plan;
+
+/* ---- library ----------------------------------------------------------- */
+
+/* Library version string (static storage). */
+const char *tcp_version(void);
+
+/* Copy the calling thread's last error message into buf (NUL-terminated,
+ * truncated to len); returns the full message length. */
+size_t tcp_last_error(char *buf, size_t len);
+
+/* Number of visible HIP devices (0 when none). */
+int tcp_device_count(int *count);
+
+/* ---- operator: PauliOperator::from_terms (core/operator.rs:117-155),
+ *      reached from native/operator.rs:206-243 pauli_operator_native_array -- */
+
+/* codes: nterms x nqubits row-major u8; coef: 2*nterms doubles. */
+int tcp_operator_from_codes(uint32_t nqubits, uint64_t nterms, const uint8_t *codes,
+                            const double *coef, tcp_operator **out);
+
+/* Packed words (nterms x ceil(n/64) each, core/word.rs:63-91 from_words:
+ * padding bits are cleared), then canonicalised like from_terms. */
+int tcp_operator_from_words(uint32_t nqubits, uint64_t nterms, const uint64_t *x_words,
+                            const uint64_t *z_words, const double *coef, tcp_operator **out);
+
+uint32_t tcp_operator_nqubits(const tcp_operator *op);
+uint64_t tcp_operator_term_count(const tcp_operator *op);
+/* distinct_x_mask_count (native/operator.rs:50-59) */
+uint64_t tcp_operator_distinct_x_mask_count(const tcp_operator *op);
+
+/* Canonical terms as codes (term_count x nqubits) and coefficients. */
+int tcp_operator_export_codes(const tcp_operator *op, uint8_t *codes, double *coef);
+
+/* Canonical terms as BackendMvpPlan arrays (core/hamiltonian.rs:501-524):
+ * x_words/z_words are term_count x ceil(n/64). */
+int tcp_operator_export_words(const tcp_operator *op, uint64_t *x_words, uint64_t *z_words,
+                              double *coef);
+
+/* PauliOperator::analyze_charge (core/operator.rs:396-445) for a diagonal
+ * qubit charge: qubit_levels holds (level of |0>, level of |1>) per qubit. */
+int tcp_operator_analyze_charge(const tcp_operator *op, const double *qubit_levels,
+                                int *is_conserved, uint64_t *nonzero);
+
+void tcp_operator_destroy(tcp_operator *op);
+
+/* ---- full-space plan: MvpPlan (core/hamiltonian.rs:52-270) via
+ *      pauli_mvp_plan_handle (native/hamiltonian.rs:270-288) ------------- */
+
+/* Build a reusable plan on the current HIP device: index-space masks with the
+ * Y phase folded in, terms regrouped by x-mask, tables uploaded to HBM.
+ * n >= 64 is TCP_ERR_OVERFLOW (core/hamiltonian.rs:544-555).  max_bytes is
+ * checked only when >= 1 GiB, against the retained plan bytes. */
+int tcp_mvp_plan_create(const tcp_operator *op, int storage, uint64_t max_bytes,
+                        tcp_mvp_plan **out);
+
+/* Same, from canonical BackendMvpPlan arrays (no re-canonicalisation). */
+int tcp_mvp_plan_create_from_words(uint32_t nqubits, uint64_t nterms, const uint64_t *x_words,
+                                   const uint64_t *z_words, const double *coef, int storage,
+                                   uint64_t max_bytes, tcp_mvp_plan **out);
+
+uint32_t tcp_mvp_plan_nqubits(const tcp_mvp_plan *plan);
+uint64_t tcp_mvp_plan_term_count(const tcp_mvp_plan *plan);
+uint64_t tcp_mvp_plan_group_count(const tcp_mvp_plan *plan);
+/* number of distinct high-part classes of the x-masks above the tile bits */
+uint64_t tcp_mvp_plan_class_count(const tcp_mvp_plan *plan);
+uint64_t tcp_mvp_plan_dimension(const tcp_mvp_plan *plan);
+int tcp_mvp_plan_storage(const tcp_mvp_plan *plan);
+/* Nonzero transitions of the eager graph (MvpPlan::transition_count,
+ * core/hamiltonian.rs:165-180); UINT64_MAX for lazy plans. */
+uint64_t tcp_mvp_plan_transition_count(const tcp_mvp_plan *plan);
+/* Retained plan bytes (host + device tables). */
+uint64_t tcp_mvp_plan_estimated_bytes(const tcp_mvp_plan *plan);
+/* Grouped-gather algorithmic bytes of one apply:
+ * 2^n * (G * s_in + s_out) + T * (16 * ceil(n/64) + s_coef). */
+uint64_t tcp_mvp_plan_bytes_alg(const tcp_mvp_plan *plan, int dtype);
+
+/* MvpPlan::apply_into (core/hamiltonian.rs:204-270) on device buffers:
+ * output <- H * input, enqueued on `stream` (NULL = default stream),
+ * asynchronous.  Lengths are in amplitudes and must equal 2^n
+ * (TCP_ERR_INVALID_STRUCTURE_LENGTH otherwise); input and output must not
+ * overlap. */
+int tcp_mvp_plan_apply_into(const tcp_mvp_plan *plan, const void *input, void *output,
+                            uint64_t input_len, uint64_t output_len, int dtype, void *stream);
+
+/* Batched right-hand sides: nrhs vectors stored contiguously (vector r at
+ * offset r * 2^n amplitudes) in input and output. */
+int tcp_mvp_plan_apply_batch(const tcp_mvp_plan *plan, const void *input, void *output,
+                             uint64_t nrhs, int dtype, void *stream);
+
+/* NativeMvpPlan.apply_into (native/hamiltonian.rs:347-365) on host complex128
+ * buffers: stages through device memory and returns when done. */
+int tcp_mvp_plan_apply_host(const tcp_mvp_plan *plan, const double *input, double *output,
+                            uint64_t input_len, uint64_t output_len);
+
+void tcp_mvp_plan_destroy(tcp_mvp_plan *plan);
+
+/* PauliOperator::mvp (core/hamiltonian.rs:481-483, native/hamiltonian.rs:254-268):
+ * one-shot plan + apply on host complex128 buffers. */
+int tcp_mvp(const tcp_operator *op, const double *input, double *output, uint64_t input_len,
+            uint64_t output_len);
+
+/* ---- sectors ------------------------------------------------------------ */
+
+/* U1Sector (core/sector.rs:79-238): dimension C(n, k); rank/unrank of an
+ * integer bitstring (qubit 0 = MSB) in ascending integer order. n <= 64. */
+int tcp_u1_dimension(uint32_t nqubits, uint32_t particle_number, uint64_t *dim);
+int tcp_u1_rank(uint32_t nqubits, uint32_t particle_number, uint64_t bitstring, uint64_t *rank);
+int tcp_u1_unrank(uint32_t nqubits, uint32_t particle_number, uint64_t index,
+                  uint64_t *bitstring);
+
+/* Contiguous fixed-population blocks, the factorized ChargeSector layout
+ * (core/charge_sector.rs:193-217, 419-456, 973-1038): block j covers the next
+ * sizes[j] qubits from qubit 0 and holds pops[j] ones; rank is mixed radix
+ * with block 0 most significant.  A single block is the U(1) sector. */
+int tcp_blocks_dimension(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops,
+                         uint64_t *dim);
+int tcp_blocks_rank(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops,
+                    uint64_t bitstring, uint64_t *rank);
+int tcp_blocks_unrank(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops,
+                      uint64_t index, uint64_t *bitstring);
+
+/* Restricted plan for U1Sector(n, k): U1LazyMvpPlan::new (core/sector.rs:515-534)
+ * behind pauli_restrict_u1_lazy_handle (native/symmetry.rs:389-402), with the
+ * exact-conservation precondition of py/symmetry.py:675-681 checked on the
+ * host (TCP_ERR_SECTOR_LEAKAGE when violated). */
+int tcp_sector_plan_create_u1(const tcp_operator *op, uint32_t particle_number,
+                              uint64_t max_bytes, tcp_sector_plan **out);
+
+/* Restricted plan for contiguous fixed-population blocks: the factorized
+ * charge path prepare_charge_mvp_plan (core/charge/generic.rs:1087-1110)
+ * behind compile_mvp_pauli_handle (native/charge_sector.rs:342-365); each
+ * block's number charge must be exactly conserved (TCP_ERR_INVALID_SECTOR). */
+int tcp_sector_plan_create_blocks(const tcp_operator *op, uint32_t nblocks, const uint32_t *sizes,
+                                  const uint32_t *pops, uint64_t max_bytes,
+                                  tcp_sector_plan **out);
+
+uint64_t tcp_sector_plan_dimension(const tcp_sector_plan *plan);
+uint32_t tcp_sector_plan_nqubits(const tcp_sector_plan *plan);
+uint64_t tcp_sector_plan_term_count(const tcp_sector_plan *plan);
+uint64_t tcp_sector_plan_group_count(const tcp_sector_plan *plan);
+uint64_t tcp_sector_plan_estimated_bytes(const tcp_sector_plan *plan);
+
+/* U1LazyMvpPlan::apply_into (core/sector.rs:579-642) /
+ * PreparedChargeMvpPlan::apply_into (core/charge/generic.rs:1231-1248) on
+ * device buffers of `dimension` amplitudes; output overwritten. */
+int tcp_sector_plan_apply_into(const tcp_sector_plan *plan, const void *input, void *output,
+                               uint64_t input_len, uint64_t output_len, int dtype, void *stream);
+
+/* Host complex128 convenience variant (stages through device memory). */
+int tcp_sector_plan_apply_host(const tcp_sector_plan *plan, const double *input, double *output,
+                               uint64_t input_len, uint64_t output_len);
+
+/* Device-side rank/unrank of `count` values (u64 device arrays), bit-exact
+ * with U1Sector::rank / unrank and ChargeSectorPlan::rank / unrank.
+ * Bitstrings outside the sector rank to UINT64_MAX. */
+int tcp_sector_plan_rank_device(const tcp_sector_plan *plan, const uint64_t *bitstrings,
+                                uint64_t *ranks, uint64_t count, void *stream);
+int tcp_sector_plan_unrank_device(const tcp_sector_plan *plan, const uint64_t *ranks,
+                                  uint64_t *bitstrings, uint64_t count, void *stream);
+
+/* States of ranks first .. first+count-1 produced by the kernels' successor
+ * step (each thread unranks once, then advances `run` times), for bit-exact
+ * sweeps of the in-kernel basis iteration against U1BasisIterator
+ * (core/sector.rs:751-834). */
+int tcp_sector_plan_walk_device(const tcp_sector_plan *plan, uint64_t first, uint64_t *bitstrings,
+                                uint64_t count, uint32_t run, void *stream);
+
+void tcp_sector_plan_destroy(tcp_sector_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TENCIRPAULI_AMD_H */

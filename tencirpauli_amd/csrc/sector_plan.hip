@@ -1,0 +1,845 @@
+// Restricted (fixed particle number) MVP: U(1) sectors and contiguous
+// fixed-population block sectors, plan builder and gfx950 kernels.
+//
+// Replaces U1LazyMvpPlan::apply_into (core/sector.rs:579-642) and the
+// factorized charge path of apply_charge_mvp_from_prepared_plan_into
+// (core/charge/generic.rs:854-1051) for contiguous fixed-population blocks
+// (core/charge_sector.rs:150-217, 419-456).
+//
+// The reference scatters serially: for each source state, for each x-mask
+// group, value = sum_k w_k (-1)^popc(z_k & src); exact zeros skipped;
+// output[rank(src ^ x)] += value * state[src].  Here each thread owns R
+// consecutive destination ranks and gathers: for destination bits b and
+// group x the source is s = b ^ x; s contributes iff it lies in the sector
+// (the pair (s, b) is then exactly one of the reference's transitions), the
+// same exact-zero skip applies, and rank(s) selects the input amplitude.
+// No atomics; the output is written once.  Exact conservation is checked when
+// the plan is built (py/symmetry.py:675-681, py/charge.py:1128-1132), so no
+// transition can leave the sector.
+//
+// Ranks.  Block j occupies a contiguous field of the index (qubit 0 = MSB,
+// block 0 highest) and holds pop_j ones; the sector rank is mixed radix over
+// blocks with block 0 most significant, and a block's rank is the
+// combinatorial number system sum_i C(p_i, i) over its set bits p_1 < p_2 < ..
+// (ascending integer order: the order of U1Sector::rank, core/sector.rs:127-167,
+// and FixedPopulationIndex::rank_contiguous_binary_word,
+// core/charge_sector.rs:973-1007).  The kernel evaluates it one byte at a
+// time from an LDS table lut[chunk][ones below][byte]; unranking walks the
+// bits from the top against C(p, needed) and later ranks of a thread follow
+// by the next-larger-combination step (ascending order).
+#include <algorithm>
+#include <memory>
+#include <mutex>
+
+#include "tcp_internal.hpp"
+
+namespace {
+
+constexpr uint32_t kMaxBlocks = 4;
+constexpr uint32_t kSectorThreads = 256;
+constexpr uint32_t kRanksPerThread = 4;
+constexpr uint32_t kLdsLutLimit = 64 * 1024;  // bytes
+
+template <class T>
+using cptr = const __attribute__((address_space(4))) T *;
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+
+struct SectorArgs {
+    uint32_t nblocks;
+    uint32_t shift[kMaxBlocks];    // field position of block j in the index
+    uint32_t size[kMaxBlocks];
+    uint32_t pop[kMaxBlocks];
+    uint32_t nchunk[kMaxBlocks];   // bytes per block field
+    uint32_t lut_off[kMaxBlocks];  // offset of block j's table in lut
+    uint32_t stride[kMaxBlocks];   // mixed-radix stride of block j
+    uint64_t bmask[kMaxBlocks];    // (1 << size) - 1
+    uint64_t dim;
+    uint32_t G;
+    uint32_t lut_words;
+    const uint64_t *gx;      // [G] index-space x-masks
+    const uint32_t *goff;    // [G+1]
+    const uint64_t *tz;      // [T] index-space z-masks
+    const void *tw;          // [T] (re, im) of w = c * i^(#Y mod 4)
+    const uint32_t *lut;     // rank tables
+    const uint64_t *choose;  // [64 * 64] C(p, j), p, j < 64
+};
+
+__device__ __forceinline__ double flip_sign(double x, uint32_t parity) {
+    return __longlong_as_double(__double_as_longlong(x) ^ (static_cast<long long>(parity) << 63));
+}
+
+// The rank/unrank/successor helpers are __host__ __device__ so the same code
+// can be exercised on the CPU against the oracle.
+__host__ __device__ __forceinline__ bool in_sector(const SectorArgs &p, uint64_t s) {
+    bool ok = true;
+    for (uint32_t j = 0; j < p.nblocks; ++j)
+        ok &= static_cast<uint32_t>(__builtin_popcountll((s >> p.shift[j]) & p.bmask[j])) == p.pop[j];
+    return ok;
+}
+
+__host__ __device__ __forceinline__ uint32_t sector_rank(const SectorArgs &p, const uint32_t *lut, uint64_t s) {
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < p.nblocks; ++j) {
+        const uint64_t w = (s >> p.shift[j]) & p.bmask[j];
+        const uint32_t width = (p.pop[j] + 1) * 256;
+        const uint32_t *table = lut + p.lut_off[j];
+        uint32_t local = 0, below = 0;
+        for (uint32_t c = 0; c < p.nchunk[j]; ++c) {
+            const uint32_t byte = static_cast<uint32_t>(w >> (8 * c)) & 0xffu;
+            local += table[c * width + below * 256 + byte];
+            below += __builtin_popcount(byte);
+        }
+        rank += local * p.stride[j];
+    }
+    return rank;
+}
+
+__host__ __device__ uint64_t sector_unrank(const SectorArgs &p, uint64_t rank) {
+    const uint64_t *choose = p.choose;
+    uint64_t bits = 0;
+    for (uint32_t j = 0; j < p.nblocks; ++j) {
+        uint64_t local = rank / p.stride[j];
+        rank -= local * p.stride[j];
+        uint32_t needed = p.pop[j];
+        uint64_t w = 0;
+        for (int q = static_cast<int>(p.size[j]) - 1; q >= 0 && needed > 0; --q) {
+            const uint64_t c = choose[q * 64 + needed];  // C(q, needed)
+            if (local >= c) {
+                local -= c;
+                w |= 1ull << q;
+                --needed;
+            }
+        }
+        bits |= w << p.shift[j];
+    }
+    return bits;
+}
+
+// Next sector state in ascending rank order: advance the lowest block to its
+// next combination (Gosper's step); on overflow reset it and carry upwards.
+__host__ __device__ __forceinline__ uint64_t sector_next(const SectorArgs &p, uint64_t bits) {
+    for (int j = static_cast<int>(p.nblocks) - 1; j >= 0; --j) {
+        const uint64_t w = (bits >> p.shift[j]) & p.bmask[j];
+        uint64_t nw = ~0ull;
+        if (p.pop[j] > 0) {
+            const uint64_t c = w & (~w + 1);
+            const uint64_t r = w + c;
+            nw = r | (((r ^ w) >> 2) >> (__builtin_ffsll(static_cast<long long>(c)) - 1));
+        }
+        bits &= ~(p.bmask[j] << p.shift[j]);
+        if (p.pop[j] > 0 && nw <= p.bmask[j]) return bits | (nw << p.shift[j]);
+        const uint64_t first = p.pop[j] ? ((1ull << p.pop[j]) - 1) : 0;
+        bits |= first << p.shift[j];
+    }
+    return bits;
+}
+
+template <typename VT>
+struct Amp;
+template <>
+struct Amp<dvec2> {
+    __device__ static double re(const dvec2 &v) { return v.x; }
+    __device__ static double im(const dvec2 &v) { return v.y; }
+    __device__ static dvec2 make(double r, double i) { return dvec2{r, i}; }
+};
+template <>
+struct Amp<fvec2> {
+    __device__ static double re(const fvec2 &v) { return static_cast<double>(v.x); }
+    __device__ static double im(const fvec2 &v) { return static_cast<double>(v.y); }
+    __device__ static fvec2 make(double r, double i) {
+        return fvec2{static_cast<float>(r), static_cast<float>(i)};
+    }
+};
+
+template <typename VT, bool LDS>
+__global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT *__restrict__ v,
+                                                                       VT *__restrict__ y,
+                                                                       SectorArgs p) {
+    extern __shared__ uint32_t lds_lut[];
+    const uint32_t *lut = p.lut;
+    if (LDS) {
+        for (uint32_t i = threadIdx.x; i < p.lut_words; i += blockDim.x) lds_lut[i] = p.lut[i];
+        __syncthreads();
+        lut = lds_lut;
+    }
+    cptr<uint64_t> gx = (cptr<uint64_t>)p.gx;
+    cptr<uint32_t> goff = (cptr<uint32_t>)p.goff;
+    cptr<uint64_t> tz = (cptr<uint64_t>)p.tz;
+    cptr<dvec2> tw = (cptr<dvec2>)p.tw;
+
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * kRanksPerThread;
+    for (uint64_t r0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kRanksPerThread;
+         r0 < p.dim; r0 += stride) {
+        uint64_t b = sector_unrank(p, r0);
+        for (uint32_t j = 0; j < kRanksPerThread; ++j) {
+            const uint64_t r = r0 + j;
+            if (r >= p.dim) break;
+            double ar = 0.0, ai = 0.0;
+            for (uint32_t g = 0; g < p.G; ++g) {
+                const uint64_t x = gx[g];
+                const uint64_t s = b ^ x;
+                if (!in_sector(p, s)) continue;
+                double dr = 0.0, di = 0.0;
+                const uint32_t k1 = goff[g + 1];
+                for (uint32_t k = goff[g]; k < k1; ++k) {
+                    const uint32_t par = __popcll(tz[k] & s) & 1u;
+                    const dvec2 w = tw[k];
+                    dr += flip_sign(w.x, par);
+                    di += flip_sign(w.y, par);
+                }
+                if (dr == 0.0 && di == 0.0) continue;  // exact-zero skip (core/sector.rs:1099)
+                const uint64_t rs = x == 0 ? r : sector_rank(p, lut, s);
+                if (rs >= p.dim) continue;  // unreachable for in-sector s; keeps the gather in bounds
+                const VT sv = v[rs];
+                const double sr = Amp<VT>::re(sv), si = Amp<VT>::im(sv);
+                ar = fma(dr, sr, ar);
+                ar = fma(-di, si, ar);
+                ai = fma(dr, si, ai);
+                ai = fma(di, sr, ai);
+            }
+            y[r] = Amp<VT>::make(ar, ai);
+            b = sector_next(p, b);
+        }
+    }
+}
+
+__global__ void sector_rank_kernel(SectorArgs p, const uint64_t *__restrict__ bits,
+                                   uint64_t *__restrict__ ranks, uint64_t count, uint32_t nqubits) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < count;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t s = bits[i];
+        const bool inside = (nqubits == 64 || (s >> nqubits) == 0) && in_sector(p, s);
+        ranks[i] = inside ? sector_rank(p, p.lut, s) : ~0ull;
+    }
+}
+
+__global__ void sector_unrank_kernel(SectorArgs p, const uint64_t *__restrict__ ranks,
+                                     uint64_t *__restrict__ bits, uint64_t count) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < count;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t r = ranks[i];
+        bits[i] = r < p.dim ? sector_unrank(p, r) : ~0ull;
+    }
+}
+
+// Consecutive ranks through the successor step, for bit-exact sweeps of the
+// iterator (core/sector.rs:751-834): bits[i] = state of rank first + i.
+__global__ void sector_walk_kernel(SectorArgs p, uint64_t first, uint64_t *__restrict__ bits,
+                                   uint64_t count, uint32_t run) {
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    const uint64_t start = t * run;
+    if (start >= count) return;
+    uint64_t b = sector_unrank(p, first + start);
+    for (uint32_t j = 0; j < run && start + j < count; ++j) {
+        bits[start + j] = b;
+        b = sector_next(p, b);
+    }
+}
+
+// Leakage scan in the reference's scatter order (core/sector.rs:1051-1127,
+// core/charge/generic.rs:1013-1041): for every source state and x-group whose
+// destination leaves the sector, the group's summed value must be an exact
+// zero.  The smallest (source rank, group) pair that is not is recorded as
+// rank * G + group, which is the first error the reference's serial loop
+// raises.  State-independent, so it runs once per plan.
+__global__ __launch_bounds__(kSectorThreads) void sector_leak_kernel(SectorArgs p,
+                                                                     unsigned long long *first) {
+    cptr<uint64_t> gx = (cptr<uint64_t>)p.gx;
+    cptr<uint32_t> goff = (cptr<uint32_t>)p.goff;
+    cptr<uint64_t> tz = (cptr<uint64_t>)p.tz;
+    cptr<dvec2> tw = (cptr<dvec2>)p.tw;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * kRanksPerThread;
+    for (uint64_t r0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kRanksPerThread;
+         r0 < p.dim; r0 += stride) {
+        uint64_t b = sector_unrank(p, r0);
+        for (uint32_t j = 0; j < kRanksPerThread; ++j) {
+            const uint64_t r = r0 + j;
+            if (r >= p.dim) break;
+            for (uint32_t g = 0; g < p.G; ++g) {
+                const uint64_t s = b ^ gx[g];
+                if (in_sector(p, s)) continue;
+                double dr = 0.0, di = 0.0;
+                const uint32_t k1 = goff[g + 1];
+                for (uint32_t k = goff[g]; k < k1; ++k) {
+                    const uint32_t par = __popcll(tz[k] & b) & 1u;
+                    const dvec2 w = tw[k];
+                    dr += flip_sign(w.x, par);
+                    di += flip_sign(w.y, par);
+                }
+                if (dr != 0.0 || di != 0.0) {
+                    atomicMin(first, static_cast<unsigned long long>(r) * p.G + g);
+                    break;
+                }
+            }
+            b = sector_next(p, b);
+        }
+    }
+}
+
+}  // namespace
+
+struct tcp_sector_plan {
+    uint32_t n = 0;
+    uint32_t nblocks = 0;
+    uint32_t sizes[kMaxBlocks] = {}, pops[kMaxBlocks] = {};
+    uint64_t dim = 0, T = 0, G = 0;
+    int device = 0;
+    bool lut_in_lds = true;
+    bool u1 = true;
+    bool leaks = false;        // the reference's apply would raise
+    uint64_t leak_source = 0;  // first leaking source rank
+    uint32_t leak_weight = 0;  // its destination weight (U(1) message)
+    SectorArgs host_args{};
+    tcp::DeviceBuffer gx, goff, tz, tw64, tw32, lut, choose;
+    uint64_t retained_bytes = 0;
+    std::mutex staging_mutex;
+    tcp::DeviceBuffer stage_in, stage_out;
+
+    SectorArgs args(int dtype) const {
+        SectorArgs a = host_args;
+        a.gx = static_cast<const uint64_t *>(gx.ptr);
+        a.goff = static_cast<const uint32_t *>(goff.ptr);
+        a.tz = static_cast<const uint64_t *>(tz.ptr);
+        a.tw = dtype == TCP_C64 ? tw32.ptr : tw64.ptr;
+        a.lut = static_cast<const uint32_t *>(lut.ptr);
+        a.choose = static_cast<const uint64_t *>(choose.ptr);
+        return a;
+    }
+};
+
+namespace {
+
+using tcp::fail;
+
+// C(n, k) with overflow detection (core/sector.rs:1342-1362).
+bool choose_checked(uint32_t n, uint32_t k, uint64_t &out) {
+    if (k > n) {
+        out = 0;
+        return true;
+    }
+    if (k > n - k) k = n - k;
+    unsigned __int128 value = 1;
+    for (uint32_t i = 1; i <= k; ++i) {
+        value = value * (n - k + i) / i;
+        if (value > static_cast<unsigned __int128>(UINT64_MAX)) return false;
+    }
+    out = static_cast<uint64_t>(value);
+    return true;
+}
+
+struct BlockSpec {
+    uint32_t nblocks;
+    uint32_t sizes[kMaxBlocks], pops[kMaxBlocks];
+};
+
+// Host-side sector geometry shared by rank/unrank entry points and plans.
+int sector_geometry(const BlockSpec &spec, SectorArgs &a, uint64_t &dim, std::vector<uint32_t> &lut,
+                    std::vector<uint64_t> &choose_table) {
+    if (spec.nblocks == 0) return fail(TCP_ERR_INVALID_SECTOR, "invalid sector: no blocks");
+    if (spec.nblocks > kMaxBlocks)
+        return fail(TCP_ERR_UNSUPPORTED, "at most 4 contiguous blocks are supported on the GPU");
+    uint32_t n = 0;
+    for (uint32_t j = 0; j < spec.nblocks; ++j) {
+        if (spec.pops[j] > spec.sizes[j])
+            return fail(TCP_ERR_INVALID_SECTOR, "invalid sector: particle_number must be between 0 and nqubits");
+        n += spec.sizes[j];
+    }
+    if (n > 64) return fail(TCP_ERR_UNSUPPORTED, "sector bitstrings are limited to 64 qubits");
+    a = SectorArgs{};
+    a.nblocks = spec.nblocks;
+    uint32_t shift = n;
+    unsigned __int128 total = 1;
+    uint64_t bdim[kMaxBlocks];
+    for (uint32_t j = 0; j < spec.nblocks; ++j) {
+        shift -= spec.sizes[j];
+        a.shift[j] = shift;
+        a.size[j] = spec.sizes[j];
+        a.pop[j] = spec.pops[j];
+        a.bmask[j] = spec.sizes[j] == 0 ? 0 : (spec.sizes[j] >= 64 ? ~0ull : (1ull << spec.sizes[j]) - 1);
+        a.nchunk[j] = (spec.sizes[j] + 7) / 8;
+        if (!choose_checked(spec.sizes[j], spec.pops[j], bdim[j]))
+            return fail(TCP_ERR_OVERFLOW, "integer overflow while computing U1 sector dimension");
+        total *= bdim[j];
+        if (total > static_cast<unsigned __int128>(UINT64_MAX))
+            return fail(TCP_ERR_OVERFLOW, "integer overflow while computing U1 sector dimension");
+    }
+    dim = static_cast<uint64_t>(total);
+    a.dim = dim;
+    uint64_t stride = 1;
+    for (int j = static_cast<int>(spec.nblocks) - 1; j >= 0; --j) {
+        a.stride[j] = static_cast<uint32_t>(stride);
+        stride *= bdim[j];
+    }
+    choose_table.assign(64 * 64, 0);
+    for (uint32_t q = 0; q < 64; ++q)
+        for (uint32_t k = 0; k < 64; ++k) {
+            uint64_t c = 0;
+            if (!choose_checked(q, k, c)) c = UINT64_MAX;
+            choose_table[q * 64 + k] = c;
+        }
+    lut.clear();
+    for (uint32_t j = 0; j < spec.nblocks; ++j) {
+        a.lut_off[j] = static_cast<uint32_t>(lut.size());
+        const uint32_t k = spec.pops[j];
+        for (uint32_t c = 0; c < a.nchunk[j]; ++c)
+            for (uint32_t below = 0; below <= k; ++below)
+                for (uint32_t byte = 0; byte < 256; ++byte) {
+                    uint64_t value = 0;
+                    uint32_t seen = 0;
+                    for (uint32_t q = 0; q < 8; ++q) {
+                        if (!((byte >> q) & 1)) continue;
+                        const uint32_t pos = 8 * c + q;
+                        const uint32_t index = below + seen + 1;
+                        ++seen;
+                        if (pos < spec.sizes[j] && index <= k) value += choose_table[pos * 64 + index];
+                    }
+                    lut.push_back(static_cast<uint32_t>(value));
+                }
+    }
+    a.lut_words = static_cast<uint32_t>(lut.size());
+    return TCP_OK;
+}
+
+int scan_leakage(tcp_sector_plan *plan, const std::vector<uint64_t> &gx);
+
+int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, uint64_t max_bytes,
+                      tcp_sector_plan **out) {
+    if (!out) return fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    uint32_t n = 0;
+    for (uint32_t j = 0; j < spec.nblocks && j < kMaxBlocks; ++j) n += spec.sizes[j];
+    if (op.nqubits != n)
+        return fail(TCP_ERR_INCOMPATIBLE_QUBIT_COUNTS, "incompatible qubit counts: " +
+                                                            std::to_string(op.nqubits) + " and " +
+                                                            std::to_string(n));
+    SectorArgs a;
+    uint64_t dim = 0;
+    std::vector<uint32_t> lut;
+    std::vector<uint64_t> choose_table;
+    int st = sector_geometry(spec, a, dim, lut, choose_table);
+    if (st) return st;
+    if (n > 63) return fail(TCP_ERR_UNSUPPORTED, "GPU sector kernels support at most 63 qubits");
+    if (dim >= (1ull << 32))
+        return fail(TCP_ERR_UNSUPPORTED, "GPU sector kernels support dimensions below 2^32");
+
+    // Exact conservation of each block's particle number: the precondition
+    // of lazy restriction (py/symmetry.py:675-681, py/charge.py:1128-1132).
+    for (uint32_t j = 0; j < spec.nblocks; ++j) {
+        std::vector<double> levels(2 * n, 0.0);
+        const uint32_t first = n - a.shift[j] - spec.sizes[j];
+        for (uint32_t q = first; q < first + spec.sizes[j]; ++q) levels[2 * q + 1] = 1.0;
+        bool conserved = false;
+        uint64_t nonzero = 0;
+        if ((st = tcp::analyze_charge(op, levels.data(), conserved, nonzero))) return st;
+        if (!conserved)
+            return fail(TCP_ERR_SECTOR_LEAKAGE,
+                        u1 ? "selected U1 sector requires an exactly conserved operator; nonzero "
+                             "U(1) sector leakage was detected"
+                           : "selected charge sector requires an exactly conserved operator");
+    }
+
+    auto plan = new tcp_sector_plan();
+    std::unique_ptr<tcp_sector_plan> guard(plan);
+    plan->n = n;
+    plan->nblocks = spec.nblocks;
+    for (uint32_t j = 0; j < spec.nblocks; ++j) {
+        plan->sizes[j] = spec.sizes[j];
+        plan->pops[j] = spec.pops[j];
+    }
+    plan->dim = dim;
+
+    // x-mask groups in first-appearance order over the canonical terms; a
+    // group's terms keep canonical order (compile_terms, core/sector.rs:926-1021).
+    std::vector<tcp::IndexTerm> terms = tcp::index_terms(op);
+    std::vector<uint64_t> gx;
+    std::vector<std::vector<uint64_t>> members;
+    for (uint64_t k = 0; k < terms.size(); ++k) {
+        size_t g = std::find(gx.begin(), gx.end(), terms[k].x) - gx.begin();
+        if (g == gx.size()) {
+            gx.push_back(terms[k].x);
+            members.emplace_back();
+        }
+        members[g].push_back(k);
+    }
+    std::vector<uint32_t> goff;
+    std::vector<uint64_t> tz;
+    std::vector<dvec2> tw64, tw32;
+    for (size_t g = 0; g < gx.size(); ++g) {
+        goff.push_back(static_cast<uint32_t>(tz.size()));
+        for (uint64_t k : members[g]) {
+            tz.push_back(terms[k].z);
+            tw64.push_back(dvec2{terms[k].w.re, terms[k].w.im});
+            tw32.push_back(dvec2{static_cast<double>(static_cast<float>(terms[k].w.re)),
+                                 static_cast<double>(static_cast<float>(terms[k].w.im))});
+        }
+    }
+    goff.push_back(static_cast<uint32_t>(tz.size()));
+    plan->T = tz.size();
+    plan->G = gx.size();
+    a.G = static_cast<uint32_t>(gx.size());
+    if (gx.empty()) gx.push_back(0);
+    if (tz.empty()) {
+        tz.push_back(0);
+        tw64.push_back(dvec2{0, 0});
+        tw32.push_back(dvec2{0, 0});
+    }
+    plan->lut_in_lds = lut.size() * 4 <= kLdsLutLimit;
+    plan->host_args = a;
+    plan->retained_bytes = sizeof(tcp_sector_plan) + gx.size() * 8 + goff.size() * 4 +
+                           tz.size() * (8 + 32) + lut.size() * 4 + choose_table.size() * 8;
+    if (tcp::exceeds_budget(plan->retained_bytes, max_bytes))
+        return tcp::memory_limit(plan->retained_bytes, max_bytes);
+
+    TCP_HIP_CHECK(hipGetDevice(&plan->device));
+    if ((st = plan->gx.upload(gx.data(), gx.size() * 8))) return st;
+    if ((st = plan->goff.upload(goff.data(), goff.size() * 4))) return st;
+    if ((st = plan->tz.upload(tz.data(), tz.size() * 8))) return st;
+    if ((st = plan->tw64.upload(tw64.data(), tw64.size() * 16))) return st;
+    if ((st = plan->tw32.upload(tw32.data(), tw32.size() * 16))) return st;
+    if ((st = plan->lut.upload(lut.data(), lut.size() * 4))) return st;
+    if ((st = plan->choose.upload(choose_table.data(), choose_table.size() * 8))) return st;
+    plan->u1 = u1;
+    if ((st = scan_leakage(plan, gx))) return st;
+    *out = guard.release();
+    return tcp::ok();
+}
+
+int host_unrank(const BlockSpec &spec, uint64_t index, uint64_t &bits);
+
+int scan_leakage(tcp_sector_plan *plan, const std::vector<uint64_t> &gx) {
+    if (plan->G == 0 || plan->dim == 0) return TCP_OK;
+    unsigned long long *dfirst = nullptr;
+    TCP_HIP_CHECK(hipMalloc(&dfirst, sizeof(unsigned long long)));
+    const unsigned long long none = ~0ull;
+    hipError_t e = hipMemcpy(dfirst, &none, sizeof none, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const uint64_t per_block = static_cast<uint64_t>(kSectorThreads) * kRanksPerThread;
+        const uint64_t blocks = std::min<uint64_t>((plan->dim + per_block - 1) / per_block, 256 * 16);
+        hipLaunchKernelGGL(sector_leak_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(kSectorThreads), 0,
+                           nullptr, plan->args(TCP_C128), dfirst);
+        e = hipGetLastError();
+    }
+    unsigned long long first = none;
+    if (e == hipSuccess) e = hipMemcpy(&first, dfirst, sizeof first, hipMemcpyDeviceToHost);
+    (void)hipFree(dfirst);
+    if (e != hipSuccess) return tcp::hip_fail(e, "sector leakage scan");
+    if (first != none) {
+        plan->leaks = true;
+        plan->leak_source = first / plan->G;
+        const uint64_t g = first % plan->G;
+        uint64_t bits = 0;
+        std::vector<uint32_t> lut;
+        std::vector<uint64_t> ch;
+        BlockSpec spec{};
+        spec.nblocks = plan->nblocks;
+        for (uint32_t j = 0; j < plan->nblocks; ++j) {
+            spec.sizes[j] = plan->sizes[j];
+            spec.pops[j] = plan->pops[j];
+        }
+        if (host_unrank(spec, plan->leak_source, bits)) return TCP_ERR_INVALID_INDEX;
+        plan->leak_weight = static_cast<uint32_t>(__builtin_popcountll(bits ^ gx[g]));
+    }
+    return TCP_OK;
+}
+
+int leak_error(const tcp_sector_plan *plan) {
+    if (plan->u1)
+        return fail(TCP_ERR_SECTOR_LEAKAGE,
+                    "U(1) sector leakage from restricted source index " + std::to_string(plan->leak_source) +
+                        ": expected particle number " + std::to_string(plan->pops[0]) +
+                        ", got destination weight " + std::to_string(plan->leak_weight));
+    return fail(TCP_ERR_INVALID_SECTOR, "invalid sector: operator leaks outside the selected charge sector");
+}
+
+int check_len(uint64_t expected, uint64_t actual) {
+    if (expected != actual)
+        return fail(TCP_ERR_INVALID_STRUCTURE_LENGTH, "expected structure length " +
+                                                          std::to_string(expected) + ", got " +
+                                                          std::to_string(actual));
+    return TCP_OK;
+}
+
+template <typename VT>
+hipError_t launch_gather(const tcp_sector_plan *plan, const void *in, void *out, int dtype,
+                         hipStream_t stream) {
+    SectorArgs a = plan->args(dtype);
+    const uint64_t per_block = static_cast<uint64_t>(kSectorThreads) * kRanksPerThread;
+    const uint64_t blocks = std::min<uint64_t>((plan->dim + per_block - 1) / per_block, 256 * 16);
+    if (plan->lut_in_lds)
+        hipLaunchKernelGGL((sector_gather_kernel<VT, true>), dim3(static_cast<uint32_t>(blocks)),
+                           dim3(kSectorThreads), a.lut_words * 4, stream, static_cast<const VT *>(in),
+                           static_cast<VT *>(out), a);
+    else
+        hipLaunchKernelGGL((sector_gather_kernel<VT, false>), dim3(static_cast<uint32_t>(blocks)),
+                           dim3(kSectorThreads), 0, stream, static_cast<const VT *>(in),
+                           static_cast<VT *>(out), a);
+    return hipGetLastError();
+}
+
+int apply_device(const tcp_sector_plan *plan, const void *in, void *out, int dtype, void *stream) {
+    if (dtype != TCP_C128 && dtype != TCP_C64)
+        return fail(TCP_ERR_INVALID_ARGUMENT, "dtype must be TCP_C128 or TCP_C64");
+    if (!in || !out) return fail(TCP_ERR_INVALID_ARGUMENT, "null state buffer");
+    if (plan->leaks) return leak_error(plan);
+    const size_t bytes = static_cast<size_t>(plan->dim) * (dtype == TCP_C128 ? 16 : 8);
+    if (tcp::ranges_overlap(in, bytes, out, bytes))
+        return fail(TCP_ERR_INVALID_ARGUMENT, "apply_into input and output must not overlap");
+    tcp::DeviceGuard dg;
+    int st = dg.enter(plan->device);
+    if (st) return st;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = dtype == TCP_C128 ? launch_gather<dvec2>(plan, in, out, dtype, s)
+                                     : launch_gather<fvec2>(plan, in, out, dtype, s);
+    if (e != hipSuccess) return tcp::hip_fail(e, "sector kernel launch");
+    return tcp::ok();
+}
+
+BlockSpec u1_spec(uint32_t n, uint32_t k) {
+    BlockSpec s{};
+    s.nblocks = 1;
+    s.sizes[0] = n;
+    s.pops[0] = k;
+    return s;
+}
+
+int blocks_spec(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops, BlockSpec &s) {
+    if (nblocks == 0 || !sizes || !pops) return fail(TCP_ERR_INVALID_SECTOR, "invalid sector: no blocks");
+    if (nblocks > kMaxBlocks)
+        return fail(TCP_ERR_UNSUPPORTED, "at most 4 contiguous blocks are supported on the GPU");
+    s = BlockSpec{};
+    s.nblocks = nblocks;
+    for (uint32_t j = 0; j < nblocks; ++j) {
+        s.sizes[j] = sizes[j];
+        s.pops[j] = pops[j];
+    }
+    return TCP_OK;
+}
+
+// Host rank/unrank with the same tables as the kernels (used by the public
+// scalar entry points).
+int host_rank(const BlockSpec &spec, uint64_t bits, uint64_t &rank) {
+    SectorArgs a;
+    uint64_t dim;
+    std::vector<uint32_t> lut;
+    std::vector<uint64_t> ch;
+    int st = sector_geometry(spec, a, dim, lut, ch);
+    if (st) return st;
+    uint32_t n = 0;
+    for (uint32_t j = 0; j < spec.nblocks; ++j) n += spec.sizes[j];
+    if (n < 64 && (bits >> n))
+        return fail(TCP_ERR_INVALID_INDEX, "bitstring is outside the computational basis");
+    unsigned __int128 r = 0;
+    for (uint32_t j = 0; j < spec.nblocks; ++j) {
+        const uint64_t w = (bits >> a.shift[j]) & a.bmask[j];
+        if (static_cast<uint32_t>(__builtin_popcountll(w)) != spec.pops[j])
+            return fail(TCP_ERR_INVALID_INDEX, "invalid index: packed basis word has the wrong Hamming weight");
+        uint64_t local = 0;
+        uint32_t seen = 0;
+        for (uint32_t q = 0; q < spec.sizes[j]; ++q)
+            if ((w >> q) & 1) local += ch[q * 64 + (++seen)];
+        uint64_t stride = 1;
+        for (uint32_t i = j + 1; i < spec.nblocks; ++i) {
+            uint64_t c;
+            choose_checked(spec.sizes[i], spec.pops[i], c);
+            stride *= c;
+        }
+        r += static_cast<unsigned __int128>(local) * stride;
+    }
+    rank = static_cast<uint64_t>(r);
+    return TCP_OK;
+}
+
+int host_unrank(const BlockSpec &spec, uint64_t index, uint64_t &bits) {
+    SectorArgs a;
+    uint64_t dim;
+    std::vector<uint32_t> lut;
+    std::vector<uint64_t> ch;
+    int st = sector_geometry(spec, a, dim, lut, ch);
+    if (st) return st;
+    if (index >= dim) return fail(TCP_ERR_INVALID_INDEX, "invalid index: restricted basis index is out of range");
+    bits = 0;
+    for (uint32_t j = 0; j < spec.nblocks; ++j) {
+        uint64_t stride = 1;
+        for (uint32_t i = j + 1; i < spec.nblocks; ++i) {
+            uint64_t c;
+            choose_checked(spec.sizes[i], spec.pops[i], c);
+            stride *= c;
+        }
+        uint64_t local = index / stride;
+        index -= local * stride;
+        uint32_t needed = spec.pops[j];
+        uint64_t w = 0;
+        for (int q = static_cast<int>(spec.sizes[j]) - 1; q >= 0 && needed > 0; --q) {
+            const uint64_t c = ch[q * 64 + needed];
+            if (local >= c) {
+                local -= c;
+                w |= 1ull << q;
+                --needed;
+            }
+        }
+        bits |= w << a.shift[j];
+    }
+    return TCP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tcp_u1_dimension(uint32_t nqubits, uint32_t particle_number, uint64_t *dim) {
+    if (!dim) return fail(TCP_ERR_INVALID_ARGUMENT, "null output");
+    if (particle_number > nqubits)
+        return fail(TCP_ERR_INVALID_SECTOR, "invalid sector: particle_number must be between 0 and nqubits");
+    if (!choose_checked(nqubits, particle_number, *dim))
+        return fail(TCP_ERR_OVERFLOW, "integer overflow while computing U1 sector dimension");
+    return tcp::ok();
+}
+
+int tcp_u1_rank(uint32_t nqubits, uint32_t particle_number, uint64_t bitstring, uint64_t *rank) {
+    if (!rank) return fail(TCP_ERR_INVALID_ARGUMENT, "null output");
+    int st = host_rank(u1_spec(nqubits, particle_number), bitstring, *rank);
+    return st ? st : tcp::ok();
+}
+
+int tcp_u1_unrank(uint32_t nqubits, uint32_t particle_number, uint64_t index, uint64_t *bitstring) {
+    if (!bitstring) return fail(TCP_ERR_INVALID_ARGUMENT, "null output");
+    int st = host_unrank(u1_spec(nqubits, particle_number), index, *bitstring);
+    return st ? st : tcp::ok();
+}
+
+int tcp_blocks_dimension(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops, uint64_t *dim) {
+    if (!dim) return fail(TCP_ERR_INVALID_ARGUMENT, "null output");
+    BlockSpec s;
+    int st = blocks_spec(nblocks, sizes, pops, s);
+    if (st) return st;
+    SectorArgs a;
+    std::vector<uint32_t> lut;
+    std::vector<uint64_t> ch;
+    st = sector_geometry(s, a, *dim, lut, ch);
+    return st ? st : tcp::ok();
+}
+
+int tcp_blocks_rank(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops, uint64_t bitstring,
+                    uint64_t *rank) {
+    if (!rank) return fail(TCP_ERR_INVALID_ARGUMENT, "null output");
+    BlockSpec s;
+    int st = blocks_spec(nblocks, sizes, pops, s);
+    if (!st) st = host_rank(s, bitstring, *rank);
+    return st ? st : tcp::ok();
+}
+
+int tcp_blocks_unrank(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops, uint64_t index,
+                      uint64_t *bitstring) {
+    if (!bitstring) return fail(TCP_ERR_INVALID_ARGUMENT, "null output");
+    BlockSpec s;
+    int st = blocks_spec(nblocks, sizes, pops, s);
+    if (!st) st = host_unrank(s, index, *bitstring);
+    return st ? st : tcp::ok();
+}
+
+int tcp_sector_plan_create_u1(const tcp_operator *op, uint32_t particle_number, uint64_t max_bytes,
+                              tcp_sector_plan **out) {
+    if (!op) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
+    const tcp::Operator &o = tcp::operator_of(op);
+    if (particle_number > o.nqubits)
+        return fail(TCP_ERR_INVALID_SECTOR, "invalid sector: particle_number must be between 0 and nqubits");
+    return build_sector_plan(o, u1_spec(o.nqubits, particle_number), true, max_bytes, out);
+}
+
+int tcp_sector_plan_create_blocks(const tcp_operator *op, uint32_t nblocks, const uint32_t *sizes,
+                                  const uint32_t *pops, uint64_t max_bytes, tcp_sector_plan **out) {
+    if (!op) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
+    BlockSpec s;
+    int st = blocks_spec(nblocks, sizes, pops, s);
+    if (st) return st;
+    return build_sector_plan(tcp::operator_of(op), s, false, max_bytes, out);
+}
+
+uint64_t tcp_sector_plan_dimension(const tcp_sector_plan *plan) { return plan ? plan->dim : 0; }
+uint32_t tcp_sector_plan_nqubits(const tcp_sector_plan *plan) { return plan ? plan->n : 0; }
+uint64_t tcp_sector_plan_term_count(const tcp_sector_plan *plan) { return plan ? plan->T : 0; }
+uint64_t tcp_sector_plan_group_count(const tcp_sector_plan *plan) { return plan ? plan->G : 0; }
+uint64_t tcp_sector_plan_estimated_bytes(const tcp_sector_plan *plan) {
+    return plan ? plan->retained_bytes : 0;
+}
+
+int tcp_sector_plan_apply_into(const tcp_sector_plan *plan, const void *input, void *output,
+                               uint64_t input_len, uint64_t output_len, int dtype, void *stream) {
+    if (!plan) return fail(TCP_ERR_INVALID_ARGUMENT, "null plan");
+    int st;
+    if ((st = check_len(plan->dim, input_len))) return st;
+    if ((st = check_len(plan->dim, output_len))) return st;
+    return apply_device(plan, input, output, dtype, stream);
+}
+
+int tcp_sector_plan_apply_host(const tcp_sector_plan *plan, const double *input, double *output,
+                               uint64_t input_len, uint64_t output_len) {
+    if (!plan) return fail(TCP_ERR_INVALID_ARGUMENT, "null plan");
+    int st;
+    if ((st = check_len(plan->dim, input_len))) return st;
+    if ((st = check_len(plan->dim, output_len))) return st;
+    if (!input || !output) return fail(TCP_ERR_INVALID_ARGUMENT, "null state buffer");
+    const size_t bytes = static_cast<size_t>(plan->dim) * 16;
+    if (tcp::ranges_overlap(input, bytes, output, bytes))
+        return fail(TCP_ERR_INVALID_ARGUMENT, "apply_into input and output must not overlap");
+    tcp::DeviceGuard dg;
+    if ((st = dg.enter(plan->device))) return st;
+    auto *mp = const_cast<tcp_sector_plan *>(plan);
+    std::lock_guard<std::mutex> lock(mp->staging_mutex);
+    if (mp->stage_in.bytes < bytes && (st = mp->stage_in.allocate(bytes))) return st;
+    if (mp->stage_out.bytes < bytes && (st = mp->stage_out.allocate(bytes))) return st;
+    TCP_HIP_CHECK(hipMemcpy(mp->stage_in.ptr, input, bytes, hipMemcpyHostToDevice));
+    if ((st = apply_device(plan, mp->stage_in.ptr, mp->stage_out.ptr, TCP_C128, nullptr))) return st;
+    TCP_HIP_CHECK(hipMemcpy(output, mp->stage_out.ptr, bytes, hipMemcpyDeviceToHost));
+    return tcp::ok();
+}
+
+int tcp_sector_plan_rank_device(const tcp_sector_plan *plan, const uint64_t *bitstrings, uint64_t *ranks,
+                                uint64_t count, void *stream) {
+    if (!plan || (count && (!bitstrings || !ranks))) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    if (count == 0) return tcp::ok();
+    tcp::DeviceGuard dg;
+    int st = dg.enter(plan->device);
+    if (st) return st;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((count + 255) / 256, 8192));
+    hipLaunchKernelGGL(sector_rank_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       plan->args(TCP_C128), bitstrings, ranks, count, plan->n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? tcp::ok() : tcp::hip_fail(e, "sector rank kernel");
+}
+
+int tcp_sector_plan_unrank_device(const tcp_sector_plan *plan, const uint64_t *ranks, uint64_t *bitstrings,
+                                  uint64_t count, void *stream) {
+    if (!plan || (count && (!bitstrings || !ranks))) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    if (count == 0) return tcp::ok();
+    tcp::DeviceGuard dg;
+    int st = dg.enter(plan->device);
+    if (st) return st;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((count + 255) / 256, 8192));
+    hipLaunchKernelGGL(sector_unrank_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       plan->args(TCP_C128), ranks, bitstrings, count);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? tcp::ok() : tcp::hip_fail(e, "sector unrank kernel");
+}
+
+int tcp_sector_plan_walk_device(const tcp_sector_plan *plan, uint64_t first, uint64_t *bitstrings,
+                                uint64_t count, uint32_t run, void *stream) {
+    if (!plan || (count && !bitstrings) || run == 0) return fail(TCP_ERR_INVALID_ARGUMENT, "bad argument");
+    if (count == 0) return tcp::ok();
+    if (first > plan->dim || count > plan->dim - first)
+        return fail(TCP_ERR_INVALID_INDEX, "invalid index: restricted basis index is out of range");
+    tcp::DeviceGuard dg;
+    int st = dg.enter(plan->device);
+    if (st) return st;
+    const uint64_t threads = (count + run - 1) / run;
+    const uint32_t blocks = static_cast<uint32_t>((threads + 255) / 256);
+    hipLaunchKernelGGL(sector_walk_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       plan->args(TCP_C128), first, bitstrings, count, run);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? tcp::ok() : tcp::hip_fail(e, "sector walk kernel");
+}
+
+void tcp_sector_plan_destroy(tcp_sector_plan *plan) { delete plan; }
+
+}  // extern "C"

@@ -1,0 +1,358 @@
+"""PauliOperator and NativeMVPPlan: the reference's operator/plan API on the GPU.
+
+Mirrors py/pauli.py (``PauliOperator.__init__`` 324-354, ``from_code_arrays``
+397-421, ``restrict_charge`` 705-736, ``mvp`` 931-952, ``backend_mvp_plan``
+954-983, ``native_mvp_plan`` 985-1036, ``to_scipy_linear_operator``
+1038-1045) and py/hamiltonian.py ``NativeMVPPlan`` (162-344).  Construction
+canonicalises in the native library (PauliOperator::from_terms,
+core/operator.rs:117-155); every apply is one call into the gfx950 kernels.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Any, Iterable, Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+from . import _lib
+from ._plan import (
+    DEFAULT_MAX_BYTES,
+    NativePlanFacade,
+    check_allocation,
+    effective_max_bytes,
+    validate_max_bytes,
+)
+
+_LETTERS = {"I": 0, "X": 1, "Y": 2, "Z": 3}
+PauliInput = Union[str, Sequence[int]]
+
+
+def _codes_of(nqubits: int, word: PauliInput) -> Tuple[int, ...]:
+    if isinstance(word, str):
+        try:
+            codes = tuple(_LETTERS[ch] for ch in word.upper())
+        except KeyError as exc:
+            raise ValueError(f"invalid Pauli letter {exc.args[0]!r}; expected I/X/Y/Z") from None
+    else:
+        codes = tuple(int(c) for c in word)
+    if len(codes) != nqubits:
+        raise ValueError(f"expected structure length {nqubits}, got {len(codes)}")
+    return codes
+
+
+class _OperatorHandle:
+    """Owns one native canonical operator."""
+
+    def __init__(self, ptr: ctypes.c_void_p) -> None:
+        self.ptr = ptr
+
+    def __del__(self) -> None:
+        if getattr(self, "ptr", None) and _lib._lib is not None:
+            _lib.lib().tcp_operator_destroy(self.ptr)
+            self.ptr = None
+
+
+@dataclass(frozen=True)
+class PauliTerm:
+    """One canonical term: codes from qubit 0 and its complex coefficient."""
+
+    codes: Tuple[int, ...]
+    coefficient: complex
+
+    def to_string(self) -> str:
+        return "".join("IXYZ"[c] for c in self.codes)
+
+
+class PauliOperator:
+    """Deterministic sparse Pauli operator with exact-zero aggregation.
+
+    Terms are canonicalised on construction: duplicate words are summed in
+    IEEE-bit-pattern order, exact zeros removed and the survivors sorted by
+    code tuple from qubit 0 (I < X < Y < Z).  Qubit 0 is the most significant
+    bit of every computational index.
+    """
+
+    def __init__(self, nqubits: int, terms: Iterable[Tuple[PauliInput, complex]], *,
+                 max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> None:
+        validate_max_bytes(max_bytes)
+        if not isinstance(nqubits, int) or isinstance(nqubits, bool) or nqubits < 0:
+            raise ValueError("nqubits must be a non-negative integer")
+        rows = []
+        coefs = []
+        for word, coefficient in terms:
+            rows.append(_codes_of(nqubits, word))
+            coefs.append(complex(coefficient))
+        codes = np.asarray(rows, dtype=np.int64).reshape(len(rows), nqubits)
+        self._init_from_arrays(nqubits, codes, np.asarray(coefs, dtype=np.complex128))
+
+    def _init_from_arrays(self, nqubits: int, codes: np.ndarray, coefficients: np.ndarray) -> None:
+        if codes.size and (codes.min() < 0 or codes.max() > 255):
+            bad = np.argwhere((codes < 0) | (codes > 255))[0]
+            raise ValueError(f"invalid Pauli code {codes[tuple(bad)]} at index {bad[1]}; expected 0..=3")
+        codes8 = np.ascontiguousarray(codes, dtype=np.uint8)
+        coef = np.ascontiguousarray(coefficients, dtype=np.complex128)
+        out = ctypes.c_void_p()
+        _lib.check(_lib.lib().tcp_operator_from_codes(
+            nqubits, coef.shape[0], codes8.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+            coef.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(out)))
+        self._handle = _OperatorHandle(out)
+        self.nqubits = nqubits
+        self._terms: Optional[Tuple[PauliTerm, ...]] = None
+        self._arrays: Optional[Tuple[np.ndarray, np.ndarray]] = None
+
+    # -- constructors ----------------------------------------------------
+    @classmethod
+    def from_terms(cls, nqubits: int, terms: Iterable[Tuple[PauliInput, complex]], *,
+                   max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "PauliOperator":
+        return cls(nqubits, terms, max_bytes=max_bytes)
+
+    @classmethod
+    def from_code_arrays(cls, structures: Any, coefficients: Any, *,
+                         max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "PauliOperator":
+        """One native canonicalisation of [T, n] code rows and T coefficients."""
+        validate_max_bytes(max_bytes)
+        codes = np.asarray(structures)
+        coef = np.asarray(coefficients, dtype=np.complex128).reshape(-1)
+        if codes.ndim != 2:
+            raise ValueError("structures must be a two-dimensional code array")
+        if codes.shape[0] != coef.shape[0]:
+            raise ValueError(f"expected structure length {codes.shape[0]}, got {coef.shape[0]}")
+        instance = object.__new__(cls)
+        instance._init_from_arrays(int(codes.shape[1]), codes.astype(np.int64), coef)
+        return instance
+
+    @classmethod
+    def empty(cls, nqubits: int) -> "PauliOperator":
+        return cls(nqubits, ())
+
+    # -- canonical content -----------------------------------------------
+    @property
+    def term_count(self) -> int:
+        return int(_lib.lib().tcp_operator_term_count(self._handle.ptr))
+
+    def code_arrays(self) -> Tuple[np.ndarray, np.ndarray]:
+        """Canonical (codes [T, n] uint8, coefficients [T] complex128), read-only."""
+        if self._arrays is None:
+            T, n = self.term_count, self.nqubits
+            codes = np.zeros((T, n), dtype=np.uint8)
+            coef = np.zeros(T, dtype=np.complex128)
+            _lib.check(_lib.lib().tcp_operator_export_codes(
+                self._handle.ptr, codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                coef.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+            codes.flags.writeable = False
+            coef.flags.writeable = False
+            self._arrays = (codes, coef)
+        return self._arrays
+
+    @property
+    def terms(self) -> Tuple[PauliTerm, ...]:
+        if self._terms is None:
+            codes, coef = self.code_arrays()
+            self._terms = tuple(PauliTerm(tuple(int(c) for c in row), complex(v))
+                                for row, v in zip(codes, coef))
+        return self._terms
+
+    def distinct_x_mask_count(self) -> int:
+        """Number of distinct X-flip masks (native/operator.rs:50-59)."""
+        return int(_lib.lib().tcp_operator_distinct_x_mask_count(self._handle.ptr))
+
+    def backend_mvp_plan(self) -> "BackendPlanArrays":
+        """Schema-v1 flat arrays (core/hamiltonian.rs:501-524, py/pauli.py:954-983)."""
+        T, n = self.term_count, self.nqubits
+        W = (n + 63) // 64
+        x = np.zeros((T, W), dtype=np.uint64)
+        z = np.zeros((T, W), dtype=np.uint64)
+        coef = np.zeros(T, dtype=np.complex128)
+        _lib.check(_lib.lib().tcp_operator_export_words(
+            self._handle.ptr, x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+            z.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+            coef.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        for a in (x, z, coef):
+            a.flags.writeable = False
+        return BackendPlanArrays(1, n, W, x, z, coef, "qubit0_msb_matrix", int(T * (W * 16 + 16)))
+
+    # -- charges ----------------------------------------------------------
+    def analyze_charge(self, charge: Any, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "ChargeAnalysis":
+        """Exact conservation analysis of a diagonal qubit charge (core/operator.rs:396-445)."""
+        from .charge import AdditiveCharge
+
+        validate_max_bytes(max_bytes)
+        if not isinstance(charge, AdditiveCharge):
+            raise TypeError("charge must be an AdditiveCharge")
+        if charge.nqubits != self.nqubits:
+            raise ValueError("Pauli operator and charge layouts are incompatible")
+        levels = np.ascontiguousarray(charge.level_array(), dtype=np.float64)
+        conserved = ctypes.c_int(0)
+        nonzero = ctypes.c_uint64(0)
+        _lib.check(_lib.lib().tcp_operator_analyze_charge(
+            self._handle.ptr, levels.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            ctypes.byref(conserved), ctypes.byref(nonzero)))
+        return ChargeAnalysis(bool(conserved.value), int(nonzero.value))
+
+    def conserves_charge(self, charge: Any, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> bool:
+        return self.analyze_charge(charge, max_bytes=max_bytes).is_conserved
+
+    def restrict_charge(self, sector: Any, *, storage: str = "lazy",
+                        max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        """Restrict an exactly conserved operator to an occupation sector.
+
+        ``U1Sector`` and single full-width number charges use the U(1) path;
+        a ``ChargeSector`` of contiguous fixed-population qubit blocks uses
+        the factorized block path.
+        """
+        from .charge import ChargeSector, restrict_charge
+        from .symmetry import U1Sector, canonical_u1_sector, restrict_u1
+
+        if isinstance(sector, U1Sector):
+            return restrict_u1(self, sector, max_bytes, storage=storage)
+        if isinstance(sector, ChargeSector):
+            u1 = canonical_u1_sector(sector)
+            if u1 is not None:
+                return restrict_u1(self, u1, max_bytes, storage=storage)
+            return restrict_charge(self, sector, storage=storage, max_bytes=max_bytes)
+        raise TypeError("sector must be a U1Sector or a ChargeSector")
+
+    # -- MVP --------------------------------------------------------------
+    def mvp(self, state: Any, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        """One-shot H @ state (core/hamiltonian.rs:481-483); output is new."""
+        validate_max_bytes(max_bytes)
+        if not hasattr(state, "is_cuda"):
+            values = np.asarray(state, dtype=np.complex128)
+            if values.ndim != 1:
+                raise ValueError(f"state must be one-dimensional, got shape {values.shape}")
+            dim = 1 << self.nqubits if self.nqubits < 64 else None
+            if dim is None:
+                raise OverflowError("integer overflow while computing matrix dimension")
+            if values.shape[0] != dim:
+                raise ValueError(f"expected structure length {dim}, got {values.shape[0]}")
+        return self.native_mvp_plan(max_bytes=max_bytes).apply(state, max_bytes=max_bytes)
+
+    def native_mvp_plan(self, *, storage: str = "lazy",
+                        max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "NativeMVPPlan":
+        """Compile a reusable matrix-free plan on the current GPU."""
+        validate_max_bytes(max_bytes)
+        if storage not in ("lazy", "eager"):
+            raise ValueError("storage must be either 'eager' or 'lazy'")
+        _lib.require_device()
+        out = ctypes.c_void_p()
+        st = _lib.TCP_STORAGE_EAGER if storage == "eager" else _lib.TCP_STORAGE_LAZY
+        _lib.check(_lib.lib().tcp_mvp_plan_create(self._handle.ptr, st, effective_max_bytes(max_bytes),
+                                                   ctypes.byref(out)))
+        return NativeMVPPlan(out, self, storage)
+
+    def to_scipy_linear_operator(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        return self.native_mvp_plan(max_bytes=max_bytes).to_scipy_linear_operator(max_bytes=max_bytes)
+
+    def __repr__(self) -> str:
+        return f"PauliOperator(nqubits={self.nqubits}, term_count={self.term_count})"
+
+
+@dataclass(frozen=True)
+class ChargeAnalysis:
+    is_conserved: bool
+    nonzero_commutator_terms: int
+
+
+@dataclass(frozen=True)
+class BackendPlanArrays:
+    """BackendMvpPlan arrays (core/hamiltonian.rs:297-309)."""
+
+    schema_version: int
+    nqubits: int
+    word_count: int
+    x_words: np.ndarray
+    z_words: np.ndarray
+    coefficients: np.ndarray
+    ordering: str
+    estimated_bytes: int
+
+
+class NativeMVPPlan(NativePlanFacade):
+    """Reusable full-space plan (py/hamiltonian.py:162-344) executed by gfx950 kernels."""
+
+    execution_family = "pauli"
+    native_variant = "pauli"
+    basis_ordering = "qubit0_msb_matrix"
+    target = "native_mvp"
+
+    def __init__(self, ptr: ctypes.c_void_p, operator: PauliOperator, storage: str) -> None:
+        self._ptr = ptr
+        self._operator = operator
+        L = _lib.lib()
+        self.nqubits = int(L.tcp_mvp_plan_nqubits(ptr))
+        self.term_count = int(L.tcp_mvp_plan_term_count(ptr))
+        self.group_count = int(L.tcp_mvp_plan_group_count(ptr))
+        self.class_count = int(L.tcp_mvp_plan_class_count(ptr))
+        self.dimension = int(L.tcp_mvp_plan_dimension(ptr))
+        self.storage = storage
+        self.retained_format = "compact_graph" if storage == "eager" else "compiled_actions"
+        self.schedule_count = self.term_count
+        word_count = (self.nqubits + 63) // 64
+        term_bytes = self.term_count * (word_count * 16 + 16)
+        if storage == "eager":  # py/pauli.py:1011-1015
+            self.estimated_bytes = term_bytes + operator.distinct_x_mask_count() * self.dimension * 16
+            count = int(L.tcp_mvp_plan_transition_count(ptr))
+            self.transition_count: Optional[int] = count
+        else:
+            self.estimated_bytes = term_bytes
+            self.transition_count = None
+        self.estimated_retained_bytes = self.estimated_bytes
+        self.estimated_construction_peak_bytes = self.estimated_bytes
+        self.device_bytes = int(L.tcp_mvp_plan_estimated_bytes(ptr))
+
+    def __del__(self) -> None:
+        if getattr(self, "_ptr", None) and _lib._lib is not None:
+            _lib.lib().tcp_mvp_plan_destroy(self._ptr)
+            self._ptr = None
+
+    @property
+    def handle(self) -> int:
+        """Raw ``tcp_mvp_plan*`` for callers of the C ABI."""
+        return int(self._ptr.value or 0)
+
+    def bytes_alg(self, dtype: str = "complex128") -> int:
+        """Grouped-gather algorithmic bytes of one apply."""
+        code = _lib.TCP_C64 if dtype in ("complex64", "c64") else _lib.TCP_C128
+        return int(_lib.lib().tcp_mvp_plan_bytes_alg(self._ptr, code))
+
+    def _native_apply_host(self, inp: np.ndarray, out: np.ndarray) -> int:
+        dp = ctypes.POINTER(ctypes.c_double)
+        return _lib.lib().tcp_mvp_plan_apply_host(self._ptr, inp.ctypes.data_as(dp), out.ctypes.data_as(dp),
+                                                  inp.shape[0], out.shape[0])
+
+    def _native_apply_device(self, inp: Any, out: Any, dtype: int, stream: int) -> int:
+        return _lib.lib().tcp_mvp_plan_apply_into(self._ptr, inp.data_ptr(), out.data_ptr(), inp.numel(),
+                                                  out.numel(), dtype, stream)
+
+    def apply_batch(self, inputs: Any, outputs: Any) -> None:
+        """outputs[r] = H inputs[r] for device tensors of shape (nrhs, 2^n)."""
+        import torch
+
+        if inputs.shape != outputs.shape or inputs.dim() != 2 or inputs.shape[1] != self.dimension:
+            raise ValueError(f"batched buffers must have shape (nrhs, {self.dimension})")
+        if not (inputs.is_contiguous() and outputs.is_contiguous()):
+            raise ValueError("batched buffers must be C-contiguous")
+        dtype = {torch.complex128: _lib.TCP_C128, torch.complex64: _lib.TCP_C64}.get(inputs.dtype)
+        if dtype is None or outputs.dtype != inputs.dtype:
+            raise TypeError("batched buffers must both be complex128 or complex64")
+        from ._plan import current_stream_handle
+
+        _lib.check(_lib.lib().tcp_mvp_plan_apply_batch(self._ptr, inputs.data_ptr(), outputs.data_ptr(),
+                                                       inputs.shape[0], dtype, current_stream_handle(inputs)))
+
+    def __repr__(self) -> str:
+        return (f"NativeMVPPlan(nqubits={self.nqubits}, term_count={self.term_count}, "
+                f"groups={self.group_count}, classes={self.class_count}, storage={self.storage!r})")
+
+
+__all__ = [
+    "DEFAULT_MAX_BYTES",
+    "BackendPlanArrays",
+    "ChargeAnalysis",
+    "NativeMVPPlan",
+    "PauliOperator",
+    "PauliTerm",
+    "check_allocation",
+]

@@ -1,0 +1,218 @@
+"""U(1) fixed-particle-number sectors (py/symmetry.py:144-241, 244-538, 658-706).
+
+``U1Sector`` ranks and unranks in ascending integer (TensorCircuit) order with
+qubit 0 as the most significant bit; the rank is the combinatorial number
+system, computed by the native library (core/sector.rs:127-238).  A restricted
+operator's lazy plan runs the gfx950 gather kernel.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+from typing import Any, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._plan import (
+    DEFAULT_MAX_BYTES,
+    NativePlanFacade,
+    check_allocation,
+    effective_max_bytes,
+    validate_max_bytes,
+)
+
+
+def _coerce_bitstring(nqubits: int, bitstring: Any) -> int:
+    if isinstance(bitstring, bool):
+        raise TypeError("bitstring must be an integer or a sequence of bits")
+    if isinstance(bitstring, (int, np.integer)):
+        value = int(bitstring)
+    else:
+        bits = tuple(bitstring)
+        if len(bits) != nqubits or any(bit not in (0, 1) for bit in bits):
+            raise ValueError("bitstring must contain exactly nqubits binary values")
+        value = 0
+        for bit in bits:
+            value = (value << 1) | int(bit)
+    if value < 0 or nqubits < value.bit_length():
+        raise ValueError("bitstring is outside the computational basis")
+    return value
+
+
+@dataclass(frozen=True)
+class U1Sector:
+    """Fixed-Hamming-weight basis with TensorCircuit integer ordering."""
+
+    nqubits: int
+    particle_number: int
+
+    def __post_init__(self) -> None:
+        if not isinstance(self.nqubits, int) or isinstance(self.nqubits, bool) or self.nqubits < 0:
+            raise ValueError("nqubits must be a non-negative integer")
+        if (not isinstance(self.particle_number, int) or isinstance(self.particle_number, bool)
+                or not 0 <= self.particle_number <= self.nqubits):
+            raise ValueError("particle_number must be between 0 and nqubits")
+        if math.comb(self.nqubits, self.particle_number) > np.iinfo(np.uint64).max:
+            raise OverflowError("U1 sector dimension exceeds uint64 restricted indices")
+
+    @property
+    def dimension(self) -> int:
+        return math.comb(self.nqubits, self.particle_number)
+
+    def rank(self, bitstring: Any) -> int:
+        value = _coerce_bitstring(self.nqubits, bitstring)
+        if bin(value).count("1") != self.particle_number:
+            raise ValueError("bitstring has the wrong Hamming weight")
+        if self.nqubits > 64:
+            raise OverflowError("ranking is limited to 64 qubits in this build")
+        out = ctypes.c_uint64()
+        _lib.check(_lib.lib().tcp_u1_rank(self.nqubits, self.particle_number, value, ctypes.byref(out)))
+        return int(out.value)
+
+    def unrank(self, index: int) -> Tuple[int, ...]:
+        if not isinstance(index, (int, np.integer)) or isinstance(index, bool) or not 0 <= index < self.dimension:
+            raise IndexError("restricted basis index is out of range")
+        out = ctypes.c_uint64()
+        _lib.check(_lib.lib().tcp_u1_unrank(self.nqubits, self.particle_number, int(index), ctypes.byref(out)))
+        value = int(out.value)
+        return tuple((value >> (self.nqubits - 1 - q)) & 1 for q in range(self.nqubits))
+
+    def unrank_int(self, index: int) -> int:
+        out = ctypes.c_uint64()
+        _lib.check(_lib.lib().tcp_u1_unrank(self.nqubits, self.particle_number, int(index), ctypes.byref(out)))
+        return int(out.value)
+
+    def basis_states(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> np.ndarray:
+        """Read-only uint8 rows (qubit 0 first) in restricted order."""
+        validate_max_bytes(max_bytes)
+        check_allocation(self.dimension * self.nqubits, max_bytes, "U1 basis states")
+        rows = np.asarray([self.unrank(i) for i in range(self.dimension)], dtype=np.uint8)
+        rows = rows.reshape(self.dimension, self.nqubits)
+        rows.flags.writeable = False
+        return rows
+
+
+class SectorMvpPlan(NativePlanFacade):
+    """Lazy restricted plan on the GPU (U1LazyMvpPlan / PreparedChargeMvpPlan)."""
+
+    def __init__(self, ptr: ctypes.c_void_p, native_variant: str, execution_family: str) -> None:
+        self._ptr = ptr
+        L = _lib.lib()
+        self.dimension = int(L.tcp_sector_plan_dimension(ptr))
+        self.nqubits = int(L.tcp_sector_plan_nqubits(ptr))
+        self.term_count = int(L.tcp_sector_plan_term_count(ptr))
+        self.group_count = int(L.tcp_sector_plan_group_count(ptr))
+        self.estimated_bytes = int(L.tcp_sector_plan_estimated_bytes(ptr))
+        self.native_variant = native_variant
+        self.execution_family = execution_family
+        self.storage = "lazy"
+        self.transition_count: Optional[int] = None
+
+    def __del__(self) -> None:
+        if getattr(self, "_ptr", None) and _lib._lib is not None:
+            _lib.lib().tcp_sector_plan_destroy(self._ptr)
+            self._ptr = None
+
+    @property
+    def handle(self) -> int:
+        return int(self._ptr.value or 0)
+
+    def _native_apply_host(self, inp: np.ndarray, out: np.ndarray) -> int:
+        dp = ctypes.POINTER(ctypes.c_double)
+        return _lib.lib().tcp_sector_plan_apply_host(self._ptr, inp.ctypes.data_as(dp), out.ctypes.data_as(dp),
+                                                     inp.shape[0], out.shape[0])
+
+    def _native_apply_device(self, inp: Any, out: Any, dtype: int, stream: int) -> int:
+        return _lib.lib().tcp_sector_plan_apply_into(self._ptr, inp.data_ptr(), out.data_ptr(), inp.numel(),
+                                                     out.numel(), dtype, stream)
+
+    def rank_device(self, bitstrings: Any) -> Any:
+        """Ranks of a device uint64 tensor of integer bitstrings (UINT64_MAX outside)."""
+        import torch
+
+        from ._plan import current_stream_handle
+
+        out = torch.empty_like(bitstrings)
+        _lib.check(_lib.lib().tcp_sector_plan_rank_device(self._ptr, bitstrings.data_ptr(), out.data_ptr(),
+                                                          bitstrings.numel(), current_stream_handle(bitstrings)))
+        return out
+
+    def unrank_device(self, ranks: Any) -> Any:
+        import torch
+
+        from ._plan import current_stream_handle
+
+        out = torch.empty_like(ranks)
+        _lib.check(_lib.lib().tcp_sector_plan_unrank_device(self._ptr, ranks.data_ptr(), out.data_ptr(),
+                                                            ranks.numel(), current_stream_handle(ranks)))
+        return out
+
+    def walk_device(self, first: int, count: int, run: int, device: Any = "cuda") -> Any:
+        import torch
+
+        from ._plan import current_stream_handle
+
+        out = torch.empty(count, dtype=torch.int64, device=device)  # bit pattern of uint64
+        _lib.check(_lib.lib().tcp_sector_plan_walk_device(self._ptr, first, out.data_ptr(), count, run,
+                                                          current_stream_handle(out)))
+        return out
+
+
+class U1RestrictedOperator(NativePlanFacade):
+    """A validated Pauli operator restricted to one U(1) sector (py/symmetry.py:244-538)."""
+
+    def __init__(self, sector: U1Sector, operator: Any, plan: SectorMvpPlan, storage: str) -> None:
+        self.sector = sector
+        self.operator = operator
+        self.dimension = sector.dimension
+        self.storage = storage
+        self.term_count = operator.term_count
+        self._plan = plan
+        self.native_variant = "qubit"
+
+    @property
+    def plan(self) -> SectorMvpPlan:
+        return self._plan
+
+    def mvp_plan(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> SectorMvpPlan:
+        validate_max_bytes(max_bytes)
+        return self._plan
+
+    def _native_apply_host(self, inp: np.ndarray, out: np.ndarray) -> int:
+        return self._plan._native_apply_host(inp, out)
+
+    def _native_apply_device(self, inp: Any, out: Any, dtype: int, stream: int) -> int:
+        return self._plan._native_apply_device(inp, out, dtype, stream)
+
+
+def canonical_u1_sector(sector: Any) -> Optional[U1Sector]:
+    """A single number charge over all qubits is the U(1) sector (py/symmetry.py:620-643)."""
+    from .charge import ChargeSector
+
+    if not isinstance(sector, ChargeSector) or len(sector.constraints) != 1:
+        return None
+    charge, requested = sector.constraints[0]
+    if charge.offset != 0 or any(levels != (0, 1) for levels in charge.qubit_levels):
+        return None
+    if requested < 0 or requested > charge.nqubits:
+        return None
+    return U1Sector(charge.nqubits, int(requested))
+
+
+def restrict_u1(operator: Any, sector: U1Sector, max_bytes: Optional[int], *,
+                storage: str = "lazy") -> U1RestrictedOperator:
+    """py/symmetry.py:658-706 on the GPU; conservation is checked natively."""
+    if storage not in ("lazy", "eager"):
+        raise ValueError("storage must be either 'eager' or 'lazy'")
+    validate_max_bytes(max_bytes)
+    if operator.nqubits != sector.nqubits:
+        raise ValueError(f"incompatible qubit counts: {operator.nqubits} and {sector.nqubits}")
+    _lib.require_device()
+    out = ctypes.c_void_p()
+    _lib.check(_lib.lib().tcp_sector_plan_create_u1(operator._handle.ptr, sector.particle_number,
+                                                     effective_max_bytes(max_bytes), ctypes.byref(out)))
+    plan = SectorMvpPlan(out, "qubit", "u1")
+    return U1RestrictedOperator(sector, operator, plan, storage)

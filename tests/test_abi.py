@@ -1,0 +1,220 @@
+"""CPU checks of the C ABI: the library loads, exports every declared symbol,
+and its host-side logic (canonicalisation, packing, charge analysis, sector
+rank/unrank, error taxonomy) matches the oracle.  No kernel is launched."""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import tencirpauli_amd as tcp
+from oracle import dense
+from tencirpauli_amd import _lib, _plan
+from tests.helpers import conserving_u1_terms
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    include = os.path.join(ROOT, "include")
+    for fname in os.listdir(include):
+        if fname.endswith(".h"):
+            text = open(os.path.join(include, fname)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names.update(re.findall(r"\b(tcp_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    names = declared_symbols()
+    assert len(names) >= 40
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+    assert set(_lib.SIGNATURES) == set(names)
+    assert lib.tcp_version().decode().startswith("tencirpauli_amd")
+
+
+def test_operator_canonicalisation_matches_oracle():
+    rng = np.random.default_rng(20260801)
+    for n in range(0, 13):
+        codes = rng.integers(0, 4, size=(60, n)).astype(np.uint8)
+        codes[30:] = codes[:30]
+        coef = rng.normal(size=60) + 1j * rng.normal(size=60)
+        coef[5] = 0.0
+        op = tcp.PauliOperator.from_code_arrays(codes, coef)
+        cc, cf = oracle.canonicalize(n, codes, coef)
+        got_codes, got_coef = op.code_arrays()
+        np.testing.assert_array_equal(got_codes, cc)
+        np.testing.assert_array_equal(got_coef, cf)  # bit-exact sums
+
+
+def test_bit_pattern_summation_drops_exact_zero():
+    op = tcp.PauliOperator.from_terms(1, [("X", 1e16), ("X", -1e16), ("X", 1.0), ("Z", 2.0)])
+    assert [t.codes for t in op.terms] == [(3,)]
+
+
+def test_string_terms_and_backend_arrays():
+    op = tcp.PauliOperator.from_terms(3, [("XYZ", 0.5), ("ZII", 1.0), ("XYZ", 0.25)])
+    assert [t.to_string() for t in op.terms] == ["XYZ", "ZII"]
+    plan = op.backend_mvp_plan()
+    assert plan.x_words.dtype == np.uint64 and plan.x_words.shape == (2, 1)
+    # qubit q at bit q (core/word.rs:94-110): XYZ -> x = 0b011, z = 0b110
+    assert plan.x_words[0, 0] == 0b011 and plan.z_words[0, 0] == 0b110
+    assert plan.z_words[1, 0] == 0b001
+    np.testing.assert_array_equal(plan.coefficients, [0.75, 1.0])
+    assert not plan.x_words.flags.writeable
+    assert op.distinct_x_mask_count() == 2
+
+
+def test_operator_from_words_round_trip():
+    rng = np.random.default_rng(3)
+    for n in (1, 7, 63, 64, 65, 130):
+        codes = rng.integers(0, 4, size=(12, n)).astype(np.uint8)
+        coef = rng.normal(size=12) + 0j
+        op = tcp.PauliOperator.from_code_arrays(codes, coef)
+        arrays = op.backend_mvp_plan()
+        out = ctypes.c_void_p()
+        u64 = ctypes.POINTER(ctypes.c_uint64)
+        cf = np.ascontiguousarray(arrays.coefficients)
+        xw = np.ascontiguousarray(arrays.x_words)
+        zw = np.ascontiguousarray(arrays.z_words)
+        _lib.check(_lib.lib().tcp_operator_from_words(n, len(cf), xw.ctypes.data_as(u64), zw.ctypes.data_as(u64),
+                                                      cf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                      ctypes.byref(out)))
+        assert _lib.lib().tcp_operator_term_count(out) == op.term_count
+        codes2 = np.zeros((op.term_count, n), dtype=np.uint8)
+        _lib.check(_lib.lib().tcp_operator_export_codes(out, codes2.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                                        None))
+        np.testing.assert_array_equal(codes2, op.code_arrays()[0])
+        _lib.lib().tcp_operator_destroy(out)
+
+
+def test_error_taxonomy():
+    with pytest.raises(ValueError, match="invalid Pauli code"):
+        tcp.PauliOperator.from_code_arrays([[0, 4]], [1.0])
+    with pytest.raises(ValueError, match="not finite"):
+        tcp.PauliOperator.from_terms(1, [("X", float("inf"))])
+    with pytest.raises(ValueError, match="expected structure length"):
+        tcp.PauliOperator.from_terms(2, [("XYZ", 1.0)])
+    # n >= 64 is an overflow before any device work (core/hamiltonian.rs:544-555)
+    op = tcp.PauliOperator.empty(64)
+    out = ctypes.c_void_p()
+    status = _lib.lib().tcp_mvp_plan_create(op._handle.ptr, 0, 0, ctypes.byref(out))
+    assert status == _lib.TCP_ERR_OVERFLOW
+    with pytest.raises(OverflowError, match="matrix dimension"):
+        _lib.check(status)
+    status = _lib.lib().tcp_mvp_plan_create(op._handle.ptr, 7, 0, ctypes.byref(out))
+    assert status == _lib.TCP_ERR_INVALID_ARGUMENT
+
+
+def test_no_cpu_fallback_without_device():
+    if _lib.device_count() > 0:
+        pytest.skip("a HIP device is visible; this checks the device-less failure path")
+    op = tcp.PauliOperator.from_terms(1, [("X", 1.0)])
+    with pytest.raises(tcp.DeviceError):
+        op.native_mvp_plan()
+    with pytest.raises(tcp.DeviceError):
+        op.mvp(np.ones(2, dtype=np.complex128))
+    with pytest.raises(tcp.DeviceError):
+        op.restrict_charge(tcp.U1Sector(1, 1))
+
+
+def test_analyze_charge_matches_oracle():
+    rng = np.random.default_rng(20260802)
+    for n in range(1, 9):
+        codes, coef = conserving_u1_terms(rng, n, 10)
+        extra = rng.integers(0, 4, size=(3, n)).astype(np.uint8)
+        for c, f in ((codes, coef), (np.concatenate([codes, extra]), np.concatenate([coef, np.ones(3)]))):
+            op = tcp.PauliOperator.from_code_arrays(c, f)
+            cc, cf = op.code_arrays()
+            levels = [(0.0, 1.0)] * n
+            got = op.analyze_charge(tcp.AdditiveCharge.number(n, range(n)))
+            assert (got.is_conserved, got.nonzero_commutator_terms) == oracle.analyze_charge(n, cc, cf, levels)
+
+
+def test_u1_host_rank_unrank_bit_exact():
+    for n in range(0, 13):
+        for k in range(n + 1):
+            sector = tcp.U1Sector(n, k)
+            basis = dense.u1_basis(n, k)
+            assert [sector.rank(b) for b in basis] == list(range(len(basis)))
+            assert [sector.unrank_int(i) for i in range(len(basis))] == basis
+    rng = np.random.default_rng(9)
+    for n in (25, 40, 63, 64):
+        for k in (1, n // 2, n // 2 + 1, n - 1):
+            dim = math.comb(n, k)
+            idx = np.asarray(sorted({0, dim - 1, *(int(i) for i in rng.integers(0, dim, size=40, dtype=np.uint64))}),
+                             dtype=np.uint64)
+            expected = oracle.u1_unrank_many(n, k, idx)
+            sector = tcp.U1Sector(n, k)
+            assert [sector.unrank_int(int(i)) for i in idx] == [int(b) for b in expected]
+            assert [sector.rank(int(b)) for b in expected] == [int(i) for i in idx]
+
+
+def test_u1_known_basis_orders():
+    for n, k, expected in [(0, 0, [0]), (3, 0, [0]), (3, 1, [1, 2, 4]), (3, 2, [3, 5, 6]), (3, 3, [7])]:
+        sector = tcp.U1Sector(n, k)
+        assert sector.dimension == len(expected)
+        bits = [tuple((v >> (n - 1 - q)) & 1 for q in range(n)) for v in expected]
+        assert [sector.unrank(i) for i in range(sector.dimension)] == bits
+        assert [sector.rank(v) for v in expected] == list(range(len(expected)))
+        basis = sector.basis_states()
+        np.testing.assert_array_equal(basis, np.asarray(bits, dtype=np.uint8).reshape(len(bits), n))
+        assert basis.flags.writeable is False
+    with pytest.raises(ValueError, match="wrong Hamming weight"):
+        tcp.U1Sector(3, 1).rank(3)
+    with pytest.raises(IndexError):
+        tcp.U1Sector(3, 1).unrank(3)
+
+
+def test_block_sector_rank_matches_oracle():
+    for blocks in ([(3, 1), (3, 2)], [(4, 2), (4, 2)], [(2, 1), (3, 0), (3, 3)], [(16, 4), (16, 4)]):
+        n = sum(s for s, _ in blocks)
+        charges = []
+        start = 0
+        for size, pop in blocks:
+            charges.append((tcp.AdditiveCharge.number(n, range(start, start + size)), pop))
+            start += size
+        sector = tcp.ChargeSector(charges[::-1])  # constraint order does not matter
+        assert sector.blocks == tuple(blocks)
+        assert sector.dimension == oracle.blocks_dimension(blocks)
+        idx = np.unique(np.linspace(0, sector.dimension - 1, num=min(sector.dimension, 300)).astype(np.uint64))
+        bits = oracle.blocks_unrank_many(blocks, idx)
+        for i, b in zip(idx, bits):
+            occ = tuple((int(b) >> (n - 1 - q)) & 1 for q in range(n))
+            assert sector.unrank(int(i)) == occ
+            assert sector.rank(occ) == int(i)
+    three = tcp.ChargeSector(((tcp.AdditiveCharge.number(3, range(3)), 1),))
+    assert [three.unrank(i) for i in range(3)] == [(0, 0, 1), (0, 1, 0), (1, 0, 0)]
+
+
+def test_apply_into_buffer_protocol():
+    good = np.zeros(4, dtype=np.complex128)
+    with pytest.raises(TypeError, match="NumPy arrays"):
+        _plan.validate_apply_into_buffers([0, 0, 0, 0], good, 4)
+    with pytest.raises(TypeError, match="complex128"):
+        _plan.validate_apply_into_buffers(np.zeros(4, dtype=np.complex64), good, 4)
+    with pytest.raises(ValueError, match="shape"):
+        _plan.validate_apply_into_buffers(np.zeros(3, dtype=np.complex128), good, 4)
+    with pytest.raises(ValueError, match="C-contiguous"):
+        _plan.validate_apply_into_buffers(np.zeros(8, dtype=np.complex128)[::2], good, 4)
+    with pytest.raises(ValueError, match="overlap"):
+        _plan.validate_apply_into_buffers(good, good, 4)
+    ro = np.zeros(4, dtype=np.complex128)
+    ro.flags.writeable = False
+    with pytest.raises(ValueError, match="writable"):
+        _plan.validate_apply_into_buffers(good, ro, 4)
+    with pytest.raises(ValueError, match="max_bytes"):
+        _plan.validate_max_bytes(-1)
+    assert _plan.enforced_max_bytes(1) is None
+    with pytest.raises(MemoryError):
+        _plan.check_allocation(2 << 30, 1 << 30, "output")
