@@ -235,6 +235,99 @@ int tcp_sector_plan_walk_device(const tcp_sector_plan *plan, uint64_t first, uin
 
 void tcp_sector_plan_destroy(tcp_sector_plan *plan);
 
+/* ---- Davidson eigensolver (no reference counterpart: the reference's
+ *      eigen entry point is SciPy eigsh over a LinearOperator,
+ *      py/integrations/scipy.py:13-47) ------------------------------------- */
+
+/* Per-state log multipliers m_i of the correction t_i = r_i e^{m_i} / (theta - H_ii):
+ * returns a device float[dimension] array valid on the solver's stream, or NULL
+ * on failure. */
+typedef const void *(*tcp_log_multiplier_fn)(void *ctx, double theta);
+
+typedef struct tcp_davidson_options {
+    uint32_t max_subspace;    /* basis size before a thick restart (4..64; default 24) */
+    uint32_t restart_keep;    /* Ritz vectors kept at a restart (default 4) */
+    uint32_t max_iterations;  /* default 1000 */
+    double residual_tol;      /* converged when ||H u - theta u|| <= residual_tol (FP64 phase) */
+    int mvp_dtype;            /* TCP_C128, or TCP_C64 for a complex64-storage first phase */
+    double switch_tol;        /* TCP_C64 phase ends (restart with FP64 MVPs) at ||r|| <= switch_tol */
+    uint64_t seed;            /* deterministic start vector */
+    tcp_log_multiplier_fn log_multiplier_fn; /* NULL: plain Jacobi preconditioner */
+    void *log_multiplier_ctx;
+    double *trace;            /* host [trace_capacity x 4]: seconds, theta, ||r||, MVP bits (32/64) */
+    uint32_t trace_capacity;
+    void *stream;             /* hipStream_t (NULL = default stream) */
+} tcp_davidson_options;
+
+typedef struct tcp_davidson_result {
+    double eigenvalue;
+    double residual_norm;
+    double seconds;           /* wall time of the solve, host clock */
+    uint32_t iterations;
+    uint32_t mvps_c64;
+    uint32_t mvps_c128;
+    uint32_t restarts;
+    uint32_t trace_len;
+    uint32_t converged;
+    uint32_t final_fp64;
+} tcp_davidson_result;
+
+void tcp_davidson_default_options(tcp_davidson_options *opt);
+
+/* Lowest eigenpair of a restricted / full-space plan; `eigenvector` is an
+ * optional device complex128 buffer of `dimension` amplitudes (normalised Ritz
+ * vector on return). */
+int tcp_davidson_sector(const tcp_sector_plan *plan, const tcp_davidson_options *opt,
+                        void *eigenvector, tcp_davidson_result *result);
+int tcp_davidson_full(const tcp_mvp_plan *plan, const tcp_davidson_options *opt, void *eigenvector,
+                      tcp_davidson_result *result);
+
+/* Real diagonal H_ii of a plan into a device double[dimension] array. */
+int tcp_sector_plan_diagonal(const tcp_sector_plan *plan, double *diag, void *stream);
+int tcp_mvp_plan_diagonal(const tcp_mvp_plan *plan, double *diag, void *stream);
+
+/* ---- learned preconditioner (no reference counterpart): a 64-128-128-1 tanh
+ *      MLP, bf16 weights, FP32 accumulation on MFMA, trained on the GPU ---- */
+
+typedef struct tcp_mlp tcp_mlp;
+typedef struct tcp_learned_precond tcp_learned_precond;
+
+/* Flat FP32 parameter vector: W1[64][128], b1[128], W2[128][128], b2[128],
+ * w3[128], b3 (in-major weight matrices). */
+uint32_t tcp_mlp_parameter_count(void);
+int tcp_mlp_create(uint64_t seed, tcp_mlp **out);
+int tcp_mlp_get_parameters(const tcp_mlp *mlp, float *params);
+int tcp_mlp_set_parameters(tcp_mlp *mlp, const float *params);
+/* out[b] = MLP(features[b]) for bf16 features [batch x 64] (device buffers). */
+int tcp_mlp_forward(const tcp_mlp *mlp, const void *features, uint64_t batch, float *out, void *stream);
+/* Parameter gradient of sum_b dloss_dout[b] * MLP(features[b]) into the
+ * MLP's gradient buffer (and grad_host when not NULL). */
+int tcp_mlp_gradient(tcp_mlp *mlp, const void *features, uint64_t batch, const float *dloss_dout,
+                     float *grad_host, void *stream);
+/* One Adam step (beta 0.9/0.999) with the last gradient. */
+int tcp_mlp_adam_step(tcp_mlp *mlp, float lr, float weight_decay, void *stream);
+void tcp_mlp_destroy(tcp_mlp *mlp);
+
+/* Correction-angle loss of one snapshot, L = 1 - <e,t>^2 / (|e|^2 |t - u<u,t>|^2)
+ * with t_i = r_i e^{y_i} / (theta - diag_i) (real parts; e orthogonal to u),
+ * and dy = weight * dL/dy.  r, e, u: device complex128 [dim]. */
+int tcp_precond_loss_gradient(const float *y, const void *r, const void *e, const void *u,
+                              const double *diag, double theta, uint64_t dim, double weight,
+                              float *dy, double *loss, void *stream);
+
+/* Per-state features of a sector plan (bf16 [dim x 64]) for `mlp`; the
+ * callback below is a tcp_log_multiplier_fn for tcp_davidson_options. */
+int tcp_learned_precond_create(const tcp_mlp *mlp, const tcp_sector_plan *plan,
+                               tcp_learned_precond **out);
+const void *tcp_learned_precond_log_multiplier(void *ctx, double theta);
+/* Writes feature 0, (H_ii - theta) / scale, of every state (the callback does
+ * this itself; training calls it before tcp_mlp_forward / tcp_mlp_gradient on
+ * the features buffer). */
+int tcp_learned_precond_prepare(tcp_learned_precond *precond, double theta, void *stream);
+int tcp_learned_precond_buffers(const tcp_learned_precond *precond, const void **features,
+                                const double **diag, double *scale);
+void tcp_learned_precond_destroy(tcp_learned_precond *precond);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,652 @@
+// Learned diagonal preconditioner for Davidson: a 64-128-128-1 tanh MLP with
+// bf16 weights and FP32 accumulation on MFMA (v_mfma_f32_16x16x16_bf16),
+// trained and evaluated on the GPU.  No reference counterpart.
+//
+// The network maps per-state features to a log multiplier m_i of the Jacobi
+// correction, t_i = r_i e^{m_i} / (theta - H_ii).  Training minimises the
+// angle between t and the true error vector on snapshots of Jacobi Davidson
+// runs on small lattices (loss kernels below).
+//
+// MFMA 16x16x16 operand layout (per 64-lane wave): A[i][k] with i = lane % 16,
+// k = 4 (lane / 16) + j; B[k][n] with n = lane % 16, k = 4 (lane / 16) + j;
+// D[i][n] with n = lane % 16, i = 4 (lane / 16) + j; j = 0..3 the element
+// within the lane's vector.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "tcp_internal.hpp"
+
+namespace {
+
+constexpr uint32_t kF = 64;   // input features
+constexpr uint32_t kH = 128;  // hidden width
+constexpr uint32_t kP = kF * kH + kH + kH * kH + kH + kH + 1;
+constexpr uint32_t kOffW1 = 0, kOffB1 = kF * kH, kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH,
+                   kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kH;
+constexpr uint32_t kThreads = 256;
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+__host__ __device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf2f(uint16_t h) {
+    return __builtin_bit_cast(float, static_cast<uint32_t>(h) << 16);
+}
+
+__device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+struct MlpDev {
+    const s4 *w1f;   // [8 n][4 ks][64 lane] fragments of W1 (bf16 x4)
+    const s4 *w2f;   // [8 n][8 ks][64 lane] fragments of W2
+    const uint16_t *w2b;  // W2 bf16, row-major [in][out]
+    const float *params;  // FP32 master parameters
+};
+
+// Fragment-major bf16 copies of W1, W2 and a plain bf16 copy of W2.
+__global__ void swizzle_kernel(const float *__restrict__ params, uint16_t *__restrict__ w1f,
+                               uint16_t *__restrict__ w2f, uint16_t *__restrict__ w2b) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    // W1: 8 * 4 * 64 * 4 = 8192 entries
+    if (tid < 8 * 4 * 64 * 4) {
+        const uint32_t j = tid & 3, lane = (tid >> 2) & 63, ks = (tid >> 8) & 3, n = tid >> 10;
+        const uint32_t k = ks * 16 + 4 * (lane >> 4) + j, col = n * 16 + (lane & 15);
+        w1f[tid] = f2bf(params[kOffW1 + k * kH + col]);
+    }
+    if (tid < 8 * 8 * 64 * 4) {
+        const uint32_t j = tid & 3, lane = (tid >> 2) & 63, ks = (tid >> 8) & 7, n = tid >> 11;
+        const uint32_t k = ks * 16 + 4 * (lane >> 4) + j, col = n * 16 + (lane & 15);
+        w2f[tid] = f2bf(params[kOffW2 + k * kH + col]);
+        w2b[tid] = f2bf(params[kOffW2 + tid]);
+    }
+}
+
+// Feature 0 of every state: (diag[r] - theta) * inv_scale, in place.
+__global__ void fill_feature0_kernel(uint16_t *__restrict__ X, const double *__restrict__ diag, double theta,
+                                     double inv_scale, uint64_t B) {
+    for (uint64_t r = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; r < B;
+         r += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        X[r * kF] = f2bf(static_cast<float>((diag[r] - theta) * inv_scale));
+}
+
+// y[r] = MLP(X[r]); X is bf16 [B x 64].  With h1/h2, the post-activation
+// hidden layers are stored (FP32 [B x 128]) for the backward pass.
+__global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const uint16_t *__restrict__ X,
+                                                               uint64_t B, float *__restrict__ y,
+                                                               float *__restrict__ h1, float *__restrict__ h2) {
+    __shared__ s4 w1s[8 * 4 * 64];
+    __shared__ s4 w2s[8 * 8 * 64];
+    __shared__ float bs1[kH], bs2[kH], ws3[kH];
+    __shared__ uint16_t hst[4][16 * kH];
+    for (uint32_t i = threadIdx.x; i < 8 * 4 * 64; i += kThreads) w1s[i] = p.w1f[i];
+    for (uint32_t i = threadIdx.x; i < 8 * 8 * 64; i += kThreads) w2s[i] = p.w2f[i];
+    for (uint32_t i = threadIdx.x; i < kH; i += kThreads) {
+        bs1[i] = p.params[kOffB1 + i];
+        bs2[i] = p.params[kOffB2 + i];
+        ws3[i] = p.params[kOffW3 + i];
+    }
+    const float b3 = p.params[kOffB3];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t li = lane & 15, lg = lane >> 4;
+    const uint64_t ntile = (B + 63) / 64;
+    for (uint64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const uint64_t row0 = tile * 64 + wave * 16;
+        const uint64_t row = row0 + li;
+        const bool valid = row < B;
+        s4 a[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+            a[ks] = valid ? *reinterpret_cast<const s4 *>(X + row * kF + ks * 16 + lg * 4) : s4{0, 0, 0, 0};
+        f4 acc[8];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (int n = 0; n < 8; ++n) acc[n] = mfma(a[ks], w1s[(n * 4 + ks) * 64 + lane], acc[n]);
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float v = tanhf(acc[n][j] + bs1[n * 16 + li]);
+                hst[wave][(lg * 4 + j) * kH + n * 16 + li] = f2bf(v);
+                const uint64_t r = row0 + lg * 4 + j;
+                if (h1 && r < B) h1[r * kH + n * 16 + li] = v;
+            }
+        __syncthreads();
+        f4 acc2[8];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) acc2[n] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+            const s4 a2 = *reinterpret_cast<const s4 *>(&hst[wave][li * kH + ks * 16 + lg * 4]);
+#pragma unroll
+            for (int n = 0; n < 8; ++n) acc2[n] = mfma(a2, w2s[(n * 8 + ks) * 64 + lane], acc2[n]);
+        }
+        float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float v = tanhf(acc2[n][j] + bs2[n * 16 + li]);
+                const uint64_t r = row0 + lg * 4 + j;
+                if (h2 && r < B) h2[r * kH + n * 16 + li] = v;
+                part[j] += v * ws3[n * 16 + li];
+            }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            for (int off = 1; off < 16; off <<= 1) part[j] += __shfl_xor(part[j], off);
+        if (li == 0)
+            for (int j = 0; j < 4; ++j) {
+                const uint64_t r = row0 + lg * 4 + j;
+                if (r < B) y[r] = part[j] + b3;
+            }
+        __syncthreads();
+    }
+}
+
+// dA2 = dy * w3 * (1 - h2^2) (bf16 out); per-block partial sums of
+// dw3 = sum dy h2, db2 = sum dA2, db3 = sum dy.
+__global__ __launch_bounds__(kThreads) void act2_backward_kernel(const float *__restrict__ params,
+                                                                 const float *__restrict__ dy,
+                                                                 const float *__restrict__ h2, uint64_t B,
+                                                                 uint16_t *__restrict__ da2,
+                                                                 float *__restrict__ partial) {
+    // 256 threads: column u = tid % 128, row phase = tid / 128 (2 rows in flight)
+    const uint32_t u = threadIdx.x & (kH - 1), phase = threadIdx.x >> 7;
+    const float w3 = params[kOffW3 + u];
+    float gw3 = 0.f, gb2 = 0.f, gb3 = 0.f;
+    for (uint64_t r = static_cast<uint64_t>(blockIdx.x) * 2 + phase; r < B; r += static_cast<uint64_t>(gridDim.x) * 2) {
+        const float d = dy[r];
+        const float v = h2[r * kH + u];
+        const float g = d * w3 * (1.f - v * v);
+        da2[r * kH + u] = f2bf(g);
+        gw3 += d * v;
+        gb2 += g;
+        if (u == 0) gb3 += d;
+    }
+    __shared__ float red[2][3 * kH + 1];
+    red[phase][u] = gw3;
+    red[phase][kH + u] = gb2;
+    if (u == 0) red[phase][3 * kH] = gb3;
+    __syncthreads();
+    if (phase == 0) {
+        float* out = partial + static_cast<uint64_t>(blockIdx.x) * (2 * kH + 1);
+        out[u] = red[0][u] + red[1][u];
+        out[kH + u] = red[0][kH + u] + red[1][kH + u];
+        if (u == 0) out[2 * kH] = red[0][3 * kH] + red[1][3 * kH];
+    }
+}
+
+// dA1 = (dA2 W2^T) * (1 - h1^2), bf16 out; per-block partial db1 = sum dA1.
+__global__ __launch_bounds__(kThreads) void h1_backward_kernel(MlpDev p, const uint16_t *__restrict__ da2,
+                                                               const float *__restrict__ h1, uint64_t B,
+                                                               uint16_t *__restrict__ da1,
+                                                               float *__restrict__ partial) {
+    __shared__ float colsum[4][kH];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t li = lane & 15, lg = lane >> 4;
+    float gb1[8][4] = {};
+    const uint64_t ntile = (B + 63) / 64;
+    for (uint64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const uint64_t row0 = tile * 64 + wave * 16;
+        const uint64_t row = row0 + li;
+        f4 acc[8];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+            const s4 a = row < B ? *reinterpret_cast<const s4 *>(da2 + row * kH + ks * 16 + lg * 4) : s4{0, 0, 0, 0};
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                // B[k = u][n = i] = W2[i][u]: contiguous along u
+                const s4 b = *reinterpret_cast<const s4 *>(p.w2b + (n * 16 + li) * kH + ks * 16 + lg * 4);
+                acc[n] = mfma(a, b, acc[n]);
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint64_t r = row0 + lg * 4 + j;
+                if (r < B) {
+                    const float v = h1[r * kH + n * 16 + li];
+                    const float g = acc[n][j] * (1.f - v * v);
+                    da1[r * kH + n * 16 + li] = f2bf(g);
+                    gb1[n][j] += g;
+                }
+            }
+    }
+    // column sums: reduce over j and over the 4 lane groups (same li), then waves
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        float s = gb1[n][0] + gb1[n][1] + gb1[n][2] + gb1[n][3];
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        if (lg == 0) colsum[wave][n * 16 + li] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kH)
+        partial[static_cast<uint64_t>(blockIdx.x) * kH + threadIdx.x] =
+            colsum[0][threadIdx.x] + colsum[1][threadIdx.x] + colsum[2][threadIdx.x] + colsum[3][threadIdx.x];
+}
+
+// Weight gradient G[M x 128] = sum_r A[r][m] * D[r][n] over this block's rows
+// (A^T D with the batch as the contraction), A given as bf16 (Abf) or FP32
+// (Af) rows of length M, D bf16 rows of length 128.  Each wave owns
+// M/64 row tiles x 8 column tiles; partial[block][M x 128].
+template <uint32_t M>
+__global__ __launch_bounds__(kThreads) void wgrad_kernel(const uint16_t *__restrict__ Abf, const float *__restrict__ Af,
+                                                         const uint16_t *__restrict__ Dm, uint64_t B, uint64_t rows_per_block,
+                                                         float *__restrict__ partial) {
+    constexpr uint32_t TM = M / 16 / 4;  // row tiles per wave
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t li = lane & 15, lg = lane >> 4;
+    f4 acc[TM][8];
+#pragma unroll
+    for (uint32_t a = 0; a < TM; ++a)
+#pragma unroll
+        for (int n = 0; n < 8; ++n) acc[a][n] = f4{0.f, 0.f, 0.f, 0.f};
+    const uint64_t r_begin = static_cast<uint64_t>(blockIdx.x) * rows_per_block;
+    const uint64_t r_end = r_begin + rows_per_block < B ? r_begin + rows_per_block : B;
+    for (uint64_t r0 = r_begin; r0 < r_end; r0 += 16) {
+        s4 bfrag[8];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            s4 b;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint64_t r = r0 + lg * 4 + j;
+                b[j] = r < r_end ? static_cast<short>(Dm[r * kH + n * 16 + li]) : 0;
+            }
+            bfrag[n] = b;
+        }
+#pragma unroll
+        for (uint32_t a = 0; a < TM; ++a) {
+            const uint32_t m = (wave * TM + a) * 16 + li;
+            s4 afrag;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint64_t r = r0 + lg * 4 + j;
+                short v = 0;
+                if (r < r_end) v = Abf ? static_cast<short>(Abf[r * M + m]) : static_cast<short>(f2bf(Af[r * M + m]));
+                afrag[j] = v;
+            }
+#pragma unroll
+            for (int n = 0; n < 8; ++n) acc[a][n] = mfma(afrag, bfrag[n], acc[a][n]);
+        }
+    }
+    float *out = partial + static_cast<uint64_t>(blockIdx.x) * M * kH;
+#pragma unroll
+    for (uint32_t a = 0; a < TM; ++a)
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t m = (wave * TM + a) * 16 + lg * 4 + j;
+                out[m * kH + n * 16 + li] = acc[a][n][j];
+            }
+}
+
+// out[i] = scale * sum_b partial[b * width + i] (fixed order)
+__global__ void reduce_columns_kernel(const float *__restrict__ partial, uint32_t nblk, uint32_t width,
+                                      float *__restrict__ out, uint32_t out_offset) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= width) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nblk; ++b) s += partial[static_cast<uint64_t>(b) * width + i];
+    out[out_offset + i] = static_cast<float>(s);
+}
+
+__global__ void adam_kernel(float *__restrict__ params, const float *__restrict__ grad, float *__restrict__ m,
+                            float *__restrict__ v, uint32_t n, float lr, float beta1, float beta2, float eps,
+                            float corr1, float corr2, float weight_decay) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = grad[i] + weight_decay * params[i];
+    const float mi = beta1 * m[i] + (1.f - beta1) * g;
+    const float vi = beta2 * v[i] + (1.f - beta2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    params[i] -= lr * (mi / corr1) / (sqrtf(vi / corr2) + eps);
+}
+
+// ---- preconditioner loss: L = 1 - <e,t>^2 / (|e|^2 |t_perp|^2), real parts,
+// t_i = r_i e^{y_i} / (theta - d_i), t_perp = t - u <u,t>, e orthogonal to u.
+__device__ __forceinline__ double precond_scale(double theta, double d, float y) {
+    double den = theta - d;
+    if (fabs(den) < 1e-8) den = den < 0.0 ? -1e-8 : (den > 0.0 ? 1e-8 : -1e-8);
+    return exp(static_cast<double>(y)) / den;
+}
+
+__global__ __launch_bounds__(kThreads) void loss_sums_kernel(const float *__restrict__ y, const dvec2 *__restrict__ r,
+                                                             const dvec2 *__restrict__ e, const dvec2 *__restrict__ u,
+                                                             const double *__restrict__ d, double theta, uint64_t B,
+                                                             double *__restrict__ partial) {
+    __shared__ double red[4][kThreads];
+    double s_et = 0.0, s_tt = 0.0, s_ut = 0.0, s_ee = 0.0;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < B;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const double t = r[i].x * precond_scale(theta, d[i], y[i]);
+        s_et += e[i].x * t;
+        s_tt += t * t;
+        s_ut += u[i].x * t;
+        s_ee += e[i].x * e[i].x;
+    }
+    red[0][threadIdx.x] = s_et;
+    red[1][threadIdx.x] = s_tt;
+    red[2][threadIdx.x] = s_ut;
+    red[3][threadIdx.x] = s_ee;
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double s = 0.0;
+        for (uint32_t k = 0; k < kThreads; ++k) s += red[threadIdx.x][k];
+        partial[blockIdx.x * 4 + threadIdx.x] = s;
+    }
+}
+
+__global__ void loss_grad_kernel(const float *__restrict__ y, const dvec2 *__restrict__ r, const dvec2 *__restrict__ e,
+                                 const dvec2 *__restrict__ u, const double *__restrict__ d, double theta, uint64_t B,
+                                 double a, double bnorm, double c, double ut, double weight,
+                                 float *__restrict__ dy) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < B;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const double t = r[i].x * precond_scale(theta, d[i], y[i]);
+        const double tperp = t - ut * u[i].x;
+        const double dt = -2.0 * a * e[i].x / (bnorm * c) + 2.0 * a * a * tperp / (bnorm * c * c);
+        dy[i] = static_cast<float>(weight * dt * t);
+    }
+}
+
+using tcp::fail;
+
+}  // namespace
+
+struct tcp_mlp {
+    int device = 0;
+    tcp::DeviceBuffer params, grad, adam_m, adam_v, w1f, w2f, w2b;
+    uint32_t step = 0;
+    // training scratch (grown on demand)
+    tcp::DeviceBuffer h1, h2, y, da2, da1, partial;
+    MlpDev dev() const {
+        return MlpDev{static_cast<const s4 *>(w1f.ptr), static_cast<const s4 *>(w2f.ptr),
+                      static_cast<const uint16_t *>(w2b.ptr), static_cast<const float *>(params.ptr)};
+    }
+    int refresh(hipStream_t s) {
+        hipLaunchKernelGGL(swizzle_kernel, dim3((8 * 8 * 64 * 4 + 255) / 256), dim3(256), 0, s,
+                           static_cast<const float *>(params.ptr), static_cast<uint16_t *>(w1f.ptr),
+                           static_cast<uint16_t *>(w2f.ptr), static_cast<uint16_t *>(w2b.ptr));
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? TCP_OK : tcp::hip_fail(e, "mlp swizzle");
+    }
+};
+
+struct tcp_learned_precond {
+    const tcp_mlp *mlp = nullptr;
+    int device = 0;
+    uint64_t dim = 0;
+    double scale = 1.0;
+    hipStream_t stream = nullptr;
+    tcp::DeviceBuffer features, diag, logm;
+};
+
+namespace {
+
+int ensure(tcp::DeviceBuffer &b, size_t bytes) {
+    if (b.bytes >= bytes) return TCP_OK;
+    return b.allocate(bytes);
+}
+
+int forward(const tcp_mlp *mlp, const uint16_t *X, uint64_t B, float *y, float *h1, float *h2, hipStream_t s) {
+    if (B == 0) return TCP_OK;
+    const uint64_t tiles = (B + 63) / 64;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(tiles, 256 * 4));
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks), dim3(kThreads), 0, s, mlp->dev(), X, B, y, h1, h2);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TCP_OK : tcp::hip_fail(e, "mlp forward");
+}
+
+// grad (device, kP floats) = dL/dparams for dL/dy = dy over a batch of B rows.
+int backward(tcp_mlp *mlp, const uint16_t *X, uint64_t B, const float *dy, hipStream_t s) {
+    int st;
+    if ((st = ensure(mlp->h1, 4 * B * kH)) || (st = ensure(mlp->h2, 4 * B * kH)) || (st = ensure(mlp->y, 4 * B)) ||
+        (st = ensure(mlp->da2, 2 * B * kH)) || (st = ensure(mlp->da1, 2 * B * kH)))
+        return st;
+    auto *h1 = static_cast<float *>(mlp->h1.ptr);
+    auto *h2 = static_cast<float *>(mlp->h2.ptr);
+    if ((st = forward(mlp, X, B, static_cast<float *>(mlp->y.ptr), h1, h2, s))) return st;
+    const uint32_t nb_act = static_cast<uint32_t>(std::min<uint64_t>((B + 1) / 2, 1024));
+    const uint32_t nb_h1 = static_cast<uint32_t>(std::min<uint64_t>((B + 63) / 64, 1024));
+    const uint64_t rows_per_block = 2048;
+    const uint32_t nb_w = static_cast<uint32_t>((B + rows_per_block - 1) / rows_per_block);
+    const size_t partial_floats = std::max<size_t>({static_cast<size_t>(nb_act) * (2 * kH + 1),
+                                                    static_cast<size_t>(nb_h1) * kH,
+                                                    static_cast<size_t>(nb_w) * kH * kH});
+    if ((st = ensure(mlp->partial, 4 * partial_floats))) return st;
+    auto *partial = static_cast<float *>(mlp->partial.ptr);
+    auto *grad = static_cast<float *>(mlp->grad.ptr);
+    auto *da2 = static_cast<uint16_t *>(mlp->da2.ptr);
+    auto *da1 = static_cast<uint16_t *>(mlp->da1.ptr);
+    hipLaunchKernelGGL(act2_backward_kernel, dim3(nb_act), dim3(kThreads), 0, s,
+                       static_cast<const float *>(mlp->params.ptr), dy, h2, B, da2, partial);
+    // gw3 -> [kOffW3, +128), gb2 -> [kOffB2, +128), gb3 -> kOffB3: columns of width 2*128+1
+    hipLaunchKernelGGL(reduce_columns_kernel, dim3((2 * kH + 1 + 127) / 128), dim3(128), 0, s, partial, nb_act,
+                       2 * kH + 1,
+                       static_cast<float *>(mlp->h2.ptr) /* scratch */, 0u);
+    // move the three segments into place (small device copies)
+    TCP_HIP_CHECK(hipMemcpyAsync(grad + kOffW3, static_cast<float *>(mlp->h2.ptr), 4 * kH, hipMemcpyDeviceToDevice, s));
+    TCP_HIP_CHECK(hipMemcpyAsync(grad + kOffB2, static_cast<float *>(mlp->h2.ptr) + kH, 4 * kH, hipMemcpyDeviceToDevice, s));
+    TCP_HIP_CHECK(hipMemcpyAsync(grad + kOffB3, static_cast<float *>(mlp->h2.ptr) + 2 * kH, 4, hipMemcpyDeviceToDevice, s));
+    // h2 is no longer needed past this point; recompute nothing.
+    hipLaunchKernelGGL(h1_backward_kernel, dim3(nb_h1), dim3(kThreads), 0, s, mlp->dev(), da2, h1, B, da1, partial);
+    hipLaunchKernelGGL(reduce_columns_kernel, dim3(1), dim3(128), 0, s, partial, nb_h1, kH, grad, kOffB1);
+    hipLaunchKernelGGL((wgrad_kernel<kH>), dim3(nb_w), dim3(kThreads), 0, s, nullptr, h1, da2, B, rows_per_block,
+                       partial);
+    hipLaunchKernelGGL(reduce_columns_kernel, dim3((kH * kH + 255) / 256), dim3(256), 0, s, partial, nb_w, kH * kH,
+                       grad, kOffW2);
+    hipLaunchKernelGGL((wgrad_kernel<kF>), dim3(nb_w), dim3(kThreads), 0, s, X, nullptr, da1, B, rows_per_block,
+                       partial);
+    hipLaunchKernelGGL(reduce_columns_kernel, dim3((kF * kH + 255) / 256), dim3(256), 0, s, partial, nb_w, kF * kH,
+                       grad, kOffW1);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TCP_OK : tcp::hip_fail(e, "mlp backward");
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t tcp_mlp_parameter_count(void) { return kP; }
+
+int tcp_mlp_create(uint64_t seed, tcp_mlp **out) {
+    if (!out) return fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    auto *m = new tcp_mlp();
+    std::unique_ptr<tcp_mlp> guard(m);
+    std::vector<float> p(kP, 0.0f);
+    std::mt19937_64 rng(seed);
+    auto glorot = [&](uint32_t off, uint32_t fan_in, uint32_t fan_out) {
+        const float lim = std::sqrt(6.0f / static_cast<float>(fan_in + fan_out));
+        std::uniform_real_distribution<float> dist(-lim, lim);
+        for (uint32_t i = 0; i < fan_in * fan_out; ++i) p[off + i] = dist(rng);
+    };
+    glorot(kOffW1, kF, kH);
+    glorot(kOffW2, kH, kH);
+    glorot(kOffW3, kH, 1);
+    for (uint32_t i = 0; i < kH; ++i) p[kOffW3 + i] *= 0.1f;  // start near the plain Jacobi correction
+    TCP_HIP_CHECK(hipGetDevice(&m->device));
+    int st;
+    std::vector<float> zeros(kP, 0.0f);
+    if ((st = m->params.upload(p.data(), 4 * kP)) || (st = m->grad.upload(zeros.data(), 4 * kP)) ||
+        (st = m->adam_m.upload(zeros.data(), 4 * kP)) || (st = m->adam_v.upload(zeros.data(), 4 * kP)) ||
+        (st = m->w1f.allocate(2 * kF * kH)) || (st = m->w2f.allocate(2 * kH * kH)) ||
+        (st = m->w2b.allocate(2 * kH * kH)) || (st = m->refresh(nullptr)))
+        return st;
+    TCP_HIP_CHECK(hipDeviceSynchronize());
+    *out = guard.release();
+    return tcp::ok();
+}
+
+int tcp_mlp_get_parameters(const tcp_mlp *mlp, float *params) {
+    if (!mlp || !params) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    tcp::DeviceGuard dg;
+    int st = dg.enter(mlp->device);
+    if (st) return st;
+    TCP_HIP_CHECK(hipMemcpy(params, mlp->params.ptr, 4 * kP, hipMemcpyDeviceToHost));
+    return tcp::ok();
+}
+
+int tcp_mlp_set_parameters(tcp_mlp *mlp, const float *params) {
+    if (!mlp || !params) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    tcp::DeviceGuard dg;
+    int st = dg.enter(mlp->device);
+    if (st) return st;
+    TCP_HIP_CHECK(hipMemcpy(mlp->params.ptr, params, 4 * kP, hipMemcpyHostToDevice));
+    if ((st = mlp->refresh(nullptr))) return st;
+    TCP_HIP_CHECK(hipDeviceSynchronize());
+    return tcp::ok();
+}
+
+int tcp_mlp_forward(const tcp_mlp *mlp, const void *features, uint64_t batch, float *out, void *stream) {
+    if (!mlp || (batch && (!features || !out))) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    tcp::DeviceGuard dg;
+    int st = dg.enter(mlp->device);
+    if (st) return st;
+    st = forward(mlp, static_cast<const uint16_t *>(features), batch, out, nullptr, nullptr,
+                 static_cast<hipStream_t>(stream));
+    return st ? st : tcp::ok();
+}
+
+int tcp_mlp_gradient(tcp_mlp *mlp, const void *features, uint64_t batch, const float *dloss_dout, float *grad_host,
+                     void *stream) {
+    if (!mlp || !batch || !features || !dloss_dout) return fail(TCP_ERR_INVALID_ARGUMENT, "bad argument");
+    tcp::DeviceGuard dg;
+    int st = dg.enter(mlp->device);
+    if (st) return st;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((st = backward(mlp, static_cast<const uint16_t *>(features), batch, dloss_dout, s))) return st;
+    if (grad_host) {
+        TCP_HIP_CHECK(hipMemcpyAsync(grad_host, mlp->grad.ptr, 4 * kP, hipMemcpyDeviceToHost, s));
+        TCP_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return tcp::ok();
+}
+
+int tcp_mlp_adam_step(tcp_mlp *mlp, float lr, float weight_decay, void *stream) {
+    if (!mlp) return fail(TCP_ERR_INVALID_ARGUMENT, "null mlp");
+    tcp::DeviceGuard dg;
+    int st = dg.enter(mlp->device);
+    if (st) return st;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ++mlp->step;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, static_cast<float>(mlp->step));
+    const float c2 = 1.f - std::pow(b2, static_cast<float>(mlp->step));
+    hipLaunchKernelGGL(adam_kernel, dim3((kP + 255) / 256), dim3(256), 0, s, static_cast<float *>(mlp->params.ptr),
+                       static_cast<const float *>(mlp->grad.ptr), static_cast<float *>(mlp->adam_m.ptr),
+                       static_cast<float *>(mlp->adam_v.ptr), kP, lr, b1, b2, 1e-8f, c1, c2, weight_decay);
+    if ((st = mlp->refresh(s))) return st;
+    return tcp::ok();
+}
+
+void tcp_mlp_destroy(tcp_mlp *mlp) { delete mlp; }
+
+int tcp_precond_loss_gradient(const float *y, const void *r, const void *e, const void *u, const double *diag,
+                              double theta, uint64_t dim, double weight, float *dy, double *loss, void *stream) {
+    if (!y || !r || !e || !u || !diag || !dy || !loss || dim == 0) return fail(TCP_ERR_INVALID_ARGUMENT, "bad argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>((dim + kThreads - 1) / kThreads, 1024));
+    tcp::DeviceBuffer partial;
+    int st = partial.allocate(8 * 4 * nb);
+    if (st) return st;
+    hipLaunchKernelGGL(loss_sums_kernel, dim3(nb), dim3(kThreads), 0, s, y, static_cast<const dvec2 *>(r),
+                       static_cast<const dvec2 *>(e), static_cast<const dvec2 *>(u), diag, theta, dim,
+                       static_cast<double *>(partial.ptr));
+    std::vector<double> h(4 * nb);
+    TCP_HIP_CHECK(hipMemcpyAsync(h.data(), partial.ptr, 8 * 4 * nb, hipMemcpyDeviceToHost, s));
+    TCP_HIP_CHECK(hipStreamSynchronize(s));
+    double a = 0, tt = 0, ut = 0, ee = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+        a += h[4 * b];
+        tt += h[4 * b + 1];
+        ut += h[4 * b + 2];
+        ee += h[4 * b + 3];
+    }
+    const double c = tt - ut * ut;
+    if (!(ee > 0.0) || !(c > 0.0)) return fail(TCP_ERR_INVALID_ARGUMENT, "degenerate preconditioner snapshot");
+    *loss = 1.0 - a * a / (ee * c);
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(nb), dim3(kThreads), 0, s, y, static_cast<const dvec2 *>(r),
+                       static_cast<const dvec2 *>(e), static_cast<const dvec2 *>(u), diag, theta, dim, a, ee, c, ut,
+                       weight, dy);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return tcp::hip_fail(err, "loss gradient");
+    TCP_HIP_CHECK(hipStreamSynchronize(s));
+    return tcp::ok();
+}
+
+int tcp_learned_precond_create(const tcp_mlp *mlp, const tcp_sector_plan *plan, tcp_learned_precond **out) {
+    if (!mlp || !plan || !out) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    auto *p = new tcp_learned_precond();
+    std::unique_ptr<tcp_learned_precond> guard(p);
+    p->mlp = mlp;
+    p->dim = tcp::plan_dimension(plan);
+    p->scale = tcp::plan_energy_scale(plan);
+    p->device = tcp::plan_device(plan);
+    tcp::DeviceGuard dg;
+    int st = dg.enter(p->device);
+    if (st) return st;
+    if ((st = p->features.allocate(2 * kF * p->dim)) || (st = p->diag.allocate(8 * p->dim)) ||
+        (st = p->logm.allocate(4 * p->dim)))
+        return st;
+    if ((st = tcp::plan_features(plan, p->scale, static_cast<uint16_t *>(p->features.ptr),
+                                 static_cast<double *>(p->diag.ptr), nullptr)))
+        return st;
+    TCP_HIP_CHECK(hipDeviceSynchronize());
+    *out = guard.release();
+    return tcp::ok();
+}
+
+int tcp_learned_precond_prepare(tcp_learned_precond *p, double theta, void *stream) {
+    if (!p) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((p->dim + 255) / 256, 4096));
+    hipLaunchKernelGGL(fill_feature0_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<uint16_t *>(p->features.ptr), static_cast<const double *>(p->diag.ptr), theta,
+                       1.0 / p->scale, p->dim);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? tcp::ok() : tcp::hip_fail(e, "feature fill");
+}
+
+// tcp_log_multiplier_fn: log multipliers for the current theta (feature 0 =
+// (H_ii - theta) / scale, then MLP inference over every state).
+const void *tcp_learned_precond_log_multiplier(void *ctx, double theta) {
+    auto *p = static_cast<tcp_learned_precond *>(ctx);
+    if (!p || tcp_learned_precond_prepare(p, theta, p->stream) != TCP_OK) return nullptr;
+    int st = forward(p->mlp, static_cast<const uint16_t *>(p->features.ptr), p->dim,
+                     static_cast<float *>(p->logm.ptr), nullptr, nullptr, p->stream);
+    return st ? nullptr : p->logm.ptr;
+}
+
+int tcp_learned_precond_buffers(const tcp_learned_precond *p, const void **features, const double **diag,
+                                double *scale) {
+    if (!p) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    if (features) *features = p->features.ptr;
+    if (diag) *diag = static_cast<const double *>(p->diag.ptr);
+    if (scale) *scale = p->scale;
+    return tcp::ok();
+}
+
+void tcp_learned_precond_destroy(tcp_learned_precond *p) { delete p; }
+
+}  // extern "C"

@@ -213,6 +213,30 @@ __global__ void mvp_count_kernel(MvpArgs p, uint32_t A, uint32_t G, uint64_t dim
     atomicAdd(count, local);
 }
 
+// Real part of the diagonal H_ii: the x = 0 group (group 0 when present).
+__global__ void mvp_diagonal_kernel(MvpArgs p, uint32_t A, uint32_t has_diag, uint64_t dim,
+                                    double *__restrict__ diag) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t tile_id = i >> p.L;
+        const uint32_t l = static_cast<uint32_t>(i & ((1ull << p.L) - 1));
+        const uint32_t t = l & (kThreads - 1);
+        const uint32_t slot = l >> kThreadBits;
+        double d = 0.0;
+        if (has_diag) {
+            for (uint32_t c = 0; c < A; ++c) {
+                const uint32_t cpar = __popc(c & slot);
+                for (uint32_t k = p.g_off[c]; k < p.g_off[c + 1]; ++k) {
+                    const uint32_t par =
+                        (__popc(p.t_zlo[k] & t) + __popcll(p.t_zhi[k] & tile_id) + cpar) & 1u;
+                    d += flip_sign(reinterpret_cast<const double2 *>(p.t_w)[k].x, par);
+                }
+            }
+        }
+        diag[i] = d;
+    }
+}
+
 }  // namespace
 
 struct tcp_mvp_plan {
@@ -224,6 +248,7 @@ struct tcp_mvp_plan {
     int device = 0;
     tcp::DeviceBuffer cls_hi, cls_goff, g_lo, g_off, t_zlo, t_zhi, t_w64, t_w32;
     uint64_t retained_bytes = 0;
+    bool has_diagonal = false;  // group 0 has x = 0
     std::mutex staging_mutex;
     tcp::DeviceBuffer stage_in, stage_out;
 
@@ -326,6 +351,7 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, tcp_mvp
     cls_goff.push_back(static_cast<uint32_t>(g_lo.size()));
     plan->G = g_lo.size();
     plan->H = cls_hi.size();
+    plan->has_diagonal = !cls_hi.empty() && cls_hi[0] == 0 && g_lo[0] == 0;
     if (cls_hi.empty()) cls_hi.push_back(0);
 
     uint64_t device_bytes = cls_hi.size() * 8 + cls_goff.size() * 4 + g_lo.size() * 4 +
@@ -417,6 +443,26 @@ int apply_device(const tcp_mvp_plan *plan, const void *in, void *out, uint64_t n
     if (e != hipSuccess) return tcp::hip_fail(e, "mvp kernel launch");
     return tcp::ok();
 }
+
+}  // namespace
+
+namespace tcp {
+uint64_t plan_dimension(const tcp_mvp_plan *plan) { return plan->dim(); }
+int plan_device(const tcp_mvp_plan *plan) { return plan->device; }
+int plan_apply(const tcp_mvp_plan *plan, const void *in, void *out, int dtype, hipStream_t stream) {
+    return apply_device(plan, in, out, 1, dtype, stream);
+}
+int plan_diagonal(const tcp_mvp_plan *plan, double *diag, hipStream_t stream) {
+    const uint64_t dim = plan->dim();
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((dim + 255) / 256, 65536));
+    hipLaunchKernelGGL(mvp_diagonal_kernel, dim3(blocks), dim3(256), 0, stream, plan->args(TCP_C128),
+                       1u << plan->ab, plan->has_diagonal ? 1u : 0u, dim, diag);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TCP_OK : hip_fail(e, "diagonal kernel");
+}
+}  // namespace tcp
+
+namespace {
 
 int check_len(uint64_t expected, uint64_t actual) {
     if (expected != actual)

@@ -28,6 +28,7 @@
 // bits from the top against C(p, needed) and later ranks of a thread follow
 // by the next-larger-combination step (ascending order).
 #include <algorithm>
+#include <cmath>
 #include <memory>
 #include <mutex>
 
@@ -277,6 +278,99 @@ __global__ __launch_bounds__(kSectorThreads) void sector_leak_kernel(SectorArgs 
     }
 }
 
+// Real part of the diagonal H_rr: the x = 0 group evaluated on each state.
+__global__ void sector_diagonal_kernel(SectorArgs p, uint32_t diag_group, double *__restrict__ diag) {
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    const uint64_t start = t * kRanksPerThread;
+    if (start >= p.dim) return;
+    uint64_t b = sector_unrank(p, start);
+    for (uint32_t j = 0; j < kRanksPerThread && start + j < p.dim; ++j) {
+        double d = 0.0;
+        if (diag_group < p.G) {
+            const dvec2 *tw = static_cast<const dvec2 *>(p.tw);
+            for (uint32_t k = p.goff[diag_group]; k < p.goff[diag_group + 1]; ++k)
+                d += flip_sign(tw[k].x, __popcll(p.tz[k] & b) & 1u);
+        }
+        diag[start + j] = d;
+        b = sector_next(p, b);
+    }
+}
+
+// Per-state features of the learned preconditioner (64 bf16 per state, state-
+// local and independent of the lattice size).  For state b with diagonal D_b:
+// every x-group g whose source s = b ^ x_g stays in the sector contributes a
+// coupling c = |sum_k w_k (-1)^popc(z_k & s)| and an excitation energy
+// dE = D_s - D_b.  Slots: 0 = (D_b - theta) / scale, filled at inference;
+// 1 = #couplings / 16; 2 = sum c^2 / scale^2 / 16; 3..26 = c^2-weighted
+// histogram of dE / scale over 24 bins on [-6, 6); 27..50 = unweighted
+// histogram / 8; 51..53 = mean, min, max of dE / scale; 54 = sum c^2 / dE
+// over dE > 0 (second-order shift) / scale; the rest zero.
+constexpr uint32_t kFeatures = 64;
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+__device__ double diag_at(const SectorArgs &p, uint32_t diag_group, uint64_t s) {
+    double d = 0.0;
+    if (diag_group < p.G) {
+        const dvec2 *tw = static_cast<const dvec2 *>(p.tw);
+        for (uint32_t k = p.goff[diag_group]; k < p.goff[diag_group + 1]; ++k)
+            d += flip_sign(tw[k].x, __popcll(p.tz[k] & s) & 1u);
+    }
+    return d;
+}
+
+__global__ void sector_features_kernel(SectorArgs p, uint32_t diag_group, double inv_scale,
+                                       uint16_t *__restrict__ features, double *__restrict__ diag) {
+    const uint64_t r = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (r >= p.dim) return;
+    const uint64_t b = sector_unrank(p, r);
+    const double db = diag_at(p, diag_group, b);
+    float f[kFeatures];
+    for (uint32_t i = 0; i < kFeatures; ++i) f[i] = 0.0f;
+    double ncoup = 0.0, c2sum = 0.0, desum = 0.0, demin = 1e300, demax = -1e300, shift = 0.0;
+    const dvec2 *tw = static_cast<const dvec2 *>(p.tw);
+    for (uint32_t g = 0; g < p.G; ++g) {
+        if (g == diag_group) continue;
+        const uint64_t s = b ^ p.gx[g];
+        if (!in_sector(p, s)) continue;
+        double vr = 0.0, vi = 0.0;
+        for (uint32_t k = p.goff[g]; k < p.goff[g + 1]; ++k) {
+            const uint32_t par = __popcll(p.tz[k] & s) & 1u;
+            vr += flip_sign(tw[k].x, par);
+            vi += flip_sign(tw[k].y, par);
+        }
+        const double c2 = (vr * vr + vi * vi) * inv_scale * inv_scale;
+        if (c2 == 0.0) continue;
+        const double de = (diag_at(p, diag_group, s) - db) * inv_scale;
+        ncoup += 1.0;
+        c2sum += c2;
+        desum += de;
+        demin = fmin(demin, de);
+        demax = fmax(demax, de);
+        if (de > 1e-3) shift += c2 / de;
+        int bin = static_cast<int>(floor((de + 6.0) * 2.0));
+        bin = bin < 0 ? 0 : (bin > 23 ? 23 : bin);
+        f[3 + bin] += static_cast<float>(c2);
+        f[27 + bin] += 1.0f / 8.0f;
+    }
+    f[1] = static_cast<float>(ncoup / 16.0);
+    f[2] = static_cast<float>(c2sum / 16.0);
+    if (ncoup > 0) {
+        f[51] = static_cast<float>(desum / ncoup);
+        f[52] = static_cast<float>(demin);
+        f[53] = static_cast<float>(demax);
+    }
+    f[54] = static_cast<float>(shift);
+    uint16_t *out = features + r * kFeatures;
+    for (uint32_t i = 0; i < kFeatures; ++i) out[i] = to_bf16(f[i]);
+    diag[r] = db;
+}
+
 }  // namespace
 
 struct tcp_sector_plan {
@@ -287,6 +381,8 @@ struct tcp_sector_plan {
     int device = 0;
     bool lut_in_lds = true;
     bool u1 = true;
+    uint32_t diag_group = UINT32_MAX;  // index of the x = 0 group
+    double energy_scale = 1.0;         // largest sum |w| of an off-diagonal group
     bool leaks = false;        // the reference's apply would raise
     uint64_t leak_source = 0;  // first leaking source rank
     uint32_t leak_weight = 0;  // its destination weight (U(1) message)
@@ -477,6 +573,17 @@ int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, u
     goff.push_back(static_cast<uint32_t>(tz.size()));
     plan->T = tz.size();
     plan->G = gx.size();
+    double scale = 0.0;
+    for (size_t g = 0; g < gx.size(); ++g) {
+        if (gx[g] == 0) {
+            plan->diag_group = static_cast<uint32_t>(g);
+            continue;
+        }
+        double sum = 0.0;
+        for (uint64_t k : members[g]) sum += std::hypot(terms[k].w.re, terms[k].w.im);
+        scale = std::max(scale, sum);
+    }
+    plan->energy_scale = scale > 0.0 ? scale : 1.0;
     a.G = static_cast<uint32_t>(gx.size());
     if (gx.empty()) gx.push_back(0);
     if (tz.empty()) {
@@ -551,6 +658,38 @@ int leak_error(const tcp_sector_plan *plan) {
                         ", got destination weight " + std::to_string(plan->leak_weight));
     return fail(TCP_ERR_INVALID_SECTOR, "invalid sector: operator leaks outside the selected charge sector");
 }
+
+int apply_device(const tcp_sector_plan *plan, const void *in, void *out, int dtype, void *stream);
+
+}  // namespace
+
+namespace tcp {
+uint64_t plan_dimension(const tcp_sector_plan *plan) { return plan->dim; }
+int plan_device(const tcp_sector_plan *plan) { return plan->device; }
+int plan_apply(const tcp_sector_plan *plan, const void *in, void *out, int dtype, hipStream_t stream) {
+    return apply_device(plan, in, out, dtype, stream);
+}
+double plan_energy_scale(const tcp_sector_plan *plan) { return plan->energy_scale; }
+int plan_features(const tcp_sector_plan *plan, double scale, uint16_t *features, double *diag, hipStream_t stream) {
+    if (plan->dim == 0) return TCP_OK;
+    const uint32_t blocks = static_cast<uint32_t>((plan->dim + 255) / 256);
+    hipLaunchKernelGGL(sector_features_kernel, dim3(blocks), dim3(256), 0, stream, plan->args(TCP_C128),
+                       plan->diag_group, 1.0 / scale, features, diag);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TCP_OK : hip_fail(e, "sector features kernel");
+}
+int plan_diagonal(const tcp_sector_plan *plan, double *diag, hipStream_t stream) {
+    if (plan->dim == 0) return TCP_OK;
+    const uint64_t threads = (plan->dim + kRanksPerThread - 1) / kRanksPerThread;
+    const uint32_t blocks = static_cast<uint32_t>((threads + 255) / 256);
+    hipLaunchKernelGGL(sector_diagonal_kernel, dim3(blocks), dim3(256), 0, stream, plan->args(TCP_C128),
+                       plan->diag_group, diag);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TCP_OK : hip_fail(e, "sector diagonal kernel");
+}
+}  // namespace tcp
+
+namespace {
 
 int check_len(uint64_t expected, uint64_t actual) {
     if (expected != actual)

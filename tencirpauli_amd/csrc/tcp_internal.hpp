@@ -85,6 +85,22 @@ std::vector<IndexTerm> index_terms(const Operator &op);
 // Exact conservation analysis of a diagonal qubit charge (core/operator.rs:396-445).
 int analyze_charge(const Operator &op, const double *levels, bool &conserved, uint64_t &nonzero);
 
+// Operator access for solvers built on the plans (device buffers, async on `stream`).
+uint64_t plan_dimension(const tcp_mvp_plan *plan);
+uint64_t plan_dimension(const tcp_sector_plan *plan);
+int plan_device(const tcp_mvp_plan *plan);
+int plan_device(const tcp_sector_plan *plan);
+int plan_apply(const tcp_mvp_plan *plan, const void *in, void *out, int dtype, hipStream_t stream);
+int plan_apply(const tcp_sector_plan *plan, const void *in, void *out, int dtype, hipStream_t stream);
+// Real part of the diagonal matrix elements H_ii (FP64, `dimension` entries).
+int plan_diagonal(const tcp_mvp_plan *plan, double *diag, hipStream_t stream);
+int plan_diagonal(const tcp_sector_plan *plan, double *diag, hipStream_t stream);
+
+// Learned-preconditioner inputs of a sector plan: 64 bf16 features per state
+// (slot 0 left for (H_ii - theta) / scale) and the diagonal.
+double plan_energy_scale(const tcp_sector_plan *plan);
+int plan_features(const tcp_sector_plan *plan, double scale, uint16_t *features, double *diag, hipStream_t stream);
+
 // RAII device buffer.
 struct DeviceBuffer {
     void *ptr = nullptr;
