@@ -235,6 +235,41 @@ int tcp_sector_plan_walk_device(const tcp_sector_plan *plan, uint64_t first, uin
 
 void tcp_sector_plan_destroy(tcp_sector_plan *plan);
 
+/* ---- batched MVP (no reference counterpart): many operators of uneven size,
+ *      each applied to a block of right-hand sides, in one persistent launch -- */
+
+typedef struct tcp_batch_plan tcp_batch_plan;
+
+/* The batch buffer holds, for operator p in order, `nrhs` vectors of 2^n_p
+ * amplitudes back to back starting at amplitude tcp_batch_plan_offset(p);
+ * input and output use the same layout.  Operators need >= 9 qubits. */
+int tcp_batch_plan_create(const tcp_operator *const *ops, uint32_t count, uint32_t nrhs,
+                          tcp_batch_plan **out);
+uint32_t tcp_batch_plan_count(const tcp_batch_plan *plan);
+uint64_t tcp_batch_plan_offset(const tcp_batch_plan *plan, uint32_t index);
+uint64_t tcp_batch_plan_total_amplitudes(const tcp_batch_plan *plan);
+/* sum over operators of terms x 2^n x nrhs (term*amp*rhs per apply) */
+double tcp_batch_plan_work(const tcp_batch_plan *plan);
+/* Tile geometries the autotuner chooses from: threads per workgroup and
+ * amplitudes per thread. */
+uint32_t tcp_batch_config_count(void);
+int tcp_batch_config(uint32_t index, uint32_t *threads, uint32_t *amplitudes);
+uint32_t tcp_batch_plan_selected(const tcp_batch_plan *plan);
+int tcp_batch_plan_select(tcp_batch_plan *plan, uint32_t config);
+/* Times every geometry on the given buffers (after one warm-up each), keeps
+ * the fastest; times_ms (optional, tcp_batch_config_count() entries) gets
+ * the per-apply times, -1 for geometries that do not fit. */
+int tcp_batch_plan_autotune(tcp_batch_plan *plan, const void *input, void *output, int dtype,
+                            uint32_t repeats, float *times_ms, uint32_t *chosen);
+/* output <- H_p input for every operator and right-hand side, on `stream`. */
+int tcp_batch_plan_apply(const tcp_batch_plan *plan, const void *input, void *output, int dtype,
+                         void *stream);
+/* Capture one apply into a hipGraph kept by the plan; replay launches it. */
+int tcp_batch_plan_capture(tcp_batch_plan *plan, const void *input, void *output, int dtype,
+                           void *stream);
+int tcp_batch_plan_replay(const tcp_batch_plan *plan, void *stream);
+void tcp_batch_plan_destroy(tcp_batch_plan *plan);
+
 /* ---- Davidson eigensolver (no reference counterpart: the reference's
  *      eigen entry point is SciPy eigsh over a LinearOperator,
  *      py/integrations/scipy.py:13-47) ------------------------------------- */

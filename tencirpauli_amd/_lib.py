@@ -97,6 +97,20 @@ SIGNATURES = {
     "tcp_sector_plan_unrank_device": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _vp]),
     "tcp_sector_plan_walk_device": (_c.c_int, [_vp, _c.c_uint64, _vp, _c.c_uint64, _c.c_uint32, _vp]),
     "tcp_sector_plan_destroy": (None, [_vp]),
+    "tcp_batch_plan_create": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint32, _c.POINTER(_vp)]),
+    "tcp_batch_plan_count": (_c.c_uint32, [_vp]),
+    "tcp_batch_plan_offset": (_c.c_uint64, [_vp, _c.c_uint32]),
+    "tcp_batch_plan_total_amplitudes": (_c.c_uint64, [_vp]),
+    "tcp_batch_plan_work": (_c.c_double, [_vp]),
+    "tcp_batch_config_count": (_c.c_uint32, []),
+    "tcp_batch_config": (_c.c_int, [_c.c_uint32, _u32p, _u32p]),
+    "tcp_batch_plan_selected": (_c.c_uint32, [_vp]),
+    "tcp_batch_plan_select": (_c.c_int, [_vp, _c.c_uint32]),
+    "tcp_batch_plan_autotune": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _c.c_uint32, _vp, _u32p]),
+    "tcp_batch_plan_apply": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _vp]),
+    "tcp_batch_plan_capture": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _vp]),
+    "tcp_batch_plan_replay": (_c.c_int, [_vp, _vp]),
+    "tcp_batch_plan_destroy": (None, [_vp]),
     "tcp_davidson_default_options": (None, [_vp]),
     "tcp_davidson_sector": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "tcp_davidson_full": (_c.c_int, [_vp, _vp, _vp, _vp]),

@@ -27,7 +27,7 @@
 #include <memory>
 #include <mutex>
 
-#include "tcp_internal.hpp"
+#include "tile_tables.hpp"
 
 namespace {
 
@@ -35,144 +35,21 @@ constexpr uint32_t kThreads = 512;  // threads per workgroup (fixed stride of sl
 constexpr uint32_t kThreadBits = 9;
 constexpr uint32_t kMaxSlotBits = 3;  // up to 8 amplitudes per thread
 
-struct MvpArgs {
-    const uint64_t *cls_hi;    // [H]   x >> L of each class
-    const uint32_t *cls_goff;  // [H+1] group range of each class
-    const uint32_t *g_lo;      // [G]   x & (2^L - 1)
-    const uint32_t *g_off;     // [G*A + 1] term start per (group, slot class)
-    const uint32_t *t_zlo;     // [T]   z & (2^tb - 1)
-    const uint64_t *t_zhi;     // [T]   z >> L
-    const void *t_w;           // [T]   (re, im) of w * (-1)^popc(z & x)
-    uint32_t H;
-    uint32_t L;
-    uint32_t active;  // threads owning an amplitude: min(512, 2^n)
-    uint64_t dim;     // 2^n, the stride between right-hand sides
-};
+using tcp::Amp;
+using tcp::cptr;
+using tcp::dvec2;
+using tcp::flip_sign;
+using tcp::fvec2;
+using MvpArgs = tcp::TileArgs;
 
-// Plan tables are read through the constant address space: every index into
-// them is wave-uniform, and constant-space loads become scalar loads (s_load)
-// whose completion is tracked apart from the in-flight tile prefetch.
-template <class T>
-using cptr = const __attribute__((address_space(4))) T *;
-
-typedef double dvec2 __attribute__((ext_vector_type(2)));
-typedef float fvec2 __attribute__((ext_vector_type(2)));
-
-template <typename VT>
-struct Amp;
-template <>
-struct Amp<dvec2> {
-    __device__ static double re(const dvec2 &v) { return v.x; }
-    __device__ static double im(const dvec2 &v) { return v.y; }
-    __device__ static dvec2 make(double r, double i) { return dvec2{r, i}; }
-};
-template <>
-struct Amp<fvec2> {
-    __device__ static double re(const fvec2 &v) { return static_cast<double>(v.x); }
-    __device__ static double im(const fvec2 &v) { return static_cast<double>(v.y); }
-    __device__ static fvec2 make(double r, double i) {
-        return fvec2{static_cast<float>(r), static_cast<float>(i)};
-    }
-};
-
-__device__ __forceinline__ double flip_sign(double x, uint32_t parity) {
-    return __longlong_as_double(__double_as_longlong(x) ^ (static_cast<long long>(parity) << 63));
-}
-
+// One workgroup per output tile (blockIdx.x) and right-hand side (blockIdx.y).
 // PARTIAL: fewer than 512 amplitudes (n < 9); threads t >= 2^n idle.
 template <int A, typename VT, bool PARTIAL>
-__global__ __launch_bounds__(kThreads) void mvp_tile_kernel(const VT *__restrict__ v,
-                                                            VT *__restrict__ y, MvpArgs p) {
+__global__ __launch_bounds__(kThreads) void mvp_tile_kernel(const VT *__restrict__ v, VT *__restrict__ y,
+                                                            MvpArgs p, uint32_t active, uint64_t dim) {
     __shared__ VT tile[kThreads * A];
-    cptr<uint64_t> cls_hi = (cptr<uint64_t>)p.cls_hi;
-    cptr<uint32_t> cls_goff = (cptr<uint32_t>)p.cls_goff;
-    cptr<uint32_t> g_lo = (cptr<uint32_t>)p.g_lo;
-    cptr<uint32_t> g_off = (cptr<uint32_t>)p.g_off;
-    cptr<uint32_t> t_zlo = (cptr<uint32_t>)p.t_zlo;
-    cptr<uint64_t> t_zhi = (cptr<uint64_t>)p.t_zhi;
-    cptr<dvec2> t_w = (cptr<dvec2>)p.t_w;
-
-    const uint32_t t = threadIdx.x;
-    const uint64_t tile_id = blockIdx.x;
-    const uint64_t base = tile_id << p.L;
-    const uint64_t rhs = static_cast<uint64_t>(blockIdx.y) * p.dim;
-    const VT *__restrict__ vin = v + rhs + t;
-    const bool on = !PARTIAL || t < p.active;
-
-    double accr[A], acci[A];
-    VT pre[A];
-#pragma unroll
-    for (int a = 0; a < A; ++a) {
-        accr[a] = 0.0;
-        acci[a] = 0.0;
-    }
-    {
-        const uint64_t src = base ^ (cls_hi[0] << p.L);
-        if (on) {
-#pragma unroll
-            for (int a = 0; a < A; ++a) pre[a] = vin[src + a * kThreads];
-        }
-    }
-    for (uint32_t h = 0; h < p.H; ++h) {
-        __syncthreads();  // every wave is done reading the previous class's tile
-        if (on) {
-#pragma unroll
-            for (int a = 0; a < A; ++a) tile[t + a * kThreads] = pre[a];
-        }
-        __syncthreads();
-        const uint32_t hn = h + 1 < p.H ? h + 1 : h;
-        const uint64_t src = base ^ (cls_hi[hn] << p.L);
-        if (h + 1 < p.H && on) {  // prefetch the next class's source tile
-#pragma unroll
-            for (int a = 0; a < A; ++a) pre[a] = vin[src + a * kThreads];
-        }
-        const uint32_t g0 = cls_goff[h], g1 = cls_goff[h + 1];
-        for (uint32_t g = g0; g < g1; ++g) {
-            const uint32_t lo = g_lo[g];
-            double sr[A], si[A];
-#pragma unroll
-            for (int c = 0; c < A; ++c) {
-                sr[c] = 0.0;
-                si[c] = 0.0;
-                const uint32_t k0 = g_off[g * A + c], k1 = g_off[g * A + c + 1];
-                for (uint32_t k = k0; k < k1; ++k) {
-                    const uint32_t tile_par = __popcll(t_zhi[k] & tile_id);
-                    const uint32_t par = (__popc(t_zlo[k] & t) + tile_par) & 1u;
-                    const dvec2 w = t_w[k];
-                    sr[c] += flip_sign(w.x, par);
-                    si[c] += flip_sign(w.y, par);
-                }
-            }
-            // d[a] = sum_c (-1)^popc(c & a) S[c]
-#pragma unroll
-            for (int m = 1; m < A; m <<= 1) {
-#pragma unroll
-                for (int c = 0; c < A; ++c) {
-                    if (!(c & m)) {
-                        const double ur = sr[c], ui = si[c], vr = sr[c | m], vi = si[c | m];
-                        sr[c] = ur + vr;
-                        si[c] = ui + vi;
-                        sr[c | m] = ur - vr;
-                        si[c | m] = ui - vi;
-                    }
-                }
-            }
-#pragma unroll
-            for (int a = 0; a < A; ++a) {
-                const VT s = tile[(t + a * kThreads) ^ lo];
-                const double s_r = Amp<VT>::re(s), s_i = Amp<VT>::im(s);
-                accr[a] = fma(sr[a], s_r, accr[a]);
-                accr[a] = fma(-si[a], s_i, accr[a]);
-                acci[a] = fma(sr[a], s_i, acci[a]);
-                acci[a] = fma(si[a], s_r, acci[a]);
-            }
-        }
-    }
-    if (on) {
-        VT *__restrict__ out = y + rhs + base + t;
-#pragma unroll
-        for (int a = 0; a < A; ++a) out[a * kThreads] = Amp<VT>::make(accr[a], acci[a]);
-    }
+    const uint64_t rhs = static_cast<uint64_t>(blockIdx.y) * dim;
+    tcp::tile_apply<A, VT, PARTIAL, kThreads>(v + rhs, y + rhs, p, blockIdx.x, active, tile);
 }
 
 // Fill kernel used instead of hipMemsetAsync (safe under graph capture).
@@ -261,13 +138,12 @@ struct tcp_mvp_plan {
         a.g_off = static_cast<const uint32_t *>(g_off.ptr);
         a.t_zlo = static_cast<const uint32_t *>(t_zlo.ptr);
         a.t_zhi = static_cast<const uint64_t *>(t_zhi.ptr);
-        a.t_w = static_cast<const double2 *>(dtype == TCP_C64 ? t_w32.ptr : t_w64.ptr);
+        a.t_w = dtype == TCP_C64 ? t_w32.ptr : t_w64.ptr;
         a.H = static_cast<uint32_t>(H);
         a.L = L;
-        a.active = n >= kThreadBits ? kThreads : (1u << n);
-        a.dim = dim();
         return a;
     }
+    uint32_t active() const { return n >= kThreadBits ? kThreads : (1u << n); }
 };
 
 namespace {
@@ -292,67 +168,24 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, tcp_mvp
     plan->tb = std::min(n, kThreadBits);
     plan->ab = n > kThreadBits ? std::min(n - kThreadBits, kMaxSlotBits) : 0;
     plan->L = plan->tb + plan->ab;
-    const uint32_t L = plan->L, A = 1u << plan->ab;
-    const uint64_t lmask = (1ull << L) - 1;
-
-    std::vector<tcp::IndexTerm> terms = tcp::index_terms(op);
-    struct Key {
-        uint64_t hi, lo;
-        uint32_t cls;
-        uint64_t index;
-    };
-    std::vector<Key> keys(T);
-    for (uint64_t k = 0; k < T; ++k)
-        keys[k] = Key{terms[k].x >> L, terms[k].x & lmask,
-                      static_cast<uint32_t>((terms[k].z >> plan->tb) & (A - 1)), k};
-    std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
-        if (a.hi != b.hi) return a.hi < b.hi;
-        if (a.lo != b.lo) return a.lo < b.lo;
-        if (a.cls != b.cls) return a.cls < b.cls;
-        return a.index < b.index;
-    });
-
-    std::vector<uint64_t> cls_hi;
-    std::vector<uint32_t> cls_goff, g_lo, g_off, t_zlo;
-    std::vector<uint64_t> t_zhi;
-    std::vector<double2> t_w64, t_w32;
-    t_zlo.reserve(T);
-    t_zhi.reserve(T);
-    t_w64.reserve(T);
-    t_w32.reserve(T);
-    uint64_t k = 0;
-    while (k < T) {
-        if (cls_hi.empty() || cls_hi.back() != keys[k].hi) {
-            cls_hi.push_back(keys[k].hi);
-            cls_goff.push_back(static_cast<uint32_t>(g_lo.size()));
-        }
-        uint64_t e = k;
-        while (e < T && keys[e].hi == keys[k].hi && keys[e].lo == keys[k].lo) ++e;
-        g_lo.push_back(static_cast<uint32_t>(keys[k].lo));
-        uint64_t j = k;
-        for (uint32_t c = 0; c < A; ++c) {
-            g_off.push_back(static_cast<uint32_t>(j));
-            while (j < e && keys[j].cls == c) ++j;
-        }
-        for (uint64_t q = k; q < e; ++q) {
-            const tcp::IndexTerm &term = terms[keys[q].index];
-            const bool neg = __builtin_popcountll(term.z & term.x) & 1;
-            const double wr = neg ? -term.w.re : term.w.re;
-            const double wi = neg ? -term.w.im : term.w.im;
-            t_zlo.push_back(static_cast<uint32_t>(term.z & ((1ull << plan->tb) - 1)));
-            t_zhi.push_back(term.z >> L);
-            t_w64.push_back(make_double2(wr, wi));
-            t_w32.push_back(make_double2(static_cast<double>(static_cast<float>(wr)),
-                                         static_cast<double>(static_cast<float>(wi))));
-        }
-        k = e;
-    }
-    g_off.push_back(static_cast<uint32_t>(T));
-    cls_goff.push_back(static_cast<uint32_t>(g_lo.size()));
-    plan->G = g_lo.size();
-    plan->H = cls_hi.size();
-    plan->has_diagonal = !cls_hi.empty() && cls_hi[0] == 0 && g_lo[0] == 0;
+    const uint32_t A = 1u << plan->ab;
+    tcp::TileTables tab;
+    tab.append(op, plan->tb, plan->ab);
+    plan->G = tab.G;
+    plan->H = tab.H;
+    plan->has_diagonal = tab.has_diagonal;
+    std::vector<uint64_t> &cls_hi = tab.cls_hi;
+    std::vector<uint32_t> &cls_goff = tab.cls_goff, &g_lo = tab.g_lo, &g_off = tab.g_off, &t_zlo = tab.t_zlo;
+    std::vector<uint64_t> &t_zhi = tab.t_zhi;
+    std::vector<double> &t_w64 = tab.t_w64, &t_w32 = tab.t_w32;
     if (cls_hi.empty()) cls_hi.push_back(0);
+    if (g_lo.empty()) g_lo.push_back(0);
+    if (t_zlo.empty()) {
+        t_zlo.push_back(0);
+        t_zhi.push_back(0);
+        t_w64.assign(2, 0.0);
+        t_w32.assign(2, 0.0);
+    }
 
     uint64_t device_bytes = cls_hi.size() * 8 + cls_goff.size() * 4 + g_lo.size() * 4 +
                             g_off.size() * 4 + T * (4 + 8 + 16 + 16);
@@ -368,8 +201,8 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, tcp_mvp
     if ((st = plan->g_off.upload(g_off.data(), g_off.size() * 4))) return st;
     if ((st = plan->t_zlo.upload(t_zlo.data(), t_zlo.size() * 4))) return st;
     if ((st = plan->t_zhi.upload(t_zhi.data(), t_zhi.size() * 8))) return st;
-    if ((st = plan->t_w64.upload(t_w64.data(), t_w64.size() * 16))) return st;
-    if ((st = plan->t_w32.upload(t_w32.data(), t_w32.size() * 16))) return st;
+    if ((st = plan->t_w64.upload(t_w64.data(), t_w64.size() * 8))) return st;
+    if ((st = plan->t_w32.upload(t_w32.data(), t_w32.size() * 8))) return st;
 
     if (storage == TCP_STORAGE_EAGER) {
         unsigned long long *dcount = nullptr;
@@ -397,7 +230,7 @@ hipError_t launch_tile(const tcp_mvp_plan *plan, const void *in, void *out, uint
     const uint64_t tiles = plan->dim() >> plan->L;
     hipLaunchKernelGGL((mvp_tile_kernel<A, VT, PARTIAL>), dim3(static_cast<uint32_t>(tiles), nrhs),
                        dim3(kThreads), 0, stream, static_cast<const VT *>(in),
-                       static_cast<VT *>(out), args);
+                       static_cast<VT *>(out), args, plan->active(), plan->dim());
     return hipGetLastError();
 }
 
