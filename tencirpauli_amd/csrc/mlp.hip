@@ -378,7 +378,7 @@ struct tcp_mlp {
     tcp::DeviceBuffer params, grad, adam_m, adam_v, w1f, w2f, w2b;
     uint32_t step = 0;
     // training scratch (grown on demand)
-    tcp::DeviceBuffer h1, h2, y, da2, da1, partial;
+    tcp::DeviceBuffer h1, h2, y, da2, da1, partial, small;
     MlpDev dev() const {
         return MlpDev{static_cast<const s4 *>(w1f.ptr), static_cast<const s4 *>(w2f.ptr),
                       static_cast<const uint16_t *>(w2b.ptr), static_cast<const float *>(params.ptr)};
@@ -433,7 +433,7 @@ int backward(tcp_mlp *mlp, const uint16_t *X, uint64_t B, const float *dy, hipSt
     const size_t partial_floats = std::max<size_t>({static_cast<size_t>(nb_act) * (2 * kH + 1),
                                                     static_cast<size_t>(nb_h1) * kH,
                                                     static_cast<size_t>(nb_w) * kH * kH});
-    if ((st = ensure(mlp->partial, 4 * partial_floats))) return st;
+    if ((st = ensure(mlp->partial, 4 * partial_floats)) || (st = ensure(mlp->small, 4 * (2 * kH + 1)))) return st;
     auto *partial = static_cast<float *>(mlp->partial.ptr);
     auto *grad = static_cast<float *>(mlp->grad.ptr);
     auto *da2 = static_cast<uint16_t *>(mlp->da2.ptr);
@@ -443,12 +443,11 @@ int backward(tcp_mlp *mlp, const uint16_t *X, uint64_t B, const float *dy, hipSt
     // gw3 -> [kOffW3, +128), gb2 -> [kOffB2, +128), gb3 -> kOffB3: columns of width 2*128+1
     hipLaunchKernelGGL(reduce_columns_kernel, dim3((2 * kH + 1 + 127) / 128), dim3(128), 0, s, partial, nb_act,
                        2 * kH + 1,
-                       static_cast<float *>(mlp->h2.ptr) /* scratch */, 0u);
+                       static_cast<float *>(mlp->small.ptr), 0u);
     // move the three segments into place (small device copies)
-    TCP_HIP_CHECK(hipMemcpyAsync(grad + kOffW3, static_cast<float *>(mlp->h2.ptr), 4 * kH, hipMemcpyDeviceToDevice, s));
-    TCP_HIP_CHECK(hipMemcpyAsync(grad + kOffB2, static_cast<float *>(mlp->h2.ptr) + kH, 4 * kH, hipMemcpyDeviceToDevice, s));
-    TCP_HIP_CHECK(hipMemcpyAsync(grad + kOffB3, static_cast<float *>(mlp->h2.ptr) + 2 * kH, 4, hipMemcpyDeviceToDevice, s));
-    // h2 is no longer needed past this point; recompute nothing.
+    TCP_HIP_CHECK(hipMemcpyAsync(grad + kOffW3, static_cast<float *>(mlp->small.ptr), 4 * kH, hipMemcpyDeviceToDevice, s));
+    TCP_HIP_CHECK(hipMemcpyAsync(grad + kOffB2, static_cast<float *>(mlp->small.ptr) + kH, 4 * kH, hipMemcpyDeviceToDevice, s));
+    TCP_HIP_CHECK(hipMemcpyAsync(grad + kOffB3, static_cast<float *>(mlp->small.ptr) + 2 * kH, 4, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(h1_backward_kernel, dim3(nb_h1), dim3(kThreads), 0, s, mlp->dev(), da2, h1, B, da1, partial);
     hipLaunchKernelGGL(reduce_columns_kernel, dim3(1), dim3(128), 0, s, partial, nb_h1, kH, grad, kOffB1);
     hipLaunchKernelGGL((wgrad_kernel<kH>), dim3(nb_w), dim3(kThreads), 0, s, nullptr, h1, da2, B, rows_per_block,
