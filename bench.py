@@ -43,6 +43,8 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "mvp_tile_28q_traffic.json"))
+    p.add_argument("--extras", default="fp32,sector,davidson",
+                   help="comma list of extra sections: fp32, sector, davidson, batched ('' for none)")
     return p.parse_args()
 
 
@@ -87,6 +89,147 @@ def cpu_baseline(op, v_host: np.ndarray, y_dev_host_rows, args, bytes_alg: int, 
                   f"{full_seconds:.1f} s)",
         "term_amp_per_s": round(op.term_count * nrows / elapsed, 1),
     }, {"rows": nrows, "rel_err": err}
+
+
+def time_applies(fn, steps: int, warmup: int = 1) -> float:
+    """Milliseconds per call of fn() (device work on torch's current stream)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+def extra_fp32(plan, v, dim) -> dict:
+    """The same MVP with complex64 storage, FP32 coefficients, FP64 accumulation."""
+    import torch
+
+    v32 = v.to(torch.complex64)
+    y32 = torch.empty_like(v32)
+    ms = time_applies(lambda: plan.apply_into(v32, y32), 5)
+    y64 = torch.empty_like(v)
+    plan.apply_into(v, y64)
+    err = (torch.linalg.vector_norm(y32.to(torch.complex128) - y64) / torch.linalg.vector_norm(y64)).item()
+    alg = plan.bytes_alg("complex64")
+    return {"ms_per_mvp": round(ms, 4), "bytes_alg_GBps": round(alg / (ms * 1e-3) / 1e9, 2),
+            "term_amp_per_s": round(plan.term_count * dim / (ms * 1e-3), 1), "rel_err_vs_fp64": err}
+
+
+def extra_sector() -> dict:
+    """Restricted MVPs: 4x4 Hubbard (32 qubits, N_up = N_dn = 4) and the 5x5 XXZ S_z = 1/2 sector."""
+    import torch
+
+    import tencirpauli_amd as tcp
+    from tencirpauli_amd import lattice
+
+    out = {}
+    codes, coef = lattice.fermi_hubbard_jw(4, 4, 1.0, 4.0, seed=SEED + 3)
+    op = tcp.PauliOperator.from_code_arrays(codes, coef)
+    up = tcp.AdditiveCharge.number(32, range(16))
+    dn = tcp.AdditiveCharge.number(32, range(16, 32))
+    cases = [("hubbard_4x4_jw32_nup4_ndn4", op.restrict_charge(tcp.ChargeSector(((up, 4), (dn, 4))))),
+             ("xxz_5x5_periodic_sz_half", None)]
+    codes, coef = lattice.heisenberg_periodic(5, 5, seed=SEED)
+    op5 = tcp.PauliOperator.from_code_arrays(codes, coef)
+    cases[1] = (cases[1][0], op5.restrict_charge(tcp.U1Sector(25, 12)))
+    for name, restricted in cases:
+        gen = torch.Generator(device="cuda").manual_seed(SEED + 7)
+        v = torch.randn(restricted.dimension, dtype=torch.complex128, device="cuda", generator=gen)
+        rec = {"dimension": restricted.dimension, "terms": restricted.plan.term_count,
+               "x_groups": restricted.plan.group_count}
+        for tag, dt in (("fp64", torch.complex128), ("fp32", torch.complex64)):
+            vv = v.to(dt)
+            yy = torch.empty_like(vv)
+            ms = time_applies(lambda: restricted.apply_into(vv, yy), 10)
+            rec[f"ms_per_mvp_{tag}"] = round(ms, 4)
+            rec[f"amp_per_s_{tag}"] = round(restricted.dimension / (ms * 1e-3), 1)
+        out[name] = rec
+    return out
+
+
+def extra_davidson() -> dict:
+    """5x5 periodic XXZ (25 qubits, S_z = 1/2, dim 5,200,300): seconds until the Ritz value stays
+    within 1e-8 of the converged FP64 energy, plain FP64 Jacobi vs learned preconditioner + FP32 MVPs."""
+    import tencirpauli_amd as tcp
+    from tencirpauli_amd import davidson, lattice, learned, train_preconditioner
+
+    codes, coef = lattice.heisenberg_periodic(5, 5, seed=SEED)
+    op = tcp.PauliOperator.from_code_arrays(codes, coef)
+    restricted = op.restrict_charge(tcp.U1Sector(25, 12))
+    trained_here = not os.path.exists(train_preconditioner.DEFAULT_WEIGHTS)
+    t0 = time.perf_counter()
+    if trained_here:
+        mlp, _, _ = train_preconditioner.train_default(epochs=20)
+    else:
+        mlp = train_preconditioner.load()
+    train_s = time.perf_counter() - t0
+    hook = learned.LearnedPreconditioner(mlp, restricted)
+    davidson.lowest_eigenpair(restricted, tol=1e-3, max_iterations=3)  # warm-up (allocations, code load)
+    jac = davidson.lowest_eigenpair(restricted, tol=1e-9)
+    e_ref = jac.eigenvalue
+    lrn = davidson.lowest_eigenpair(restricted, tol=1e-7, mvp_dtype="complex64", switch_tol=3e-4,
+                                    preconditioner=hook)
+    t_jac = jac.seconds_to(e_ref, 1e-8)
+    t_lrn = lrn.seconds_to(e_ref, 1e-8)
+    return {
+        "energy_fp64": e_ref,
+        "jacobi_fp64": {"seconds_to_1e-8": t_jac, "iterations": jac.iterations, "mvps": jac.mvps_c128,
+                        "seconds_total": jac.seconds},
+        "learned_fp32mvp": {"seconds_to_1e-8": t_lrn, "iterations": lrn.iterations, "mvps_c64": lrn.mvps_c64,
+                            "mvps_c128": lrn.mvps_c128, "abs_dE": abs(lrn.eigenvalue - e_ref),
+                            "seconds_total": lrn.seconds},
+        "speedup": (t_jac / t_lrn) if (t_jac and t_lrn) else None,
+        "preconditioner": "trained in this run" if trained_here else "loaded " +
+                          os.path.relpath(train_preconditioner.DEFAULT_WEIGHTS, ROOT),
+        "training_or_load_s": round(train_s, 2),
+    }
+
+
+def extra_batched() -> dict:
+    """512 random Pauli sums (n ~ U{10..22}, T ~ U{64..4096}), 8 right-hand sides each,
+    complex64 storage with FP64 accumulation, one persistent launch replayed as a hipGraph."""
+    import torch
+
+    import tencirpauli_amd as tcp
+    from tencirpauli_amd import lattice
+    from tencirpauli_amd.batched import BatchedMVP
+
+    rng = np.random.default_rng(SEED + 2)
+    ops = []
+    for i in range(512):
+        n = int(rng.integers(10, 23))
+        t = int(rng.integers(64, 4097))
+        codes, coef = lattice.random_pauli_sum(n, t, seed=SEED + 10000 + i)
+        ops.append(tcp.PauliOperator.from_code_arrays(codes, coef))
+    t0 = time.perf_counter()
+    batch = BatchedMVP(ops, nrhs=8)
+    build_s = time.perf_counter() - t0
+    gen = torch.Generator(device="cuda").manual_seed(SEED + 3)
+    v = torch.randn(batch.total_amplitudes, dtype=torch.complex64, device="cuda", generator=gen)
+    y = torch.empty_like(v)
+    tune = batch.autotune(v, y, repeats=1)
+    stream = torch.cuda.Stream()
+    batch.capture(v, y, stream)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        batch.replay(stream)  # warm-up replay
+        start.record(stream)
+        for _ in range(3):
+            batch.replay(stream)
+        stop.record(stream)
+    stream.synchronize()
+    ms = start.elapsed_time(stop) / 3
+    return {"problems": 512, "rhs": 8, "amplitudes": batch.total_amplitudes, "term_amp_rhs": batch.work,
+            "ms_per_batch": round(ms, 3), "term_amp_rhs_per_s": round(batch.work / (ms * 1e-3), 1),
+            "plan_build_s": round(build_s, 2), "autotune": tune}
 
 
 def main() -> int:
@@ -177,6 +320,20 @@ def main() -> int:
             return y_full[:count].cpu().numpy()
 
         record["cpu_baseline"], record["parity"] = cpu_baseline(op, v_host, gpu_rows, args, bytes_alg, dim)
+        del v_host
+    extras = [e for e in args.extras.split(",") if e]
+    if "fp32" in extras and tag == "complex128":
+        record["fp32_storage"] = extra_fp32(plan, v, dim)
+    del y
+    if "sector" in extras or "davidson" in extras or "batched" in extras:
+        del v, plan
+        torch.cuda.empty_cache()
+    if "sector" in extras:
+        record["sector_mvp"] = extra_sector()
+    if "davidson" in extras:
+        record["davidson_5x5_xxz"] = extra_davidson()
+    if "batched" in extras:
+        record["batched_512x8"] = extra_batched()
     print(json.dumps(record), flush=True)
     return 0
 

@@ -157,6 +157,13 @@ def lib() -> ctypes.CDLL:
         return _lib
     with _lock:
         if _lib is None:
+            # torch ships its own libamdhip64.so.7 (same soname as the system
+            # ROCm one).  Only one runtime can be resident in a process: load
+            # torch's first so device tensors and this library share it.
+            try:
+                import torch  # noqa: F401
+            except ImportError:  # pragma: no cover - torch-less C-ABI users
+                pass
             if not os.path.exists(LIB_PATH):
                 raise NativeLibraryError(
                     f"{LIB_PATH} is not built; run tencirpauli_amd.build.build() "

@@ -376,7 +376,8 @@ using tcp::fail;
 struct tcp_mlp {
     int device = 0;
     tcp::DeviceBuffer params, grad, adam_m, adam_v, w1f, w2f, w2b;
-    uint32_t step = 0;
+    uint32_t step = 0;     // Adam steps taken
+    uint32_t version = 0;  // bumped whenever the weights change
     // training scratch (grown on demand)
     tcp::DeviceBuffer h1, h2, y, da2, da1, partial, small;
     MlpDev dev() const {
@@ -399,6 +400,12 @@ struct tcp_learned_precond {
     double scale = 1.0;
     hipStream_t stream = nullptr;
     tcp::DeviceBuffer features, diag, logm;
+    // the callback's last evaluation: reused while theta moves less than
+    // 1e-3 * scale (below the bf16 resolution of feature 0) and the weights
+    // are unchanged
+    bool cached = false;
+    double cached_theta = 0.0;
+    uint32_t cached_step = 0;
 };
 
 namespace {
@@ -512,6 +519,7 @@ int tcp_mlp_set_parameters(tcp_mlp *mlp, const float *params) {
     int st = dg.enter(mlp->device);
     if (st) return st;
     TCP_HIP_CHECK(hipMemcpy(mlp->params.ptr, params, 4 * kP, hipMemcpyHostToDevice));
+    ++mlp->version;
     if ((st = mlp->refresh(nullptr))) return st;
     TCP_HIP_CHECK(hipDeviceSynchronize());
     return tcp::ok();
@@ -549,6 +557,7 @@ int tcp_mlp_adam_step(tcp_mlp *mlp, float lr, float weight_decay, void *stream) 
     if (st) return st;
     hipStream_t s = static_cast<hipStream_t>(stream);
     ++mlp->step;
+    ++mlp->version;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.f - std::pow(b1, static_cast<float>(mlp->step));
     const float c2 = 1.f - std::pow(b2, static_cast<float>(mlp->step));
@@ -631,10 +640,17 @@ int tcp_learned_precond_prepare(tcp_learned_precond *p, double theta, void *stre
 // (H_ii - theta) / scale, then MLP inference over every state).
 const void *tcp_learned_precond_log_multiplier(void *ctx, double theta) {
     auto *p = static_cast<tcp_learned_precond *>(ctx);
-    if (!p || tcp_learned_precond_prepare(p, theta, p->stream) != TCP_OK) return nullptr;
+    if (!p) return nullptr;
+    if (p->cached && p->cached_step == p->mlp->version && std::fabs(theta - p->cached_theta) < 1e-3 * p->scale)
+        return p->logm.ptr;
+    if (tcp_learned_precond_prepare(p, theta, p->stream) != TCP_OK) return nullptr;
     int st = forward(p->mlp, static_cast<const uint16_t *>(p->features.ptr), p->dim,
                      static_cast<float *>(p->logm.ptr), nullptr, nullptr, p->stream);
-    return st ? nullptr : p->logm.ptr;
+    if (st) return nullptr;
+    p->cached = true;
+    p->cached_theta = theta;
+    p->cached_step = p->mlp->version;
+    return p->logm.ptr;
 }
 
 int tcp_learned_precond_buffers(const tcp_learned_precond *p, const void **features, const double **diag,
