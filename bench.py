@@ -43,6 +43,9 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "mvp_tile_28q_traffic.json"))
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="run one elementwise copy (y = -v, 2^n amplitudes each way) before the MVPs, "
+                        "the known-size reference for HBM counter scaling")
     p.add_argument("--extras", default="fp32,sector,davidson",
                    help="comma list of extra sections: fp32, sector, davidson, batched ('' for none)")
     return p.parse_args()
@@ -256,6 +259,9 @@ def main() -> int:
     v = v.to(cdtype)
     y = torch.empty_like(v)
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        torch.neg(v, out=y)
+        torch.cuda.synchronize()
 
     for _ in range(args.warmup):
         plan.apply_into(v, y)

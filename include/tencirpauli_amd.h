@@ -126,11 +126,31 @@ int tcp_mvp_plan_create_from_words(uint32_t nqubits, uint64_t nterms, const uint
                                    const uint64_t *z_words, const double *coef, int storage,
                                    uint64_t max_bytes, tcp_mvp_plan **out);
 
+/* Tile geometry of a full-space plan (no reference counterpart: the reference
+ * has one CPU loop).  Threads per workgroup x amplitudes per thread; OR-ing
+ * TCP_TILE_XCD_ORDER walks the tiles XCD by XCD.  Spaces smaller than the
+ * geometry's tile fall back to a smaller one.  tcp_mvp_plan_create uses
+ * TCP_TILE_AUTO (512x8 with the XCD order). */
+enum {
+    TCP_TILE_AUTO = 0,
+    TCP_TILE_512x8 = 1,  /* 2^12-amplitude tiles, two workgroups per CU */
+    TCP_TILE_1024x8 = 2, /* 2^13-amplitude tiles */
+    TCP_TILE_512x16 = 3  /* 2^13-amplitude tiles */
+};
+#define TCP_TILE_XCD_ORDER 0x100u
+int tcp_mvp_plan_create_tiled(const tcp_operator *op, int storage, uint64_t max_bytes, uint32_t tiling,
+                              tcp_mvp_plan **out);
+/* Geometry actually used (after the small-space fallback) and log2 of its tile. */
+uint32_t tcp_mvp_plan_tiling(const tcp_mvp_plan *plan);
+uint32_t tcp_mvp_plan_tile_bits(const tcp_mvp_plan *plan);
+
 uint32_t tcp_mvp_plan_nqubits(const tcp_mvp_plan *plan);
 uint64_t tcp_mvp_plan_term_count(const tcp_mvp_plan *plan);
 uint64_t tcp_mvp_plan_group_count(const tcp_mvp_plan *plan);
 /* number of distinct high-part classes of the x-masks above the tile bits */
 uint64_t tcp_mvp_plan_class_count(const tcp_mvp_plan *plan);
+/* Passes of the tile kernel: (x-mask, slot pattern) pairs (>= group_count). */
+uint64_t tcp_mvp_plan_pass_count(const tcp_mvp_plan *plan);
 uint64_t tcp_mvp_plan_dimension(const tcp_mvp_plan *plan);
 int tcp_mvp_plan_storage(const tcp_mvp_plan *plan);
 /* Nonzero transitions of the eager graph (MvpPlan::transition_count,

@@ -37,6 +37,11 @@ TCP_C128 = 0
 TCP_C64 = 1
 TCP_STORAGE_LAZY = 0
 TCP_STORAGE_EAGER = 1
+TCP_TILE_AUTO = 0
+TCP_TILE_512x8 = 1
+TCP_TILE_1024x8 = 2
+TCP_TILE_512x16 = 3
+TCP_TILE_XCD_ORDER = 0x100
 
 # Every exported entry point: (name, restype, argtypes).
 _c = ctypes
@@ -67,6 +72,10 @@ SIGNATURES = {
     "tcp_mvp_plan_term_count": (_c.c_uint64, [_vp]),
     "tcp_mvp_plan_group_count": (_c.c_uint64, [_vp]),
     "tcp_mvp_plan_class_count": (_c.c_uint64, [_vp]),
+    "tcp_mvp_plan_pass_count": (_c.c_uint64, [_vp]),
+    "tcp_mvp_plan_create_tiled": (_c.c_int, [_vp, _c.c_int, _c.c_uint64, _c.c_uint32, _c.POINTER(_vp)]),
+    "tcp_mvp_plan_tiling": (_c.c_uint32, [_vp]),
+    "tcp_mvp_plan_tile_bits": (_c.c_uint32, [_vp]),
     "tcp_mvp_plan_dimension": (_c.c_uint64, [_vp]),
     "tcp_mvp_plan_storage": (_c.c_int, [_vp]),
     "tcp_mvp_plan_transition_count": (_c.c_uint64, [_vp]),

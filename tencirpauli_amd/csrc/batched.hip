@@ -33,8 +33,8 @@ struct BatchProblem {
     uint32_t tiles;       // output tiles per right-hand side
     uint32_t H;           // classes
     uint32_t L;           // tile bits
-    uint32_t cls_base;    // first entry in cls_hi
-    uint32_t goff_base;   // first entry in cls_goff
+    uint32_t cls_base;    // first entry in the class table
+    uint32_t active;      // amplitudes per tile (2^L, or 2^n when the space is smaller than a tile)
     uint32_t pad;
 };
 
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(TPB) void batch_kernel(const VT *__restrict__ v, VT
                                                     const BatchProblem *__restrict__ probs, uint32_t nprob,
                                                     TileArgs shared, uint64_t total_items,
                                                     unsigned int *__restrict__ counter) {
-    __shared__ VT tile[TPB * A];
+    __shared__ dvec2 lds[TPB * A];
     __shared__ unsigned int item_s;
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(counter, 1u);
@@ -70,18 +70,18 @@ __global__ __launch_bounds__(TPB) void batch_kernel(const VT *__restrict__ v, VT
         const uint64_t rhs = local / P.tiles;
         const uint64_t tile_id = local - rhs * P.tiles;
         TileArgs a = shared;
-        a.cls_hi += P.cls_base;
-        a.cls_goff += P.goff_base;
+        a.cls += P.cls_base;
+        a.zoff += static_cast<uint64_t>(P.cls_base) * tcp::zoff_stride(A);
         a.H = P.H;
         a.L = P.L;
         const uint64_t off = P.in_off + rhs * P.dim;
-        tcp::tile_apply<A, VT, false, TPB>(v + off, y + off, a, tile_id, TPB, tile);
+        tcp::tile_apply<A, VT, TPB, true>(v + off, y + off, a, tile_id, P.active, lds);
     }
 }
 
 struct Tables {
     bool built = false;
-    tcp::DeviceBuffer cls_hi, cls_goff, g_lo, g_off, t_zlo, t_zhi, t_w64, t_w32, probs;
+    tcp::DeviceBuffer cls, zoff, grp64, grp32, term64, term32, probs;
     uint64_t total_items = 0;
 };
 
@@ -93,7 +93,6 @@ struct tcp_batch_plan {
     uint32_t nrhs = 1;
     uint64_t total = 0;             // amplitudes of the whole batch buffer
     double work = 0;                // sum of terms x 2^n x nrhs
-    uint32_t min_n = 64;
     int device = 0;
     uint32_t config = 0;
     Tables tables[kConfigCount];
@@ -132,7 +131,6 @@ int build_tables(tcp_batch_plan *plan, uint32_t cfg) {
     if (tb.built) return TCP_OK;
     const uint32_t tbits = log2u(kConfigs[cfg].threads), abits = log2u(kConfigs[cfg].amps);
     const uint32_t L = tbits + abits;
-    if (L > plan->min_n) return fail(TCP_ERR_UNSUPPORTED, "tile larger than the smallest operator");
     tcp::TileTables tab;
     const size_t P = plan->ops.size();
     std::vector<BatchProblem> probs(P);
@@ -142,11 +140,11 @@ int build_tables(tcp_batch_plan *plan, uint32_t cfg) {
         BatchProblem &bp = probs[p];
         bp.in_off = plan->offsets[p];
         bp.dim = 1ull << plan->ops[p].nqubits;
-        bp.tiles = static_cast<uint32_t>(bp.dim >> L);
-        bp.H = static_cast<uint32_t>(tab.H);
+        bp.tiles = static_cast<uint32_t>(std::max<uint64_t>(1, bp.dim >> L));
+        bp.H = static_cast<uint32_t>(tab.Hk);
         bp.L = L;
         bp.cls_base = static_cast<uint32_t>(tab.cls_base);
-        bp.goff_base = static_cast<uint32_t>(tab.goff_base);
+        bp.active = static_cast<uint32_t>(std::min<uint64_t>(bp.dim, 1ull << L));
         cost[p] = static_cast<double>(plan->ops[p].term_count() + 1) * static_cast<double>(bp.dim);
     }
     // longest processing time first
@@ -162,23 +160,14 @@ int build_tables(tcp_batch_plan *plan, uint32_t cfg) {
     }
     if (item >= (1ull << 32)) return fail(TCP_ERR_UNSUPPORTED, "more than 2^32 work items");
     tb.total_items = item;
-    if (tab.cls_hi.empty()) tab.cls_hi.push_back(0);
-    if (tab.g_lo.empty()) tab.g_lo.push_back(0);
-    if (tab.t_zlo.empty()) {
-        tab.t_zlo.push_back(0);
-        tab.t_zhi.push_back(0);
-        tab.t_w64.assign(2, 0.0);
-        tab.t_w32.assign(2, 0.0);
-    }
+    tab.pad_empty();
     int st;
-    if ((st = tb.cls_hi.upload(tab.cls_hi.data(), 8 * tab.cls_hi.size())) ||
-        (st = tb.cls_goff.upload(tab.cls_goff.data(), 4 * tab.cls_goff.size())) ||
-        (st = tb.g_lo.upload(tab.g_lo.data(), 4 * tab.g_lo.size())) ||
-        (st = tb.g_off.upload(tab.g_off.data(), 4 * tab.g_off.size())) ||
-        (st = tb.t_zlo.upload(tab.t_zlo.data(), 4 * tab.t_zlo.size())) ||
-        (st = tb.t_zhi.upload(tab.t_zhi.data(), 8 * tab.t_zhi.size())) ||
-        (st = tb.t_w64.upload(tab.t_w64.data(), 8 * tab.t_w64.size())) ||
-        (st = tb.t_w32.upload(tab.t_w32.data(), 8 * tab.t_w32.size())) ||
+    if ((st = tb.cls.upload(tab.cls.data(), sizeof(tcp::TileClass) * tab.cls.size())) ||
+        (st = tb.zoff.upload(tab.zoff.data(), sizeof(uint32_t) * tab.zoff.size())) ||
+        (st = tb.grp64.upload(tab.grp64.data(), sizeof(tcp::TileGroup) * tab.grp64.size())) ||
+        (st = tb.grp32.upload(tab.grp32.data(), sizeof(tcp::TileGroup) * tab.grp32.size())) ||
+        (st = tb.term64.upload(tab.term64.data(), sizeof(tcp::TileTerm) * tab.term64.size())) ||
+        (st = tb.term32.upload(tab.term32.data(), sizeof(tcp::TileTerm) * tab.term32.size())) ||
         (st = tb.probs.upload(sorted.data(), sizeof(BatchProblem) * sorted.size())))
         return st;
     // persistent grid: every resident workgroup of the device
@@ -199,13 +188,11 @@ template <int A, typename VT, uint32_t TPB>
 void launch_cfg(const tcp_batch_plan *plan, uint32_t cfg, const void *in, void *out, int dtype, hipStream_t s) {
     const Tables &tb = plan->tables[cfg];
     TileArgs a;
-    a.cls_hi = static_cast<const uint64_t *>(tb.cls_hi.ptr);
-    a.cls_goff = static_cast<const uint32_t *>(tb.cls_goff.ptr);
-    a.g_lo = static_cast<const uint32_t *>(tb.g_lo.ptr);
-    a.g_off = static_cast<const uint32_t *>(tb.g_off.ptr);
-    a.t_zlo = static_cast<const uint32_t *>(tb.t_zlo.ptr);
-    a.t_zhi = static_cast<const uint64_t *>(tb.t_zhi.ptr);
-    a.t_w = dtype == TCP_C64 ? tb.t_w32.ptr : tb.t_w64.ptr;
+    a.cls = static_cast<const tcp::TileClass *>(tb.cls.ptr);
+    a.zoff = static_cast<const uint32_t *>(tb.zoff.ptr);
+    a.xorder = nullptr;
+    a.grp = static_cast<const tcp::TileGroup *>(dtype == TCP_C64 ? tb.grp32.ptr : tb.grp64.ptr);
+    a.term = static_cast<const tcp::TileTerm *>(dtype == TCP_C64 ? tb.term32.ptr : tb.term64.ptr);
     a.H = 0;
     a.L = 0;
     const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(plan->grid[cfg], tb.total_items));
@@ -263,19 +250,12 @@ int tcp_batch_plan_create(const tcp_operator *const *ops, uint32_t count, uint32
         plan->offsets.push_back(plan->total);
         plan->total += (1ull << op.nqubits) * nrhs;
         plan->work += static_cast<double>(op.term_count()) * static_cast<double>(1ull << op.nqubits) * nrhs;
-        plan->min_n = std::min(plan->min_n, op.nqubits);
     }
     TCP_HIP_CHECK(hipGetDevice(&plan->device));
     int st = plan->counter.allocate(sizeof(unsigned int));
     if (st) return st;
-    plan->config = kConfigCount;
-    for (uint32_t c = 0; c < kConfigCount; ++c)  // default: the first geometry that fits
-        if (log2u(kConfigs[c].threads) + log2u(kConfigs[c].amps) <= plan->min_n) {
-            plan->config = c;
-            break;
-        }
-    if (count && plan->config == kConfigCount)
-        return fail(TCP_ERR_UNSUPPORTED, "batched operators need at least 9 qubits");
+    // default: 512 x 4 tiles (operators smaller than a tile use a partial one)
+    plan->config = kConfigCount - 1;
     if (count && (st = build_tables(plan, plan->config))) return st;
     *out = guard.release();
     return tcp::ok();
@@ -334,7 +314,6 @@ int tcp_batch_plan_autotune(tcp_batch_plan *plan, const void *input, void *outpu
     uint32_t best_cfg = plan->config;
     for (uint32_t c = 0; c < kConfigCount; ++c) {
         if (times_ms) times_ms[c] = -1.0f;
-        if (log2u(kConfigs[c].threads) + log2u(kConfigs[c].amps) > plan->min_n) continue;
         if ((st = build_tables(plan, c))) break;
         if ((st = enqueue(plan, c, input, output, dtype, nullptr))) break;  // warm-up
         TCP_HIP_CHECK(hipEventRecord(e0, nullptr));

@@ -8,21 +8,19 @@
 // so w'_t = w_t (-1)^popc(z_t & x_t) and every term of one x-mask group
 // shares one source amplitude v[r ^ x].
 //
-// Layout.  The index space is cut into tiles of 2^L amplitudes (L = 12 for
-// n >= 12): a 512-thread workgroup owns one output tile, each thread holding
-// A = 2^(L-9) amplitudes l = t + a*512 as FP64 accumulators in registers.
-// Group g reads source tile (tile ^ (x_g >> L)) permuted by x_g & (2^L-1).
-// Groups are sorted by that high part ("class"): the source tile of a class
-// is streamed from HBM into LDS once and every group of the class gathers
-// from LDS, so HBM reads per apply are 2^n * (classes) amplitudes instead of
-// 2^n * (groups).  The next class's tile is prefetched into registers while
-// the current one is consumed.  Term masks and coefficients are wave-uniform
-// and are read through scalar loads.
-//
-// Per term only the thread-part parity popc(z & t) is per-lane; the tile-part
-// parity popc((z >> L) & tile) is scalar, and the slot part (bits 9..L-1,
-// amplitude a of the thread) is folded by sorting each group's terms by those
-// z bits and applying a length-A Walsh-Hadamard transform once per group.
+// Layout.  The index space is cut into tiles of 2^L amplitudes: a workgroup
+// of TPB threads owns one output tile, each thread holding A = 2^L / TPB
+// amplitudes l = t + a*TPB as FP64 accumulators in registers (geometries
+// 512x8 with L = 12, 1024x8 and 512x16 with L = 13; smaller spaces use 512
+// threads and 1-4 amplitudes).  Group g reads source tile (tile ^ (x_g >> L))
+// permuted by x_g & (2^L-1).  Groups are sorted by that high part ("class"):
+// the source tile of a class is streamed from HBM into LDS once (widened to
+// FP64) and every group of the class gathers from LDS, so HBM reads per apply
+// are 2^n * (classes) amplitudes instead of 2^n * (groups).  The next class's
+// tile is prefetched into registers while the current one is consumed.
+// Group descriptors and terms are wave-uniform and read through scalar loads;
+// tile_tables.hpp describes the two group kinds (uniform slot pattern, or a
+// Walsh-Hadamard fold over the slot bits).
 #include <algorithm>
 #include <memory>
 #include <mutex>
@@ -31,25 +29,50 @@
 
 namespace {
 
-constexpr uint32_t kThreads = 512;  // threads per workgroup (fixed stride of slots)
-constexpr uint32_t kThreadBits = 9;
-constexpr uint32_t kMaxSlotBits = 3;  // up to 8 amplitudes per thread
-
 using tcp::Amp;
-using tcp::cptr;
 using tcp::dvec2;
 using tcp::flip_sign;
 using tcp::fvec2;
 using MvpArgs = tcp::TileArgs;
 
-// One workgroup per output tile (blockIdx.x) and right-hand side (blockIdx.y).
-// PARTIAL: fewer than 512 amplitudes (n < 9); threads t >= 2^n idle.
-template <int A, typename VT, bool PARTIAL>
-__global__ __launch_bounds__(kThreads) void mvp_tile_kernel(const VT *__restrict__ v, VT *__restrict__ y,
-                                                            MvpArgs p, uint32_t active, uint64_t dim) {
-    __shared__ VT tile[kThreads * A];
+// Tile geometries: threads per workgroup x amplitudes per thread.
+struct Geometry {
+    uint32_t threads, amps;
+};
+constexpr Geometry kGeometries[] = {{0, 0}, {512, 8}, {1024, 8}, {512, 16}};
+constexpr uint32_t kGeometryCount = sizeof(kGeometries) / sizeof(kGeometries[0]);
+constexpr uint32_t kSmallThreads = 512;  // below 2^12 amplitudes: 512 threads, 1-4 amplitudes each
+constexpr uint32_t kSmallBits = 9;
+
+uint32_t log2u(uint32_t x) { return 31u - static_cast<uint32_t>(__builtin_clz(x)); }
+
+// One workgroup per output tile and right-hand side (blockIdx.y).  With
+// `xcd_order` the tile of workgroup b is (b mod 8) * (tiles / 8) + b / 8:
+// workgroups are dealt to the eight XCDs round-robin, so each XCD walks one
+// contiguous eighth of the tiles and its concurrent workgroups read source
+// tiles that lie close together (shared in that XCD's L2).
+// PARTIAL: the whole space is smaller than one tile; l >= dim idle.
+// Waves per SIMD the register budget is sized for: two 512-thread or one
+// 1024-thread workgroup per CU (4 waves, <= 128 VGPRs); 512 x 16 runs one
+// workgroup per CU (2 waves, <= 256 VGPRs).
+template <int A, uint32_t TPB>
+constexpr int waves_per_simd() {
+    return A * TPB <= 4096 ? (TPB >= 256 ? 4 : 8) : (TPB == 1024 ? 4 : 2);
+}
+
+template <int A, typename VT, uint32_t TPB, bool PARTIAL>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(waves_per_simd<A, TPB>())))
+void mvp_tile_kernel(const VT *__restrict__ v, VT *__restrict__ y, MvpArgs p,
+                                                       uint64_t dim, uint32_t xcd_order) {
+    __shared__ dvec2 lds[TPB * A];
     const uint64_t rhs = static_cast<uint64_t>(blockIdx.y) * dim;
-    tcp::tile_apply<A, VT, PARTIAL, kThreads>(v + rhs, y + rhs, p, blockIdx.x, active, tile);
+    uint64_t tile = blockIdx.x;
+    if (xcd_order) {
+        const uint64_t per = gridDim.x >> 3;
+        tile = (tile & 7u) * per + (tile >> 3);
+    }
+    const uint32_t active = PARTIAL ? static_cast<uint32_t>(dim) : TPB * A;
+    tcp::tile_apply<A, VT, TPB, PARTIAL>(v + rhs, y + rhs, p, tile, active, lds);
 }
 
 // Fill kernel used instead of hipMemsetAsync (safe under graph capture).
@@ -63,53 +86,40 @@ __global__ void zero_kernel(VT *__restrict__ y, uint64_t count) {
 // Eager transition count: number of (row, group) pairs whose diagonal sum is
 // nonzero (MvpPlan::transition_count, core/hamiltonian.rs:165-180; the
 // destination-form diagonal is a permutation of the source-form one).
-__global__ void mvp_count_kernel(MvpArgs p, uint32_t A, uint32_t G, uint64_t dim,
+__global__ void mvp_count_kernel(MvpArgs p, uint32_t tb, uint32_t passes, uint64_t dim,
                                  unsigned long long *count) {
     unsigned long long local = 0;
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
          i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
         const uint64_t tile_id = i >> p.L;
         const uint32_t l = static_cast<uint32_t>(i & ((1ull << p.L) - 1));
-        const uint32_t t = l & (kThreads - 1);
-        const uint32_t slot = l >> kThreadBits;
-        for (uint32_t g = 0; g < G; ++g) {
-            double dr = 0.0, di = 0.0;
-            for (uint32_t c = 0; c < A; ++c) {
-                const uint32_t cpar = __popc(c & slot);
-                for (uint32_t k = p.g_off[g * A + c]; k < p.g_off[g * A + c + 1]; ++k) {
-                    const uint32_t par =
-                        (__popc(p.t_zlo[k] & t) + __popcll(p.t_zhi[k] & tile_id) + cpar) & 1u;
-                    const double2 w = reinterpret_cast<const double2 *>(p.t_w)[k];
-                    dr += flip_sign(w.x, par);
-                    di += flip_sign(w.y, par);
-                }
-            }
-            local += (dr != 0.0 || di != 0.0);
+        const uint32_t t = l & ((1u << tb) - 1);
+        const uint32_t slot = l >> tb;
+        // in x-mask order the passes of one x-mask are consecutive (`same`
+        // marks the continuation)
+        dvec2 d = dvec2{0.0, 0.0};
+        for (uint32_t i = 0; i < passes; ++i) {
+            const uint32_t g = p.xorder[i];
+            if (i > 0 && !p.grp[g].same) local += (d.x != 0.0 || d.y != 0.0);
+            if (i == 0 || !p.grp[g].same) d = dvec2{0.0, 0.0};
+            d += tcp::group_sum(p, g, t, slot, tile_id);
         }
+        if (passes > 0) local += (d.x != 0.0 || d.y != 0.0);
     }
     atomicAdd(count, local);
 }
 
-// Real part of the diagonal H_ii: the x = 0 group (group 0 when present).
-__global__ void mvp_diagonal_kernel(MvpArgs p, uint32_t A, uint32_t has_diag, uint64_t dim,
+// Real part of the diagonal H_ii: the passes of the x = 0 group (the first
+// `diag_passes` passes in x-mask order).
+__global__ void mvp_diagonal_kernel(MvpArgs p, uint32_t tb, uint32_t diag_passes, uint64_t dim,
                                     double *__restrict__ diag) {
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
          i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
         const uint64_t tile_id = i >> p.L;
         const uint32_t l = static_cast<uint32_t>(i & ((1ull << p.L) - 1));
-        const uint32_t t = l & (kThreads - 1);
-        const uint32_t slot = l >> kThreadBits;
+        const uint32_t t = l & ((1u << tb) - 1);
         double d = 0.0;
-        if (has_diag) {
-            for (uint32_t c = 0; c < A; ++c) {
-                const uint32_t cpar = __popc(c & slot);
-                for (uint32_t k = p.g_off[c]; k < p.g_off[c + 1]; ++k) {
-                    const uint32_t par =
-                        (__popc(p.t_zlo[k] & t) + __popcll(p.t_zhi[k] & tile_id) + cpar) & 1u;
-                    d += flip_sign(reinterpret_cast<const double2 *>(p.t_w)[k].x, par);
-                }
-            }
-        }
+        for (uint32_t i = 0; i < diag_passes; ++i) d += tcp::group_sum(p, p.xorder[i], t, l >> tb, tile_id).x;
         diag[i] = d;
     }
 }
@@ -118,43 +128,55 @@ __global__ void mvp_diagonal_kernel(MvpArgs p, uint32_t A, uint32_t has_diag, ui
 
 struct tcp_mvp_plan {
     uint32_t n = 0;
-    uint64_t T = 0, G = 0, H = 0;
+    uint64_t T = 0, G = 0, H = 0, Hk = 0, passes = 0;
     int storage = TCP_STORAGE_LAZY;
     uint64_t transitions = UINT64_MAX;
+    uint32_t tiling = 0;  // geometry index | TCP_TILE_XCD_ORDER
     uint32_t tb = 0, ab = 0, L = 0;
     int device = 0;
-    tcp::DeviceBuffer cls_hi, cls_goff, g_lo, g_off, t_zlo, t_zhi, t_w64, t_w32;
+    tcp::DeviceBuffer cls, zoff, xorder, grp64, grp32, term64, term32;
     uint64_t retained_bytes = 0;
-    bool has_diagonal = false;  // group 0 has x = 0
+    uint32_t diag_passes = 0;  // leading passes with x = 0
     std::mutex staging_mutex;
     tcp::DeviceBuffer stage_in, stage_out;
 
     uint64_t dim() const { return 1ull << n; }
     MvpArgs args(int dtype) const {
         MvpArgs a;
-        a.cls_hi = static_cast<const uint64_t *>(cls_hi.ptr);
-        a.cls_goff = static_cast<const uint32_t *>(cls_goff.ptr);
-        a.g_lo = static_cast<const uint32_t *>(g_lo.ptr);
-        a.g_off = static_cast<const uint32_t *>(g_off.ptr);
-        a.t_zlo = static_cast<const uint32_t *>(t_zlo.ptr);
-        a.t_zhi = static_cast<const uint64_t *>(t_zhi.ptr);
-        a.t_w = dtype == TCP_C64 ? t_w32.ptr : t_w64.ptr;
-        a.H = static_cast<uint32_t>(H);
+        a.cls = static_cast<const tcp::TileClass *>(cls.ptr);
+        a.zoff = static_cast<const uint32_t *>(zoff.ptr);
+        a.xorder = static_cast<const uint32_t *>(xorder.ptr);
+        a.grp = static_cast<const tcp::TileGroup *>(dtype == TCP_C64 ? grp32.ptr : grp64.ptr);
+        a.term = static_cast<const tcp::TileTerm *>(dtype == TCP_C64 ? term32.ptr : term64.ptr);
+        a.H = static_cast<uint32_t>(Hk);
         a.L = L;
         return a;
     }
-    uint32_t active() const { return n >= kThreadBits ? kThreads : (1u << n); }
+    uint32_t threads() const { return 1u << tb; }
+    bool xcd_order() const { return (tiling & TCP_TILE_XCD_ORDER) && (dim() >> L) >= 8; }
 };
 
 namespace {
 
 using tcp::fail;
 
-int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, tcp_mvp_plan **out) {
+// Geometry index for `tiling` (0 = automatic) on an n-qubit space; 0 means
+// the small-space geometry (fewer than 2^12 amplitudes).
+uint32_t resolve_geometry(uint32_t n, uint32_t tiling) {
+    uint32_t g = tiling & 0xffu;
+    if (g == TCP_TILE_AUTO) g = TCP_TILE_512x8;  // measured fastest on the 28-qubit lattice workload
+    const uint32_t L = log2u(kGeometries[g].threads) + log2u(kGeometries[g].amps);
+    if (n < L) g = n >= 12 ? TCP_TILE_512x8 : 0;
+    return g;
+}
+
+int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, uint32_t tiling, tcp_mvp_plan **out) {
     if (!out) return fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
     *out = nullptr;
     if (storage != TCP_STORAGE_LAZY && storage != TCP_STORAGE_EAGER)
         return fail(TCP_ERR_INVALID_ARGUMENT, "storage must be either 'eager' or 'lazy'");
+    if ((tiling & 0xffu) >= kGeometryCount || (tiling & ~(0xffu | TCP_TILE_XCD_ORDER)))
+        return fail(TCP_ERR_INVALID_ARGUMENT, "unknown tile geometry");
     const uint32_t n = op.nqubits;
     if (n >= 64) return fail(TCP_ERR_OVERFLOW, "integer overflow while computing matrix dimension");
     const uint64_t T = op.term_count();
@@ -165,44 +187,40 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, tcp_mvp
     plan->n = n;
     plan->T = T;
     plan->storage = storage;
-    plan->tb = std::min(n, kThreadBits);
-    plan->ab = n > kThreadBits ? std::min(n - kThreadBits, kMaxSlotBits) : 0;
+    const uint32_t geo = resolve_geometry(n, tiling);
+    // automatic tiling walks the tiles XCD by XCD (measured faster)
+    const bool xcd = (tiling & TCP_TILE_XCD_ORDER) || (tiling & 0xffu) == TCP_TILE_AUTO;
+    plan->tiling = geo | (xcd ? TCP_TILE_XCD_ORDER : 0u);
+    if (geo == 0) {
+        plan->tb = std::min(n, kSmallBits);
+        plan->ab = n > kSmallBits ? n - kSmallBits : 0;
+    } else {
+        plan->tb = log2u(kGeometries[geo].threads);
+        plan->ab = log2u(kGeometries[geo].amps);
+    }
     plan->L = plan->tb + plan->ab;
-    const uint32_t A = 1u << plan->ab;
     tcp::TileTables tab;
     tab.append(op, plan->tb, plan->ab);
     plan->G = tab.G;
     plan->H = tab.H;
-    plan->has_diagonal = tab.has_diagonal;
-    std::vector<uint64_t> &cls_hi = tab.cls_hi;
-    std::vector<uint32_t> &cls_goff = tab.cls_goff, &g_lo = tab.g_lo, &g_off = tab.g_off, &t_zlo = tab.t_zlo;
-    std::vector<uint64_t> &t_zhi = tab.t_zhi;
-    std::vector<double> &t_w64 = tab.t_w64, &t_w32 = tab.t_w32;
-    if (cls_hi.empty()) cls_hi.push_back(0);
-    if (g_lo.empty()) g_lo.push_back(0);
-    if (t_zlo.empty()) {
-        t_zlo.push_back(0);
-        t_zhi.push_back(0);
-        t_w64.assign(2, 0.0);
-        t_w32.assign(2, 0.0);
-    }
+    plan->Hk = tab.Hk;
+    plan->passes = tab.passes;
+    plan->diag_passes = static_cast<uint32_t>(tab.diag_passes);
+    tab.pad_empty();
 
-    uint64_t device_bytes = cls_hi.size() * 8 + cls_goff.size() * 4 + g_lo.size() * 4 +
-                            g_off.size() * 4 + T * (4 + 8 + 16 + 16);
-    plan->retained_bytes = device_bytes + sizeof(tcp_mvp_plan);
+    plan->retained_bytes = tab.device_bytes() + sizeof(tcp_mvp_plan);
     if (tcp::exceeds_budget(plan->retained_bytes, max_bytes))
         return tcp::memory_limit(plan->retained_bytes, max_bytes);
 
     TCP_HIP_CHECK(hipGetDevice(&plan->device));
     int st;
-    if ((st = plan->cls_hi.upload(cls_hi.data(), cls_hi.size() * 8))) return st;
-    if ((st = plan->cls_goff.upload(cls_goff.data(), cls_goff.size() * 4))) return st;
-    if ((st = plan->g_lo.upload(g_lo.data(), g_lo.size() * 4))) return st;
-    if ((st = plan->g_off.upload(g_off.data(), g_off.size() * 4))) return st;
-    if ((st = plan->t_zlo.upload(t_zlo.data(), t_zlo.size() * 4))) return st;
-    if ((st = plan->t_zhi.upload(t_zhi.data(), t_zhi.size() * 8))) return st;
-    if ((st = plan->t_w64.upload(t_w64.data(), t_w64.size() * 8))) return st;
-    if ((st = plan->t_w32.upload(t_w32.data(), t_w32.size() * 8))) return st;
+    if ((st = plan->cls.upload(tab.cls.data(), tab.cls.size() * sizeof(tcp::TileClass)))) return st;
+    if ((st = plan->zoff.upload(tab.zoff.data(), tab.zoff.size() * sizeof(uint32_t)))) return st;
+    if ((st = plan->xorder.upload(tab.xorder.data(), tab.xorder.size() * sizeof(uint32_t)))) return st;
+    if ((st = plan->grp64.upload(tab.grp64.data(), tab.grp64.size() * sizeof(tcp::TileGroup)))) return st;
+    if ((st = plan->grp32.upload(tab.grp32.data(), tab.grp32.size() * sizeof(tcp::TileGroup)))) return st;
+    if ((st = plan->term64.upload(tab.term64.data(), tab.term64.size() * sizeof(tcp::TileTerm)))) return st;
+    if ((st = plan->term32.upload(tab.term32.data(), tab.term32.size() * sizeof(tcp::TileTerm)))) return st;
 
     if (storage == TCP_STORAGE_EAGER) {
         unsigned long long *dcount = nullptr;
@@ -211,7 +229,7 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, tcp_mvp
         const uint64_t dim = plan->dim();
         const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((dim + 255) / 256, 4096));
         hipLaunchKernelGGL(mvp_count_kernel, dim3(blocks), dim3(256), 0, nullptr, plan->args(TCP_C128),
-                           A, static_cast<uint32_t>(plan->G), dim, dcount);
+                           plan->tb, static_cast<uint32_t>(plan->passes), dim, dcount);
         hipError_t e = hipGetLastError();
         unsigned long long hcount = 0;
         if (e == hipSuccess) e = hipMemcpy(&hcount, dcount, sizeof hcount, hipMemcpyDeviceToHost);
@@ -223,38 +241,40 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, tcp_mvp
     return tcp::ok();
 }
 
-template <int A, typename VT, bool PARTIAL = false>
-hipError_t launch_tile(const tcp_mvp_plan *plan, const void *in, void *out, uint32_t nrhs,
-                       int dtype, hipStream_t stream) {
-    MvpArgs args = plan->args(dtype);
-    const uint64_t tiles = plan->dim() >> plan->L;
-    hipLaunchKernelGGL((mvp_tile_kernel<A, VT, PARTIAL>), dim3(static_cast<uint32_t>(tiles), nrhs),
-                       dim3(kThreads), 0, stream, static_cast<const VT *>(in),
-                       static_cast<VT *>(out), args, plan->active(), plan->dim());
+template <int A, typename VT, uint32_t TPB, bool PARTIAL = false>
+hipError_t launch_tile(const tcp_mvp_plan *plan, const void *in, void *out, uint32_t nrhs, int dtype,
+                       hipStream_t stream) {
+    const uint64_t tiles = PARTIAL ? 1 : plan->dim() >> plan->L;
+    hipLaunchKernelGGL((mvp_tile_kernel<A, VT, TPB, PARTIAL>), dim3(static_cast<uint32_t>(tiles), nrhs),
+                       dim3(TPB), 0, stream, static_cast<const VT *>(in), static_cast<VT *>(out),
+                       plan->args(dtype), plan->dim(), plan->xcd_order() ? 1u : 0u);
     return hipGetLastError();
 }
 
 template <typename VT>
-hipError_t launch_typed(const tcp_mvp_plan *plan, const void *in, void *out, uint32_t nrhs,
-                        int dtype, hipStream_t stream) {
+hipError_t launch_typed(const tcp_mvp_plan *plan, const void *in, void *out, uint32_t nrhs, int dtype,
+                        hipStream_t stream) {
     if (plan->H == 0) {  // empty operator: H v = 0
         const uint64_t count = plan->dim() * nrhs;
         const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((count + 255) / 256, 65536));
-        hipLaunchKernelGGL(zero_kernel<VT>, dim3(blocks), dim3(256), 0, stream, static_cast<VT *>(out),
-                           count);
+        hipLaunchKernelGGL(zero_kernel<VT>, dim3(blocks), dim3(256), 0, stream, static_cast<VT *>(out), count);
         return hipGetLastError();
     }
-    if (plan->n < kThreadBits) return launch_tile<1, VT, true>(plan, in, out, nrhs, dtype, stream);
+    switch (plan->tiling & 0xffu) {
+        case TCP_TILE_512x8: return launch_tile<8, VT, 512>(plan, in, out, nrhs, dtype, stream);
+        case TCP_TILE_1024x8: return launch_tile<8, VT, 1024>(plan, in, out, nrhs, dtype, stream);
+        case TCP_TILE_512x16: return launch_tile<16, VT, 512>(plan, in, out, nrhs, dtype, stream);
+        default: break;
+    }
+    if (plan->n < kSmallBits) return launch_tile<1, VT, kSmallThreads, true>(plan, in, out, nrhs, dtype, stream);
     switch (plan->ab) {
-        case 0: return launch_tile<1, VT>(plan, in, out, nrhs, dtype, stream);
-        case 1: return launch_tile<2, VT>(plan, in, out, nrhs, dtype, stream);
-        case 2: return launch_tile<4, VT>(plan, in, out, nrhs, dtype, stream);
-        default: return launch_tile<8, VT>(plan, in, out, nrhs, dtype, stream);
+        case 0: return launch_tile<1, VT, kSmallThreads>(plan, in, out, nrhs, dtype, stream);
+        case 1: return launch_tile<2, VT, kSmallThreads>(plan, in, out, nrhs, dtype, stream);
+        default: return launch_tile<4, VT, kSmallThreads>(plan, in, out, nrhs, dtype, stream);
     }
 }
 
-int apply_device(const tcp_mvp_plan *plan, const void *in, void *out, uint64_t nrhs, int dtype,
-                 void *stream) {
+int apply_device(const tcp_mvp_plan *plan, const void *in, void *out, uint64_t nrhs, int dtype, void *stream) {
     if (dtype != TCP_C128 && dtype != TCP_C64)
         return fail(TCP_ERR_INVALID_ARGUMENT, "dtype must be TCP_C128 or TCP_C64");
     if (!in || !out) return fail(TCP_ERR_INVALID_ARGUMENT, "null state buffer");
@@ -264,15 +284,13 @@ int apply_device(const tcp_mvp_plan *plan, const void *in, void *out, uint64_t n
     const size_t bytes = static_cast<size_t>(plan->dim()) * amp * nrhs;
     if (tcp::ranges_overlap(in, bytes, out, bytes))
         return fail(TCP_ERR_INVALID_ARGUMENT, "apply_into input and output must not overlap");
-    if ((plan->dim() >> plan->L) > 0x7fffffffull)
-        return fail(TCP_ERR_UNSUPPORTED, "too many tiles for one launch");
+    if ((plan->dim() >> plan->L) > 0x7fffffffull) return fail(TCP_ERR_UNSUPPORTED, "too many tiles for one launch");
     tcp::DeviceGuard dg;
     int st = dg.enter(plan->device);
     if (st) return st;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipError_t e = dtype == TCP_C128
-                       ? launch_typed<dvec2>(plan, in, out, static_cast<uint32_t>(nrhs), dtype, s)
-                       : launch_typed<fvec2>(plan, in, out, static_cast<uint32_t>(nrhs), dtype, s);
+    hipError_t e = dtype == TCP_C128 ? launch_typed<dvec2>(plan, in, out, static_cast<uint32_t>(nrhs), dtype, s)
+                                     : launch_typed<fvec2>(plan, in, out, static_cast<uint32_t>(nrhs), dtype, s);
     if (e != hipSuccess) return tcp::hip_fail(e, "mvp kernel launch");
     return tcp::ok();
 }
@@ -289,7 +307,7 @@ int plan_diagonal(const tcp_mvp_plan *plan, double *diag, hipStream_t stream) {
     const uint64_t dim = plan->dim();
     const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((dim + 255) / 256, 65536));
     hipLaunchKernelGGL(mvp_diagonal_kernel, dim3(blocks), dim3(256), 0, stream, plan->args(TCP_C128),
-                       1u << plan->ab, plan->has_diagonal ? 1u : 0u, dim, diag);
+                       plan->tb, plan->diag_passes, dim, diag);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TCP_OK : hip_fail(e, "diagonal kernel");
 }
@@ -312,8 +330,17 @@ extern "C" {
 int tcp_mvp_plan_create(const tcp_operator *op, int storage, uint64_t max_bytes,
                         tcp_mvp_plan **out) {
     if (!op) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
-    return build_plan(tcp::operator_of(op), storage, max_bytes, out);
+    return build_plan(tcp::operator_of(op), storage, max_bytes, TCP_TILE_AUTO, out);
 }
+
+int tcp_mvp_plan_create_tiled(const tcp_operator *op, int storage, uint64_t max_bytes, uint32_t tiling,
+                              tcp_mvp_plan **out) {
+    if (!op) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
+    return build_plan(tcp::operator_of(op), storage, max_bytes, tiling, out);
+}
+
+uint32_t tcp_mvp_plan_tiling(const tcp_mvp_plan *plan) { return plan ? plan->tiling : 0; }
+uint32_t tcp_mvp_plan_tile_bits(const tcp_mvp_plan *plan) { return plan ? plan->L : 0; }
 
 int tcp_mvp_plan_create_from_words(uint32_t nqubits, uint64_t nterms, const uint64_t *x_words,
                                    const uint64_t *z_words, const double *coef, int storage,
@@ -321,7 +348,7 @@ int tcp_mvp_plan_create_from_words(uint32_t nqubits, uint64_t nterms, const uint
     tcp_operator *op = nullptr;
     int st = tcp_operator_from_words(nqubits, nterms, x_words, z_words, coef, &op);
     if (st) return st;
-    st = build_plan(tcp::operator_of(op), storage, max_bytes, out);
+    st = build_plan(tcp::operator_of(op), storage, max_bytes, TCP_TILE_AUTO, out);
     tcp_operator_destroy(op);
     return st;
 }
@@ -330,6 +357,7 @@ uint32_t tcp_mvp_plan_nqubits(const tcp_mvp_plan *plan) { return plan ? plan->n 
 uint64_t tcp_mvp_plan_term_count(const tcp_mvp_plan *plan) { return plan ? plan->T : 0; }
 uint64_t tcp_mvp_plan_group_count(const tcp_mvp_plan *plan) { return plan ? plan->G : 0; }
 uint64_t tcp_mvp_plan_class_count(const tcp_mvp_plan *plan) { return plan ? plan->H : 0; }
+uint64_t tcp_mvp_plan_pass_count(const tcp_mvp_plan *plan) { return plan ? plan->passes : 0; }
 uint64_t tcp_mvp_plan_dimension(const tcp_mvp_plan *plan) { return plan ? plan->dim() : 0; }
 int tcp_mvp_plan_storage(const tcp_mvp_plan *plan) { return plan ? plan->storage : -1; }
 uint64_t tcp_mvp_plan_transition_count(const tcp_mvp_plan *plan) {

@@ -2,12 +2,21 @@
 // on the host and the device routine that applies them to one output tile.
 //
 // Tile geometry: a workgroup of TPB = 2^tb threads owns 2^L amplitudes,
-// L = tb + ab, thread t holding A = 2^ab amplitudes l = t + a * TPB.  Terms
-// are grouped by x-mask; groups are ordered by their high part x >> L (the
-// "class": one source tile per class) and, inside a group, terms by the slot
-// bits (z >> tb) & (A - 1) so one length-A Walsh-Hadamard transform per group
-// folds the slot-dependent signs.  Coefficients carry the destination-index
-// sign: w' = w (-1)^popc(z & x).
+// L = tb + ab, thread t holding A = 2^ab amplitudes l = t + a * TPB ("slot"
+// a).  Terms are grouped by x-mask; groups are ordered by their high part
+// x >> L (the "class": one source tile per class).  Coefficients carry the
+// destination-index sign, w' = w (-1)^popc(z & x), so a group contributes
+//     y[l] += D(l) v[l ^ lo],   D(l) = sum_k w'_k (-1)^popc(z_k & l)
+// to every amplitude of the tile.  The sign of a term splits into a tile part
+// (popc((z >> L) & tile), wave-uniform), a thread part (popc(z & (TPB-1) & t))
+// and a slot part (popc(((z >> tb) & (A-1)) & a)).
+//
+// Each x-group is split by slot pattern zs = (z >> tb) & (A-1) into
+// "passes" whose terms share zs (a 2D lattice with random terms has ~1.15
+// passes per x-group): a pass contributes D(l) = S(t) (-1)^popc(zs & a) with
+// one per-thread sum S.  Inside a class the passes are ordered by zs and the
+// kernel runs one loop per slot pattern, so the slot signs are compile-time
+// constants of straight-line code.
 #pragma once
 
 #include <algorithm>
@@ -27,14 +36,14 @@ template <typename VT>
 struct Amp;
 template <>
 struct Amp<dvec2> {
-    __device__ static double re(const dvec2 &v) { return v.x; }
-    __device__ static double im(const dvec2 &v) { return v.y; }
+    __device__ static dvec2 widen(const dvec2 &v) { return v; }
     __device__ static dvec2 make(double r, double i) { return dvec2{r, i}; }
 };
 template <>
 struct Amp<fvec2> {
-    __device__ static double re(const fvec2 &v) { return static_cast<double>(v.x); }
-    __device__ static double im(const fvec2 &v) { return static_cast<double>(v.y); }
+    __device__ static dvec2 widen(const fvec2 &v) {
+        return dvec2{static_cast<double>(v.x), static_cast<double>(v.y)};
+    }
     __device__ static fvec2 make(double r, double i) {
         return fvec2{static_cast<float>(r), static_cast<float>(i)};
     }
@@ -44,193 +53,362 @@ __device__ __forceinline__ double flip_sign(double x, uint32_t parity) {
     return __longlong_as_double(__double_as_longlong(x) ^ (static_cast<long long>(parity) << 63));
 }
 
-// Device view of one operator's tables.  Group and term indices stored in
-// cls_goff / g_off are absolute (several operators may share the arrays).
+// One source tile per class; its passes are [gbegin, gend), and those with
+// slot pattern z are [zoff[c*S + z], zoff[c*S + z + 1]) with the record
+// stride S = zoff_stride(A) (one 64- or 128-byte scalar load per class).
+struct alignas(16) TileClass {
+    uint64_t hi;  // x >> L
+    uint32_t gbegin, gend;
+};
+
+// 64-byte pass descriptor (one scalar load).  The first term is stored
+// inline, the others are [tbegin, tend) of the term array.
+struct alignas(64) TileGroup {
+    uint32_t lo;    // x & (2^L - 1)
+    uint32_t zs;    // the shared slot pattern
+    uint32_t same;  // 1 if the previous pass in x-mask order (xorder) has the same x-mask
+    uint32_t zlo;   // first term: z & (TPB - 1)
+    uint64_t zhi;   // first term: z >> L
+    double wr, wi;  // first term: w'
+    uint32_t tbegin, tend;
+    uint32_t pad[4];
+};
+
+struct alignas(32) TileTerm {
+    uint32_t zlo;  // z & (TPB - 1)
+    uint32_t zs;   // (z >> tb) & (A - 1)
+    uint64_t zhi;  // z >> L
+    double wr, wi;
+};
+
+// Copies of the descriptors out of the constant address space (scalar loads).
+__device__ __forceinline__ TileGroup load_group(cptr<TileGroup> p) {
+    TileGroup g;
+    g.lo = p->lo;
+    g.zs = p->zs;
+    g.zlo = p->zlo;
+    g.zhi = p->zhi;
+    g.wr = p->wr;
+    g.wi = p->wi;
+    g.tbegin = p->tbegin;
+    g.tend = p->tend;
+    return g;
+}
+__device__ __forceinline__ TileTerm load_term(cptr<TileTerm> p) {
+    TileTerm t;
+    t.zlo = p->zlo;
+    t.zs = p->zs;
+    t.zhi = p->zhi;
+    t.wr = p->wr;
+    t.wi = p->wi;
+    return t;
+}
+
+// Device view of one operator's tables.  Group and term indices are absolute
+// (several operators may share the arrays).
 struct TileArgs {
-    const uint64_t *cls_hi;    // [H]    x >> L of each class
-    const uint32_t *cls_goff;  // [H+1]  group range of each class
-    const uint32_t *g_lo;      // [G]    x & (2^L - 1)
-    const uint32_t *g_off;     // [G*A+1] term start per (group, slot class)
-    const uint32_t *t_zlo;     // [T]    z & (2^tb - 1)
-    const uint64_t *t_zhi;     // [T]    z >> L
-    const void *t_w;           // [T]    (re, im) of w * (-1)^popc(z & x), as doubles
+    const TileClass *cls;   // [H]; H even (an odd class list ends with an empty class)
+    const uint32_t *zoff;   // [H * zoff_stride(A)] pass ranges per slot pattern
+    const TileGroup *grp;   // passes
+    const TileTerm *term;   // extra terms
+    const uint32_t *xorder; // passes in x-mask order (plan builders only)
     uint32_t H;
     uint32_t L;
 };
 
-// y[tile] = sum over classes/groups of the gathered source tiles.  `tile` is
-// the workgroup's LDS buffer of TPB * A amplitudes; `active` threads own an
-// amplitude (fewer than TPB only when the whole space is smaller than TPB).
-// Every thread of the workgroup must call this (it synchronises).
-template <int A, typename VT, bool PARTIAL, uint32_t TPB>
+// Words per class in the pass-range table: A + 1 entries padded to 16 or 32.
+__host__ __device__ constexpr uint32_t zoff_stride(uint32_t A) { return A + 1 <= 16 ? 16 : 32; }
+
+template <int A>
+struct SlotSign {  // (-1)^popc(zs & a) as a compile-time bit
+    static constexpr bool neg(int zs, int a) { return __builtin_popcount(zs & a) & 1; }
+};
+
+// Byte offset in the LDS tile of source amplitude (t + a*TPB) ^ lo, given
+// tbase = (t ^ (lo & (TPB-1))) * 16 and the slot part ls = lo >> tb:
+// (a ^ ls) selects the slot, a wave-uniform offset.
+template <uint32_t TPB>
+__device__ __forceinline__ uint32_t src_offset(uint32_t tbase, uint32_t ls, int a) {
+    return tbase + ((static_cast<uint32_t>(a) ^ ls) * TPB * static_cast<uint32_t>(sizeof(dvec2)));
+}
+
+// acc[a] += (-1)^popc(ZS & a) * S * src[a] for every pass of slot pattern ZS
+// in [g0, g1) (straight-line code: the slot signs are compile-time).
+template <int ZS, int A, uint32_t TPB>
+__device__ __forceinline__ void passes_uniform(cptr<TileGroup> grp, cptr<TileTerm> term, uint32_t g0, uint32_t g1,
+                                               uint32_t gend, TileGroup &next, uint32_t t, uint32_t t16,
+                                               uint64_t tile_id, const char *tile, double *accr, double *acci) {
+    constexpr uint32_t TB = __builtin_ctz(TPB);
+    for (uint32_t g = g0; g < g1; ++g) {
+        const TileGroup G = next;  // the next descriptor loads while this pass computes
+        if (g + 1 < gend) next = load_group(grp + g + 1);
+        uint32_t par = __popc(G.zlo & t) + __popcll(G.zhi & tile_id);
+        double sr = flip_sign(G.wr, par & 1u), si = flip_sign(G.wi, par & 1u);
+        for (uint32_t k = G.tbegin; k < G.tend; ++k) {
+            const TileTerm T = load_term(term + k);
+            par = __popc(T.zlo & t) + __popcll(T.zhi & tile_id);
+            sr += flip_sign(T.wr, par & 1u);
+            si += flip_sign(T.wi, par & 1u);
+        }
+        const uint32_t tbase = t16 ^ ((G.lo & (TPB - 1)) * static_cast<uint32_t>(sizeof(dvec2)));
+        const uint32_t ls = G.lo >> TB;
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+            const dvec2 s = *reinterpret_cast<const dvec2 *>(tile + src_offset<TPB>(tbase, ls, a));
+            if (SlotSign<A>::neg(ZS, a)) {
+                accr[a] = fma(-sr, s.x, accr[a]);
+                accr[a] = fma(si, s.y, accr[a]);
+                acci[a] = fma(-sr, s.y, acci[a]);
+                acci[a] = fma(-si, s.x, acci[a]);
+            } else {
+                accr[a] = fma(sr, s.x, accr[a]);
+                accr[a] = fma(-si, s.y, accr[a]);
+                acci[a] = fma(sr, s.y, acci[a]);
+                acci[a] = fma(si, s.x, acci[a]);
+            }
+        }
+    }
+}
+
+// All passes of one class, slot pattern by slot pattern; `next` holds the
+// descriptor of pass zr[0] on entry.
+template <int A, uint32_t TPB, int ZS = 0>
+__device__ __forceinline__ void passes_all(cptr<TileGroup> grp, cptr<TileTerm> term, const uint32_t *zr,
+                                           TileGroup &next, uint32_t t, uint32_t t16, uint64_t tile_id,
+                                           const char *tile, double *accr, double *acci) {
+    passes_uniform<ZS, A, TPB>(grp, term, zr[ZS], zr[ZS + 1], zr[A], next, t, t16, tile_id, tile, accr, acci);
+    if constexpr (ZS + 1 < A)
+        passes_all<A, TPB, ZS + 1>(grp, term, zr, next, t, t16, tile_id, tile, accr, acci);
+}
+
+// y[tile] = sum over classes/groups of the gathered source tiles.  `lds` is
+// the workgroup's buffer of TPB * A FP64 amplitudes (FP32 storage is widened
+// once per class).  `active` amplitudes of the tile exist (2^L except when
+// the whole space is smaller than one tile; then only l < active are read
+// and written).  Every thread of the workgroup must call this (it
+// synchronises).
+template <int A, typename VT, uint32_t TPB, bool PARTIAL>
 __device__ __forceinline__ void tile_apply(const VT *__restrict__ vin, VT *__restrict__ yout, const TileArgs &p,
-                                           uint64_t tile_id, uint32_t active, VT *tile) {
-    cptr<uint64_t> cls_hi = (cptr<uint64_t>)p.cls_hi;
-    cptr<uint32_t> cls_goff = (cptr<uint32_t>)p.cls_goff;
-    cptr<uint32_t> g_lo = (cptr<uint32_t>)p.g_lo;
-    cptr<uint32_t> g_off = (cptr<uint32_t>)p.g_off;
-    cptr<uint32_t> t_zlo = (cptr<uint32_t>)p.t_zlo;
-    cptr<uint64_t> t_zhi = (cptr<uint64_t>)p.t_zhi;
-    cptr<dvec2> t_w = (cptr<dvec2>)p.t_w;
+                                           uint64_t tile_id, uint32_t active, dvec2 *lds) {
+    cptr<TileClass> cls = (cptr<TileClass>)p.cls;
+    cptr<TileGroup> grp = (cptr<TileGroup>)p.grp;
+    cptr<TileTerm> term = (cptr<TileTerm>)p.term;
+    cptr<uint32_t> zoff = (cptr<uint32_t>)p.zoff;
 
     const uint32_t t = threadIdx.x;
     const uint64_t base = tile_id << p.L;
     const VT *__restrict__ src_base = vin + t;
-    const bool on = !PARTIAL || t < active;
+    auto live = [&](int a) { return !PARTIAL || t + a * TPB < active; };
     if (p.H == 0) {  // empty operator: H v = 0
-        if (on) {
 #pragma unroll
-            for (int a = 0; a < A; ++a) yout[base + t + a * TPB] = Amp<VT>::make(0.0, 0.0);
-        }
+        for (int a = 0; a < A; ++a)
+            if (live(a)) yout[base + t + a * TPB] = Amp<VT>::make(0.0, 0.0);
         return;
     }
 
+    const uint32_t t16 = t * static_cast<uint32_t>(sizeof(dvec2));
+    const char *tile = reinterpret_cast<const char *>(lds);
+
     double accr[A], acci[A];
-    VT pre[A];
 #pragma unroll
     for (int a = 0; a < A; ++a) {
         accr[a] = 0.0;
         acci[a] = 0.0;
     }
-    {
-        const uint64_t src = base ^ (cls_hi[0] << p.L);
-        if (on) {
+    // Two classes of source tiles are in flight: buffers pa / pb alternate
+    // (the class loop is unrolled by two so neither is copied while its
+    // loads are outstanding).
+    VT pa[A], pb[A];
+    auto fetch = [&](uint32_t h, VT *buf) {
+        const uint32_t hc = h < p.H ? h : p.H - 1;  // past the end: re-read the last tile
+        const uint64_t src = base ^ (static_cast<uint64_t>(cls[hc].hi) << p.L);
 #pragma unroll
-            for (int a = 0; a < A; ++a) pre[a] = src_base[src + a * TPB];
-        }
-    }
-    for (uint32_t h = 0; h < p.H; ++h) {
+        for (int a = 0; a < A; ++a) buf[a] = live(a) ? src_base[src + a * TPB] : Amp<VT>::make(0.0, 0.0);
+    };
+    constexpr uint32_t ZSTRIDE = zoff_stride(A);
+    auto step = [&](uint32_t h, VT *buf) {
         __syncthreads();  // every wave is done reading the previous class's tile
-        if (on) {
 #pragma unroll
-            for (int a = 0; a < A; ++a) tile[t + a * TPB] = pre[a];
-        }
+        for (int a = 0; a < A; ++a) lds[t + a * TPB] = Amp<VT>::widen(buf[a]);
         __syncthreads();
-        const uint32_t hn = h + 1 < p.H ? h + 1 : h;
-        const uint64_t src = base ^ (cls_hi[hn] << p.L);
-        if (h + 1 < p.H && on) {  // prefetch the next class's source tile
+        // this class's pass ranges, its first pass and the tile two classes
+        // ahead are requested together after the barriers (a barrier waits
+        // for every outstanding scalar load)
+        uint32_t zr[A + 1];
 #pragma unroll
-            for (int a = 0; a < A; ++a) pre[a] = src_base[src + a * TPB];
-        }
-        const uint32_t g0 = cls_goff[h], g1 = cls_goff[h + 1];
-        for (uint32_t g = g0; g < g1; ++g) {
-            const uint32_t lo = g_lo[g];
-            double sr[A], si[A];
-#pragma unroll
-            for (int c = 0; c < A; ++c) {
-                sr[c] = 0.0;
-                si[c] = 0.0;
-                const uint32_t k0 = g_off[g * A + c], k1 = g_off[g * A + c + 1];
-                for (uint32_t k = k0; k < k1; ++k) {
-                    const uint32_t tile_par = __popcll(t_zhi[k] & tile_id);
-                    const uint32_t par = (__popc(t_zlo[k] & t) + tile_par) & 1u;
-                    const dvec2 w = t_w[k];
-                    sr[c] += flip_sign(w.x, par);
-                    si[c] += flip_sign(w.y, par);
-                }
-            }
-            // d[a] = sum_c (-1)^popc(c & a) S[c]
-#pragma unroll
-            for (int m = 1; m < A; m <<= 1) {
-#pragma unroll
-                for (int c = 0; c < A; ++c) {
-                    if (!(c & m)) {
-                        const double ur = sr[c], ui = si[c], vr = sr[c | m], vi = si[c | m];
-                        sr[c] = ur + vr;
-                        si[c] = ui + vi;
-                        sr[c | m] = ur - vr;
-                        si[c | m] = ui - vi;
-                    }
-                }
-            }
-#pragma unroll
-            for (int a = 0; a < A; ++a) {
-                const VT s = tile[(t + a * TPB) ^ lo];
-                const double s_r = Amp<VT>::re(s), s_i = Amp<VT>::im(s);
-                accr[a] = fma(sr[a], s_r, accr[a]);
-                accr[a] = fma(-si[a], s_i, accr[a]);
-                acci[a] = fma(sr[a], s_i, acci[a]);
-                acci[a] = fma(si[a], s_r, acci[a]);
-            }
-        }
+        for (int z = 0; z <= A; ++z) zr[z] = zoff[h * ZSTRIDE + z];
+        fetch(h + 2, buf);
+        // keep the prefetch here, ahead of this class's passes (the compiler
+        // would otherwise sink the loads toward their use)
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        TileGroup next = load_group(grp + zr[0]);  // every class has a pass
+        passes_all<A, TPB>(grp, term, zr, next, t, t16, tile_id, tile, accr, acci);
+    };
+    // the first two tiles are requested in buffer order (pa strictly before
+    // pb): the waits of the loop rely on pa's loads being the older ones
+    fetch(0, pa);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    fetch(1, pb);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    for (uint32_t h = 0; h < p.H; h += 2) {  // p.H is even (the tables pad it)
+        step(h, pa);
+        step(h + 1, pb);
     }
-    if (on) {
-        VT *__restrict__ out = yout + base + t;
+    VT *__restrict__ out = yout + base + t;
 #pragma unroll
-        for (int a = 0; a < A; ++a) out[a * TPB] = Amp<VT>::make(accr[a], acci[a]);
-    }
+    for (int a = 0; a < A; ++a)
+        if (live(a)) out[a * TPB] = Amp<VT>::make(accr[a], acci[a]);
 }
 
-// Host-side tables of one operator for tile geometry (tb, ab).
-struct TileTables {
-    std::vector<uint64_t> cls_hi;
-    std::vector<uint32_t> cls_goff, g_lo, g_off, t_zlo;
-    std::vector<uint64_t> t_zhi;
-    std::vector<double> t_w64, t_w32;  // interleaved (re, im); t_w32 rounded to FP32
-    uint64_t G = 0, H = 0;      // of the last appended operator
-    bool has_diagonal = false;  // its first group has x = 0
-    size_t cls_base = 0;        // its first entry in cls_hi
-    size_t goff_base = 0;       // its first entry in cls_goff
+// Diagonal sum of pass g at tile-local position (t, slot) of tile `tile_id`,
+// in FP64 (plan builders: eager transition counts, diagonals).
+__device__ inline dvec2 group_sum(const TileArgs &p, uint32_t g, uint32_t t, uint32_t slot, uint64_t tile_id) {
+    const TileGroup &G = p.grp[g];
+    const uint32_t spar = __popc(G.zs & slot);
+    uint32_t par = __popc(G.zlo & t) + __popcll(G.zhi & tile_id) + spar;
+    double dr = flip_sign(G.wr, par & 1u), di = flip_sign(G.wi, par & 1u);
+    for (uint32_t k = G.tbegin; k < G.tend; ++k) {
+        const TileTerm &T = p.term[k];
+        par = __popc(T.zlo & t) + __popcll(T.zhi & tile_id) + spar;
+        dr += flip_sign(T.wr, par & 1u);
+        di += flip_sign(T.wi, par & 1u);
+    }
+    return dvec2{dr, di};
+}
 
-    // Appends the tables of `op`; group and term indices are offset by what
-    // the arrays already hold.
+// Host-side tables of one or more operators for tile geometry (tb, ab).
+struct TileTables {
+    std::vector<TileClass> cls;
+    std::vector<uint32_t> zoff;            // (A+1) per class
+    std::vector<uint32_t> xorder;          // passes in x-mask order
+    std::vector<TileGroup> grp64, grp32;   // grp32: coefficients rounded to FP32
+    std::vector<TileTerm> term64, term32;
+    uint64_t G = 0, H = 0;      // of the last appended operator: distinct x-masks, classes
+    uint64_t Hk = 0;            // its classes as the kernel walks them (H rounded up to even)
+    uint64_t passes = 0;        // its passes (x-mask, slot pattern)
+    uint64_t diag_passes = 0;   // its leading passes with x = 0
+    size_t cls_base = 0;        // its first entry in cls
+    size_t grp_base = 0;        // its first pass
+
+    // Appends the tables of `op`.
     void append(const Operator &op, uint32_t tb, uint32_t ab) {
         const uint32_t L = tb + ab, A = 1u << ab;
-        const uint64_t lmask = (1ull << L) - 1;
+        const uint64_t lmask = (1ull << L) - 1, tmask = (1ull << tb) - 1;
         const std::vector<IndexTerm> terms = index_terms(op);
         const uint64_t T = terms.size();
         struct Key {
             uint64_t hi, lo;
-            uint32_t cls;
+            uint32_t zs;
             uint64_t index;
         };
         std::vector<Key> keys(T);
         for (uint64_t k = 0; k < T; ++k)
             keys[k] = Key{terms[k].x >> L, terms[k].x & lmask, static_cast<uint32_t>((terms[k].z >> tb) & (A - 1)), k};
+        // kernel order: class, slot pattern, x-mask, canonical term order
         std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
             if (a.hi != b.hi) return a.hi < b.hi;
+            if (a.zs != b.zs) return a.zs < b.zs;
             if (a.lo != b.lo) return a.lo < b.lo;
-            if (a.cls != b.cls) return a.cls < b.cls;
             return a.index < b.index;
         });
-        const size_t cls_start = cls_hi.size();
-        cls_base = cls_start;
-        goff_base = cls_goff.size();
-        const uint32_t term_base = static_cast<uint32_t>(t_zlo.size());
-        if (!g_off.empty()) g_off.pop_back();  // the previous operator's end marker
+        cls_base = cls.size();
+        grp_base = grp64.size();
+        auto make_term = [&](const IndexTerm &term, bool fp32) {
+            const bool neg = __builtin_popcountll(term.z & term.x) & 1;
+            double wr = neg ? -term.w.re : term.w.re, wi = neg ? -term.w.im : term.w.im;
+            if (fp32) {
+                wr = static_cast<double>(static_cast<float>(wr));
+                wi = static_cast<double>(static_cast<float>(wi));
+            }
+            return TileTerm{static_cast<uint32_t>(term.z & tmask), static_cast<uint32_t>((term.z >> tb) & (A - 1)),
+                            term.z >> L, wr, wi};
+        };
+        struct PassKey {
+            uint64_t hi, lo;
+            uint32_t zs, index;
+        };
+        std::vector<PassKey> pkeys;
         uint64_t k = 0;
         while (k < T) {
-            if (cls_hi.size() == cls_start || cls_hi.back() != keys[k].hi) {
-                cls_hi.push_back(keys[k].hi);
-                cls_goff.push_back(static_cast<uint32_t>(g_lo.size()));
+            if (cls.size() == cls_base || cls.back().hi != keys[k].hi) {
+                if (cls.size() > cls_base) cls.back().gend = static_cast<uint32_t>(grp64.size());
+                cls.push_back(TileClass{keys[k].hi, static_cast<uint32_t>(grp64.size()), 0});
             }
             uint64_t e = k;
-            while (e < T && keys[e].hi == keys[k].hi && keys[e].lo == keys[k].lo) ++e;
-            g_lo.push_back(static_cast<uint32_t>(keys[k].lo));
-            uint64_t j = k;
-            for (uint32_t c = 0; c < A; ++c) {
-                g_off.push_back(static_cast<uint32_t>(term_base + j));
-                while (j < e && keys[j].cls == c) ++j;
-            }
-            for (uint64_t q = k; q < e; ++q) {
-                const IndexTerm &term = terms[keys[q].index];
-                const bool neg = __builtin_popcountll(term.z & term.x) & 1;
-                const double wr = neg ? -term.w.re : term.w.re;
-                const double wi = neg ? -term.w.im : term.w.im;
-                t_zlo.push_back(static_cast<uint32_t>(term.z & ((1ull << tb) - 1)));
-                t_zhi.push_back(term.z >> L);
-                t_w64.push_back(wr);
-                t_w64.push_back(wi);
-                t_w32.push_back(static_cast<double>(static_cast<float>(wr)));
-                t_w32.push_back(static_cast<double>(static_cast<float>(wi)));
+            while (e < T && keys[e].hi == keys[k].hi && keys[e].zs == keys[k].zs && keys[e].lo == keys[k].lo) ++e;
+            pkeys.push_back(PassKey{keys[k].hi, keys[k].lo, keys[k].zs, static_cast<uint32_t>(grp64.size())});
+            for (int fp32 = 0; fp32 < 2; ++fp32) {
+                std::vector<TileGroup> &grp = fp32 ? grp32 : grp64;
+                std::vector<TileTerm> &tv = fp32 ? term32 : term64;
+                TileGroup P{};
+                const TileTerm head = make_term(terms[keys[k].index], fp32);
+                P.lo = static_cast<uint32_t>(keys[k].lo);
+                P.zs = keys[k].zs;
+                P.zlo = head.zlo;
+                P.zhi = head.zhi;
+                P.wr = head.wr;
+                P.wi = head.wi;
+                P.tbegin = static_cast<uint32_t>(tv.size());
+                for (uint64_t q = k + 1; q < e; ++q) tv.push_back(make_term(terms[keys[q].index], fp32));
+                P.tend = static_cast<uint32_t>(tv.size());
+                grp.push_back(P);
             }
             k = e;
         }
-        g_off.push_back(static_cast<uint32_t>(t_zlo.size()));
-        cls_goff.push_back(static_cast<uint32_t>(g_lo.size()));
-        H = cls_hi.size() - cls_start;
-        G = cls_goff.back() - cls_goff[goff_base];
-        has_diagonal = H > 0 && cls_hi[cls_start] == 0 && g_lo[cls_goff[goff_base]] == 0;
+        if (cls.size() > cls_base) cls.back().gend = static_cast<uint32_t>(grp64.size());
+        H = cls.size() - cls_base;
+        if (H & 1)  // the kernel walks classes in pairs: pad with an empty class reading the last tile
+            cls.push_back(TileClass{cls.back().hi, static_cast<uint32_t>(grp64.size()),
+                                    static_cast<uint32_t>(grp64.size())});
+        Hk = cls.size() - cls_base;
+        passes = grp64.size() - grp_base;
+        // pass ranges per slot pattern
+        const uint32_t stride = zoff_stride(A);
+        for (size_t c = cls_base; c < cls.size(); ++c) {
+            uint32_t g = cls[c].gbegin;
+            for (uint32_t z = 0; z < stride; ++z) {
+                while (z <= A && g < cls[c].gend && grp64[g].zs < z) ++g;
+                zoff.push_back(z <= A ? g : cls[c].gend);
+            }
+        }
+        // x-mask order for the plan builders
+        std::sort(pkeys.begin(), pkeys.end(), [](const PassKey &a, const PassKey &b) {
+            if (a.hi != b.hi) return a.hi < b.hi;
+            if (a.lo != b.lo) return a.lo < b.lo;
+            return a.zs < b.zs;
+        });
+        G = 0;
+        diag_passes = 0;
+        for (size_t i = 0; i < pkeys.size(); ++i) {
+            const bool same = i > 0 && pkeys[i].hi == pkeys[i - 1].hi && pkeys[i].lo == pkeys[i - 1].lo;
+            grp64[pkeys[i].index].same = grp32[pkeys[i].index].same = same ? 1u : 0u;
+            if (!same) ++G;
+            if (pkeys[i].hi == 0 && pkeys[i].lo == 0) ++diag_passes;
+            xorder.push_back(pkeys[i].index);
+        }
+    }
+
+    // Keeps every device array non-empty (a zero-size upload has no pointer).
+    void pad_empty() {
+        if (cls.empty()) cls.push_back(TileClass{0, 0, 0});
+        if (zoff.empty()) zoff.push_back(0);
+        if (xorder.empty()) xorder.push_back(0);
+        if (grp64.empty()) {
+            grp64.push_back(TileGroup{});
+            grp32.push_back(TileGroup{});
+        }
+        if (term64.empty()) {
+            term64.push_back(TileTerm{});
+            term32.push_back(TileTerm{});
+        }
+    }
+    uint64_t device_bytes() const {
+        return cls.size() * sizeof(TileClass) + (zoff.size() + xorder.size()) * sizeof(uint32_t) +
+               (grp64.size() + grp32.size()) * sizeof(TileGroup) +
+               (term64.size() + term32.size()) * sizeof(TileTerm);
     }
 };
 

@@ -25,6 +25,9 @@ from ._plan import (
     validate_max_bytes,
 )
 
+# tile geometries of the full-space kernel (threads x amplitudes per thread)
+TILINGS = {"512x8": _lib.TCP_TILE_512x8, "1024x8": _lib.TCP_TILE_1024x8, "512x16": _lib.TCP_TILE_512x16}
+
 _LETTERS = {"I": 0, "X": 1, "Y": 2, "Z": 3}
 PauliInput = Union[str, Sequence[int]]
 
@@ -230,16 +233,30 @@ class PauliOperator:
         return self.native_mvp_plan(max_bytes=max_bytes).apply(state, max_bytes=max_bytes)
 
     def native_mvp_plan(self, *, storage: str = "lazy",
-                        max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "NativeMVPPlan":
-        """Compile a reusable matrix-free plan on the current GPU."""
+                        max_bytes: Optional[int] = DEFAULT_MAX_BYTES,
+                        tiling: Optional[str] = None, xcd_order: Optional[bool] = None) -> "NativeMVPPlan":
+        """Compile a reusable matrix-free plan on the current GPU.
+
+        ``tiling`` picks the kernel's tile geometry ("512x8", "1024x8",
+        "512x16"; None = automatic: 512x8 with the XCD order) and
+        ``xcd_order`` walks the tiles XCD by XCD; neither changes the result
+        beyond floating-point summation order.
+        """
         validate_max_bytes(max_bytes)
         if storage not in ("lazy", "eager"):
             raise ValueError("storage must be either 'eager' or 'lazy'")
+        if tiling is not None and tiling not in TILINGS:
+            raise ValueError(f"tiling must be one of {sorted(TILINGS)} or None")
         _lib.require_device()
         out = ctypes.c_void_p()
         st = _lib.TCP_STORAGE_EAGER if storage == "eager" else _lib.TCP_STORAGE_LAZY
-        _lib.check(_lib.lib().tcp_mvp_plan_create(self._handle.ptr, st, effective_max_bytes(max_bytes),
-                                                   ctypes.byref(out)))
+        code = TILINGS.get(tiling, _lib.TCP_TILE_AUTO)
+        if xcd_order is False and code == _lib.TCP_TILE_AUTO:
+            code = _lib.TCP_TILE_512x8  # automatic geometry without the XCD order
+        if xcd_order:
+            code |= _lib.TCP_TILE_XCD_ORDER
+        _lib.check(_lib.lib().tcp_mvp_plan_create_tiled(self._handle.ptr, st, effective_max_bytes(max_bytes),
+                                                         code, ctypes.byref(out)))
         return NativeMVPPlan(out, self, storage)
 
     def to_scipy_linear_operator(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
@@ -285,6 +302,11 @@ class NativeMVPPlan(NativePlanFacade):
         self.term_count = int(L.tcp_mvp_plan_term_count(ptr))
         self.group_count = int(L.tcp_mvp_plan_group_count(ptr))
         self.class_count = int(L.tcp_mvp_plan_class_count(ptr))
+        self.pass_count = int(L.tcp_mvp_plan_pass_count(ptr))
+        self.tile_bits = int(L.tcp_mvp_plan_tile_bits(ptr))
+        code = int(L.tcp_mvp_plan_tiling(ptr))
+        self.tiling = {v: k for k, v in TILINGS.items()}.get(code & 0xFF, "small")
+        self.xcd_order = bool(code & _lib.TCP_TILE_XCD_ORDER)
         self.dimension = int(L.tcp_mvp_plan_dimension(ptr))
         self.storage = storage
         self.retained_format = "compact_graph" if storage == "eager" else "compiled_actions"

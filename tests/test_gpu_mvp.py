@@ -71,6 +71,43 @@ def test_random_operators_every_tile_shape(n):
         assert rel_err(out, full) <= FP64_TOL
 
 
+@pytest.mark.parametrize("xcd_order", [False, True])
+@pytest.mark.parametrize("tiling", [None, "512x8", "1024x8", "512x16"])
+def test_every_tiling_against_oracle(tiling, xcd_order):
+    import torch
+
+    codes, coef = lattice.random_pauli_sum(17, 500, seed=91, p_identity=0.55)
+    op = tcp.PauliOperator.from_code_arrays(codes, coef * (1 - 0.4j))
+    plan = op.native_mvp_plan(tiling=tiling, xcd_order=xcd_order)
+    assert plan.tiling == (tiling or "512x8") and plan.xcd_order == xcd_order
+    assert plan.tile_bits == (12 if plan.tiling == "512x8" else 13)
+    assert plan.pass_count >= plan.group_count
+    state = _random_state(17, 92)
+    expected = _oracle_apply(op, state)
+    assert rel_err(plan.apply(state), expected) <= FP64_TOL
+    y32 = plan.apply(torch.from_numpy(state).cuda().to(torch.complex64))
+    torch.cuda.synchronize()
+    assert rel_err(y32.cpu().numpy().astype(np.complex128), expected) <= FP32_TOL
+
+
+def test_default_tiling_is_xcd_ordered_512x8():
+    codes, coef = lattice.random_pauli_sum(14, 50, seed=95)
+    plan = tcp.PauliOperator.from_code_arrays(codes, coef).native_mvp_plan()
+    assert plan.tiling == "512x8" and plan.xcd_order and plan.tile_bits == 12
+
+
+def test_tiling_falls_back_on_small_spaces():
+    codes, coef = lattice.random_pauli_sum(12, 80, seed=93)
+    op = tcp.PauliOperator.from_code_arrays(codes, coef)
+    for tiling in ("1024x8", "512x16"):
+        plan = op.native_mvp_plan(tiling=tiling, xcd_order=True)
+        assert plan.tiling == "512x8" and plan.tile_bits == 12
+        state = _random_state(12, 94)
+        assert rel_err(plan.apply(state), _oracle_apply(op, state)) <= FP64_TOL
+    with pytest.raises(ValueError):
+        op.native_mvp_plan(tiling="64x64")
+
+
 def test_many_classes_and_large_groups():
     # full-width random strings: nearly every term is its own x-group and class
     codes, coef = lattice.random_pauli_sum(18, 700, seed=5, p_identity=0.3)
