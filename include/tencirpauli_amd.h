@@ -128,21 +128,26 @@ int tcp_mvp_plan_create_from_words(uint32_t nqubits, uint64_t nterms, const uint
 
 /* Tile geometry of a full-space plan (no reference counterpart: the reference
  * has one CPU loop).  Threads per workgroup x amplitudes per thread; OR-ing
- * TCP_TILE_XCD_ORDER walks the tiles XCD by XCD.  Spaces smaller than the
+ * TCP_TILE_XCD_ORDER walks the tiles XCD by XCD (the default of
+ * TCP_TILE_AUTO; TCP_TILE_LINEAR_ORDER turns it off).  Spaces smaller than the
  * geometry's tile fall back to a smaller one.  tcp_mvp_plan_create uses
- * TCP_TILE_AUTO (512x8 with the XCD order). */
+ * TCP_TILE_AUTO: 512x8 for complex128 and 512x4 for complex64 storage, both
+ * with the XCD order. */
 enum {
     TCP_TILE_AUTO = 0,
     TCP_TILE_512x8 = 1,  /* 2^12-amplitude tiles, two workgroups per CU */
     TCP_TILE_1024x8 = 2, /* 2^13-amplitude tiles */
-    TCP_TILE_512x16 = 3  /* 2^13-amplitude tiles */
+    TCP_TILE_512x16 = 3, /* 2^13-amplitude tiles */
+    TCP_TILE_512x4 = 4   /* 2^11-amplitude tiles, four workgroups per CU */
 };
 #define TCP_TILE_XCD_ORDER 0x100u
+#define TCP_TILE_LINEAR_ORDER 0x200u /* plain tile order, also for TCP_TILE_AUTO */
 int tcp_mvp_plan_create_tiled(const tcp_operator *op, int storage, uint64_t max_bytes, uint32_t tiling,
                               tcp_mvp_plan **out);
-/* Geometry actually used (after the small-space fallback) and log2 of its tile. */
-uint32_t tcp_mvp_plan_tiling(const tcp_mvp_plan *plan);
-uint32_t tcp_mvp_plan_tile_bits(const tcp_mvp_plan *plan);
+/* Geometry the given dtype's applies use (after the small-space fallback;
+ * 0 = small-space kernel) and log2 of its tile. */
+uint32_t tcp_mvp_plan_tiling(const tcp_mvp_plan *plan, int dtype);
+uint32_t tcp_mvp_plan_tile_bits(const tcp_mvp_plan *plan, int dtype);
 
 uint32_t tcp_mvp_plan_nqubits(const tcp_mvp_plan *plan);
 uint64_t tcp_mvp_plan_term_count(const tcp_mvp_plan *plan);

@@ -41,7 +41,9 @@ TCP_TILE_AUTO = 0
 TCP_TILE_512x8 = 1
 TCP_TILE_1024x8 = 2
 TCP_TILE_512x16 = 3
+TCP_TILE_512x4 = 4
 TCP_TILE_XCD_ORDER = 0x100
+TCP_TILE_LINEAR_ORDER = 0x200
 
 # Every exported entry point: (name, restype, argtypes).
 _c = ctypes
@@ -74,8 +76,8 @@ SIGNATURES = {
     "tcp_mvp_plan_class_count": (_c.c_uint64, [_vp]),
     "tcp_mvp_plan_pass_count": (_c.c_uint64, [_vp]),
     "tcp_mvp_plan_create_tiled": (_c.c_int, [_vp, _c.c_int, _c.c_uint64, _c.c_uint32, _c.POINTER(_vp)]),
-    "tcp_mvp_plan_tiling": (_c.c_uint32, [_vp]),
-    "tcp_mvp_plan_tile_bits": (_c.c_uint32, [_vp]),
+    "tcp_mvp_plan_tiling": (_c.c_uint32, [_vp, _c.c_int]),
+    "tcp_mvp_plan_tile_bits": (_c.c_uint32, [_vp, _c.c_int]),
     "tcp_mvp_plan_dimension": (_c.c_uint64, [_vp]),
     "tcp_mvp_plan_storage": (_c.c_int, [_vp]),
     "tcp_mvp_plan_transition_count": (_c.c_uint64, [_vp]),

@@ -141,7 +141,7 @@ int build_tables(tcp_batch_plan *plan, uint32_t cfg) {
         bp.in_off = plan->offsets[p];
         bp.dim = 1ull << plan->ops[p].nqubits;
         bp.tiles = static_cast<uint32_t>(std::max<uint64_t>(1, bp.dim >> L));
-        bp.H = static_cast<uint32_t>(tab.Hk);
+        bp.H = static_cast<uint32_t>(tab.H);
         bp.L = L;
         bp.cls_base = static_cast<uint32_t>(tab.cls_base);
         bp.active = static_cast<uint32_t>(std::min<uint64_t>(bp.dim, 1ull << L));

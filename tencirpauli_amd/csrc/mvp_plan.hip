@@ -39,7 +39,7 @@ using MvpArgs = tcp::TileArgs;
 struct Geometry {
     uint32_t threads, amps;
 };
-constexpr Geometry kGeometries[] = {{0, 0}, {512, 8}, {1024, 8}, {512, 16}};
+constexpr Geometry kGeometries[] = {{0, 0}, {512, 8}, {1024, 8}, {512, 16}, {512, 4}};
 constexpr uint32_t kGeometryCount = sizeof(kGeometries) / sizeof(kGeometries[0]);
 constexpr uint32_t kSmallThreads = 512;  // below 2^12 amplitudes: 512 threads, 1-4 amplitudes each
 constexpr uint32_t kSmallBits = 9;
@@ -52,12 +52,13 @@ uint32_t log2u(uint32_t x) { return 31u - static_cast<uint32_t>(__builtin_clz(x)
 // contiguous eighth of the tiles and its concurrent workgroups read source
 // tiles that lie close together (shared in that XCD's L2).
 // PARTIAL: the whole space is smaller than one tile; l >= dim idle.
-// Waves per SIMD the register budget is sized for: two 512-thread or one
-// 1024-thread workgroup per CU (4 waves, <= 128 VGPRs); 512 x 16 runs one
-// workgroup per CU (2 waves, <= 256 VGPRs).
+// Waves per SIMD the register budget is sized for: 2^11-amplitude and
+// smaller tiles run four 512-thread workgroups per CU (8 waves, <= 64 VGPRs);
+// 2^12-amplitude tiles (64 KiB of LDS) two per CU or one 1024-thread
+// workgroup (4 waves, <= 128 VGPRs); 512 x 16 one per CU (2 waves).
 template <int A, uint32_t TPB>
 constexpr int waves_per_simd() {
-    return A * TPB <= 4096 ? (TPB >= 256 ? 4 : 8) : (TPB == 1024 ? 4 : 2);
+    return A * TPB <= 2048 ? 8 : (A * TPB <= 4096 || TPB == 1024 ? 4 : 2);
 }
 
 template <int A, typename VT, uint32_t TPB, bool PARTIAL>
@@ -126,21 +127,15 @@ __global__ void mvp_diagonal_kernel(MvpArgs p, uint32_t tb, uint32_t diag_passes
 
 }  // namespace
 
-struct tcp_mvp_plan {
-    uint32_t n = 0;
-    uint64_t T = 0, G = 0, H = 0, Hk = 0, passes = 0;
-    int storage = TCP_STORAGE_LAZY;
-    uint64_t transitions = UINT64_MAX;
-    uint32_t tiling = 0;  // geometry index | TCP_TILE_XCD_ORDER
+// Device tables of one tile geometry.
+struct PlanTiles {
+    uint32_t tiling = 0;  // geometry index (0: small space) | TCP_TILE_XCD_ORDER
     uint32_t tb = 0, ab = 0, L = 0;
-    int device = 0;
+    uint64_t H = 0, passes = 0;
+    uint32_t diag_passes = 0;  // passes with x = 0
     tcp::DeviceBuffer cls, zoff, xorder, grp64, grp32, term64, term32;
-    uint64_t retained_bytes = 0;
-    uint32_t diag_passes = 0;  // leading passes with x = 0
-    std::mutex staging_mutex;
-    tcp::DeviceBuffer stage_in, stage_out;
+    uint64_t bytes = 0;
 
-    uint64_t dim() const { return 1ull << n; }
     MvpArgs args(int dtype) const {
         MvpArgs a;
         a.cls = static_cast<const tcp::TileClass *>(cls.ptr);
@@ -148,26 +143,78 @@ struct tcp_mvp_plan {
         a.xorder = static_cast<const uint32_t *>(xorder.ptr);
         a.grp = static_cast<const tcp::TileGroup *>(dtype == TCP_C64 ? grp32.ptr : grp64.ptr);
         a.term = static_cast<const tcp::TileTerm *>(dtype == TCP_C64 ? term32.ptr : term64.ptr);
-        a.H = static_cast<uint32_t>(Hk);
+        a.H = static_cast<uint32_t>(H);
         a.L = L;
         return a;
     }
-    uint32_t threads() const { return 1u << tb; }
-    bool xcd_order() const { return (tiling & TCP_TILE_XCD_ORDER) && (dim() >> L) >= 8; }
+};
+
+struct tcp_mvp_plan {
+    uint32_t n = 0;
+    uint64_t T = 0, G = 0;
+    int storage = TCP_STORAGE_LAZY;
+    uint64_t transitions = UINT64_MAX;
+    int device = 0;
+    // complex128 tables, and complex64 tables when that storage uses another geometry
+    PlanTiles t128, t64;
+    bool own64 = false;
+    uint64_t retained_bytes = 0;
+    std::mutex staging_mutex;
+    tcp::DeviceBuffer stage_in, stage_out;
+
+    uint64_t dim() const { return 1ull << n; }
+    const PlanTiles &tiles(int dtype) const { return dtype == TCP_C64 && own64 ? t64 : t128; }
+    MvpArgs args(int dtype) const { return tiles(dtype).args(dtype); }
+    bool xcd_order(const PlanTiles &pt) const { return (pt.tiling & TCP_TILE_XCD_ORDER) && (dim() >> pt.L) >= 8; }
 };
 
 namespace {
 
 using tcp::fail;
 
-// Geometry index for `tiling` (0 = automatic) on an n-qubit space; 0 means
-// the small-space geometry (fewer than 2^12 amplitudes).
-uint32_t resolve_geometry(uint32_t n, uint32_t tiling) {
+// Geometry index for `tiling` on an n-qubit space; 0 means the small-space
+// geometry (512 threads, 1-4 amplitudes each).  Automatic: 512 x 8 for
+// complex128 and 512 x 4 for complex64 storage (measured fastest on the
+// 28-qubit lattice workload: FP64 tiles are bounded by HBM traffic, which the
+// larger tile lowers; FP32 ones by LDS and issue latency, which the higher
+// occupancy of the smaller tile hides).
+uint32_t resolve_geometry(uint32_t n, uint32_t tiling, int dtype) {
     uint32_t g = tiling & 0xffu;
-    if (g == TCP_TILE_AUTO) g = TCP_TILE_512x8;  // measured fastest on the 28-qubit lattice workload
+    if (g == TCP_TILE_AUTO) g = dtype == TCP_C64 ? TCP_TILE_512x4 : TCP_TILE_512x8;
     const uint32_t L = log2u(kGeometries[g].threads) + log2u(kGeometries[g].amps);
-    if (n < L) g = n >= 12 ? TCP_TILE_512x8 : 0;
+    if (n < L) g = n >= 12 ? TCP_TILE_512x8 : (n >= 11 ? TCP_TILE_512x4 : 0);
     return g;
+}
+
+int build_tiles(const tcp::Operator &op, uint32_t geo, bool xcd, PlanTiles &pt, tcp::TileTables &tab) {
+    const uint32_t n = op.nqubits;
+    pt.tiling = geo | (xcd ? TCP_TILE_XCD_ORDER : 0u);
+    if (geo == 0) {
+        pt.tb = std::min(n, kSmallBits);
+        pt.ab = n > kSmallBits ? n - kSmallBits : 0;
+    } else {
+        pt.tb = log2u(kGeometries[geo].threads);
+        pt.ab = log2u(kGeometries[geo].amps);
+    }
+    pt.L = pt.tb + pt.ab;
+    tab.append(op, pt.tb, pt.ab);
+    pt.H = tab.H;
+    pt.passes = tab.passes;
+    pt.diag_passes = static_cast<uint32_t>(tab.diag_passes);
+    tab.pad_empty();
+    pt.bytes = tab.device_bytes();
+    return TCP_OK;
+}
+
+int upload_tiles(PlanTiles &pt, const tcp::TileTables &tab) {
+    int st;
+    if ((st = pt.cls.upload(tab.cls.data(), tab.cls.size() * sizeof(tcp::TileClass)))) return st;
+    if ((st = pt.zoff.upload(tab.zoff.data(), tab.zoff.size() * sizeof(uint32_t)))) return st;
+    if ((st = pt.xorder.upload(tab.xorder.data(), tab.xorder.size() * sizeof(uint32_t)))) return st;
+    if ((st = pt.grp64.upload(tab.grp64.data(), tab.grp64.size() * sizeof(tcp::TileGroup)))) return st;
+    if ((st = pt.grp32.upload(tab.grp32.data(), tab.grp32.size() * sizeof(tcp::TileGroup)))) return st;
+    if ((st = pt.term64.upload(tab.term64.data(), tab.term64.size() * sizeof(tcp::TileTerm)))) return st;
+    return pt.term32.upload(tab.term32.data(), tab.term32.size() * sizeof(tcp::TileTerm));
 }
 
 int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, uint32_t tiling, tcp_mvp_plan **out) {
@@ -175,7 +222,8 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, uint32_
     *out = nullptr;
     if (storage != TCP_STORAGE_LAZY && storage != TCP_STORAGE_EAGER)
         return fail(TCP_ERR_INVALID_ARGUMENT, "storage must be either 'eager' or 'lazy'");
-    if ((tiling & 0xffu) >= kGeometryCount || (tiling & ~(0xffu | TCP_TILE_XCD_ORDER)))
+    if ((tiling & 0xffu) >= kGeometryCount || (tiling & ~(0xffu | TCP_TILE_XCD_ORDER | TCP_TILE_LINEAR_ORDER)) ||
+        ((tiling & TCP_TILE_XCD_ORDER) && (tiling & TCP_TILE_LINEAR_ORDER)))
         return fail(TCP_ERR_INVALID_ARGUMENT, "unknown tile geometry");
     const uint32_t n = op.nqubits;
     if (n >= 64) return fail(TCP_ERR_OVERFLOW, "integer overflow while computing matrix dimension");
@@ -187,40 +235,24 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, uint32_
     plan->n = n;
     plan->T = T;
     plan->storage = storage;
-    const uint32_t geo = resolve_geometry(n, tiling);
-    // automatic tiling walks the tiles XCD by XCD (measured faster)
-    const bool xcd = (tiling & TCP_TILE_XCD_ORDER) || (tiling & 0xffu) == TCP_TILE_AUTO;
-    plan->tiling = geo | (xcd ? TCP_TILE_XCD_ORDER : 0u);
-    if (geo == 0) {
-        plan->tb = std::min(n, kSmallBits);
-        plan->ab = n > kSmallBits ? n - kSmallBits : 0;
-    } else {
-        plan->tb = log2u(kGeometries[geo].threads);
-        plan->ab = log2u(kGeometries[geo].amps);
-    }
-    plan->L = plan->tb + plan->ab;
-    tcp::TileTables tab;
-    tab.append(op, plan->tb, plan->ab);
-    plan->G = tab.G;
-    plan->H = tab.H;
-    plan->Hk = tab.Hk;
-    plan->passes = tab.passes;
-    plan->diag_passes = static_cast<uint32_t>(tab.diag_passes);
-    tab.pad_empty();
+    // automatic tiling walks the tiles XCD by XCD (measured faster) unless told otherwise
+    const bool xcd = (tiling & TCP_TILE_XCD_ORDER) ||
+                     ((tiling & 0xffu) == TCP_TILE_AUTO && !(tiling & TCP_TILE_LINEAR_ORDER));
+    const uint32_t g128 = resolve_geometry(n, tiling, TCP_C128), g64 = resolve_geometry(n, tiling, TCP_C64);
+    tcp::TileTables tab128, tab64;
+    build_tiles(op, g128, xcd, plan->t128, tab128);
+    plan->G = tab128.G;
+    plan->own64 = g64 != g128;
+    if (plan->own64) build_tiles(op, g64, xcd, plan->t64, tab64);
 
-    plan->retained_bytes = tab.device_bytes() + sizeof(tcp_mvp_plan);
+    plan->retained_bytes = plan->t128.bytes + (plan->own64 ? plan->t64.bytes : 0) + sizeof(tcp_mvp_plan);
     if (tcp::exceeds_budget(plan->retained_bytes, max_bytes))
         return tcp::memory_limit(plan->retained_bytes, max_bytes);
 
     TCP_HIP_CHECK(hipGetDevice(&plan->device));
     int st;
-    if ((st = plan->cls.upload(tab.cls.data(), tab.cls.size() * sizeof(tcp::TileClass)))) return st;
-    if ((st = plan->zoff.upload(tab.zoff.data(), tab.zoff.size() * sizeof(uint32_t)))) return st;
-    if ((st = plan->xorder.upload(tab.xorder.data(), tab.xorder.size() * sizeof(uint32_t)))) return st;
-    if ((st = plan->grp64.upload(tab.grp64.data(), tab.grp64.size() * sizeof(tcp::TileGroup)))) return st;
-    if ((st = plan->grp32.upload(tab.grp32.data(), tab.grp32.size() * sizeof(tcp::TileGroup)))) return st;
-    if ((st = plan->term64.upload(tab.term64.data(), tab.term64.size() * sizeof(tcp::TileTerm)))) return st;
-    if ((st = plan->term32.upload(tab.term32.data(), tab.term32.size() * sizeof(tcp::TileTerm)))) return st;
+    if ((st = upload_tiles(plan->t128, tab128))) return st;
+    if (plan->own64 && (st = upload_tiles(plan->t64, tab64))) return st;
 
     if (storage == TCP_STORAGE_EAGER) {
         unsigned long long *dcount = nullptr;
@@ -229,7 +261,7 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, uint32_
         const uint64_t dim = plan->dim();
         const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((dim + 255) / 256, 4096));
         hipLaunchKernelGGL(mvp_count_kernel, dim3(blocks), dim3(256), 0, nullptr, plan->args(TCP_C128),
-                           plan->tb, static_cast<uint32_t>(plan->passes), dim, dcount);
+                           plan->t128.tb, static_cast<uint32_t>(plan->t128.passes), dim, dcount);
         hipError_t e = hipGetLastError();
         unsigned long long hcount = 0;
         if (e == hipSuccess) e = hipMemcpy(&hcount, dcount, sizeof hcount, hipMemcpyDeviceToHost);
@@ -242,35 +274,37 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, uint32_
 }
 
 template <int A, typename VT, uint32_t TPB, bool PARTIAL = false>
-hipError_t launch_tile(const tcp_mvp_plan *plan, const void *in, void *out, uint32_t nrhs, int dtype,
-                       hipStream_t stream) {
-    const uint64_t tiles = PARTIAL ? 1 : plan->dim() >> plan->L;
+hipError_t launch_tile(const tcp_mvp_plan *plan, const PlanTiles &pt, const void *in, void *out, uint32_t nrhs,
+                       int dtype, hipStream_t stream) {
+    const uint64_t tiles = PARTIAL ? 1 : plan->dim() >> pt.L;
     hipLaunchKernelGGL((mvp_tile_kernel<A, VT, TPB, PARTIAL>), dim3(static_cast<uint32_t>(tiles), nrhs),
-                       dim3(TPB), 0, stream, static_cast<const VT *>(in), static_cast<VT *>(out),
-                       plan->args(dtype), plan->dim(), plan->xcd_order() ? 1u : 0u);
+                       dim3(TPB), 0, stream, static_cast<const VT *>(in), static_cast<VT *>(out), pt.args(dtype),
+                       plan->dim(), plan->xcd_order(pt) ? 1u : 0u);
     return hipGetLastError();
 }
 
 template <typename VT>
 hipError_t launch_typed(const tcp_mvp_plan *plan, const void *in, void *out, uint32_t nrhs, int dtype,
                         hipStream_t stream) {
-    if (plan->H == 0) {  // empty operator: H v = 0
+    const PlanTiles &pt = plan->tiles(dtype);
+    if (pt.H == 0) {  // empty operator: H v = 0
         const uint64_t count = plan->dim() * nrhs;
         const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((count + 255) / 256, 65536));
         hipLaunchKernelGGL(zero_kernel<VT>, dim3(blocks), dim3(256), 0, stream, static_cast<VT *>(out), count);
         return hipGetLastError();
     }
-    switch (plan->tiling & 0xffu) {
-        case TCP_TILE_512x8: return launch_tile<8, VT, 512>(plan, in, out, nrhs, dtype, stream);
-        case TCP_TILE_1024x8: return launch_tile<8, VT, 1024>(plan, in, out, nrhs, dtype, stream);
-        case TCP_TILE_512x16: return launch_tile<16, VT, 512>(plan, in, out, nrhs, dtype, stream);
+    switch (pt.tiling & 0xffu) {
+        case TCP_TILE_512x8: return launch_tile<8, VT, 512>(plan, pt, in, out, nrhs, dtype, stream);
+        case TCP_TILE_1024x8: return launch_tile<8, VT, 1024>(plan, pt, in, out, nrhs, dtype, stream);
+        case TCP_TILE_512x16: return launch_tile<16, VT, 512>(plan, pt, in, out, nrhs, dtype, stream);
+        case TCP_TILE_512x4: return launch_tile<4, VT, 512>(plan, pt, in, out, nrhs, dtype, stream);
         default: break;
     }
-    if (plan->n < kSmallBits) return launch_tile<1, VT, kSmallThreads, true>(plan, in, out, nrhs, dtype, stream);
-    switch (plan->ab) {
-        case 0: return launch_tile<1, VT, kSmallThreads>(plan, in, out, nrhs, dtype, stream);
-        case 1: return launch_tile<2, VT, kSmallThreads>(plan, in, out, nrhs, dtype, stream);
-        default: return launch_tile<4, VT, kSmallThreads>(plan, in, out, nrhs, dtype, stream);
+    if (plan->n < kSmallBits) return launch_tile<1, VT, kSmallThreads, true>(plan, pt, in, out, nrhs, dtype, stream);
+    switch (pt.ab) {
+        case 0: return launch_tile<1, VT, kSmallThreads>(plan, pt, in, out, nrhs, dtype, stream);
+        case 1: return launch_tile<2, VT, kSmallThreads>(plan, pt, in, out, nrhs, dtype, stream);
+        default: return launch_tile<4, VT, kSmallThreads>(plan, pt, in, out, nrhs, dtype, stream);
     }
 }
 
@@ -284,7 +318,8 @@ int apply_device(const tcp_mvp_plan *plan, const void *in, void *out, uint64_t n
     const size_t bytes = static_cast<size_t>(plan->dim()) * amp * nrhs;
     if (tcp::ranges_overlap(in, bytes, out, bytes))
         return fail(TCP_ERR_INVALID_ARGUMENT, "apply_into input and output must not overlap");
-    if ((plan->dim() >> plan->L) > 0x7fffffffull) return fail(TCP_ERR_UNSUPPORTED, "too many tiles for one launch");
+    if ((plan->dim() >> plan->tiles(dtype).L) > 0x7fffffffull)
+        return fail(TCP_ERR_UNSUPPORTED, "too many tiles for one launch");
     tcp::DeviceGuard dg;
     int st = dg.enter(plan->device);
     if (st) return st;
@@ -307,7 +342,7 @@ int plan_diagonal(const tcp_mvp_plan *plan, double *diag, hipStream_t stream) {
     const uint64_t dim = plan->dim();
     const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((dim + 255) / 256, 65536));
     hipLaunchKernelGGL(mvp_diagonal_kernel, dim3(blocks), dim3(256), 0, stream, plan->args(TCP_C128),
-                       plan->tb, plan->diag_passes, dim, diag);
+                       plan->t128.tb, plan->t128.diag_passes, dim, diag);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TCP_OK : hip_fail(e, "diagonal kernel");
 }
@@ -339,8 +374,8 @@ int tcp_mvp_plan_create_tiled(const tcp_operator *op, int storage, uint64_t max_
     return build_plan(tcp::operator_of(op), storage, max_bytes, tiling, out);
 }
 
-uint32_t tcp_mvp_plan_tiling(const tcp_mvp_plan *plan) { return plan ? plan->tiling : 0; }
-uint32_t tcp_mvp_plan_tile_bits(const tcp_mvp_plan *plan) { return plan ? plan->L : 0; }
+uint32_t tcp_mvp_plan_tiling(const tcp_mvp_plan *plan, int dtype) { return plan ? plan->tiles(dtype).tiling : 0; }
+uint32_t tcp_mvp_plan_tile_bits(const tcp_mvp_plan *plan, int dtype) { return plan ? plan->tiles(dtype).L : 0; }
 
 int tcp_mvp_plan_create_from_words(uint32_t nqubits, uint64_t nterms, const uint64_t *x_words,
                                    const uint64_t *z_words, const double *coef, int storage,
@@ -356,8 +391,8 @@ int tcp_mvp_plan_create_from_words(uint32_t nqubits, uint64_t nterms, const uint
 uint32_t tcp_mvp_plan_nqubits(const tcp_mvp_plan *plan) { return plan ? plan->n : 0; }
 uint64_t tcp_mvp_plan_term_count(const tcp_mvp_plan *plan) { return plan ? plan->T : 0; }
 uint64_t tcp_mvp_plan_group_count(const tcp_mvp_plan *plan) { return plan ? plan->G : 0; }
-uint64_t tcp_mvp_plan_class_count(const tcp_mvp_plan *plan) { return plan ? plan->H : 0; }
-uint64_t tcp_mvp_plan_pass_count(const tcp_mvp_plan *plan) { return plan ? plan->passes : 0; }
+uint64_t tcp_mvp_plan_class_count(const tcp_mvp_plan *plan) { return plan ? plan->t128.H : 0; }
+uint64_t tcp_mvp_plan_pass_count(const tcp_mvp_plan *plan) { return plan ? plan->t128.passes : 0; }
 uint64_t tcp_mvp_plan_dimension(const tcp_mvp_plan *plan) { return plan ? plan->dim() : 0; }
 int tcp_mvp_plan_storage(const tcp_mvp_plan *plan) { return plan ? plan->storage : -1; }
 uint64_t tcp_mvp_plan_transition_count(const tcp_mvp_plan *plan) {

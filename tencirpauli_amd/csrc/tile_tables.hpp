@@ -81,17 +81,21 @@ struct alignas(32) TileTerm {
     double wr, wi;
 };
 
-// Copies of the descriptors out of the constant address space (scalar loads).
+typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
+
+// A pass descriptor in one 64-byte scalar load.
 __device__ __forceinline__ TileGroup load_group(cptr<TileGroup> p) {
+    const u32x16 w = *reinterpret_cast<cptr<u32x16>>(p);
     TileGroup g;
-    g.lo = p->lo;
-    g.zs = p->zs;
-    g.zlo = p->zlo;
-    g.zhi = p->zhi;
-    g.wr = p->wr;
-    g.wi = p->wi;
-    g.tbegin = p->tbegin;
-    g.tend = p->tend;
+    g.lo = w[0];
+    g.zs = w[1];
+    g.same = w[2];
+    g.zlo = w[3];
+    g.zhi = w[4] | (static_cast<uint64_t>(w[5]) << 32);
+    g.wr = __longlong_as_double(static_cast<long long>(w[6] | (static_cast<uint64_t>(w[7]) << 32)));
+    g.wi = __longlong_as_double(static_cast<long long>(w[8] | (static_cast<uint64_t>(w[9]) << 32)));
+    g.tbegin = w[10];
+    g.tend = w[11];
     return g;
 }
 __device__ __forceinline__ TileTerm load_term(cptr<TileTerm> p) {
@@ -107,7 +111,7 @@ __device__ __forceinline__ TileTerm load_term(cptr<TileTerm> p) {
 // Device view of one operator's tables.  Group and term indices are absolute
 // (several operators may share the arrays).
 struct TileArgs {
-    const TileClass *cls;   // [H]; H even (an odd class list ends with an empty class)
+    const TileClass *cls;   // [H]
     const uint32_t *zoff;   // [H * zoff_stride(A)] pass ranges per slot pattern
     const TileGroup *grp;   // passes
     const TileTerm *term;   // extra terms
@@ -133,15 +137,21 @@ __device__ __forceinline__ uint32_t src_offset(uint32_t tbase, uint32_t ls, int 
 }
 
 // acc[a] += (-1)^popc(ZS & a) * S * src[a] for every pass of slot pattern ZS
-// in [g0, g1) (straight-line code: the slot signs are compile-time).
+// in [g0, g1) (straight-line code: the slot signs are compile-time).  The A
+// LDS reads of a pass are issued together, then the per-thread sum S of the
+// pass's terms, then the FMAs.
 template <int ZS, int A, uint32_t TPB>
 __device__ __forceinline__ void passes_uniform(cptr<TileGroup> grp, cptr<TileTerm> term, uint32_t g0, uint32_t g1,
-                                               uint32_t gend, TileGroup &next, uint32_t t, uint32_t t16,
-                                               uint64_t tile_id, const char *tile, double *accr, double *acci) {
+                                               uint32_t t, uint32_t t16, uint64_t tile_id, const char *tile,
+                                               double *accr, double *acci) {
     constexpr uint32_t TB = __builtin_ctz(TPB);
     for (uint32_t g = g0; g < g1; ++g) {
-        const TileGroup G = next;  // the next descriptor loads while this pass computes
-        if (g + 1 < gend) next = load_group(grp + g + 1);
+        const TileGroup G = load_group(grp + g);
+        const uint32_t tbase = t16 ^ ((G.lo & (TPB - 1)) * static_cast<uint32_t>(sizeof(dvec2)));
+        const uint32_t ls = G.lo >> TB;
+        dvec2 s[A];
+#pragma unroll
+        for (int a = 0; a < A; ++a) s[a] = *reinterpret_cast<const dvec2 *>(tile + src_offset<TPB>(tbase, ls, a));
         uint32_t par = __popc(G.zlo & t) + __popcll(G.zhi & tile_id);
         double sr = flip_sign(G.wr, par & 1u), si = flip_sign(G.wi, par & 1u);
         for (uint32_t k = G.tbegin; k < G.tend; ++k) {
@@ -150,43 +160,40 @@ __device__ __forceinline__ void passes_uniform(cptr<TileGroup> grp, cptr<TileTer
             sr += flip_sign(T.wr, par & 1u);
             si += flip_sign(T.wi, par & 1u);
         }
-        const uint32_t tbase = t16 ^ ((G.lo & (TPB - 1)) * static_cast<uint32_t>(sizeof(dvec2)));
-        const uint32_t ls = G.lo >> TB;
 #pragma unroll
         for (int a = 0; a < A; ++a) {
-            const dvec2 s = *reinterpret_cast<const dvec2 *>(tile + src_offset<TPB>(tbase, ls, a));
             if (SlotSign<A>::neg(ZS, a)) {
-                accr[a] = fma(-sr, s.x, accr[a]);
-                accr[a] = fma(si, s.y, accr[a]);
-                acci[a] = fma(-sr, s.y, acci[a]);
-                acci[a] = fma(-si, s.x, acci[a]);
+                accr[a] = fma(-sr, s[a].x, accr[a]);
+                accr[a] = fma(si, s[a].y, accr[a]);
+                acci[a] = fma(-sr, s[a].y, acci[a]);
+                acci[a] = fma(-si, s[a].x, acci[a]);
             } else {
-                accr[a] = fma(sr, s.x, accr[a]);
-                accr[a] = fma(-si, s.y, accr[a]);
-                acci[a] = fma(sr, s.y, acci[a]);
-                acci[a] = fma(si, s.x, acci[a]);
+                accr[a] = fma(sr, s[a].x, accr[a]);
+                accr[a] = fma(-si, s[a].y, accr[a]);
+                acci[a] = fma(sr, s[a].y, acci[a]);
+                acci[a] = fma(si, s[a].x, acci[a]);
             }
         }
     }
 }
 
-// All passes of one class, slot pattern by slot pattern; `next` holds the
-// descriptor of pass zr[0] on entry.
+// All passes of one class, slot pattern by slot pattern (zr: the class's
+// pass-range record).
 template <int A, uint32_t TPB, int ZS = 0>
-__device__ __forceinline__ void passes_all(cptr<TileGroup> grp, cptr<TileTerm> term, const uint32_t *zr,
-                                           TileGroup &next, uint32_t t, uint32_t t16, uint64_t tile_id,
-                                           const char *tile, double *accr, double *acci) {
-    passes_uniform<ZS, A, TPB>(grp, term, zr[ZS], zr[ZS + 1], zr[A], next, t, t16, tile_id, tile, accr, acci);
-    if constexpr (ZS + 1 < A)
-        passes_all<A, TPB, ZS + 1>(grp, term, zr, next, t, t16, tile_id, tile, accr, acci);
+__device__ __forceinline__ void passes_all(cptr<TileGroup> grp, cptr<TileTerm> term, cptr<uint32_t> zr, uint32_t t,
+                                           uint32_t t16, uint64_t tile_id, const char *tile, double *accr,
+                                           double *acci) {
+    passes_uniform<ZS, A, TPB>(grp, term, zr[ZS], zr[ZS + 1], t, t16, tile_id, tile, accr, acci);
+    if constexpr (ZS + 1 < A) passes_all<A, TPB, ZS + 1>(grp, term, zr, t, t16, tile_id, tile, accr, acci);
 }
 
-// y[tile] = sum over classes/groups of the gathered source tiles.  `lds` is
+// y[tile] = sum over classes/passes of the gathered source tiles.  `lds` is
 // the workgroup's buffer of TPB * A FP64 amplitudes (FP32 storage is widened
 // once per class).  `active` amplitudes of the tile exist (2^L except when
 // the whole space is smaller than one tile; then only l < active are read
-// and written).  Every thread of the workgroup must call this (it
-// synchronises).
+// and written).  The next class's source tile is loaded into registers while
+// the current class is applied.  Every thread of the workgroup must call
+// this (it synchronises).
 template <int A, typename VT, uint32_t TPB, bool PARTIAL>
 __device__ __forceinline__ void tile_apply(const VT *__restrict__ vin, VT *__restrict__ yout, const TileArgs &p,
                                            uint64_t tile_id, uint32_t active, dvec2 *lds) {
@@ -208,54 +215,33 @@ __device__ __forceinline__ void tile_apply(const VT *__restrict__ vin, VT *__res
 
     const uint32_t t16 = t * static_cast<uint32_t>(sizeof(dvec2));
     const char *tile = reinterpret_cast<const char *>(lds);
-
     double accr[A], acci[A];
+    VT pre[A];
 #pragma unroll
     for (int a = 0; a < A; ++a) {
         accr[a] = 0.0;
         acci[a] = 0.0;
+        pre[a] = Amp<VT>::make(0.0, 0.0);
     }
-    // Two classes of source tiles are in flight: buffers pa / pb alternate
-    // (the class loop is unrolled by two so neither is copied while its
-    // loads are outstanding).
-    VT pa[A], pb[A];
-    auto fetch = [&](uint32_t h, VT *buf) {
-        const uint32_t hc = h < p.H ? h : p.H - 1;  // past the end: re-read the last tile
-        const uint64_t src = base ^ (static_cast<uint64_t>(cls[hc].hi) << p.L);
+    {
+        const uint64_t src = base ^ (static_cast<uint64_t>(cls[0].hi) << p.L);
 #pragma unroll
-        for (int a = 0; a < A; ++a) buf[a] = live(a) ? src_base[src + a * TPB] : Amp<VT>::make(0.0, 0.0);
-    };
+        for (int a = 0; a < A; ++a)
+            if (live(a)) pre[a] = src_base[src + a * TPB];
+    }
     constexpr uint32_t ZSTRIDE = zoff_stride(A);
-    auto step = [&](uint32_t h, VT *buf) {
+    for (uint32_t h = 0; h < p.H; ++h) {
         __syncthreads();  // every wave is done reading the previous class's tile
 #pragma unroll
-        for (int a = 0; a < A; ++a) lds[t + a * TPB] = Amp<VT>::widen(buf[a]);
+        for (int a = 0; a < A; ++a) lds[t + a * TPB] = Amp<VT>::widen(pre[a]);
         __syncthreads();
-        // this class's pass ranges, its first pass and the tile two classes
-        // ahead are requested together after the barriers (a barrier waits
-        // for every outstanding scalar load)
-        uint32_t zr[A + 1];
+        if (h + 1 < p.H) {  // prefetch the next class's source tile
+            const uint64_t src = base ^ (static_cast<uint64_t>(cls[h + 1].hi) << p.L);
 #pragma unroll
-        for (int z = 0; z <= A; ++z) zr[z] = zoff[h * ZSTRIDE + z];
-        fetch(h + 2, buf);
-        // keep the prefetch here, ahead of this class's passes (the compiler
-        // would otherwise sink the loads toward their use)
-        asm volatile("" ::: "memory");
-        __builtin_amdgcn_sched_barrier(0);
-        TileGroup next = load_group(grp + zr[0]);  // every class has a pass
-        passes_all<A, TPB>(grp, term, zr, next, t, t16, tile_id, tile, accr, acci);
-    };
-    // the first two tiles are requested in buffer order (pa strictly before
-    // pb): the waits of the loop rely on pa's loads being the older ones
-    fetch(0, pa);
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_sched_barrier(0);
-    fetch(1, pb);
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_sched_barrier(0);
-    for (uint32_t h = 0; h < p.H; h += 2) {  // p.H is even (the tables pad it)
-        step(h, pa);
-        step(h + 1, pb);
+            for (int a = 0; a < A; ++a)
+                if (live(a)) pre[a] = src_base[src + a * TPB];
+        }
+        passes_all<A, TPB>(grp, term, zoff + h * ZSTRIDE, t, t16, tile_id, tile, accr, acci);
     }
     VT *__restrict__ out = yout + base + t;
 #pragma unroll
@@ -287,7 +273,6 @@ struct TileTables {
     std::vector<TileGroup> grp64, grp32;   // grp32: coefficients rounded to FP32
     std::vector<TileTerm> term64, term32;
     uint64_t G = 0, H = 0;      // of the last appended operator: distinct x-masks, classes
-    uint64_t Hk = 0;            // its classes as the kernel walks them (H rounded up to even)
     uint64_t passes = 0;        // its passes (x-mask, slot pattern)
     uint64_t diag_passes = 0;   // its leading passes with x = 0
     size_t cls_base = 0;        // its first entry in cls
@@ -360,10 +345,6 @@ struct TileTables {
         }
         if (cls.size() > cls_base) cls.back().gend = static_cast<uint32_t>(grp64.size());
         H = cls.size() - cls_base;
-        if (H & 1)  // the kernel walks classes in pairs: pad with an empty class reading the last tile
-            cls.push_back(TileClass{cls.back().hi, static_cast<uint32_t>(grp64.size()),
-                                    static_cast<uint32_t>(grp64.size())});
-        Hk = cls.size() - cls_base;
         passes = grp64.size() - grp_base;
         // pass ranges per slot pattern
         const uint32_t stride = zoff_stride(A);

@@ -26,7 +26,8 @@ from ._plan import (
 )
 
 # tile geometries of the full-space kernel (threads x amplitudes per thread)
-TILINGS = {"512x8": _lib.TCP_TILE_512x8, "1024x8": _lib.TCP_TILE_1024x8, "512x16": _lib.TCP_TILE_512x16}
+TILINGS = {"512x8": _lib.TCP_TILE_512x8, "1024x8": _lib.TCP_TILE_1024x8, "512x16": _lib.TCP_TILE_512x16,
+           "512x4": _lib.TCP_TILE_512x4}
 
 _LETTERS = {"I": 0, "X": 1, "Y": 2, "Z": 3}
 PauliInput = Union[str, Sequence[int]]
@@ -237,10 +238,11 @@ class PauliOperator:
                         tiling: Optional[str] = None, xcd_order: Optional[bool] = None) -> "NativeMVPPlan":
         """Compile a reusable matrix-free plan on the current GPU.
 
-        ``tiling`` picks the kernel's tile geometry ("512x8", "1024x8",
-        "512x16"; None = automatic: 512x8 with the XCD order) and
-        ``xcd_order`` walks the tiles XCD by XCD; neither changes the result
-        beyond floating-point summation order.
+        ``tiling`` picks the kernel's tile geometry ("512x8", "512x4",
+        "1024x8", "512x16"; None = automatic: 512x8 for complex128 and 512x4
+        for complex64 storage, with the XCD order) and ``xcd_order`` walks the
+        tiles XCD by XCD; neither changes the result beyond floating-point
+        summation order.
         """
         validate_max_bytes(max_bytes)
         if storage not in ("lazy", "eager"):
@@ -251,10 +253,10 @@ class PauliOperator:
         out = ctypes.c_void_p()
         st = _lib.TCP_STORAGE_EAGER if storage == "eager" else _lib.TCP_STORAGE_LAZY
         code = TILINGS.get(tiling, _lib.TCP_TILE_AUTO)
-        if xcd_order is False and code == _lib.TCP_TILE_AUTO:
-            code = _lib.TCP_TILE_512x8  # automatic geometry without the XCD order
         if xcd_order:
             code |= _lib.TCP_TILE_XCD_ORDER
+        elif xcd_order is False:
+            code |= _lib.TCP_TILE_LINEAR_ORDER
         _lib.check(_lib.lib().tcp_mvp_plan_create_tiled(self._handle.ptr, st, effective_max_bytes(max_bytes),
                                                          code, ctypes.byref(out)))
         return NativeMVPPlan(out, self, storage)
@@ -303,9 +305,12 @@ class NativeMVPPlan(NativePlanFacade):
         self.group_count = int(L.tcp_mvp_plan_group_count(ptr))
         self.class_count = int(L.tcp_mvp_plan_class_count(ptr))
         self.pass_count = int(L.tcp_mvp_plan_pass_count(ptr))
-        self.tile_bits = int(L.tcp_mvp_plan_tile_bits(ptr))
-        code = int(L.tcp_mvp_plan_tiling(ptr))
-        self.tiling = {v: k for k, v in TILINGS.items()}.get(code & 0xFF, "small")
+        names = {v: k for k, v in TILINGS.items()}
+        self.tile_bits = int(L.tcp_mvp_plan_tile_bits(ptr, _lib.TCP_C128))
+        self.tile_bits_c64 = int(L.tcp_mvp_plan_tile_bits(ptr, _lib.TCP_C64))
+        code, code64 = int(L.tcp_mvp_plan_tiling(ptr, _lib.TCP_C128)), int(L.tcp_mvp_plan_tiling(ptr, _lib.TCP_C64))
+        self.tiling = names.get(code & 0xFF, "small")        # complex128 applies
+        self.tiling_c64 = names.get(code64 & 0xFF, "small")  # complex64 applies
         self.xcd_order = bool(code & _lib.TCP_TILE_XCD_ORDER)
         self.dimension = int(L.tcp_mvp_plan_dimension(ptr))
         self.storage = storage

@@ -72,7 +72,7 @@ def test_random_operators_every_tile_shape(n):
 
 
 @pytest.mark.parametrize("xcd_order", [False, True])
-@pytest.mark.parametrize("tiling", [None, "512x8", "1024x8", "512x16"])
+@pytest.mark.parametrize("tiling", [None, "512x8", "512x4", "1024x8", "512x16"])
 def test_every_tiling_against_oracle(tiling, xcd_order):
     import torch
 
@@ -80,7 +80,9 @@ def test_every_tiling_against_oracle(tiling, xcd_order):
     op = tcp.PauliOperator.from_code_arrays(codes, coef * (1 - 0.4j))
     plan = op.native_mvp_plan(tiling=tiling, xcd_order=xcd_order)
     assert plan.tiling == (tiling or "512x8") and plan.xcd_order == xcd_order
-    assert plan.tile_bits == (12 if plan.tiling == "512x8" else 13)
+    assert plan.tiling_c64 == (tiling or "512x4")
+    bits = {"512x4": 11, "512x8": 12, "1024x8": 13, "512x16": 13}
+    assert plan.tile_bits == bits[plan.tiling] and plan.tile_bits_c64 == bits[plan.tiling_c64]
     assert plan.pass_count >= plan.group_count
     state = _random_state(17, 92)
     expected = _oracle_apply(op, state)
@@ -90,10 +92,11 @@ def test_every_tiling_against_oracle(tiling, xcd_order):
     assert rel_err(y32.cpu().numpy().astype(np.complex128), expected) <= FP32_TOL
 
 
-def test_default_tiling_is_xcd_ordered_512x8():
+def test_default_tiling_is_xcd_ordered():
     codes, coef = lattice.random_pauli_sum(14, 50, seed=95)
     plan = tcp.PauliOperator.from_code_arrays(codes, coef).native_mvp_plan()
     assert plan.tiling == "512x8" and plan.xcd_order and plan.tile_bits == 12
+    assert plan.tiling_c64 == "512x4" and plan.tile_bits_c64 == 11
 
 
 def test_tiling_falls_back_on_small_spaces():
