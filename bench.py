@@ -44,7 +44,7 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "mvp_tile_28q_traffic.json"))
     p.add_argument("--calibrate-copy", action="store_true",
-                   help="run one elementwise copy (y = -v, 2^n amplitudes each way) before the MVPs, "
+                   help="run one plain device copy (y <- v, 2^n amplitudes each way) before the MVPs, "
                         "the known-size reference for HBM counter scaling")
     p.add_argument("--extras", default="fp32,sector,davidson",
                    help="comma list of extra sections: fp32, sector, davidson, batched ('' for none)")
@@ -94,6 +94,16 @@ def cpu_baseline(op, v_host: np.ndarray, y_dev_host_rows, args, bytes_alg: int, 
     }, {"rows": nrows, "rel_err": err}
 
 
+def tile_model(plan, tag: str, dim: int, ms: float) -> dict:
+    """Source-tile traffic of the kernel that ran: 2^n * (classes * s_in + s_out)."""
+    s = 8 if tag == "complex64" else 16
+    classes = plan.class_count if tag == "complex128" else plan.class_count_c64
+    tile_bytes = dim * (classes * s + s)
+    gbs = tile_bytes / (ms * 1e-3) / 1e9
+    return {"tiling": plan.tiling if tag == "complex128" else plan.tiling_c64, "classes": classes,
+            "bytes": tile_bytes, "GBps": round(gbs, 2), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def time_applies(fn, steps: int, warmup: int = 1) -> float:
     """Milliseconds per call of fn() (device work on torch's current stream)."""
     import torch
@@ -123,7 +133,8 @@ def extra_fp32(plan, v, dim) -> dict:
     err = (torch.linalg.vector_norm(y32.to(torch.complex128) - y64) / torch.linalg.vector_norm(y64)).item()
     alg = plan.bytes_alg("complex64")
     return {"ms_per_mvp": round(ms, 4), "bytes_alg_GBps": round(alg / (ms * 1e-3) / 1e9, 2),
-            "term_amp_per_s": round(plan.term_count * dim / (ms * 1e-3), 1), "rel_err_vs_fp64": err}
+            "term_amp_per_s": round(plan.term_count * dim / (ms * 1e-3), 1), "rel_err_vs_fp64": err,
+            "tile_model": tile_model(plan, "complex64", dim, ms)}
 
 
 def extra_sector() -> dict:
@@ -260,7 +271,7 @@ def main() -> int:
     y = torch.empty_like(v)
     torch.cuda.synchronize()
     if args.calibrate_copy:
-        torch.neg(v, out=y)
+        y.copy_(v)
         torch.cuda.synchronize()
 
     for _ in range(args.warmup):
@@ -316,6 +327,10 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
         },
+        # The grouped-gather model above charges one full read of v per x-group;
+        # the tile kernel reads one source tile per (output tile, class), so its
+        # own minimum HBM traffic is 2^n (classes * s_in + s_out).
+        "tile_model": tile_model(plan, tag, dim, ms_per_step),
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

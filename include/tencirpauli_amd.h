@@ -156,6 +156,8 @@ uint64_t tcp_mvp_plan_group_count(const tcp_mvp_plan *plan);
 uint64_t tcp_mvp_plan_class_count(const tcp_mvp_plan *plan);
 /* Passes of the tile kernel: (x-mask, slot pattern) pairs (>= group_count). */
 uint64_t tcp_mvp_plan_pass_count(const tcp_mvp_plan *plan);
+/* Classes of the tables the given dtype's applies use (complex64 storage may use another tile). */
+uint64_t tcp_mvp_plan_class_count_for(const tcp_mvp_plan *plan, int dtype);
 uint64_t tcp_mvp_plan_dimension(const tcp_mvp_plan *plan);
 int tcp_mvp_plan_storage(const tcp_mvp_plan *plan);
 /* Nonzero transitions of the eager graph (MvpPlan::transition_count,

@@ -75,6 +75,7 @@ SIGNATURES = {
     "tcp_mvp_plan_group_count": (_c.c_uint64, [_vp]),
     "tcp_mvp_plan_class_count": (_c.c_uint64, [_vp]),
     "tcp_mvp_plan_pass_count": (_c.c_uint64, [_vp]),
+    "tcp_mvp_plan_class_count_for": (_c.c_uint64, [_vp, _c.c_int]),
     "tcp_mvp_plan_create_tiled": (_c.c_int, [_vp, _c.c_int, _c.c_uint64, _c.c_uint32, _c.POINTER(_vp)]),
     "tcp_mvp_plan_tiling": (_c.c_uint32, [_vp, _c.c_int]),
     "tcp_mvp_plan_tile_bits": (_c.c_uint32, [_vp, _c.c_int]),

@@ -393,6 +393,7 @@ uint64_t tcp_mvp_plan_term_count(const tcp_mvp_plan *plan) { return plan ? plan-
 uint64_t tcp_mvp_plan_group_count(const tcp_mvp_plan *plan) { return plan ? plan->G : 0; }
 uint64_t tcp_mvp_plan_class_count(const tcp_mvp_plan *plan) { return plan ? plan->t128.H : 0; }
 uint64_t tcp_mvp_plan_pass_count(const tcp_mvp_plan *plan) { return plan ? plan->t128.passes : 0; }
+uint64_t tcp_mvp_plan_class_count_for(const tcp_mvp_plan *plan, int dtype) { return plan ? plan->tiles(dtype).H : 0; }
 uint64_t tcp_mvp_plan_dimension(const tcp_mvp_plan *plan) { return plan ? plan->dim() : 0; }
 int tcp_mvp_plan_storage(const tcp_mvp_plan *plan) { return plan ? plan->storage : -1; }
 uint64_t tcp_mvp_plan_transition_count(const tcp_mvp_plan *plan) {

@@ -305,6 +305,7 @@ class NativeMVPPlan(NativePlanFacade):
         self.group_count = int(L.tcp_mvp_plan_group_count(ptr))
         self.class_count = int(L.tcp_mvp_plan_class_count(ptr))
         self.pass_count = int(L.tcp_mvp_plan_pass_count(ptr))
+        self.class_count_c64 = int(L.tcp_mvp_plan_class_count_for(ptr, _lib.TCP_C64))
         names = {v: k for k, v in TILINGS.items()}
         self.tile_bits = int(L.tcp_mvp_plan_tile_bits(ptr, _lib.TCP_C128))
         self.tile_bits_c64 = int(L.tcp_mvp_plan_tile_bits(ptr, _lib.TCP_C64))
