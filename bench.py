@@ -46,8 +46,8 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run one plain device copy (y <- v, 2^n amplitudes each way) before the MVPs, "
                         "the known-size reference for HBM counter scaling")
-    p.add_argument("--extras", default="fp32,sector,davidson",
-                   help="comma list of extra sections: fp32, sector, davidson, batched ('' for none)")
+    p.add_argument("--extras", default="fp32,sector,eager,davidson",
+                   help="comma list of extra sections: fp32, sector, eager, davidson, batched ('' for none)")
     return p.parse_args()
 
 
@@ -167,6 +167,46 @@ def extra_sector() -> dict:
             rec[f"amp_per_s_{tag}"] = round(restricted.dimension / (ms * 1e-3), 1)
         out[name] = rec
     return out
+
+
+def extra_eager() -> dict:
+    """5x5 periodic XXZ, S_z = 1/2 (dim 5,200,300): the eager restricted operator (device CSR)
+    against the lazy sector gather.  CSR bytes per apply: indptr 8 (D+1) + columns 4 nnz +
+    values 16|8 nnz + y 16|8 D, plus one read of v (gathers are assumed to hit in cache)."""
+    import torch
+
+    import tencirpauli_amd as tcp
+    from tencirpauli_amd import lattice
+
+    codes, coef = lattice.heisenberg_periodic(5, 5, seed=SEED)
+    op = tcp.PauliOperator.from_code_arrays(codes, coef)
+    sector = tcp.U1Sector(25, 12)
+    lazy = op.restrict_charge(sector)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eager = op.restrict_charge(sector, storage="eager")
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    dim, nnz = eager.dimension, eager.transition_count
+    gen = torch.Generator(device="cuda").manual_seed(SEED + 9)
+    v = torch.randn(dim, dtype=torch.complex128, device="cuda", generator=gen)
+    rec = {"dimension": dim, "nnz": nnz, "build_s": round(build_s, 3),
+           "retained_device_bytes": eager.mvp_plan(storage="eager").device_bytes}
+    y_ref = torch.empty_like(v)
+    lazy.apply_into(v, y_ref)
+    for tag, dt, s in (("fp64", torch.complex128, 16), ("fp32", torch.complex64, 8)):
+        vv = v.to(dt)
+        yy = torch.empty_like(vv)
+        ms = time_applies(lambda: eager.apply_into(vv, yy), 10)
+        ms_lazy = time_applies(lambda: lazy.apply_into(vv, yy), 10)
+        eager.apply_into(vv, yy)
+        err = (torch.linalg.vector_norm(yy.to(torch.complex128) - y_ref) / torch.linalg.vector_norm(y_ref)).item()
+        alg = 8 * (dim + 1) + (4 + s) * nnz + 2 * s * dim
+        rec[tag] = {"ms_per_mvp": round(ms, 4), "lazy_ms_per_mvp": round(ms_lazy, 4),
+                    "bytes_alg_GBps": round(alg / (ms * 1e-3) / 1e9, 2),
+                    "frac_of_hbm_peak": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "rel_err_vs_lazy_fp64": err}
+    return rec
 
 
 def extra_davidson() -> dict:
@@ -346,11 +386,13 @@ def main() -> int:
     if "fp32" in extras and tag == "complex128":
         record["fp32_storage"] = extra_fp32(plan, v, dim)
     del y
-    if "sector" in extras or "davidson" in extras or "batched" in extras:
+    if any(e in extras for e in ("sector", "eager", "davidson", "batched")):
         del v, plan
         torch.cuda.empty_cache()
     if "sector" in extras:
         record["sector_mvp"] = extra_sector()
+    if "eager" in extras:
+        record["eager_csr_5x5_xxz"] = extra_eager()
     if "davidson" in extras:
         record["davidson_5x5_xxz"] = extra_davidson()
     if "batched" in extras:

@@ -262,6 +262,35 @@ int tcp_sector_plan_walk_device(const tcp_sector_plan *plan, uint64_t first, uin
 
 void tcp_sector_plan_destroy(tcp_sector_plan *plan);
 
+/* ---- eager restricted operator: U1RestrictedOperator::new + CsrGraph
+ *      (core/sector.rs:345-432, core/csr.rs:196-240) and the charge eager
+ *      build (core/charge/eager.rs, native/charge_sector.rs:569-845) ------ */
+
+typedef struct tcp_csr_plan tcp_csr_plan;
+
+/* Materialise the sector-restricted operator as CSR on the device: rows are
+ * destination ranks, columns ascend inside a row, exact zeros are skipped.
+ * No conservation precondition (as the reference's eager builders): every
+ * aggregated transition is checked and leakage fails the build with the
+ * reference's error (TCP_ERR_SECTOR_LEAKAGE for U(1), TCP_ERR_INVALID_SECTOR
+ * for charge blocks).  max_bytes (>= 1 GiB) bounds the retained CSR bytes. */
+int tcp_csr_plan_create_u1(const tcp_operator *op, uint32_t particle_number, uint64_t max_bytes,
+                           tcp_csr_plan **out);
+int tcp_csr_plan_create_blocks(const tcp_operator *op, uint32_t nblocks, const uint32_t *sizes,
+                               const uint32_t *pops, uint64_t max_bytes, tcp_csr_plan **out);
+uint64_t tcp_csr_plan_dimension(const tcp_csr_plan *plan);
+uint64_t tcp_csr_plan_transition_count(const tcp_csr_plan *plan); /* stored entries */
+uint64_t tcp_csr_plan_estimated_bytes(const tcp_csr_plan *plan);
+/* y = A v: row-parallel CSR gather (CsrGraph::apply_into), FP64 accumulation;
+ * TCP_C64 uses complex64 vectors and FP32-rounded values. */
+int tcp_csr_plan_apply_into(const tcp_csr_plan *plan, const void *input, void *output, uint64_t input_len,
+                            uint64_t output_len, int dtype, void *stream);
+int tcp_csr_plan_apply_host(const tcp_csr_plan *plan, const double *input, double *output, uint64_t input_len,
+                            uint64_t output_len);
+/* CsrGraph::to_u64_arrays: host arrays of dimension+1, nnz and 2*nnz entries. */
+int tcp_csr_plan_export(const tcp_csr_plan *plan, uint64_t *indptr, uint64_t *indices, double *values);
+void tcp_csr_plan_destroy(tcp_csr_plan *plan);
+
 /* ---- batched MVP (no reference counterpart): many operators of uneven size,
  *      each applied to a block of right-hand sides, in one persistent launch -- */
 

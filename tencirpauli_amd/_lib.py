@@ -109,6 +109,16 @@ SIGNATURES = {
     "tcp_sector_plan_unrank_device": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _vp]),
     "tcp_sector_plan_walk_device": (_c.c_int, [_vp, _c.c_uint64, _vp, _c.c_uint64, _c.c_uint32, _vp]),
     "tcp_sector_plan_destroy": (None, [_vp]),
+    "tcp_csr_plan_create_u1": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_csr_plan_create_blocks": (_c.c_int, [_vp, _c.c_uint32, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint32),
+                                               _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_csr_plan_dimension": (_c.c_uint64, [_vp]),
+    "tcp_csr_plan_transition_count": (_c.c_uint64, [_vp]),
+    "tcp_csr_plan_estimated_bytes": (_c.c_uint64, [_vp]),
+    "tcp_csr_plan_apply_into": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _c.c_uint64, _c.c_int, _vp]),
+    "tcp_csr_plan_apply_host": (_c.c_int, [_vp, _dp, _dp, _c.c_uint64, _c.c_uint64]),
+    "tcp_csr_plan_export": (_c.c_int, [_vp, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64), _dp]),
+    "tcp_csr_plan_destroy": (None, [_vp]),
     "tcp_batch_plan_create": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint32, _c.POINTER(_vp)]),
     "tcp_batch_plan_count": (_c.c_uint32, [_vp]),
     "tcp_batch_plan_offset": (_c.c_uint64, [_vp, _c.c_uint32]),

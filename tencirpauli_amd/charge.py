@@ -149,39 +149,69 @@ def _factorized_blocks(constraints: Sequence[Tuple[AdditiveCharge, int]]) -> Opt
 
 
 class ChargeRestrictedOperator(NativePlanFacade):
-    """Pauli operator restricted to a factorized charge sector (py/charge.py:1075-1412)."""
+    """Pauli operator restricted to a factorized charge sector (py/charge.py:1075-1412).
 
-    def __init__(self, operator: Any, sector: ChargeSector, plan: Any, storage: str) -> None:
+    ``storage="eager"`` materialises the restricted operator as device CSR
+    (core/charge/eager.rs); ``"lazy"`` keeps the matrix-free plan.
+    """
+
+    def __init__(self, operator: Any, sector: ChargeSector, plan: Any, storage: str, csr_plan: Any = None) -> None:
         self.operator = operator
         self.sector = sector
         self.dimension = sector.dimension
         self.storage = storage
         self.term_count = operator.term_count
         self._plan = plan
+        self._csr = csr_plan
         self.native_variant = "factorized"
         self.execution_family = "factorized"
 
     @property
     def plan(self) -> Any:
+        return self._csr if self.storage == "eager" else self._plan
+
+    @property
+    def transition_count(self) -> Optional[int]:
+        return None if self._csr is None else self._csr.transition_count
+
+    def _ensure_lazy(self, max_bytes: Optional[int]) -> Any:
+        if self._plan is None:
+            self._plan = _lazy_blocks_plan(self.operator, self.sector, max_bytes)
         return self._plan
 
+    def _ensure_eager(self, max_bytes: Optional[int]) -> Any:
+        if self._csr is None:
+            from .eager import csr_plan_blocks
+
+            sizes, pops = self.sector._arrays()
+            self._csr = csr_plan_blocks(self.operator, sizes, pops, max_bytes)
+        return self._csr
+
+    def mvp_plan(self, *, storage: str = "lazy", max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        validate_max_bytes(max_bytes)
+        if storage not in ("lazy", "eager"):
+            raise ValueError("storage must be either 'eager' or 'lazy'")
+        return self._ensure_eager(max_bytes) if storage == "eager" else self._ensure_lazy(max_bytes)
+
+    def dense(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> np.ndarray:
+        return self._ensure_eager(max_bytes).materialize("dense", max_bytes)
+
+    def coo(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        return self._ensure_eager(max_bytes).materialize("coo", max_bytes)
+
+    def csr(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        return self._ensure_eager(max_bytes).materialize("csr", max_bytes)
+
     def _native_apply_host(self, inp: np.ndarray, out: np.ndarray) -> int:
-        return self._plan._native_apply_host(inp, out)
+        return self.plan._native_apply_host(inp, out)
 
     def _native_apply_device(self, inp: Any, out: Any, dtype: int, stream: int) -> int:
-        return self._plan._native_apply_device(inp, out, dtype, stream)
+        return self.plan._native_apply_device(inp, out, dtype, stream)
 
 
-def restrict_charge(operator: Any, sector: ChargeSector, *, storage: str = "lazy",
-                    max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> ChargeRestrictedOperator:
+def _lazy_blocks_plan(operator: Any, sector: ChargeSector, max_bytes: Optional[int]) -> Any:
     from .symmetry import SectorMvpPlan
 
-    if storage not in ("lazy", "eager"):
-        raise ValueError("storage must be either 'eager' or 'lazy'")
-    if operator.nqubits != sector.nqubits:
-        raise ValueError("Pauli operator and charge sector layouts are incompatible")
-    if sector.blocks is None:
-        raise NotImplementedError("only contiguous fixed-population qubit blocks are implemented")
     _lib.require_device()
     sizes, pops = sector._arrays()
     out = ctypes.c_void_p()
@@ -189,5 +219,22 @@ def restrict_charge(operator: Any, sector: ChargeSector, *, storage: str = "lazy
     _lib.check(_lib.lib().tcp_sector_plan_create_blocks(operator._handle.ptr, len(sizes), sizes.ctypes.data_as(u32),
                                                          pops.ctypes.data_as(u32), effective_max_bytes(max_bytes),
                                                          ctypes.byref(out)))
-    plan = SectorMvpPlan(out, "factorized", "factorized")
-    return ChargeRestrictedOperator(operator, sector, plan, storage)
+    return SectorMvpPlan(out, "factorized", "factorized")
+
+
+def restrict_charge(operator: Any, sector: ChargeSector, *, storage: str = "lazy",
+                    max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> ChargeRestrictedOperator:
+    if storage not in ("lazy", "eager"):
+        raise ValueError("storage must be either 'eager' or 'lazy'")
+    validate_max_bytes(max_bytes)
+    if operator.nqubits != sector.nqubits:
+        raise ValueError("Pauli operator and charge sector layouts are incompatible")
+    if sector.blocks is None:
+        raise NotImplementedError("only contiguous fixed-population qubit blocks are implemented")
+    if storage == "eager":
+        from .eager import csr_plan_blocks
+
+        sizes, pops = sector._arrays()
+        return ChargeRestrictedOperator(operator, sector, None, storage,
+                                        csr_plan_blocks(operator, sizes, pops, max_bytes))
+    return ChargeRestrictedOperator(operator, sector, _lazy_blocks_plan(operator, sector, max_bytes), storage)

@@ -371,6 +371,172 @@ __global__ void sector_features_kernel(SectorArgs p, uint32_t diag_group, double
     diag[r] = db;
 }
 
+// ---- eager restricted operator: CSR with rows = destination ranks ----------
+//
+// Replaces U1RestrictedOperator::new / CsrGraph (core/sector.rs:345-432,
+// core/csr.rs:196-240) and the charge eager build (core/charge/eager.rs).
+// Row r holds, for every in-sector source s = b ^ x_g with a nonzero
+// aggregated value, (rank(s), value); columns ascend inside a row, the order
+// the reference's source-major two-pass build produces.
+
+constexpr uint32_t kCsrSortCap = 64;  // rows up to this length are sorted in registers/scratch
+
+template <bool LDS>
+__global__ __launch_bounds__(kSectorThreads) void csr_count_kernel(SectorArgs p, uint32_t *__restrict__ counts) {
+    extern __shared__ uint32_t lds_lut[];
+    if (LDS) {
+        for (uint32_t i = threadIdx.x; i < p.lut_words; i += blockDim.x) lds_lut[i] = p.lut[i];
+        __syncthreads();
+    }
+    cptr<uint64_t> gx = (cptr<uint64_t>)p.gx;
+    cptr<uint32_t> goff = (cptr<uint32_t>)p.goff;
+    cptr<uint64_t> tz = (cptr<uint64_t>)p.tz;
+    cptr<dvec2> tw = (cptr<dvec2>)p.tw;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * kRanksPerThread;
+    for (uint64_t r0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kRanksPerThread;
+         r0 < p.dim; r0 += stride) {
+        uint64_t b = sector_unrank(p, r0);
+        for (uint32_t j = 0; j < kRanksPerThread; ++j) {
+            const uint64_t r = r0 + j;
+            if (r >= p.dim) break;
+            uint32_t count = 0;
+            for (uint32_t g = 0; g < p.G; ++g) {
+                const uint64_t s = b ^ gx[g];
+                if (!in_sector(p, s)) continue;
+                double dr = 0.0, di = 0.0;
+                for (uint32_t k = goff[g]; k < goff[g + 1]; ++k) {
+                    const uint32_t par = __popcll(tz[k] & s) & 1u;
+                    dr += flip_sign(tw[k].x, par);
+                    di += flip_sign(tw[k].y, par);
+                }
+                count += (dr != 0.0 || di != 0.0);
+            }
+            counts[r] = count;
+            b = sector_next(p, b);
+        }
+    }
+}
+
+template <bool LDS>
+__global__ __launch_bounds__(kSectorThreads) void csr_fill_kernel(SectorArgs p, const uint64_t *__restrict__ indptr,
+                                                                  uint32_t *__restrict__ cols, dvec2 *__restrict__ vals,
+                                                                  fvec2 *__restrict__ vals32) {
+    extern __shared__ uint32_t lds_lut[];
+    const uint32_t *lut = p.lut;
+    if (LDS) {
+        for (uint32_t i = threadIdx.x; i < p.lut_words; i += blockDim.x) lds_lut[i] = p.lut[i];
+        __syncthreads();
+        lut = lds_lut;
+    }
+    cptr<uint64_t> gx = (cptr<uint64_t>)p.gx;
+    cptr<uint32_t> goff = (cptr<uint32_t>)p.goff;
+    cptr<uint64_t> tz = (cptr<uint64_t>)p.tz;
+    cptr<dvec2> tw = (cptr<dvec2>)p.tw;
+    uint32_t lc[kCsrSortCap];
+    dvec2 lv[kCsrSortCap];
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * kRanksPerThread;
+    for (uint64_t r0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kRanksPerThread;
+         r0 < p.dim; r0 += stride) {
+        uint64_t b = sector_unrank(p, r0);
+        for (uint32_t j = 0; j < kRanksPerThread; ++j) {
+            const uint64_t r = r0 + j;
+            if (r >= p.dim) break;
+            const uint64_t pos = indptr[r];
+            const uint64_t len = indptr[r + 1] - pos;
+            const bool local = len <= kCsrSortCap;
+            uint32_t n = 0;
+            for (uint32_t g = 0; g < p.G && n < len; ++g) {
+                const uint64_t x = gx[g];
+                const uint64_t s = b ^ x;
+                if (!in_sector(p, s)) continue;
+                double dr = 0.0, di = 0.0;
+                for (uint32_t k = goff[g]; k < goff[g + 1]; ++k) {
+                    const uint32_t par = __popcll(tz[k] & s) & 1u;
+                    dr += flip_sign(tw[k].x, par);
+                    di += flip_sign(tw[k].y, par);
+                }
+                if (dr == 0.0 && di == 0.0) continue;
+                const uint32_t c = x == 0 ? static_cast<uint32_t>(r) : sector_rank(p, lut, s);
+                if (local) {  // insertion into the sorted prefix
+                    uint32_t q = n;
+                    while (q > 0 && lc[q - 1] > c) {
+                        lc[q] = lc[q - 1];
+                        lv[q] = lv[q - 1];
+                        --q;
+                    }
+                    lc[q] = c;
+                    lv[q] = dvec2{dr, di};
+                } else {
+                    cols[pos + n] = c;
+                    vals[pos + n] = dvec2{dr, di};
+                }
+                ++n;
+            }
+            if (local) {
+                for (uint32_t q = 0; q < n; ++q) {
+                    cols[pos + q] = lc[q];
+                    vals[pos + q] = lv[q];
+                }
+            } else {  // long row: insertion sort in place
+                for (uint32_t q = 1; q < n; ++q) {
+                    const uint32_t c = cols[pos + q];
+                    const dvec2 v = vals[pos + q];
+                    uint32_t i = q;
+                    while (i > 0 && cols[pos + i - 1] > c) {
+                        cols[pos + i] = cols[pos + i - 1];
+                        vals[pos + i] = vals[pos + i - 1];
+                        --i;
+                    }
+                    cols[pos + i] = c;
+                    vals[pos + i] = v;
+                }
+            }
+            for (uint32_t q = 0; q < n; ++q) {
+                const dvec2 v = vals[pos + q];
+                vals32[pos + q] = fvec2{static_cast<float>(v.x), static_cast<float>(v.y)};
+            }
+            b = sector_next(p, b);
+        }
+    }
+}
+
+// y[row] = sum_k vals[k] v[cols[k]] over the row, LANES lanes per row (an
+// aligned group inside one wave), FP64 accumulation, lanes reduced by
+// cross-lane exchange; every output row is written once.
+template <typename VT, typename WT, uint32_t LANES>
+__global__ __launch_bounds__(256) void csr_spmv_kernel(const uint64_t *__restrict__ indptr,
+                                                       const uint32_t *__restrict__ cols,
+                                                       const WT *__restrict__ vals, const VT *__restrict__ v,
+                                                       VT *__restrict__ y, uint64_t dim) {
+    const uint32_t lane = threadIdx.x % LANES;
+    const uint64_t rows_per_step = static_cast<uint64_t>(gridDim.x) * (blockDim.x / LANES);
+    // the LANES lanes of a row share `row`, so they enter and leave the loop
+    // together and every exchange partner is active
+    for (uint64_t row = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / LANES) + threadIdx.x / LANES; row < dim;
+         row += rows_per_step) {
+        double ar = 0.0, ai = 0.0;
+        {
+            const uint64_t k1 = indptr[row + 1];
+            for (uint64_t k = indptr[row] + lane; k < k1; k += LANES) {
+                const WT c = vals[k];
+                const VT s = v[cols[k]];
+                const double cr = static_cast<double>(c.x), ci = static_cast<double>(c.y);
+                const double sr = static_cast<double>(s.x), si = static_cast<double>(s.y);
+                ar = fma(cr, sr, ar);
+                ar = fma(-ci, si, ar);
+                ai = fma(cr, si, ai);
+                ai = fma(ci, sr, ai);
+            }
+        }
+#pragma unroll
+        for (uint32_t off = LANES / 2; off > 0; off >>= 1) {
+            ar += __shfl_xor(ar, off, LANES);
+            ai += __shfl_xor(ai, off, LANES);
+        }
+        if (lane == 0) y[row] = Amp<VT>::make(ar, ai);
+    }
+}
+
 }  // namespace
 
 struct tcp_sector_plan {
@@ -499,8 +665,12 @@ int sector_geometry(const BlockSpec &spec, SectorArgs &a, uint64_t &dim, std::ve
 
 int scan_leakage(tcp_sector_plan *plan, const std::vector<uint64_t> &gx);
 
+// require_conserved: lazy restriction's precondition (exact conservation of
+// every block's particle number).  The eager builders skip it, as the
+// reference's do (core/sector.rs:345-432 checks each aggregated transition
+// instead), and rely on the leakage scan.
 int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, uint64_t max_bytes,
-                      tcp_sector_plan **out) {
+                      tcp_sector_plan **out, bool require_conserved = true) {
     if (!out) return fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
     *out = nullptr;
     uint32_t n = 0;
@@ -521,7 +691,7 @@ int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, u
 
     // Exact conservation of each block's particle number: the precondition
     // of lazy restriction (py/symmetry.py:675-681, py/charge.py:1128-1132).
-    for (uint32_t j = 0; j < spec.nblocks; ++j) {
+    for (uint32_t j = 0; require_conserved && j < spec.nblocks; ++j) {
         std::vector<double> levels(2 * n, 0.0);
         const uint32_t first = n - a.shift[j] - spec.sizes[j];
         for (uint32_t q = first; q < first + spec.sizes[j]; ++q) levels[2 * q + 1] = 1.0;
@@ -980,5 +1150,191 @@ int tcp_sector_plan_walk_device(const tcp_sector_plan *plan, uint64_t first, uin
 }
 
 void tcp_sector_plan_destroy(tcp_sector_plan *plan) { delete plan; }
+
+}  // extern "C"
+
+// ---- eager restricted operator (CSR) ------------------------------------
+
+struct tcp_csr_plan {
+    uint32_t n = 0;
+    uint64_t dim = 0, nnz = 0;
+    int device = 0;
+    tcp::DeviceBuffer indptr, cols, vals64, vals32;
+    uint64_t retained_bytes = 0;
+    std::mutex staging_mutex;
+    tcp::DeviceBuffer stage_in, stage_out;
+};
+
+namespace {
+
+constexpr uint32_t kCsrLanes = 8;  // lanes per row in the SpMV (rows average ~27 entries on lattice models)
+
+int build_csr(const tcp_sector_plan *sp, uint64_t max_bytes, tcp_csr_plan **out) {
+    if (sp->leaks) return leak_error(sp);
+    auto plan = new tcp_csr_plan();
+    std::unique_ptr<tcp_csr_plan> guard(plan);
+    plan->n = sp->n;
+    plan->dim = sp->dim;
+    plan->device = sp->device;
+    const uint64_t dim = sp->dim;
+    const SectorArgs a = sp->args(TCP_C128);
+    const uint64_t per_block = static_cast<uint64_t>(kSectorThreads) * kRanksPerThread;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((dim + per_block - 1) / per_block, 256 * 16));
+    const size_t lds = sp->lut_in_lds ? a.lut_words * 4 : 0;
+
+    // pass 1: entries per row; exclusive scan on the host
+    tcp::DeviceBuffer counts;
+    int st;
+    if ((st = counts.allocate(4 * dim))) return st;
+    if (sp->lut_in_lds)
+        hipLaunchKernelGGL(csr_count_kernel<true>, dim3(blocks), dim3(kSectorThreads), lds, nullptr, a,
+                           static_cast<uint32_t *>(counts.ptr));
+    else
+        hipLaunchKernelGGL(csr_count_kernel<false>, dim3(blocks), dim3(kSectorThreads), 0, nullptr, a,
+                           static_cast<uint32_t *>(counts.ptr));
+    TCP_HIP_CHECK(hipGetLastError());
+    std::vector<uint32_t> hcounts(dim);
+    TCP_HIP_CHECK(hipMemcpy(hcounts.data(), counts.ptr, 4 * dim, hipMemcpyDeviceToHost));
+    counts.release();
+    std::vector<uint64_t> hptr(dim + 1, 0);
+    for (uint64_t r = 0; r < dim; ++r) hptr[r + 1] = hptr[r] + hcounts[r];
+    plan->nnz = hptr[dim];
+    plan->retained_bytes = sizeof(tcp_csr_plan) + (dim + 1) * 8 + plan->nnz * (4 + 16 + 8);
+    if (tcp::exceeds_budget(plan->retained_bytes, max_bytes))
+        return tcp::memory_limit(plan->retained_bytes, max_bytes);
+
+    // pass 2: fill rows, columns ascending
+    if ((st = plan->indptr.upload(hptr.data(), 8 * (dim + 1)))) return st;
+    if ((st = plan->cols.allocate(4 * plan->nnz))) return st;
+    if ((st = plan->vals64.allocate(16 * plan->nnz))) return st;
+    if ((st = plan->vals32.allocate(8 * plan->nnz))) return st;
+    if (sp->lut_in_lds)
+        hipLaunchKernelGGL(csr_fill_kernel<true>, dim3(blocks), dim3(kSectorThreads), lds, nullptr, a,
+                           static_cast<const uint64_t *>(plan->indptr.ptr), static_cast<uint32_t *>(plan->cols.ptr),
+                           static_cast<dvec2 *>(plan->vals64.ptr), static_cast<fvec2 *>(plan->vals32.ptr));
+    else
+        hipLaunchKernelGGL(csr_fill_kernel<false>, dim3(blocks), dim3(kSectorThreads), 0, nullptr, a,
+                           static_cast<const uint64_t *>(plan->indptr.ptr), static_cast<uint32_t *>(plan->cols.ptr),
+                           static_cast<dvec2 *>(plan->vals64.ptr), static_cast<fvec2 *>(plan->vals32.ptr));
+    TCP_HIP_CHECK(hipGetLastError());
+    TCP_HIP_CHECK(hipDeviceSynchronize());
+    *out = guard.release();
+    return TCP_OK;
+}
+
+int csr_apply_device(const tcp_csr_plan *plan, const void *in, void *out, int dtype, void *stream) {
+    if (dtype != TCP_C128 && dtype != TCP_C64)
+        return fail(TCP_ERR_INVALID_ARGUMENT, "dtype must be TCP_C128 or TCP_C64");
+    if (!in || !out) return fail(TCP_ERR_INVALID_ARGUMENT, "null state buffer");
+    const size_t bytes = static_cast<size_t>(plan->dim) * (dtype == TCP_C128 ? 16 : 8);
+    if (tcp::ranges_overlap(in, bytes, out, bytes))
+        return fail(TCP_ERR_INVALID_ARGUMENT, "apply_into input and output must not overlap");
+    tcp::DeviceGuard dg;
+    int st = dg.enter(plan->device);
+    if (st) return st;
+    const uint64_t rows_per_block = 256 / kCsrLanes;
+    const uint32_t blocks = static_cast<uint32_t>(
+        std::max<uint64_t>(1, std::min<uint64_t>((plan->dim + rows_per_block - 1) / rows_per_block, 256 * 64)));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint64_t *ip = static_cast<const uint64_t *>(plan->indptr.ptr);
+    const uint32_t *cp = static_cast<const uint32_t *>(plan->cols.ptr);
+    if (dtype == TCP_C128)
+        hipLaunchKernelGGL((csr_spmv_kernel<dvec2, dvec2, kCsrLanes>), dim3(blocks), dim3(256), 0, s, ip, cp,
+                           static_cast<const dvec2 *>(plan->vals64.ptr), static_cast<const dvec2 *>(in),
+                           static_cast<dvec2 *>(out), plan->dim);
+    else
+        hipLaunchKernelGGL((csr_spmv_kernel<fvec2, fvec2, kCsrLanes>), dim3(blocks), dim3(256), 0, s, ip, cp,
+                           static_cast<const fvec2 *>(plan->vals32.ptr), static_cast<const fvec2 *>(in),
+                           static_cast<fvec2 *>(out), plan->dim);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? tcp::ok() : tcp::hip_fail(e, "csr kernel launch");
+}
+
+int csr_from_spec(const tcp::Operator &op, const BlockSpec &spec, bool u1, uint64_t max_bytes, tcp_csr_plan **out) {
+    if (!out) return fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    tcp_sector_plan *sp = nullptr;
+    int st = build_sector_plan(op, spec, u1, max_bytes, &sp, false);
+    if (st) return st;
+    std::unique_ptr<tcp_sector_plan> guard(sp);
+    tcp::DeviceGuard dg;
+    if ((st = dg.enter(sp->device))) return st;
+    return build_csr(sp, max_bytes, out);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tcp_csr_plan_create_u1(const tcp_operator *op, uint32_t particle_number, uint64_t max_bytes, tcp_csr_plan **out) {
+    if (!op) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
+    const tcp::Operator &o = tcp::operator_of(op);
+    if (particle_number > o.nqubits)
+        return fail(TCP_ERR_INVALID_SECTOR, "invalid sector: particle_number must be between 0 and nqubits");
+    const int st = csr_from_spec(o, u1_spec(o.nqubits, particle_number), true, max_bytes, out);
+    return st ? st : tcp::ok();
+}
+
+int tcp_csr_plan_create_blocks(const tcp_operator *op, uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops,
+                               uint64_t max_bytes, tcp_csr_plan **out) {
+    if (!op) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
+    BlockSpec s;
+    int st = blocks_spec(nblocks, sizes, pops, s);
+    if (!st) st = csr_from_spec(tcp::operator_of(op), s, false, max_bytes, out);
+    return st ? st : tcp::ok();
+}
+
+uint64_t tcp_csr_plan_dimension(const tcp_csr_plan *plan) { return plan ? plan->dim : 0; }
+uint64_t tcp_csr_plan_transition_count(const tcp_csr_plan *plan) { return plan ? plan->nnz : 0; }
+uint64_t tcp_csr_plan_estimated_bytes(const tcp_csr_plan *plan) { return plan ? plan->retained_bytes : 0; }
+
+int tcp_csr_plan_apply_into(const tcp_csr_plan *plan, const void *input, void *output, uint64_t input_len,
+                            uint64_t output_len, int dtype, void *stream) {
+    if (!plan) return fail(TCP_ERR_INVALID_ARGUMENT, "null plan");
+    int st;
+    if ((st = check_len(plan->dim, input_len))) return st;
+    if ((st = check_len(plan->dim, output_len))) return st;
+    return csr_apply_device(plan, input, output, dtype, stream);
+}
+
+int tcp_csr_plan_apply_host(const tcp_csr_plan *plan, const double *input, double *output, uint64_t input_len,
+                            uint64_t output_len) {
+    if (!plan) return fail(TCP_ERR_INVALID_ARGUMENT, "null plan");
+    int st;
+    if ((st = check_len(plan->dim, input_len))) return st;
+    if ((st = check_len(plan->dim, output_len))) return st;
+    if (!input || !output) return fail(TCP_ERR_INVALID_ARGUMENT, "null state buffer");
+    const size_t bytes = static_cast<size_t>(plan->dim) * 16;
+    if (tcp::ranges_overlap(input, bytes, output, bytes))
+        return fail(TCP_ERR_INVALID_ARGUMENT, "apply_into input and output must not overlap");
+    tcp::DeviceGuard dg;
+    if ((st = dg.enter(plan->device))) return st;
+    auto *mp = const_cast<tcp_csr_plan *>(plan);
+    std::lock_guard<std::mutex> lock(mp->staging_mutex);
+    if (mp->stage_in.bytes < bytes && (st = mp->stage_in.allocate(bytes))) return st;
+    if (mp->stage_out.bytes < bytes && (st = mp->stage_out.allocate(bytes))) return st;
+    TCP_HIP_CHECK(hipMemcpy(mp->stage_in.ptr, input, bytes, hipMemcpyHostToDevice));
+    if ((st = csr_apply_device(plan, mp->stage_in.ptr, mp->stage_out.ptr, TCP_C128, nullptr))) return st;
+    TCP_HIP_CHECK(hipMemcpy(output, mp->stage_out.ptr, bytes, hipMemcpyDeviceToHost));
+    return tcp::ok();
+}
+
+int tcp_csr_plan_export(const tcp_csr_plan *plan, uint64_t *indptr, uint64_t *indices, double *values) {
+    if (!plan) return fail(TCP_ERR_INVALID_ARGUMENT, "null plan");
+    if (!indptr || (plan->nnz && (!indices || !values))) return fail(TCP_ERR_INVALID_ARGUMENT, "null output array");
+    tcp::DeviceGuard dg;
+    int st = dg.enter(plan->device);
+    if (st) return st;
+    TCP_HIP_CHECK(hipMemcpy(indptr, plan->indptr.ptr, 8 * (plan->dim + 1), hipMemcpyDeviceToHost));
+    if (plan->nnz) {
+        std::vector<uint32_t> c(plan->nnz);
+        TCP_HIP_CHECK(hipMemcpy(c.data(), plan->cols.ptr, 4 * plan->nnz, hipMemcpyDeviceToHost));
+        for (uint64_t k = 0; k < plan->nnz; ++k) indices[k] = c[k];
+        TCP_HIP_CHECK(hipMemcpy(values, plan->vals64.ptr, 16 * plan->nnz, hipMemcpyDeviceToHost));
+    }
+    return tcp::ok();
+}
+
+void tcp_csr_plan_destroy(tcp_csr_plan *plan) { delete plan; }
 
 }  // extern "C"

@@ -80,7 +80,9 @@ class DavidsonResult:
 
 
 def _plan_handle(plan: Any):
-    inner = getattr(plan, "plan", plan)
+    from .symmetry import lazy_plan_of
+
+    inner = lazy_plan_of(plan)
     handle = getattr(inner, "_ptr", None)
     if handle is None:
         raise TypeError("expected a NativeMVPPlan, SectorMvpPlan or restricted operator")

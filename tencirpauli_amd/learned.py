@@ -108,7 +108,11 @@ class LearnedPreconditioner:
     """Features of one restricted operator plus the trained MLP, as a Davidson hook."""
 
     def __init__(self, mlp: MLP, restricted: Any) -> None:
-        plan = getattr(restricted, "plan", restricted)
+        from .symmetry import SectorMvpPlan, lazy_plan_of
+
+        plan = lazy_plan_of(restricted)
+        if not isinstance(plan, SectorMvpPlan):
+            raise TypeError("the learned preconditioner needs a restricted (sector) operator")
         out = ctypes.c_void_p()
         _lib.check(_lib.lib().tcp_learned_precond_create(mlp._ptr, plan._ptr, ctypes.byref(out)))
         self._ptr = out

@@ -162,30 +162,92 @@ class SectorMvpPlan(NativePlanFacade):
 
 
 class U1RestrictedOperator(NativePlanFacade):
-    """A validated Pauli operator restricted to one U(1) sector (py/symmetry.py:244-538)."""
+    """A Pauli operator restricted to one U(1) sector (py/symmetry.py:244-538).
 
-    def __init__(self, sector: U1Sector, operator: Any, plan: SectorMvpPlan, storage: str) -> None:
+    ``storage="lazy"`` keeps the matrix-free GPU plan (exact conservation
+    required up front); ``storage="eager"`` materialises the restricted
+    operator as CSR on the device (no conservation precondition: leakage of any
+    aggregated transition fails the build, as core/sector.rs:345-432 does).
+    """
+
+    def __init__(self, sector: U1Sector, operator: Any, plan: Optional[SectorMvpPlan], storage: str,
+                 csr_plan: Any = None) -> None:
         self.sector = sector
         self.operator = operator
         self.dimension = sector.dimension
         self.storage = storage
         self.term_count = operator.term_count
         self._plan = plan
+        self._csr = csr_plan
         self.native_variant = "qubit"
 
     @property
-    def plan(self) -> SectorMvpPlan:
+    def plan(self) -> Any:
+        return self._csr if self.storage == "eager" else self._plan
+
+    @property
+    def transition_count(self) -> Optional[int]:
+        return None if self._csr is None else self._csr.transition_count
+
+    @property
+    def estimated_bytes(self) -> int:
+        lazy = self.dimension * 16 + (self.dimension + 1) * 8 + self.term_count * (self.sector.nqubits // 8 + 32)
+        return lazy + (0 if self._csr is None else self._csr.transition_count * 32)
+
+    def _ensure_lazy(self, max_bytes: Optional[int]) -> SectorMvpPlan:
+        if self._plan is None:
+            self._plan = _lazy_u1_plan(self.operator, self.sector, max_bytes)
         return self._plan
 
-    def mvp_plan(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> SectorMvpPlan:
+    def _ensure_eager(self, max_bytes: Optional[int]) -> Any:
+        if self._csr is None:
+            from .eager import csr_plan_u1
+
+            self._csr = csr_plan_u1(self.operator, self.sector.particle_number, max_bytes)
+        return self._csr
+
+    def mvp_plan(self, *, storage: str = "lazy", max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        """Reusable restricted plan (py/symmetry.py:344-382)."""
         validate_max_bytes(max_bytes)
-        return self._plan
+        if storage not in ("lazy", "eager"):
+            raise ValueError("storage must be either 'eager' or 'lazy'")
+        return self._ensure_eager(max_bytes) if storage == "eager" else self._ensure_lazy(max_bytes)
+
+    def dense(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> np.ndarray:
+        return self._ensure_eager(max_bytes).materialize("dense", max_bytes)
+
+    def coo(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        return self._ensure_eager(max_bytes).materialize("coo", max_bytes)
+
+    def csr(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        return self._ensure_eager(max_bytes).materialize("csr", max_bytes)
+
+    def compile(self, target: str, *, storage: str = "lazy", max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        """py/symmetry.py:514-538."""
+        if target in ("dense", "coo", "csr"):
+            return getattr(self, target)(max_bytes=max_bytes)
+        if target == "native_mvp":
+            return self.mvp_plan(storage=storage, max_bytes=max_bytes)
+        raise ValueError("target must be one of 'dense', 'coo', 'csr', or 'native_mvp'")
 
     def _native_apply_host(self, inp: np.ndarray, out: np.ndarray) -> int:
-        return self._plan._native_apply_host(inp, out)
+        return self.plan._native_apply_host(inp, out)
 
     def _native_apply_device(self, inp: Any, out: Any, dtype: int, stream: int) -> int:
-        return self._plan._native_apply_device(inp, out, dtype, stream)
+        return self.plan._native_apply_device(inp, out, dtype, stream)
+
+
+def lazy_plan_of(obj: Any) -> Any:
+    """The matrix-free plan behind a plan or restricted operator (what the
+    Davidson solver and the learned preconditioner run on); device CSR plans
+    are rejected."""
+    from .eager import CsrMvpPlan
+
+    if hasattr(obj, "_ensure_lazy"):
+        return obj._ensure_lazy(DEFAULT_MAX_BYTES)
+    if isinstance(obj, CsrMvpPlan):
+        raise TypeError("this solver runs on full-space plans and lazy restricted plans, not on eager CSR plans")
+    return obj
 
 
 def canonical_u1_sector(sector: Any) -> Optional[U1Sector]:
@@ -202,17 +264,26 @@ def canonical_u1_sector(sector: Any) -> Optional[U1Sector]:
     return U1Sector(charge.nqubits, int(requested))
 
 
+def _lazy_u1_plan(operator: Any, sector: U1Sector, max_bytes: Optional[int]) -> SectorMvpPlan:
+    _lib.require_device()
+    out = ctypes.c_void_p()
+    _lib.check(_lib.lib().tcp_sector_plan_create_u1(operator._handle.ptr, sector.particle_number,
+                                                     effective_max_bytes(max_bytes), ctypes.byref(out)))
+    return SectorMvpPlan(out, "qubit", "u1")
+
+
 def restrict_u1(operator: Any, sector: U1Sector, max_bytes: Optional[int], *,
                 storage: str = "lazy") -> U1RestrictedOperator:
-    """py/symmetry.py:658-706 on the GPU; conservation is checked natively."""
+    """py/symmetry.py:658-706 on the GPU; lazy restriction checks conservation
+    natively, eager restriction builds the device CSR."""
     if storage not in ("lazy", "eager"):
         raise ValueError("storage must be either 'eager' or 'lazy'")
     validate_max_bytes(max_bytes)
     if operator.nqubits != sector.nqubits:
         raise ValueError(f"incompatible qubit counts: {operator.nqubits} and {sector.nqubits}")
-    _lib.require_device()
-    out = ctypes.c_void_p()
-    _lib.check(_lib.lib().tcp_sector_plan_create_u1(operator._handle.ptr, sector.particle_number,
-                                                     effective_max_bytes(max_bytes), ctypes.byref(out)))
-    plan = SectorMvpPlan(out, "qubit", "u1")
-    return U1RestrictedOperator(sector, operator, plan, storage)
+    if storage == "eager":
+        from .eager import csr_plan_u1
+
+        return U1RestrictedOperator(sector, operator, None, storage,
+                                    csr_plan_u1(operator, sector.particle_number, max_bytes))
+    return U1RestrictedOperator(sector, operator, _lazy_u1_plan(operator, sector, max_bytes), storage)
