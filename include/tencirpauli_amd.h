@@ -330,18 +330,22 @@ void tcp_batch_plan_destroy(tcp_batch_plan *plan);
  *      eigen entry point is SciPy eigsh over a LinearOperator,
  *      py/integrations/scipy.py:13-47) ------------------------------------- */
 
-/* Per-state log multipliers m_i of the correction t_i = r_i e^{m_i} / (theta - H_ii):
- * returns a device float[dimension] array valid on the solver's stream, or NULL
+/* Learned diagonal preconditioner: per-state log values m_i of the correction
+ * t_i = r_i e^{m_i} (replacing Jacobi's r_i / (theta - H_ii)).  Returns a device
+ * float[dimension] array written in order on `stream` (the solver's), or NULL
  * on failure. */
-typedef const void *(*tcp_log_multiplier_fn)(void *ctx, double theta);
+typedef const void *(*tcp_log_multiplier_fn)(void *ctx, double theta, void *stream);
 
 typedef struct tcp_davidson_options {
-    uint32_t max_subspace;    /* basis size before a thick restart (4..64; default 24) */
-    uint32_t restart_keep;    /* Ritz vectors kept at a restart (default 4) */
+    uint32_t max_subspace;    /* basis size before a thick restart (4..32; default 24) */
+    uint32_t restart_keep;    /* Ritz vectors kept at a restart (1..8, default 4) */
     uint32_t max_iterations;  /* default 1000 */
     double residual_tol;      /* converged when ||H u - theta u|| <= residual_tol (FP64 phase) */
-    int mvp_dtype;            /* TCP_C128, or TCP_C64 for a complex64-storage first phase */
-    double switch_tol;        /* TCP_C64 phase ends (restart with FP64 MVPs) at ||r|| <= switch_tol */
+    int mvp_dtype;            /* TCP_C128, or TCP_C64: a first phase with complex64 basis
+                                 storage and complex64 MVPs (FP64 accumulation throughout) */
+    double switch_tol;        /* the TCP_C64 phase ends at ||r|| <= switch_tol: its Ritz vector
+                                 is formed in FP64 and checked with one FP64 MVP (converged if
+                                 that residual meets residual_tol, else FP64 iterations follow) */
     uint64_t seed;            /* deterministic start vector */
     tcp_log_multiplier_fn log_multiplier_fn; /* NULL: plain Jacobi preconditioner */
     void *log_multiplier_ctx;
@@ -365,17 +369,21 @@ typedef struct tcp_davidson_result {
 
 void tcp_davidson_default_options(tcp_davidson_options *opt);
 
-/* Lowest eigenpair of a restricted / full-space plan; `eigenvector` is an
- * optional device complex128 buffer of `dimension` amplitudes (normalised Ritz
- * vector on return). */
+/* Lowest eigenpair of a restricted (lazy or eager CSR) / full-space plan;
+ * `eigenvector` is an optional device complex128 buffer of `dimension`
+ * amplitudes (normalised Ritz vector on return).  Device memory: two pools of
+ * (max_subspace + restart_keep) complex128 columns. */
 int tcp_davidson_sector(const tcp_sector_plan *plan, const tcp_davidson_options *opt,
                         void *eigenvector, tcp_davidson_result *result);
 int tcp_davidson_full(const tcp_mvp_plan *plan, const tcp_davidson_options *opt, void *eigenvector,
                       tcp_davidson_result *result);
+int tcp_davidson_csr(const tcp_csr_plan *plan, const tcp_davidson_options *opt, void *eigenvector,
+                     tcp_davidson_result *result);
 
 /* Real diagonal H_ii of a plan into a device double[dimension] array. */
 int tcp_sector_plan_diagonal(const tcp_sector_plan *plan, double *diag, void *stream);
 int tcp_mvp_plan_diagonal(const tcp_mvp_plan *plan, double *diag, void *stream);
+int tcp_csr_plan_diagonal(const tcp_csr_plan *plan, double *diag, void *stream);
 
 /* ---- learned preconditioner (no reference counterpart): a 64-128-128-1 tanh
  *      MLP, bf16 weights, FP32 accumulation on MFMA, trained on the GPU ---- */
@@ -399,24 +407,27 @@ int tcp_mlp_gradient(tcp_mlp *mlp, const void *features, uint64_t batch, const f
 int tcp_mlp_adam_step(tcp_mlp *mlp, float lr, float weight_decay, void *stream);
 void tcp_mlp_destroy(tcp_mlp *mlp);
 
-/* Correction-angle loss of one snapshot, L = 1 - <e,t>^2 / (|e|^2 |t - u<u,t>|^2)
- * with t_i = r_i e^{y_i} / (theta - diag_i) (real parts; e orthogonal to u),
- * and dy = weight * dL/dy.  r, e, u: device complex128 [dim]. */
-int tcp_precond_loss_gradient(const float *y, const void *r, const void *e, const void *u,
-                              const double *diag, double theta, uint64_t dim, double weight,
-                              float *dy, double *loss, void *stream);
-
 /* Per-state features of a sector plan (bf16 [dim x 64]) for `mlp`; the
  * callback below is a tcp_log_multiplier_fn for tcp_davidson_options. */
 int tcp_learned_precond_create(const tcp_mlp *mlp, const tcp_sector_plan *plan,
                                tcp_learned_precond **out);
-const void *tcp_learned_precond_log_multiplier(void *ctx, double theta);
+const void *tcp_learned_precond_log_multiplier(void *ctx, double theta, void *stream);
 /* Writes feature 0, (H_ii - theta) / scale, of every state (the callback does
  * this itself; training calls it before tcp_mlp_forward / tcp_mlp_gradient on
  * the features buffer). */
 int tcp_learned_precond_prepare(tcp_learned_precond *precond, double theta, void *stream);
 int tcp_learned_precond_buffers(const tcp_learned_precond *precond, const void **features,
                                 const double **diag, double *scale);
+/* One-step Ritz-gain loss of a Davidson state (normalised Ritz vector u,
+ * Ritz value theta, residual r = H u - theta u; device complex128 [dim]) for the
+ * correction t_i = r_i e^{logm_i}: with q = t - u <u, t>, the lowest Ritz value
+ * of span{u, q} is theta - gain, and loss = -log(gain).  dlogm (device float
+ * [dim]) receives dloss/dlogm.  With logm NULL the Jacobi correction
+ * r_i / (theta - H_ii) is scored instead and dlogm is not written.  The
+ * sector plan the object was created from must still exist. */
+int tcp_learned_precond_ritz_loss(tcp_learned_precond *precond, double theta, const void *u,
+                                  const void *r, const float *logm, float *dlogm, double *loss,
+                                  double *gain, void *stream);
 void tcp_learned_precond_destroy(tcp_learned_precond *precond);
 
 #ifdef __cplusplus

@@ -136,8 +136,10 @@ SIGNATURES = {
     "tcp_davidson_default_options": (None, [_vp]),
     "tcp_davidson_sector": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "tcp_davidson_full": (_c.c_int, [_vp, _vp, _vp, _vp]),
+    "tcp_davidson_csr": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "tcp_sector_plan_diagonal": (_c.c_int, [_vp, _vp, _vp]),
     "tcp_mvp_plan_diagonal": (_c.c_int, [_vp, _vp, _vp]),
+    "tcp_csr_plan_diagonal": (_c.c_int, [_vp, _vp, _vp]),
     "tcp_mlp_parameter_count": (_c.c_uint32, []),
     "tcp_mlp_create": (_c.c_int, [_c.c_uint64, _c.POINTER(_vp)]),
     "tcp_mlp_get_parameters": (_c.c_int, [_vp, _vp]),
@@ -146,10 +148,9 @@ SIGNATURES = {
     "tcp_mlp_gradient": (_c.c_int, [_vp, _vp, _c.c_uint64, _vp, _vp, _vp]),
     "tcp_mlp_adam_step": (_c.c_int, [_vp, _c.c_float, _c.c_float, _vp]),
     "tcp_mlp_destroy": (None, [_vp]),
-    "tcp_precond_loss_gradient": (
-        _c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_double, _c.c_uint64, _c.c_double, _vp, _dp, _vp]),
     "tcp_learned_precond_create": (_c.c_int, [_vp, _vp, _c.POINTER(_vp)]),
-    "tcp_learned_precond_log_multiplier": (_vp, [_vp, _c.c_double]),
+    "tcp_learned_precond_log_multiplier": (_vp, [_vp, _c.c_double, _vp]),
+    "tcp_learned_precond_ritz_loss": (_c.c_int, [_vp, _c.c_double, _vp, _vp, _vp, _vp, _dp, _dp, _vp]),
     "tcp_learned_precond_prepare": (_c.c_int, [_vp, _c.c_double, _vp]),
     "tcp_learned_precond_buffers": (_c.c_int, [_vp, _c.POINTER(_vp), _c.POINTER(_vp), _dp]),
     "tcp_learned_precond_destroy": (None, [_vp]),

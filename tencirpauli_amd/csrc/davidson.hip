@@ -1,18 +1,35 @@
 // Davidson eigensolver for the lowest eigenpair of an MVP plan, on the GPU.
 //
 // No reference counterpart: the reference's eigen entry point is SciPy
-// eigsh over a LinearOperator (py/integrations/scipy.py:13-47).  This solver
-// keeps the search basis V, the products W = H V and every vector operation on
-// the device in FP64 (orthogonalisation, Rayleigh-quotient matrix entries,
-// residual norms); only the m x m projected eigenproblem is solved on the
-// host.  The MVP may run with complex64 storage (FP32 coefficients, FP64
-// accumulation) in a first phase; once the residual reaches the switch
-// tolerance the solver restarts from the Ritz vector with FP64 MVPs, so the
-// final eigenvalue belongs to the FP64 operator.
+// eigsh over a LinearOperator (py/integrations/scipy.py:13-47).
 //
-// Correction: t = P (r), P_i = e^{m_i} / (theta - H_ii) -- the Jacobi
-// (diagonal) preconditioner, optionally scaled by a per-state log multiplier
-// m_i produced by the learned preconditioner for the current theta.
+// Storage and precision.  The search basis V and the products W = H V live on
+// the device in the storage precision of the current phase: complex128, or
+// complex64 in a first phase whose MVPs use FP32 coefficients with FP64
+// accumulation.  Every reduction (residual norms, Gram-Schmidt coefficients,
+// Rayleigh-quotient entries) accumulates in FP64 in a fixed order, and only
+// the m x m projected problem is solved on the host.  When the complex64 phase
+// reaches `switch_tol`, its Ritz vector is formed in FP64 and one FP64 MVP
+// gives the FP64 Rayleigh quotient and residual; the solve ends there if that
+// residual meets `residual_tol`, otherwise FP64 iterations continue from it.
+// Reported eigenvalues of a finished solve therefore belong to the FP64
+// operator.
+//
+// One iteration is three streaming passes plus the MVP, with one host
+// synchronisation:
+//   K1  u = V s, Hu = W s, r = Hu - theta u, t = P r, and the Gram-Schmidt
+//       coefficients <V_j, t> in the same sweep (V_j[i] stays in registers);
+//   K2  t <- t - sum_j <V_j, t> V_j, with |t|^2 and <V_j, t> of the result; a
+//       second projection (K2 again) runs only if the norm fell below
+//       1/sqrt(2) of its value (the test is made on the device);
+//   MVP W_m = H t;
+//   K4  the new column <V_j, W_m> of the projected matrix.
+// Columns are kept unnormalised; their norms live on the host and enter every
+// combination as coefficients.  A thick restart writes the kept Ritz vectors
+// into free column slots (no copies).
+//
+// Correction: t_i = r_i / (theta - H_ii) (Jacobi), or t_i = r_i e^{m_i} with
+// per-state log values m_i from a learned preconditioner.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -26,9 +43,36 @@ typedef double dvec2 __attribute__((ext_vector_type(2)));
 typedef float fvec2 __attribute__((ext_vector_type(2)));
 
 constexpr uint32_t kThreads = 256;
-constexpr uint32_t kChunk = 4096;      // elements per block of the block-dot kernel
+constexpr uint32_t kWaves = kThreads / 64;
 constexpr uint32_t kStreamBlocks = 1024;
-constexpr uint32_t kMaxSubspace = 64;
+constexpr uint32_t kMaxBasis = 32;  // basis columns
+constexpr uint32_t kMaxKeep = 8;    // Ritz vectors kept at a restart
+constexpr double kEta2 = 0.5;  // second projection when |t'|^2 < kEta2 |t|^2
+constexpr uint32_t kStallIterations = 20;  // complex64 phase: iterations without a new best residual
+
+// Device scalar slots (dvec2 each): reductions land here, kernels read their
+// coefficients from here, and one copy per iteration brings them to the host.
+constexpr uint32_t kScR2 = 0;                    // |r|^2
+constexpr uint32_t kScT2 = 1;                    // |t|^2 after preconditioning
+constexpr uint32_t kScC = 2;                     // <V_j, t>, j < m
+constexpr uint32_t kScP1 = kScC + kMaxBasis;     // |t'|^2, then <V_j, t'>
+constexpr uint32_t kScP2 = kScP1 + 1 + kMaxBasis;  // |t''|^2, then <V_j, t''>
+constexpr uint32_t kScE = kScP2 + 1 + kMaxBasis;   // <V_j, W_m>, j <= m
+constexpr uint32_t kScX = kScE + kMaxBasis + 1;    // three spare slots (pair dots)
+constexpr uint32_t kScCount = kScX + 3;
+
+__device__ __forceinline__ dvec2 widen(dvec2 x) { return x; }
+__device__ __forceinline__ dvec2 widen(fvec2 x) { return dvec2{x.x, x.y}; }
+template <typename ST>
+__device__ __forceinline__ ST narrow(dvec2 x);
+template <>
+__device__ __forceinline__ dvec2 narrow<dvec2>(dvec2 x) {
+    return x;
+}
+template <>
+__device__ __forceinline__ fvec2 narrow<fvec2>(dvec2 x) {
+    return fvec2{static_cast<float>(x.x), static_cast<float>(x.y)};
+}
 
 __device__ __forceinline__ dvec2 cmul(dvec2 a, dvec2 b) {
     return dvec2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
@@ -37,141 +81,228 @@ __device__ __forceinline__ dvec2 cmulc(dvec2 a, dvec2 b) {  // conj(a) * b
     return dvec2{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
 }
 
-// Sum over the block; the result is valid in thread 0.
-__device__ __forceinline__ dvec2 block_sum(dvec2 v, dvec2 *scratch) {
-    for (int off = 32; off > 0; off >>= 1) {
-        v.x += __shfl_xor(v.x, off);
-        v.y += __shfl_xor(v.y, off);
-    }
+template <typename ST>
+struct Cols {
+    ST *p[kMaxBasis + 1];
+};
+struct Coef {
+    dvec2 c[kMaxBasis];
+};
+struct Sig2 {
+    double s[kMaxBasis];
+};
+
+// Per-block sums of acc[0..nval) into partial[block * nval + j].
+template <int NV>
+__device__ __forceinline__ void block_partials(dvec2 (&acc)[NV], uint32_t nval, dvec2 *__restrict__ partial) {
+    __shared__ dvec2 red[kWaves][NV];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) scratch[wave] = v;
-    __syncthreads();
-    dvec2 total{0.0, 0.0};
-    if (threadIdx.x == 0)
-        for (uint32_t w = 0; w < blockDim.x / 64; ++w) total += scratch[w];
-    __syncthreads();
-    return total;
-}
-
-// partial[b * m + j] = sum over chunk b of conj(V_j[i]) w[i]
-__global__ __launch_bounds__(kThreads) void dot_partials_kernel(const dvec2 *__restrict__ V, uint32_t m,
-                                                                uint64_t ld, const dvec2 *__restrict__ w,
-                                                                uint64_t D, dvec2 *__restrict__ partial) {
-    __shared__ dvec2 ws[kChunk];
-    __shared__ dvec2 red[kThreads / 64];
-    const uint64_t i0 = static_cast<uint64_t>(blockIdx.x) * kChunk;
-    const uint32_t len = static_cast<uint32_t>(D - i0 < kChunk ? D - i0 : kChunk);
-    for (uint32_t e = threadIdx.x; e < len; e += kThreads) ws[e] = w[i0 + e];
-    __syncthreads();
-    for (uint32_t j = 0; j < m; ++j) {
-        const dvec2 *vj = V + j * ld + i0;
-        dvec2 acc{0.0, 0.0};
-        for (uint32_t e = threadIdx.x; e < len; e += kThreads) acc += cmulc(vj[e], ws[e]);
-        acc = block_sum(acc, red);
-        if (threadIdx.x == 0) partial[static_cast<uint64_t>(blockIdx.x) * m + j] = acc;
-    }
-}
-
-// out[j] = sum over blocks (fixed order: deterministic)
-__global__ void reduce_partials_kernel(const dvec2 *__restrict__ partial, uint32_t nblk, uint32_t m,
-                                       dvec2 *__restrict__ out) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= m) return;
-    dvec2 s{0.0, 0.0};
-    for (uint32_t b = 0; b < nblk; ++b) s += partial[static_cast<uint64_t>(b) * m + j];
-    out[j] = s;
-}
-
-// out = beta * out + sum_j c_j V_j
-__global__ __launch_bounds__(kThreads) void combine_kernel(const dvec2 *__restrict__ V, uint32_t m, uint64_t ld,
-                                                           const dvec2 *__restrict__ c, double beta,
-                                                           dvec2 *__restrict__ out, uint64_t D) {
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        dvec2 acc = beta != 0.0 ? out[i] * beta : dvec2{0.0, 0.0};
-        for (uint32_t j = 0; j < m; ++j) acc += cmul(c[j], V[j * ld + i]);
-        out[i] = acc;
-    }
-}
-
-// u = V s, hu = W s, r = hu - theta u; partial[b] = block sum of |r|^2
-__global__ __launch_bounds__(kThreads) void residual_kernel(const dvec2 *__restrict__ V,
-                                                            const dvec2 *__restrict__ W, uint32_t m,
-                                                            uint64_t ld, const dvec2 *__restrict__ s,
-                                                            double theta, dvec2 *__restrict__ u,
-                                                            dvec2 *__restrict__ hu, dvec2 *__restrict__ r,
-                                                            uint64_t D, double *__restrict__ partial) {
-    __shared__ dvec2 red[kThreads / 64];
-    double local = 0.0;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        dvec2 uu{0.0, 0.0}, hh{0.0, 0.0};
-        for (uint32_t j = 0; j < m; ++j) {
-            uu += cmul(s[j], V[j * ld + i]);
-            hh += cmul(s[j], W[j * ld + i]);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        if (j < static_cast<int>(nval)) {
+            dvec2 v = acc[j];
+            for (int off = 32; off > 0; off >>= 1) {
+                v.x += __shfl_xor(v.x, off);
+                v.y += __shfl_xor(v.y, off);
+            }
+            if (lane == 0) red[wave][j] = v;
         }
-        const dvec2 rr = hh - uu * theta;
-        u[i] = uu;
-        hu[i] = hh;
-        r[i] = rr;
-        local += rr.x * rr.x + rr.y * rr.y;
     }
-    const dvec2 total = block_sum(dvec2{local, 0.0}, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = total.x;
-}
-
-__global__ __launch_bounds__(kThreads) void norm_kernel(const dvec2 *__restrict__ x, uint64_t D,
-                                                        double *__restrict__ partial) {
-    __shared__ dvec2 red[kThreads / 64];
-    double local = 0.0;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-        local += x[i].x * x[i].x + x[i].y * x[i].y;
-    const dvec2 total = block_sum(dvec2{local, 0.0}, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = total.x;
-}
-
-__global__ void scale_kernel(dvec2 *__restrict__ x, double alpha, uint64_t D) {
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-        x[i] *= alpha;
-}
-
-// t_i = r_i e^{m_i} / (theta - d_i), the denominator kept away from zero.
-__global__ void precondition_kernel(const dvec2 *__restrict__ r, const double *__restrict__ diag,
-                                    const float *__restrict__ logm, double theta, double floor_,
-                                    dvec2 *__restrict__ t, uint64_t D) {
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        double den = theta - diag[i];
-        if (fabs(den) < floor_) den = den < 0.0 ? -floor_ : (den > 0.0 ? floor_ : -floor_);
-        double scale = 1.0 / den;
-        if (logm) scale *= exp(static_cast<double>(logm[i]));
-        t[i] = r[i] * scale;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < nval; j += kThreads) {
+        dvec2 s = red[0][j];
+        for (uint32_t w = 1; w < kWaves; ++w) s += red[w][j];
+        partial[static_cast<uint64_t>(blockIdx.x) * nval + j] = s;
     }
 }
 
-__global__ void to_c64_kernel(const dvec2 *__restrict__ x, fvec2 *__restrict__ y, uint64_t D) {
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-        y[i] = fvec2{static_cast<float>(x[i].x), static_cast<float>(x[i].y)};
+// Second projection gate: true when |t'|^2 >= kEta2 |t|^2 (no second pass).
+__device__ __forceinline__ bool gate_skips(const dvec2 *__restrict__ sc) {
+    return sc[kScP1].x >= kEta2 * sc[kScT2].x;
 }
 
-__global__ void to_c128_kernel(const fvec2 *__restrict__ x, dvec2 *__restrict__ y, uint64_t D) {
+// out[j] = sum over blocks of partial[b * nval + j], fixed order (deterministic).
+__global__ __launch_bounds__(kThreads) void reduce_kernel(const dvec2 *__restrict__ partial, uint32_t nblk,
+                                                          uint32_t nval, dvec2 *__restrict__ out,
+                                                          const dvec2 *__restrict__ gate) {
+    if (gate && gate_skips(gate)) return;
+    __shared__ dvec2 red[kThreads];
+    const uint32_t j = blockIdx.x;
+    dvec2 s{0.0, 0.0};
+    for (uint32_t b = threadIdx.x; b < nblk; b += kThreads) s += partial[static_cast<uint64_t>(b) * nval + j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = red[0];
+}
+
+// K1: residual, preconditioner and the first Gram-Schmidt coefficients.
+// Partials: [|r|^2, |t|^2, <V_0, t>, ..., <V_{m-1}, t>].
+template <typename ST, int MAXM, bool LEARNED>
+__global__ __launch_bounds__(kThreads) void residual_kernel(Cols<ST> V, Cols<ST> W, uint32_t m, Coef s,
+                                                            double theta, const double *__restrict__ diag,
+                                                            const float *__restrict__ logm, ST *__restrict__ t,
+                                                            uint64_t D, dvec2 *__restrict__ partial) {
+    dvec2 acc[MAXM + 2];
+#pragma unroll
+    for (int j = 0; j < MAXM + 2; ++j) acc[j] = dvec2{0.0, 0.0};
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
+         i += static_cast<uint64_t>(gridDim.x) * kThreads) {
+        ST vv[MAXM];
+        dvec2 u{0.0, 0.0}, hu{0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < MAXM; ++j) {
+            if (j < static_cast<int>(m)) {
+                vv[j] = V.p[j][i];
+                u += cmul(s.c[j], widen(vv[j]));
+                hu += cmul(s.c[j], widen(W.p[j][i]));
+            }
+        }
+        const dvec2 r = hu - u * theta;
+        double scale;
+        if (LEARNED) {
+            scale = exp(static_cast<double>(logm[i]));
+        } else {
+            double den = theta - diag[i];
+            if (fabs(den) < 1e-8) den = den < 0.0 ? -1e-8 : 1e-8;
+            scale = 1.0 / den;
+        }
+        const ST ts = narrow<ST>(r * scale);
+        t[i] = ts;
+        const dvec2 tw = widen(ts);
+        acc[0].x += r.x * r.x + r.y * r.y;
+        acc[1].x += tw.x * tw.x + tw.y * tw.y;
+#pragma unroll
+        for (int j = 0; j < MAXM; ++j)
+            if (j < static_cast<int>(m)) acc[2 + j] += cmulc(widen(vv[j]), tw);
+    }
+    block_partials<MAXM + 2>(acc, m + 2, partial);
+}
+
+// K2: t <- t - sum_j sig2_j c_j V_j with c_j = sc[coef + j]; partials
+// [|t|^2, <V_0, t>, ..., <V_{m-1}, t>].  With `gate`, does nothing when the
+// second projection is not needed.
+template <typename ST, int MAXM>
+__global__ __launch_bounds__(kThreads) void project_kernel(Cols<ST> V, uint32_t m, Sig2 sig2,
+                                                           const dvec2 *__restrict__ sc, uint32_t coef,
+                                                           ST *__restrict__ t, uint64_t D,
+                                                           dvec2 *__restrict__ partial, bool gate) {
+    if (gate && gate_skips(sc)) return;
+    __shared__ dvec2 c[MAXM];  // uniform coefficients: LDS broadcasts, not per-lane registers
+    if (threadIdx.x < MAXM)
+        c[threadIdx.x] = threadIdx.x < m ? sc[coef + threadIdx.x] * sig2.s[threadIdx.x] : dvec2{0.0, 0.0};
+    __syncthreads();
+    dvec2 acc[MAXM + 1];
+#pragma unroll
+    for (int j = 0; j < MAXM + 1; ++j) acc[j] = dvec2{0.0, 0.0};
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
+         i += static_cast<uint64_t>(gridDim.x) * kThreads) {
+        ST vv[MAXM];
+        dvec2 x = widen(t[i]);
+#pragma unroll
+        for (int j = 0; j < MAXM; ++j) {
+            if (j < static_cast<int>(m)) {
+                vv[j] = V.p[j][i];
+                x -= cmul(c[j], widen(vv[j]));
+            }
+        }
+        const ST xs = narrow<ST>(x);
+        t[i] = xs;
+        const dvec2 xw = widen(xs);
+        acc[0].x += xw.x * xw.x + xw.y * xw.y;
+#pragma unroll
+        for (int j = 0; j < MAXM; ++j)
+            if (j < static_cast<int>(m)) acc[1 + j] += cmulc(widen(vv[j]), xw);
+    }
+    block_partials<MAXM + 1>(acc, m + 1, partial);
+}
+
+// K4: partials <V_j, w>, j = 0..m (V has m + 1 columns here).
+template <typename ST, int MAXM>
+__global__ __launch_bounds__(kThreads) void column_kernel(Cols<ST> V, uint32_t ncols, const ST *__restrict__ w,
+                                                          uint64_t D, dvec2 *__restrict__ partial) {
+    dvec2 acc[MAXM + 1];
+#pragma unroll
+    for (int j = 0; j < MAXM + 1; ++j) acc[j] = dvec2{0.0, 0.0};
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
+         i += static_cast<uint64_t>(gridDim.x) * kThreads) {
+        const dvec2 wi = widen(w[i]);
+#pragma unroll
+        for (int j = 0; j < MAXM + 1; ++j)
+            if (j < static_cast<int>(ncols)) acc[j] += cmulc(widen(V.p[j][i]), wi);
+    }
+    block_partials<MAXM + 1>(acc, ncols, partial);
+}
+
+// Partials [<a, a>, <a, b>, <b, b>] of two vectors.
+template <typename ST>
+__global__ __launch_bounds__(kThreads) void pair_kernel(const ST *__restrict__ a, const ST *__restrict__ b, uint64_t D,
+                                                        dvec2 *__restrict__ partial) {
+    dvec2 acc[3] = {dvec2{0.0, 0.0}, dvec2{0.0, 0.0}, dvec2{0.0, 0.0}};
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
+         i += static_cast<uint64_t>(gridDim.x) * kThreads) {
+        const dvec2 x = widen(a[i]), y = widen(b[i]);
+        acc[0].x += x.x * x.x + x.y * x.y;
+        acc[1] += cmulc(x, y);
+        acc[2].x += y.x * y.x + y.y * y.y;
+    }
+    block_partials<3>(acc, 3, partial);
+}
+
+// Partials [|b - theta a|^2]: the explicit residual norm of a Rayleigh pair.
+__global__ __launch_bounds__(kThreads) void pair_residual_kernel(const dvec2 *__restrict__ a,
+                                                                 const dvec2 *__restrict__ b, double theta,
+                                                                 uint64_t D, dvec2 *__restrict__ partial) {
+    dvec2 acc[1] = {dvec2{0.0, 0.0}};
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
+         i += static_cast<uint64_t>(gridDim.x) * kThreads) {
+        const dvec2 r = b[i] - a[i] * theta;
+        acc[0].x += r.x * r.x + r.y * r.y;
+    }
+    block_partials<1>(acc, 1, partial);
+}
+
+// out_k = sum_j Y[k * m + j] in_j for k < K (the kept Ritz vectors of a
+// restart, or one Ritz vector formed in another precision).
+template <typename SI, typename SO, int MAXM>
+__global__ __launch_bounds__(kThreads) void combine_kernel(Cols<SI> in, uint32_t m, const dvec2 *__restrict__ Y,
+                                                           uint32_t K, Cols<SO> out, uint64_t D) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
+         i += static_cast<uint64_t>(gridDim.x) * kThreads) {
+        dvec2 x[MAXM];
+#pragma unroll
+        for (int j = 0; j < MAXM; ++j)
+            if (j < static_cast<int>(m)) x[j] = widen(in.p[j][i]);
+        for (uint32_t k = 0; k < K; ++k) {
+            dvec2 acc{0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < MAXM; ++j)
+                if (j < static_cast<int>(m)) acc += cmul(Y[k * m + j], x[j]);
+            out.p[k][i] = narrow<SO>(acc);
+        }
+    }
+}
+
+template <typename ST>
+__global__ void scale_kernel(ST *__restrict__ x, double alpha, uint64_t D) {
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
          i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-        y[i] = dvec2{static_cast<double>(x[i].x), static_cast<double>(x[i].y)};
+        x[i] = narrow<ST>(widen(x[i]) * alpha);
 }
 
 // Deterministic start vector: splitmix64 hash of (seed, i) -> U(-1, 1), real.
-__global__ void init_kernel(dvec2 *__restrict__ x, uint64_t seed, uint64_t D) {
+template <typename ST>
+__global__ void init_kernel(ST *__restrict__ x, uint64_t seed, uint64_t D) {
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
          i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
         uint64_t z = seed + 0x9e3779b97f4a7c15ull * (i + 1);
         z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
         z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
         z ^= z >> 31;
-        x[i] = dvec2{static_cast<double>(z >> 11) * (2.0 / 9007199254740992.0) - 1.0, 0.0};
+        x[i] = narrow<ST>(dvec2{static_cast<double>(z >> 11) * (2.0 / 9007199254740992.0) - 1.0, 0.0});
     }
 }
 
@@ -295,115 +426,207 @@ void hermitian_lowest(int m, const std::vector<double> &g, int count, std::vecto
     }
 }
 
-struct Work {
-    uint64_t D = 0;
-    uint32_t nblk_dot = 0;
-    tcp::DeviceBuffer V, W, Vn, Wn, u, hu, r, x32, y32, diag, partial, small, out;
-    std::vector<double> partial_host;
-};
+// The smallest register-array bound that holds m columns.
+template <class F>
+int with_maxm(uint32_t m, F &&f) {
+    if (m <= 8) return f(std::integral_constant<int, 8>());
+    if (m <= 16) return f(std::integral_constant<int, 16>());
+    if (m <= 24) return f(std::integral_constant<int, 24>());
+    return f(std::integral_constant<int, 32>());
+}
 
 template <class Plan>
 struct Solver {
     const Plan *plan;
     const tcp_davidson_options &opt;
     hipStream_t stream;
-    Work w;
-    dvec2 *Vp(uint32_t j) { return static_cast<dvec2 *>(w.V.ptr) + j * w.D; }
-    dvec2 *Wp(uint32_t j) { return static_cast<dvec2 *>(w.W.ptr) + j * w.D; }
+    uint64_t D = 0;
+    uint32_t nb = 0;
+    // column slot pools (each slot holds a complex128 column; complex64
+    // columns use its first half)
+    tcp::DeviceBuffer vpool, wpool, diag, partial, sc, ybuf, eig;
+    std::vector<uint32_t> vslot, wslot;  // column j -> slot
+    std::vector<uint32_t> vfree, wfree;
+    std::vector<double> sig;  // column j holds sig[j]^-1 times its normalised vector
+    std::vector<dvec2> hsc;
+    int phase = TCP_C128;
+    tcp_davidson_result *res = nullptr;
 
     Solver(const Plan *p, const tcp_davidson_options &o, hipStream_t s) : plan(p), opt(o), stream(s) {}
 
-    int dots(const dvec2 *V, uint32_t m, const dvec2 *x, std::vector<double> &out) {
-        hipLaunchKernelGGL(dot_partials_kernel, dim3(w.nblk_dot), dim3(kThreads), 0, stream, V, m, w.D, x, w.D,
-                           static_cast<dvec2 *>(w.partial.ptr));
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kMaxSubspace), 0, stream,
-                           static_cast<const dvec2 *>(w.partial.ptr), w.nblk_dot, m,
-                           static_cast<dvec2 *>(w.out.ptr));
-        out.resize(2 * m);
-        TCP_HIP_CHECK(hipMemcpyAsync(out.data(), w.out.ptr, 16 * m, hipMemcpyDeviceToHost, stream));
-        TCP_HIP_CHECK(hipStreamSynchronize(stream));
-        return TCP_OK;
+    void *vptr(uint32_t slot) { return static_cast<char *>(vpool.ptr) + static_cast<size_t>(slot) * 16 * D; }
+    void *wptr(uint32_t slot) { return static_cast<char *>(wpool.ptr) + static_cast<size_t>(slot) * 16 * D; }
+    dvec2 *scp(uint32_t k) { return static_cast<dvec2 *>(sc.ptr) + k; }
+    dvec2 *part() { return static_cast<dvec2 *>(partial.ptr); }
+
+    template <typename ST>
+    Cols<ST> vcols(uint32_t n) {
+        Cols<ST> c{};
+        for (uint32_t j = 0; j < n; ++j) c.p[j] = static_cast<ST *>(vptr(vslot[j]));
+        return c;
     }
-    int norm(const dvec2 *x, double &value) {
-        const uint32_t nb = stream_blocks(w.D);
-        hipLaunchKernelGGL(norm_kernel, dim3(nb), dim3(kThreads), 0, stream, x, w.D,
-                           static_cast<double *>(w.partial.ptr));
-        w.partial_host.resize(nb);
-        TCP_HIP_CHECK(hipMemcpyAsync(w.partial_host.data(), w.partial.ptr, 8 * nb, hipMemcpyDeviceToHost, stream));
-        TCP_HIP_CHECK(hipStreamSynchronize(stream));
-        double s = 0.0;
-        for (double v : w.partial_host) s += v;
-        value = std::sqrt(s);
-        return TCP_OK;
+    template <typename ST>
+    Cols<ST> wcols(uint32_t n) {
+        Cols<ST> c{};
+        for (uint32_t j = 0; j < n; ++j) c.p[j] = static_cast<ST *>(wptr(wslot[j]));
+        return c;
     }
-    int upload_small(const std::vector<double> &c) {
-        TCP_HIP_CHECK(hipMemcpyAsync(w.small.ptr, c.data(), 8 * c.size(), hipMemcpyHostToDevice, stream));
-        return TCP_OK;
-    }
-    int combine(const dvec2 *V, uint32_t m, const std::vector<double> &c, double beta, dvec2 *out) {
-        int st = upload_small(c);
-        if (st) return st;
-        hipLaunchKernelGGL(combine_kernel, dim3(stream_blocks(w.D)), dim3(kThreads), 0, stream, V, m, w.D,
-                           static_cast<const dvec2 *>(w.small.ptr), beta, out, w.D);
-        return TCP_OK;
-    }
-    // y = H x in the requested precision (complex128 in and out).
-    int apply(const dvec2 *x, dvec2 *y, int dtype, tcp_davidson_result &res) {
-        if (dtype == TCP_C64) {
-            hipLaunchKernelGGL(to_c64_kernel, dim3(stream_blocks(w.D)), dim3(kThreads), 0, stream, x,
-                               static_cast<fvec2 *>(w.x32.ptr), w.D);
-            int st = tcp::plan_apply(plan, w.x32.ptr, w.y32.ptr, TCP_C64, stream);
-            if (st) return st;
-            hipLaunchKernelGGL(to_c128_kernel, dim3(stream_blocks(w.D)), dim3(kThreads), 0, stream,
-                               static_cast<const fvec2 *>(w.y32.ptr), y, w.D);
-            ++res.mvps_c64;
-            return TCP_OK;
-        }
-        ++res.mvps_c128;
-        return tcp::plan_apply(plan, x, y, TCP_C128, stream);
+    uint32_t take(std::vector<uint32_t> &pool) {
+        const uint32_t s = pool.back();
+        pool.pop_back();
+        return s;
     }
 
-    int run(void *evec_out, tcp_davidson_result &res) {
+    int reduce(uint32_t nval, uint32_t dst, bool gated) {
+        hipLaunchKernelGGL(reduce_kernel, dim3(nval), dim3(kThreads), 0, stream, part(), nb, nval, scp(dst),
+                           gated ? scp(0) : nullptr);
+        return TCP_OK;
+    }
+    int fetch() {
+        hsc.resize(kScCount);
+        TCP_HIP_CHECK(hipMemcpyAsync(hsc.data(), sc.ptr, sizeof(dvec2) * kScCount, hipMemcpyDeviceToHost, stream));
+        TCP_HIP_CHECK(hipStreamSynchronize(stream));
+        return TCP_OK;
+    }
+    int apply(const void *x, void *y, tcp_davidson_result &r) {
+        if (phase == TCP_C64)
+            ++r.mvps_c64;
+        else
+            ++r.mvps_c128;
+        return tcp::plan_apply(plan, x, y, phase, stream);
+    }
+    // <a, a>, <a, b>, <b, b> into sc[kScX..kScX+3)
+    template <typename ST>
+    int pair_dots(const void *a, const void *b) {
+        hipLaunchKernelGGL((pair_kernel<ST>), dim3(nb), dim3(kThreads), 0, stream, static_cast<const ST *>(a),
+                           static_cast<const ST *>(b), D, part());
+        return reduce(3, kScX, false);
+    }
+    int pair_dots_phase(const void *a, const void *b) {
+        return phase == TCP_C64 ? pair_dots<fvec2>(a, b) : pair_dots<dvec2>(a, b);
+    }
+    // out_k = sum_j Y[k][j] * (column j of the basis, normalised), k < K
+    template <typename SI, typename SO>
+    int combine(uint32_t m, const std::vector<dvec2> &Y, uint32_t K, const Cols<SO> &out) {
+        TCP_HIP_CHECK(hipMemcpyAsync(ybuf.ptr, Y.data(), sizeof(dvec2) * Y.size(), hipMemcpyHostToDevice, stream));
+        const Cols<SI> in = vcols<SI>(m);
+        return with_maxm(m, [&](auto MM) {
+            hipLaunchKernelGGL((combine_kernel<SI, SO, decltype(MM)::value>), dim3(nb), dim3(kThreads), 0, stream, in,
+                               m, static_cast<const dvec2 *>(ybuf.ptr), K, out, D);
+            return TCP_OK;
+        });
+    }
+    template <typename SI, typename SO>
+    int combine_w(uint32_t m, const std::vector<dvec2> &Y, uint32_t K, const Cols<SO> &out) {
+        TCP_HIP_CHECK(hipMemcpyAsync(ybuf.ptr, Y.data(), sizeof(dvec2) * Y.size(), hipMemcpyHostToDevice, stream));
+        const Cols<SI> in = wcols<SI>(m);
+        return with_maxm(m, [&](auto MM) {
+            hipLaunchKernelGGL((combine_kernel<SI, SO, decltype(MM)::value>), dim3(nb), dim3(kThreads), 0, stream, in,
+                               m, static_cast<const dvec2 *>(ybuf.ptr), K, out, D);
+            return TCP_OK;
+        });
+    }
+
+    // One iteration's device work after the projected solve: K1, K2 (x2,
+    // gated), MVP, K4.  Column m is the new direction.
+    template <typename ST>
+    int iterate(uint32_t m, const Coef &s, double theta, const float *logm) {
+        const Cols<ST> V = vcols<ST>(m + 1), W = wcols<ST>(m + 1);
+        ST *t = V.p[m];
+        Sig2 sig2{};
+        for (uint32_t j = 0; j < m; ++j) sig2.s[j] = sig[j] * sig[j];
+        const double *dg = static_cast<const double *>(diag.ptr);
+        int st = with_maxm(m, [&](auto MM) {
+            constexpr int M = decltype(MM)::value;
+            if (logm)
+                hipLaunchKernelGGL((residual_kernel<ST, M, true>), dim3(nb), dim3(kThreads), 0, stream, V, W, m, s,
+                                   theta, dg, logm, t, D, part());
+            else
+                hipLaunchKernelGGL((residual_kernel<ST, M, false>), dim3(nb), dim3(kThreads), 0, stream, V, W, m, s,
+                                   theta, dg, logm, t, D, part());
+            reduce(m + 2, kScR2, false);
+            hipLaunchKernelGGL((project_kernel<ST, M>), dim3(nb), dim3(kThreads), 0, stream, V, m, sig2, scp(0), kScC,
+                               t, D, part(), false);
+            reduce(m + 1, kScP1, false);
+            hipLaunchKernelGGL((project_kernel<ST, M>), dim3(nb), dim3(kThreads), 0, stream, V, m, sig2, scp(0),
+                               kScP1 + 1, t, D, part(), true);
+            reduce(m + 1, kScP2, true);
+            return TCP_OK;
+        });
+        if (st) return st;
+        if ((st = apply(t, W.p[m], *res))) return st;
+        return with_maxm(m + 1, [&](auto MM) {
+            constexpr int M = decltype(MM)::value;
+            hipLaunchKernelGGL((column_kernel<ST, M>), dim3(nb), dim3(kThreads), 0, stream, V, m + 1, W.p[m], D,
+                               part());
+            return reduce(m + 1, kScE, false);
+        });
+    }
+
+    int run(void *evec_out, tcp_davidson_result &result) {
         using clock = std::chrono::steady_clock;
         const auto t0 = clock::now();
-        const uint64_t D = tcp::plan_dimension(plan);
-        w.D = D;
+        res = &result;
+        D = tcp::plan_dimension(plan);
+        nb = stream_blocks(D);
         const uint32_t mmax = std::min<uint32_t>(std::max<uint32_t>(opt.max_subspace ? opt.max_subspace : 24, 4),
-                                                 kMaxSubspace);
-        const uint32_t keep = std::min<uint32_t>(std::max<uint32_t>(opt.restart_keep ? opt.restart_keep : 4, 1),
-                                                 mmax - 2);
+                                                 kMaxBasis);
+        const uint32_t keep = std::min<uint32_t>(
+            std::min<uint32_t>(std::max<uint32_t>(opt.restart_keep ? opt.restart_keep : 4, 1), mmax - 2), kMaxKeep);
         const uint32_t max_it = opt.max_iterations ? opt.max_iterations : 1000;
         const double tol = opt.residual_tol > 0 ? opt.residual_tol : 1e-8;
-        int phase = opt.mvp_dtype == TCP_C64 ? TCP_C64 : TCP_C128;
         const double switch_tol = opt.switch_tol > 0 ? opt.switch_tol : 1e-4;
-        w.nblk_dot = static_cast<uint32_t>((D + kChunk - 1) / kChunk);
+        phase = opt.mvp_dtype == TCP_C64 ? TCP_C64 : TCP_C128;
+        const uint32_t slots = mmax + keep;
         int st;
-        const size_t vb = static_cast<size_t>(D) * 16;
-        if ((st = w.V.allocate(vb * mmax)) || (st = w.W.allocate(vb * mmax)) || (st = w.Vn.allocate(vb * keep)) ||
-            (st = w.Wn.allocate(vb * keep)) || (st = w.u.allocate(vb)) || (st = w.hu.allocate(vb)) ||
-            (st = w.r.allocate(vb)) || (st = w.diag.allocate(8 * D)) ||
-            (st = w.partial.allocate(16 * std::max<uint64_t>(static_cast<uint64_t>(w.nblk_dot) * mmax,
-                                                               kStreamBlocks))) ||
-            (st = w.small.allocate(16 * kMaxSubspace)) || (st = w.out.allocate(16 * kMaxSubspace)))
+        if ((st = vpool.allocate(static_cast<size_t>(16) * D * slots)) ||
+            (st = wpool.allocate(static_cast<size_t>(16) * D * slots)) || (st = diag.allocate(8 * D)) ||
+            (st = partial.allocate(sizeof(dvec2) * static_cast<size_t>(nb) * (kMaxBasis + 2))) ||
+            (st = sc.allocate(sizeof(dvec2) * kScCount)) ||
+            (st = ybuf.allocate(sizeof(dvec2) * kMaxBasis * kMaxKeep)))
             return st;
-        if (phase == TCP_C64 && ((st = w.x32.allocate(vb / 2)) || (st = w.y32.allocate(vb / 2)))) return st;
-        if ((st = tcp::plan_diagonal(plan, static_cast<double *>(w.diag.ptr), stream))) return st;
+        TCP_HIP_CHECK(hipMemsetAsync(sc.ptr, 0, sizeof(dvec2) * kScCount, stream));
+        if ((st = tcp::plan_diagonal(plan, static_cast<double *>(diag.ptr), stream))) return st;
+        for (uint32_t k = slots; k-- > 0;) {
+            vfree.push_back(k);
+            wfree.push_back(k);
+        }
 
-        const uint32_t nb = stream_blocks(D);
-        hipLaunchKernelGGL(init_kernel, dim3(nb), dim3(kThreads), 0, stream, Vp(0), opt.seed, D);
-        double nv;
-        if ((st = norm(Vp(0), nv))) return st;
-        hipLaunchKernelGGL(scale_kernel, dim3(nb), dim3(kThreads), 0, stream, Vp(0), 1.0 / nv, D);
-        if ((st = apply(Vp(0), Wp(0), phase, res))) return st;
-        std::vector<double> G(2 * kMaxSubspace * kMaxSubspace, 0.0), col, vals, vecs;
-        auto Gat = [&](uint32_t i, uint32_t j) -> double * { return &G[2 * (i * kMaxSubspace + j)]; };
-        if ((st = dots(Vp(0), 1, Wp(0), col))) return st;
-        Gat(0, 0)[0] = col[0];
-        Gat(0, 0)[1] = 0.0;
+        // start vector and its product
+        vslot.assign(1, take(vfree));
+        wslot.assign(1, take(wfree));
+        if (phase == TCP_C64)
+            hipLaunchKernelGGL((init_kernel<fvec2>), dim3(nb), dim3(kThreads), 0, stream,
+                               static_cast<fvec2 *>(vptr(vslot[0])), opt.seed, D);
+        else
+            hipLaunchKernelGGL((init_kernel<dvec2>), dim3(nb), dim3(kThreads), 0, stream,
+                               static_cast<dvec2 *>(vptr(vslot[0])), opt.seed, D);
+        if ((st = apply(vptr(vslot[0]), wptr(wslot[0]), result))) return st;
+        if ((st = pair_dots_phase(vptr(vslot[0]), wptr(wslot[0]))) || (st = fetch())) return st;
+        sig.assign(1, 1.0 / std::sqrt(hsc[kScX].x));
+        std::vector<double> G(2 * kMaxBasis * kMaxBasis, 0.0), vals, vecs;
+        auto Gat = [&](uint32_t i, uint32_t j) -> double * { return &G[2 * (i * kMaxBasis + j)]; };
+        Gat(0, 0)[0] = hsc[kScX + 1].x * sig[0] * sig[0];
         uint32_t m = 1;
-        double theta = col[0], rnorm = 0.0;
-        res.trace_len = 0;
+        double theta = Gat(0, 0)[0], rnorm = 0.0;
+        std::vector<double> s_host;  // current Ritz coefficients (interleaved complex)
+        result.trace_len = 0;
+        auto trace = [&](double th, double rn, int bits) {
+            if (opt.trace && result.trace_len < opt.trace_capacity) {
+                double *row = opt.trace + 4 * result.trace_len;
+                row[0] = std::chrono::duration<double>(clock::now() - t0).count();
+                row[1] = th;
+                row[2] = rn;
+                row[3] = bits;
+                ++result.trace_len;
+            }
+        };
+        double best32 = HUGE_VAL;  // best complex64-phase residual and its iteration
+        uint32_t best32_it = 0;
+        bool have_u64 = false;  // the refined FP64 Ritz vector sits in vector slot u64_slot
+        uint32_t u64_slot = 0;
         for (uint32_t it = 0; it < max_it; ++it) {
+            // projected problem
             std::vector<double> g(2 * m * m);
             for (uint32_t i = 0; i < m; ++i)
                 for (uint32_t j = 0; j < m; ++j) {
@@ -413,117 +636,174 @@ struct Solver {
             const int want = static_cast<int>(std::min<uint32_t>(m, keep));
             hermitian_lowest(static_cast<int>(m), g, want, vals, vecs);
             theta = vals[0];
-            std::vector<double> s(vecs.begin(), vecs.begin() + 2 * m);
-            if ((st = upload_small(s))) return st;
-            hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(kThreads), 0, stream, Vp(0), Wp(0), m, D,
-                               static_cast<const dvec2 *>(w.small.ptr), theta, static_cast<dvec2 *>(w.u.ptr),
-                               static_cast<dvec2 *>(w.hu.ptr), static_cast<dvec2 *>(w.r.ptr), D,
-                               static_cast<double *>(w.partial.ptr));
-            w.partial_host.resize(nb);
-            TCP_HIP_CHECK(hipMemcpyAsync(w.partial_host.data(), w.partial.ptr, 8 * nb, hipMemcpyDeviceToHost, stream));
-            TCP_HIP_CHECK(hipStreamSynchronize(stream));
-            double r2 = 0.0;
-            for (double v : w.partial_host) r2 += v;
-            rnorm = std::sqrt(r2);
-            res.iterations = it + 1;
-            const double elapsed = std::chrono::duration<double>(clock::now() - t0).count();
-            if (opt.trace && res.trace_len < opt.trace_capacity) {
-                double *row = opt.trace + 4 * res.trace_len;
-                row[0] = elapsed;
-                row[1] = theta;
-                row[2] = rnorm;
-                row[3] = phase == TCP_C64 ? 32.0 : 64.0;
-                ++res.trace_len;
-            }
-            if (phase == TCP_C128 && rnorm <= tol) {
-                res.converged = 1;
-                break;
-            }
-            if (phase == TCP_C64 && rnorm <= switch_tol) {
-                // restart from the Ritz vector with FP64 MVPs
-                phase = TCP_C128;
-                TCP_HIP_CHECK(hipMemcpyAsync(Vp(0), w.u.ptr, 16 * D, hipMemcpyDeviceToDevice, stream));
-                if ((st = norm(Vp(0), nv))) return st;
-                hipLaunchKernelGGL(scale_kernel, dim3(nb), dim3(kThreads), 0, stream, Vp(0), 1.0 / nv, D);
-                if ((st = apply(Vp(0), Wp(0), phase, res))) return st;
-                if ((st = dots(Vp(0), 1, Wp(0), col))) return st;
-                Gat(0, 0)[0] = col[0];
-                Gat(0, 0)[1] = 0.0;
-                m = 1;
-                ++res.restarts;
-                continue;
-            }
-            if (m + 1 > mmax) {
-                // thick restart: keep the `want` lowest Ritz vectors
+            if (m == mmax) {
+                // thick restart: the `want` lowest Ritz vectors into free slots
+                std::vector<dvec2> Y(static_cast<size_t>(want) * m);
+                for (int k = 0; k < want; ++k)
+                    for (uint32_t j = 0; j < m; ++j)
+                        Y[k * m + j] = dvec2{vecs[2 * (k * m + j)] * sig[j], vecs[2 * (k * m + j) + 1] * sig[j]};
+                std::vector<uint32_t> nv(want), nw(want);
                 for (int k = 0; k < want; ++k) {
-                    std::vector<double> c(vecs.begin() + 2 * m * k, vecs.begin() + 2 * m * (k + 1));
-                    if ((st = combine(Vp(0), m, c, 0.0, static_cast<dvec2 *>(w.Vn.ptr) + k * D))) return st;
-                    if ((st = combine(Wp(0), m, c, 0.0, static_cast<dvec2 *>(w.Wn.ptr) + k * D))) return st;
+                    nv[k] = take(vfree);
+                    nw[k] = take(wfree);
                 }
-                TCP_HIP_CHECK(hipMemcpyAsync(Vp(0), w.Vn.ptr, 16 * D * want, hipMemcpyDeviceToDevice, stream));
-                TCP_HIP_CHECK(hipMemcpyAsync(Wp(0), w.Wn.ptr, 16 * D * want, hipMemcpyDeviceToDevice, stream));
-                m = static_cast<uint32_t>(want);
+                if (phase == TCP_C64) {
+                    Cols<fvec2> ov{}, ow{};
+                    for (int k = 0; k < want; ++k) {
+                        ov.p[k] = static_cast<fvec2 *>(vptr(nv[k]));
+                        ow.p[k] = static_cast<fvec2 *>(wptr(nw[k]));
+                    }
+                    if ((st = combine<fvec2, fvec2>(m, Y, want, ov)) || (st = combine_w<fvec2, fvec2>(m, Y, want, ow)))
+                        return st;
+                } else {
+                    Cols<dvec2> ov{}, ow{};
+                    for (int k = 0; k < want; ++k) {
+                        ov.p[k] = static_cast<dvec2 *>(vptr(nv[k]));
+                        ow.p[k] = static_cast<dvec2 *>(wptr(nw[k]));
+                    }
+                    if ((st = combine<dvec2, dvec2>(m, Y, want, ov)) || (st = combine_w<dvec2, dvec2>(m, Y, want, ow)))
+                        return st;
+                }
                 for (uint32_t j = 0; j < m; ++j) {
-                    if ((st = dots(Vp(0), m, Wp(j), col))) return st;
-                    for (uint32_t i = 0; i < m; ++i) {
-                        Gat(i, j)[0] = col[2 * i];
-                        Gat(i, j)[1] = col[2 * i + 1];
-                    }
+                    vfree.push_back(vslot[j]);
+                    wfree.push_back(wslot[j]);
                 }
-                for (uint32_t i = 0; i < m; ++i)
-                    for (uint32_t j = i + 1; j < m; ++j) {  // Hermitian average
-                        const double re = 0.5 * (Gat(i, j)[0] + Gat(j, i)[0]);
-                        const double im = 0.5 * (Gat(i, j)[1] - Gat(j, i)[1]);
-                        Gat(i, j)[0] = re;
-                        Gat(i, j)[1] = im;
-                        Gat(j, i)[0] = re;
-                        Gat(j, i)[1] = -im;
-                    }
-                for (uint32_t i = 0; i < m; ++i) Gat(i, i)[1] = 0.0;
-                ++res.restarts;
+                vslot = nv;
+                wslot = nw;
+                m = static_cast<uint32_t>(want);
+                sig.assign(m, 1.0);
+                std::fill(G.begin(), G.end(), 0.0);
+                for (uint32_t k = 0; k < m; ++k) Gat(k, k)[0] = vals[k];
+                vecs.assign(static_cast<size_t>(2) * m, 0.0);
+                vecs[0] = 1.0;
+                ++result.restarts;
             }
-            // correction vector into the next basis slot
-            dvec2 *t = Vp(m);
+            Coef s{};
+            for (uint32_t j = 0; j < m; ++j) s.c[j] = dvec2{vecs[2 * j] * sig[j], vecs[2 * j + 1] * sig[j]};
             const float *logm = nullptr;
             if (opt.log_multiplier_fn) {
-                logm = static_cast<const float *>(opt.log_multiplier_fn(opt.log_multiplier_ctx, theta));
+                logm = static_cast<const float *>(opt.log_multiplier_fn(opt.log_multiplier_ctx, theta, stream));
                 if (!logm) return fail(TCP_ERR_INVALID_ARGUMENT, "learned preconditioner callback failed");
             }
-            hipLaunchKernelGGL(precondition_kernel, dim3(nb), dim3(kThreads), 0, stream,
-                               static_cast<const dvec2 *>(w.r.ptr), static_cast<const double *>(w.diag.ptr), logm,
-                               theta, 1e-8, t, D);
-            for (int pass = 0; pass < 2; ++pass) {  // classical Gram-Schmidt, twice
-                if ((st = dots(Vp(0), m, t, col))) return st;
-                for (double &v : col) v = -v;
-                if ((st = combine(Vp(0), m, col, 1.0, t))) return st;
+            vslot.push_back(take(vfree));
+            wslot.push_back(take(wfree));
+            st = phase == TCP_C64 ? iterate<fvec2>(m, s, theta, logm) : iterate<dvec2>(m, s, theta, logm);
+            if (st || (st = fetch())) return st;
+            rnorm = std::sqrt(hsc[kScR2].x);
+            result.iterations = it + 1;
+            trace(theta, rnorm, phase == TCP_C64 ? 32 : 64);
+            if (phase == TCP_C64 && rnorm < best32) {
+                best32 = rnorm;
+                best32_it = it;
             }
-            double nt;
-            if ((st = norm(t, nt))) return st;
-            if (!(nt > 1e-14)) break;  // correction lies in the basis: stagnated
-            hipLaunchKernelGGL(scale_kernel, dim3(nb), dim3(kThreads), 0, stream, t, 1.0 / nt, D);
-            if ((st = apply(t, Wp(m), phase, res))) return st;
-            if ((st = dots(Vp(0), m + 1, Wp(m), col))) return st;
-            for (uint32_t i = 0; i <= m; ++i) {
-                Gat(i, m)[0] = col[2 * i];
-                Gat(i, m)[1] = col[2 * i + 1];
-                Gat(m, i)[0] = col[2 * i];
-                Gat(m, i)[1] = -col[2 * i + 1];
+            const bool done64 = phase == TCP_C128 && rnorm <= tol;
+            // the complex64 phase also ends when its residual stops improving
+            // (the storage rounding floor, ~1e-7 |theta|, lies above switch_tol)
+            const bool switch32 =
+                phase == TCP_C64 && (rnorm <= switch_tol || it - best32_it >= kStallIterations);
+            if (done64 || switch32) {
+                // drop the direction just built; form the Ritz vector in FP64
+                vfree.push_back(vslot.back());
+                wfree.push_back(wslot.back());
+                vslot.pop_back();
+                wslot.pop_back();
+                std::vector<dvec2> Y(m);
+                for (uint32_t j = 0; j < m; ++j) Y[j] = s.c[j];
+                u64_slot = take(vfree);
+                Cols<dvec2> out{};
+                out.p[0] = static_cast<dvec2 *>(vptr(u64_slot));
+                st = phase == TCP_C64 ? combine<fvec2, dvec2>(m, Y, 1, out) : combine<dvec2, dvec2>(m, Y, 1, out);
+                if (st) return st;
+                have_u64 = true;
+                if (done64) {
+                    result.converged = 1;
+                    break;
+                }
+                // FP64 Rayleigh quotient and residual of the FP32-phase Ritz vector
+                phase = TCP_C128;
+                const uint32_t hu_slot = take(wfree);
+                if ((st = apply(vptr(u64_slot), wptr(hu_slot), result))) return st;
+                if ((st = pair_dots<dvec2>(vptr(u64_slot), wptr(hu_slot))) || (st = fetch())) return st;
+                const double uu = hsc[kScX].x;
+                theta = hsc[kScX + 1].x / uu;
+                hipLaunchKernelGGL(pair_residual_kernel, dim3(nb), dim3(kThreads), 0, stream,
+                                   static_cast<const dvec2 *>(vptr(u64_slot)),
+                                   static_cast<const dvec2 *>(wptr(hu_slot)), theta, D, part());
+                if ((st = reduce(1, kScX, false)) || (st = fetch())) return st;
+                rnorm = std::sqrt(hsc[kScX].x / uu);
+                trace(theta, rnorm, 64);
+                ++result.restarts;
+                // continue in FP64 from the refined vector (also when converged:
+                // the basis below is what a caller resuming would need)
+                for (uint32_t j = 0; j < m; ++j) {
+                    vfree.push_back(vslot[j]);
+                    wfree.push_back(wslot[j]);
+                }
+                vslot.assign(1, u64_slot);
+                wslot.assign(1, hu_slot);
+                sig.assign(1, 1.0 / std::sqrt(uu));
+                m = 1;
+                std::fill(G.begin(), G.end(), 0.0);
+                Gat(0, 0)[0] = theta;
+                if (rnorm <= tol) {
+                    result.converged = 1;
+                    break;
+                }
+                have_u64 = false;
+                continue;
+            }
+            // accept the new direction
+            const double t2 = hsc[kScT2].x, p1 = hsc[kScP1].x;
+            const double n2 = p1 >= kEta2 * t2 ? p1 : hsc[kScP2].x;
+            if (!(n2 > 1e-28 * t2) || !(n2 > 0.0)) {
+                // the correction lies in the basis: stagnated
+                vfree.push_back(vslot.back());
+                wfree.push_back(wslot.back());
+                vslot.pop_back();
+                wslot.pop_back();
+                break;
+            }
+            sig.push_back(1.0 / std::sqrt(n2));
+            for (uint32_t j = 0; j <= m; ++j) {
+                const dvec2 e = hsc[kScE + j];
+                const double f = sig[j] * sig[m];
+                Gat(j, m)[0] = e.x * f;
+                Gat(j, m)[1] = e.y * f;
+                Gat(m, j)[0] = e.x * f;
+                Gat(m, j)[1] = -e.y * f;
             }
             Gat(m, m)[1] = 0.0;
             ++m;
         }
-        res.eigenvalue = theta;
-        res.residual_norm = rnorm;
-        res.final_fp64 = phase == TCP_C128 ? 1 : 0;
+        result.eigenvalue = theta;
+        result.residual_norm = rnorm;
+        result.final_fp64 = phase == TCP_C128 ? 1 : 0;
         if (evec_out) {
-            double nu;
-            if ((st = norm(static_cast<dvec2 *>(w.u.ptr), nu))) return st;
-            hipLaunchKernelGGL(scale_kernel, dim3(nb), dim3(kThreads), 0, stream, static_cast<dvec2 *>(w.u.ptr),
-                               1.0 / nu, D);
-            TCP_HIP_CHECK(hipMemcpyAsync(evec_out, w.u.ptr, 16 * D, hipMemcpyDeviceToDevice, stream));
+            if (!have_u64) {
+                // Ritz vector of the current basis
+                std::vector<double> gm(2 * m * m);
+                for (uint32_t i = 0; i < m; ++i)
+                    for (uint32_t j = 0; j < m; ++j) {
+                        gm[2 * (i * m + j)] = Gat(i, j)[0];
+                        gm[2 * (i * m + j) + 1] = Gat(i, j)[1];
+                    }
+                hermitian_lowest(static_cast<int>(m), gm, 1, vals, vecs);
+                std::vector<dvec2> Y(m);
+                for (uint32_t j = 0; j < m; ++j) Y[j] = dvec2{vecs[2 * j] * sig[j], vecs[2 * j + 1] * sig[j]};
+                u64_slot = take(vfree);
+                Cols<dvec2> out{};
+                out.p[0] = static_cast<dvec2 *>(vptr(u64_slot));
+                st = phase == TCP_C64 ? combine<fvec2, dvec2>(m, Y, 1, out) : combine<dvec2, dvec2>(m, Y, 1, out);
+                if (st) return st;
+            }
+            if ((st = pair_dots<dvec2>(vptr(u64_slot), vptr(u64_slot))) || (st = fetch())) return st;
+            hipLaunchKernelGGL((scale_kernel<dvec2>), dim3(nb), dim3(kThreads), 0, stream,
+                               static_cast<dvec2 *>(vptr(u64_slot)), 1.0 / std::sqrt(hsc[kScX].x), D);
+            TCP_HIP_CHECK(hipMemcpyAsync(evec_out, vptr(u64_slot), 16 * D, hipMemcpyDeviceToDevice, stream));
         }
+        TCP_HIP_CHECK(hipGetLastError());
         TCP_HIP_CHECK(hipStreamSynchronize(stream));
-        res.seconds = std::chrono::duration<double>(clock::now() - t0).count();
+        result.seconds = std::chrono::duration<double>(clock::now() - t0).count();
         return TCP_OK;
     }
 };
@@ -533,6 +813,8 @@ int davidson(const Plan *plan, const tcp_davidson_options *opt, void *evec, tcp_
     if (!plan || !opt || !res) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
     if (opt->mvp_dtype != TCP_C128 && opt->mvp_dtype != TCP_C64)
         return fail(TCP_ERR_INVALID_ARGUMENT, "mvp_dtype must be TCP_C128 or TCP_C64");
+    if (opt->max_subspace > kMaxBasis)
+        return fail(TCP_ERR_INVALID_ARGUMENT, "max_subspace must be at most " + std::to_string(kMaxBasis));
     *res = tcp_davidson_result{};
     if (tcp::plan_dimension(plan) < 2) return fail(TCP_ERR_INVALID_ARGUMENT, "dimension must be at least 2");
     tcp::DeviceGuard dg;
@@ -569,6 +851,11 @@ int tcp_davidson_full(const tcp_mvp_plan *plan, const tcp_davidson_options *opt,
     return davidson(plan, opt, eigenvector, result);
 }
 
+int tcp_davidson_csr(const tcp_csr_plan *plan, const tcp_davidson_options *opt, void *eigenvector,
+                     tcp_davidson_result *result) {
+    return davidson(plan, opt, eigenvector, result);
+}
+
 int tcp_sector_plan_diagonal(const tcp_sector_plan *plan, double *diag, void *stream) {
     if (!plan || !diag) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
     tcp::DeviceGuard dg;
@@ -579,6 +866,15 @@ int tcp_sector_plan_diagonal(const tcp_sector_plan *plan, double *diag, void *st
 }
 
 int tcp_mvp_plan_diagonal(const tcp_mvp_plan *plan, double *diag, void *stream) {
+    if (!plan || !diag) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    tcp::DeviceGuard dg;
+    int st = dg.enter(tcp::plan_device(plan));
+    if (st) return st;
+    st = tcp::plan_diagonal(plan, diag, static_cast<hipStream_t>(stream));
+    return st ? st : tcp::ok();
+}
+
+int tcp_csr_plan_diagonal(const tcp_csr_plan *plan, double *diag, void *stream) {
     if (!plan || !diag) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
     tcp::DeviceGuard dg;
     int st = dg.enter(tcp::plan_device(plan));

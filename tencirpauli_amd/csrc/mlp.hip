@@ -2,10 +2,10 @@
 // bf16 weights and FP32 accumulation on MFMA (v_mfma_f32_16x16x16_bf16),
 // trained and evaluated on the GPU.  No reference counterpart.
 //
-// The network maps per-state features to a log multiplier m_i of the Jacobi
-// correction, t_i = r_i e^{m_i} / (theta - H_ii).  Training minimises the
-// angle between t and the true error vector on snapshots of Jacobi Davidson
-// runs on small lattices (loss kernels below).
+// The network maps per-state features to the log of a diagonal
+// preconditioner, t_i = r_i e^{m_i}, replacing Jacobi's 1 / (theta - H_ii).
+// Training maximises the one-step Ritz gain of t (loss kernels below) on
+// snapshots of Jacobi Davidson runs on small lattices.
 //
 // MFMA 16x16x16 operand layout (per 64-lane wave): A[i][k] with i = lane % 16,
 // k = 4 (lane / 16) + j; B[k][n] with n = lane % 16, k = 4 (lane / 16) + j;
@@ -322,50 +322,132 @@ __global__ void adam_kernel(float *__restrict__ params, const float *__restrict_
     params[i] -= lr * (mi / corr1) / (sqrtf(vi / corr2) + eps);
 }
 
-// ---- preconditioner loss: L = 1 - <e,t>^2 / (|e|^2 |t_perp|^2), real parts,
-// t_i = r_i e^{y_i} / (theta - d_i), t_perp = t - u <u,t>, e orthogonal to u.
-__device__ __forceinline__ double precond_scale(double theta, double d, float y) {
-    double den = theta - d;
-    if (fabs(den) < 1e-8) den = den < 0.0 ? -1e-8 : (den > 0.0 ? 1e-8 : -1e-8);
-    return exp(static_cast<double>(y)) / den;
+// ---- preconditioner loss: one-step Ritz gain ------------------------------
+//
+// For a Davidson state (normalised Ritz vector u, Ritz value theta, residual
+// r = H u - theta u) and a diagonal preconditioner t_i = r_i e^{m_i}, the new
+// direction q = t - u <u, t> extends the subspace span{u} to span{u, q}; the
+// lowest Ritz value of that 2 x 2 problem is theta - Delta with
+//   gamma = <q, H q> / |q|^2,  |rho|^2 = |<q, r>|^2 / |q|^2,  a = (gamma - theta) / 2,
+//   Delta = sqrt(a^2 + |rho|^2) - a.
+// The loss is L = -log Delta (scale-free in t).  Its gradient with respect to
+// q (real embedding, g = dL/dRe q + i dL/dIm q) is
+//   g_q = A1 (Hq - gamma q) + A2 conj(c) r + A3 q,   c = <q, r>, n2 = |q|^2,
+//   A1 = (1 - a/R) / (n2 Delta),  A2 = -1 / (n2 R Delta),  A3 = |c|^2 / (n2^2 R Delta),
+// R = sqrt(a^2 + |rho|^2); then g_t = g_q - u <u, g_q> and dL/dm_i = Re(conj(g_t,i) t_i).
+
+__device__ __forceinline__ dvec2 cmulc(dvec2 a, dvec2 b) {  // conj(a) * b
+    return dvec2{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
 }
 
-__global__ __launch_bounds__(kThreads) void loss_sums_kernel(const float *__restrict__ y, const dvec2 *__restrict__ r,
-                                                             const dvec2 *__restrict__ e, const dvec2 *__restrict__ u,
-                                                             const double *__restrict__ d, double theta, uint64_t B,
-                                                             double *__restrict__ partial) {
-    __shared__ double red[4][kThreads];
-    double s_et = 0.0, s_tt = 0.0, s_ut = 0.0, s_ee = 0.0;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < B;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        const double t = r[i].x * precond_scale(theta, d[i], y[i]);
-        s_et += e[i].x * t;
-        s_tt += t * t;
-        s_ut += u[i].x * t;
-        s_ee += e[i].x * e[i].x;
+// Block sum of two complex values, written to partial[2 * blockIdx + {0, 1}].
+__device__ __forceinline__ void block_sum2(dvec2 v0, dvec2 v1, dvec2 *__restrict__ partial) {
+    __shared__ dvec2 red[2][kThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        v0.x += __shfl_xor(v0.x, off);
+        v0.y += __shfl_xor(v0.y, off);
+        v1.x += __shfl_xor(v1.x, off);
+        v1.y += __shfl_xor(v1.y, off);
     }
-    red[0][threadIdx.x] = s_et;
-    red[1][threadIdx.x] = s_tt;
-    red[2][threadIdx.x] = s_ut;
-    red[3][threadIdx.x] = s_ee;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wave] = v0;
+        red[1][wave] = v1;
+    }
     __syncthreads();
-    if (threadIdx.x < 4) {
-        double s = 0.0;
-        for (uint32_t k = 0; k < kThreads; ++k) s += red[threadIdx.x][k];
-        partial[blockIdx.x * 4 + threadIdx.x] = s;
+    if (threadIdx.x < 2) {
+        dvec2 s{0.0, 0.0};
+        for (uint32_t w = 0; w < kThreads / 64; ++w) s += red[threadIdx.x][w];
+        partial[2 * blockIdx.x + threadIdx.x] = s;
     }
 }
 
-__global__ void loss_grad_kernel(const float *__restrict__ y, const dvec2 *__restrict__ r, const dvec2 *__restrict__ e,
-                                 const dvec2 *__restrict__ u, const double *__restrict__ d, double theta, uint64_t B,
-                                 double a, double bnorm, double c, double ut, double weight,
-                                 float *__restrict__ dy) {
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < B;
+// t = P r (learned: e^{m_i}; Jacobi: 1 / (theta - d_i), floored); partial <u, t>
+__global__ __launch_bounds__(kThreads) void ritz_t_kernel(const float *__restrict__ logm,
+                                                          const double *__restrict__ d, double theta,
+                                                          const dvec2 *__restrict__ r, const dvec2 *__restrict__ u,
+                                                          uint64_t D, dvec2 *__restrict__ t,
+                                                          dvec2 *__restrict__ partial) {
+    dvec2 ut{0.0, 0.0};
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
          i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        const double t = r[i].x * precond_scale(theta, d[i], y[i]);
-        const double tperp = t - ut * u[i].x;
-        const double dt = -2.0 * a * e[i].x / (bnorm * c) + 2.0 * a * a * tperp / (bnorm * c * c);
-        dy[i] = static_cast<float>(weight * dt * t);
+        double scale;
+        if (logm) {
+            scale = exp(static_cast<double>(logm[i]));
+        } else {
+            double den = theta - d[i];
+            if (fabs(den) < 1e-8) den = den < 0.0 ? -1e-8 : 1e-8;
+            scale = 1.0 / den;
+        }
+        const dvec2 ti = r[i] * scale;
+        t[i] = ti;
+        ut += cmulc(u[i], ti);
+    }
+    block_sum2(ut, dvec2{0.0, 0.0}, partial);
+}
+
+// q = t - alpha u; partials |q|^2 and <q, r>
+__global__ __launch_bounds__(kThreads) void ritz_q_kernel(const dvec2 *__restrict__ t, const dvec2 *__restrict__ u,
+                                                          dvec2 alpha, const dvec2 *__restrict__ r, uint64_t D,
+                                                          dvec2 *__restrict__ q, dvec2 *__restrict__ partial) {
+    dvec2 qq{0.0, 0.0}, qr{0.0, 0.0};
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const dvec2 ui = u[i];
+        const dvec2 qi = t[i] - dvec2{alpha.x * ui.x - alpha.y * ui.y, alpha.x * ui.y + alpha.y * ui.x};
+        q[i] = qi;
+        qq.x += qi.x * qi.x + qi.y * qi.y;
+        qr += cmulc(qi, r[i]);
+    }
+    block_sum2(qq, qr, partial);
+}
+
+// partial <q, Hq>
+__global__ __launch_bounds__(kThreads) void ritz_gamma_kernel(const dvec2 *__restrict__ q,
+                                                              const dvec2 *__restrict__ hq, uint64_t D,
+                                                              dvec2 *__restrict__ partial) {
+    dvec2 s{0.0, 0.0};
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        s += cmulc(q[i], hq[i]);
+    block_sum2(s, dvec2{0.0, 0.0}, partial);
+}
+
+struct RitzCoef {
+    double A1, gamma, A3;
+    dvec2 A2c;  // A2 * conj(c)
+};
+
+__device__ __forceinline__ dvec2 ritz_gq(const RitzCoef &k, dvec2 q, dvec2 hq, dvec2 r) {
+    const dvec2 a2r{k.A2c.x * r.x - k.A2c.y * r.y, k.A2c.x * r.y + k.A2c.y * r.x};
+    return (hq - q * k.gamma) * k.A1 + a2r + q * k.A3;
+}
+
+// partial <u, g_q>
+__global__ __launch_bounds__(kThreads) void ritz_beta_kernel(RitzCoef k, const dvec2 *__restrict__ q,
+                                                             const dvec2 *__restrict__ hq,
+                                                             const dvec2 *__restrict__ r, const dvec2 *__restrict__ u,
+                                                             uint64_t D, dvec2 *__restrict__ partial) {
+    dvec2 s{0.0, 0.0};
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        s += cmulc(u[i], ritz_gq(k, q[i], hq[i], r[i]));
+    block_sum2(s, dvec2{0.0, 0.0}, partial);
+}
+
+// dL/dm_i = Re(conj(g_q,i - beta u_i) t_i)
+__global__ __launch_bounds__(kThreads) void ritz_grad_kernel(RitzCoef k, dvec2 beta, const dvec2 *__restrict__ q,
+                                                             const dvec2 *__restrict__ hq,
+                                                             const dvec2 *__restrict__ r, const dvec2 *__restrict__ u,
+                                                             const dvec2 *__restrict__ t, uint64_t D,
+                                                             float *__restrict__ dm) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const dvec2 ui = u[i];
+        const dvec2 g = ritz_gq(k, q[i], hq[i], r[i]) -
+                        dvec2{beta.x * ui.x - beta.y * ui.y, beta.x * ui.y + beta.y * ui.x};
+        const dvec2 ti = t[i];
+        dm[i] = static_cast<float>(g.x * ti.x + g.y * ti.y);
     }
 }
 
@@ -395,11 +477,12 @@ struct tcp_mlp {
 
 struct tcp_learned_precond {
     const tcp_mlp *mlp = nullptr;
+    const tcp_sector_plan *plan = nullptr;  // borrowed: must outlive this object
     int device = 0;
     uint64_t dim = 0;
     double scale = 1.0;
-    hipStream_t stream = nullptr;
     tcp::DeviceBuffer features, diag, logm;
+    tcp::DeviceBuffer t, q, hq, partial;  // Ritz-gain loss workspace (allocated on first use)
     // the callback's last evaluation: reused while theta moves less than
     // 1e-3 * scale (below the bf16 resolution of feature 0) and the weights
     // are unchanged
@@ -409,6 +492,20 @@ struct tcp_learned_precond {
 };
 
 namespace {
+
+// Deterministic sum of per-block partials (fixed order on the host).
+int sum_partials(const tcp::DeviceBuffer &partial, uint32_t nb, hipStream_t s, dvec2 &v0, dvec2 &v1) {
+    std::vector<dvec2> h(2 * nb);
+    TCP_HIP_CHECK(hipMemcpyAsync(h.data(), partial.ptr, sizeof(dvec2) * 2 * nb, hipMemcpyDeviceToHost, s));
+    TCP_HIP_CHECK(hipStreamSynchronize(s));
+    v0 = dvec2{0.0, 0.0};
+    v1 = dvec2{0.0, 0.0};
+    for (uint32_t b = 0; b < nb; ++b) {
+        v0 += h[2 * b];
+        v1 += h[2 * b + 1];
+    }
+    return TCP_OK;
+}
 
 int ensure(tcp::DeviceBuffer &b, size_t bytes) {
     if (b.bytes >= bytes) return TCP_OK;
@@ -570,35 +667,58 @@ int tcp_mlp_adam_step(tcp_mlp *mlp, float lr, float weight_decay, void *stream) 
 
 void tcp_mlp_destroy(tcp_mlp *mlp) { delete mlp; }
 
-int tcp_precond_loss_gradient(const float *y, const void *r, const void *e, const void *u, const double *diag,
-                              double theta, uint64_t dim, double weight, float *dy, double *loss, void *stream) {
-    if (!y || !r || !e || !u || !diag || !dy || !loss || dim == 0) return fail(TCP_ERR_INVALID_ARGUMENT, "bad argument");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>((dim + kThreads - 1) / kThreads, 1024));
-    tcp::DeviceBuffer partial;
-    int st = partial.allocate(8 * 4 * nb);
+int tcp_learned_precond_ritz_loss(tcp_learned_precond *p, double theta, const void *u, const void *r,
+                                  const float *logm, float *dlogm, double *loss, double *gain, void *stream) {
+    if (!p || !u || !r || !loss) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    if (logm && !dlogm) return fail(TCP_ERR_INVALID_ARGUMENT, "learned multipliers need a gradient buffer");
+    tcp::DeviceGuard dg;
+    int st = dg.enter(p->device);
     if (st) return st;
-    hipLaunchKernelGGL(loss_sums_kernel, dim3(nb), dim3(kThreads), 0, s, y, static_cast<const dvec2 *>(r),
-                       static_cast<const dvec2 *>(e), static_cast<const dvec2 *>(u), diag, theta, dim,
-                       static_cast<double *>(partial.ptr));
-    std::vector<double> h(4 * nb);
-    TCP_HIP_CHECK(hipMemcpyAsync(h.data(), partial.ptr, 8 * 4 * nb, hipMemcpyDeviceToHost, s));
-    TCP_HIP_CHECK(hipStreamSynchronize(s));
-    double a = 0, tt = 0, ut = 0, ee = 0;
-    for (uint32_t b = 0; b < nb; ++b) {
-        a += h[4 * b];
-        tt += h[4 * b + 1];
-        ut += h[4 * b + 2];
-        ee += h[4 * b + 3];
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint64_t D = p->dim;
+    const size_t vb = 16 * D;
+    const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>((D + kThreads - 1) / kThreads, 1024));
+    if ((st = ensure(p->t, vb)) || (st = ensure(p->q, vb)) || (st = ensure(p->hq, vb)) ||
+        (st = ensure(p->partial, sizeof(dvec2) * 2 * nb)))
+        return st;
+    auto *T = static_cast<dvec2 *>(p->t.ptr);
+    auto *Q = static_cast<dvec2 *>(p->q.ptr);
+    auto *HQ = static_cast<dvec2 *>(p->hq.ptr);
+    auto *U = static_cast<const dvec2 *>(u);
+    auto *Rv = static_cast<const dvec2 *>(r);
+    auto *part = static_cast<dvec2 *>(p->partial.ptr);
+    dvec2 alpha, unused, qq, qr, qhq;
+    hipLaunchKernelGGL(ritz_t_kernel, dim3(nb), dim3(kThreads), 0, s, logm,
+                       static_cast<const double *>(p->diag.ptr), theta, Rv, U, D, T, part);
+    if ((st = sum_partials(p->partial, nb, s, alpha, unused))) return st;
+    hipLaunchKernelGGL(ritz_q_kernel, dim3(nb), dim3(kThreads), 0, s, T, U, alpha, Rv, D, Q, part);
+    if ((st = sum_partials(p->partial, nb, s, qq, qr))) return st;
+    const double n2 = qq.x;
+    if (!(n2 > 0.0)) return fail(TCP_ERR_INVALID_ARGUMENT, "degenerate snapshot: the correction lies along u");
+    if ((st = tcp::plan_apply(p->plan, Q, HQ, TCP_C128, s))) return st;
+    hipLaunchKernelGGL(ritz_gamma_kernel, dim3(nb), dim3(kThreads), 0, s, Q, HQ, D, part);
+    if ((st = sum_partials(p->partial, nb, s, qhq, unused))) return st;
+    const double c2 = qr.x * qr.x + qr.y * qr.y;
+    const double gamma = qhq.x / n2, rho2 = c2 / n2, a = 0.5 * (gamma - theta);
+    const double R = std::sqrt(a * a + rho2);
+    const double delta = a > 0.0 ? rho2 / (R + a) : R - a;
+    if (!(delta > 0.0)) return fail(TCP_ERR_INVALID_ARGUMENT, "degenerate snapshot: zero Ritz gain");
+    *loss = -std::log(delta);
+    if (gain) *gain = delta;
+    if (logm) {
+        RitzCoef k;
+        k.A1 = (1.0 - a / R) / (n2 * delta);
+        k.gamma = gamma;
+        k.A3 = c2 / (n2 * n2 * R * delta);
+        const double A2 = -1.0 / (n2 * R * delta);
+        k.A2c = dvec2{A2 * qr.x, -A2 * qr.y};
+        dvec2 beta;
+        hipLaunchKernelGGL(ritz_beta_kernel, dim3(nb), dim3(kThreads), 0, s, k, Q, HQ, Rv, U, D, part);
+        if ((st = sum_partials(p->partial, nb, s, beta, unused))) return st;
+        hipLaunchKernelGGL(ritz_grad_kernel, dim3(nb), dim3(kThreads), 0, s, k, beta, Q, HQ, Rv, U, T, D, dlogm);
     }
-    const double c = tt - ut * ut;
-    if (!(ee > 0.0) || !(c > 0.0)) return fail(TCP_ERR_INVALID_ARGUMENT, "degenerate preconditioner snapshot");
-    *loss = 1.0 - a * a / (ee * c);
-    hipLaunchKernelGGL(loss_grad_kernel, dim3(nb), dim3(kThreads), 0, s, y, static_cast<const dvec2 *>(r),
-                       static_cast<const dvec2 *>(e), static_cast<const dvec2 *>(u), diag, theta, dim, a, ee, c, ut,
-                       weight, dy);
     hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return tcp::hip_fail(err, "loss gradient");
+    if (err != hipSuccess) return tcp::hip_fail(err, "Ritz-gain loss");
     TCP_HIP_CHECK(hipStreamSynchronize(s));
     return tcp::ok();
 }
@@ -609,6 +729,7 @@ int tcp_learned_precond_create(const tcp_mlp *mlp, const tcp_sector_plan *plan, 
     auto *p = new tcp_learned_precond();
     std::unique_ptr<tcp_learned_precond> guard(p);
     p->mlp = mlp;
+    p->plan = plan;
     p->dim = tcp::plan_dimension(plan);
     p->scale = tcp::plan_energy_scale(plan);
     p->device = tcp::plan_device(plan);
@@ -638,14 +759,15 @@ int tcp_learned_precond_prepare(tcp_learned_precond *p, double theta, void *stre
 
 // tcp_log_multiplier_fn: log multipliers for the current theta (feature 0 =
 // (H_ii - theta) / scale, then MLP inference over every state).
-const void *tcp_learned_precond_log_multiplier(void *ctx, double theta) {
+const void *tcp_learned_precond_log_multiplier(void *ctx, double theta, void *stream) {
     auto *p = static_cast<tcp_learned_precond *>(ctx);
     if (!p) return nullptr;
     if (p->cached && p->cached_step == p->mlp->version && std::fabs(theta - p->cached_theta) < 1e-3 * p->scale)
         return p->logm.ptr;
-    if (tcp_learned_precond_prepare(p, theta, p->stream) != TCP_OK) return nullptr;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (tcp_learned_precond_prepare(p, theta, s) != TCP_OK) return nullptr;
     int st = forward(p->mlp, static_cast<const uint16_t *>(p->features.ptr), p->dim,
-                     static_cast<float *>(p->logm.ptr), nullptr, nullptr, p->stream);
+                     static_cast<float *>(p->logm.ptr), nullptr, nullptr, s);
     if (st) return nullptr;
     p->cached = true;
     p->cached_theta = theta;

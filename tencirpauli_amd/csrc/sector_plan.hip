@@ -1250,6 +1250,21 @@ int csr_apply_device(const tcp_csr_plan *plan, const void *in, void *out, int dt
     return e == hipSuccess ? tcp::ok() : tcp::hip_fail(e, "csr kernel launch");
 }
 
+// Real part of the stored diagonal entry of every row (0 where the row has none).
+__global__ void csr_diagonal_kernel(const uint64_t *__restrict__ indptr, const uint32_t *__restrict__ cols,
+                                    const dvec2 *__restrict__ vals, uint64_t dim, double *__restrict__ diag) {
+    for (uint64_t r = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; r < dim;
+         r += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        double d = 0.0;
+        for (uint64_t k = indptr[r]; k < indptr[r + 1]; ++k)
+            if (cols[k] == r) {
+                d = vals[k].x;
+                break;
+            }
+        diag[r] = d;
+    }
+}
+
 int csr_from_spec(const tcp::Operator &op, const BlockSpec &spec, bool u1, uint64_t max_bytes, tcp_csr_plan **out) {
     if (!out) return fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
     *out = nullptr;
@@ -1263,6 +1278,23 @@ int csr_from_spec(const tcp::Operator &op, const BlockSpec &spec, bool u1, uint6
 }
 
 }  // namespace
+
+namespace tcp {
+uint64_t plan_dimension(const tcp_csr_plan *plan) { return plan->dim; }
+int plan_device(const tcp_csr_plan *plan) { return plan->device; }
+int plan_apply(const tcp_csr_plan *plan, const void *in, void *out, int dtype, hipStream_t stream) {
+    return csr_apply_device(plan, in, out, dtype, stream);
+}
+int plan_diagonal(const tcp_csr_plan *plan, double *diag, hipStream_t stream) {
+    if (plan->dim == 0) return TCP_OK;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((plan->dim + 255) / 256, 4096));
+    hipLaunchKernelGGL(csr_diagonal_kernel, dim3(blocks), dim3(256), 0, stream,
+                       static_cast<const uint64_t *>(plan->indptr.ptr), static_cast<const uint32_t *>(plan->cols.ptr),
+                       static_cast<const dvec2 *>(plan->vals64.ptr), plan->dim, diag);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TCP_OK : hip_fail(e, "csr diagonal kernel");
+}
+}  // namespace tcp
 
 extern "C" {
 

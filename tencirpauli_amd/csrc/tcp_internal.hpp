@@ -92,9 +92,13 @@ int plan_device(const tcp_mvp_plan *plan);
 int plan_device(const tcp_sector_plan *plan);
 int plan_apply(const tcp_mvp_plan *plan, const void *in, void *out, int dtype, hipStream_t stream);
 int plan_apply(const tcp_sector_plan *plan, const void *in, void *out, int dtype, hipStream_t stream);
+uint64_t plan_dimension(const tcp_csr_plan *plan);
+int plan_device(const tcp_csr_plan *plan);
+int plan_apply(const tcp_csr_plan *plan, const void *in, void *out, int dtype, hipStream_t stream);
 // Real part of the diagonal matrix elements H_ii (FP64, `dimension` entries).
 int plan_diagonal(const tcp_mvp_plan *plan, double *diag, hipStream_t stream);
 int plan_diagonal(const tcp_sector_plan *plan, double *diag, hipStream_t stream);
+int plan_diagonal(const tcp_csr_plan *plan, double *diag, hipStream_t stream);
 
 // Learned-preconditioner inputs of a sector plan: 64 bf16 features per state
 // (slot 0 left for (H_ii - theta) / scale) and the diagonal.

@@ -1,12 +1,15 @@
 """Davidson ground-state solver on the GPU (``tcp_davidson_*`` in the C ABI).
 
 No reference counterpart: the reference computes eigenvalues with SciPy
-``eigsh`` over a ``LinearOperator`` (py/integrations/scipy.py:13-47).  This
-solver keeps the basis, the Rayleigh-quotient entries, orthogonalisation and
-residual norms in FP64 on the device; the MVP can run in complex64 storage in
-a first phase, then the solver restarts with FP64 MVPs so the result belongs
-to the FP64 operator.  The correction uses the Jacobi (diagonal)
-preconditioner, optionally scaled per state by a learned preconditioner.
+``eigsh`` over a ``LinearOperator`` (py/integrations/scipy.py:13-47).  The
+basis and its products stay on the device; orthogonalisation coefficients,
+Rayleigh-quotient entries and residual norms accumulate in FP64.  With
+``mvp_dtype="complex64"`` a first phase stores the basis in complex64 and runs
+complex64 MVPs; at ``switch_tol`` its Ritz vector is formed in FP64 and checked
+with one FP64 MVP, so a converged result belongs to the FP64 operator.  The
+correction uses the Jacobi (diagonal) preconditioner or a learned diagonal
+(``learned.LearnedPreconditioner``).  Full-space plans, lazy restricted plans
+and eager (CSR) restricted operators are accepted.
 """
 
 from __future__ import annotations
@@ -19,7 +22,7 @@ import numpy as np
 
 from . import _lib
 
-LOG_MULTIPLIER_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double)
+LOG_MULTIPLIER_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p)
 
 
 class DavidsonOptions(ctypes.Structure):
@@ -80,15 +83,23 @@ class DavidsonResult:
 
 
 def _plan_handle(plan: Any):
+    """(handle, solver entry point, diagonal entry point, plan object)."""
+    from .eager import CsrMvpPlan
+    from .pauli import NativeMVPPlan
     from .symmetry import lazy_plan_of
 
+    L = _lib.lib()
+    if getattr(plan, "storage", None) == "eager" and hasattr(plan, "_ensure_eager"):
+        plan = plan.plan
+    if isinstance(plan, CsrMvpPlan):
+        return plan._ptr, L.tcp_davidson_csr, L.tcp_csr_plan_diagonal, plan
     inner = lazy_plan_of(plan)
     handle = getattr(inner, "_ptr", None)
     if handle is None:
-        raise TypeError("expected a NativeMVPPlan, SectorMvpPlan or restricted operator")
-    from .pauli import NativeMVPPlan
-
-    return handle, isinstance(inner, NativeMVPPlan), inner
+        raise TypeError("expected a NativeMVPPlan, SectorMvpPlan, CsrMvpPlan or restricted operator")
+    if isinstance(inner, NativeMVPPlan):
+        return handle, L.tcp_davidson_full, L.tcp_mvp_plan_diagonal, inner
+    return handle, L.tcp_davidson_sector, L.tcp_sector_plan_diagonal, inner
 
 
 def lowest_eigenpair(plan: Any, *, tol: float = 1e-8, mvp_dtype: str = "complex128", switch_tol: float = 1e-4,
@@ -96,7 +107,7 @@ def lowest_eigenpair(plan: Any, *, tol: float = 1e-8, mvp_dtype: str = "complex1
                      seed: int = 20260801, preconditioner: Any = None, return_vector: bool = False,
                      trace_capacity: int = 4096) -> DavidsonResult:
     """Lowest eigenpair of a full-space or restricted plan by Davidson iteration."""
-    handle, full, inner = _plan_handle(plan)
+    handle, solve, _, inner = _plan_handle(plan)
     opt = DavidsonOptions()
     _lib.lib().tcp_davidson_default_options(ctypes.byref(opt))
     opt.max_subspace = max_subspace
@@ -119,8 +130,7 @@ def lowest_eigenpair(plan: Any, *, tol: float = 1e-8, mvp_dtype: str = "complex1
         vector = torch.empty(inner.dimension, dtype=torch.complex128, device="cuda")
         vec_ptr = vector.data_ptr()
     res = DavidsonResultStruct()
-    fn = _lib.lib().tcp_davidson_full if full else _lib.lib().tcp_davidson_sector
-    _lib.check(fn(handle, ctypes.byref(opt), vec_ptr, ctypes.byref(res)))
+    _lib.check(solve(handle, ctypes.byref(opt), vec_ptr, ctypes.byref(res)))
     return DavidsonResult(
         eigenvalue=res.eigenvalue, residual_norm=res.residual_norm, seconds=res.seconds,
         iterations=res.iterations, mvps_c64=res.mvps_c64, mvps_c128=res.mvps_c128, restarts=res.restarts,
@@ -134,8 +144,7 @@ def diagonal(plan: Any) -> Any:
 
     from ._plan import current_stream_handle
 
-    handle, full, inner = _plan_handle(plan)
+    handle, _, diag_fn, inner = _plan_handle(plan)
     out = torch.empty(inner.dimension, dtype=torch.float64, device="cuda")
-    fn = _lib.lib().tcp_mvp_plan_diagonal if full else _lib.lib().tcp_sector_plan_diagonal
-    _lib.check(fn(handle, out.data_ptr(), current_stream_handle(out)))
+    _lib.check(diag_fn(handle, out.data_ptr(), current_stream_handle(out)))
     return out

@@ -2,15 +2,18 @@
 
 No reference counterpart (the reference has no eigensolver of its own).  A
 64-128-128-1 tanh MLP (bf16 weights, FP32 accumulation on MFMA, C ABI
-``tcp_mlp_*``) maps per-state features of a restricted operator to a log
-multiplier m_i of the Jacobi correction, t_i = r_i e^{m_i} / (theta - H_ii).
+``tcp_mlp_*``) maps per-state features of a restricted operator to the log of
+a diagonal preconditioner: the correction is t_i = r_i e^{m_i}, in place of
+Jacobi's r_i / (theta - H_ii).
 
 Training data are snapshots of plain Jacobi Davidson runs on small lattices:
-the Ritz vector u, its residual r and Ritz value theta after a few
-iterations, and the true error e = x / <u, x> - u towards the converged
-eigenvector x.  The loss of a snapshot is the squared sine of the angle
-between t and e in the complement of u (``tcp_precond_loss_gradient``); the
-forward pass, backward pass and Adam update run as HIP kernels.
+the Ritz vector u, its Ritz value theta and residual r = H u - theta u after a
+few iterations, and the lattice's ground energy E0.  The loss of a snapshot is
+the one-step Ritz gain of t: adding t to span{u} lowers the Ritz value by
+gain, and the loss is -log(gain / (theta - E0)), the log of the fraction of
+the energy error one step removes (``tcp_learned_precond_ritz_loss``, which
+applies H to the projected correction).  Forward pass, loss gradient,
+backward pass and Adam update all run as HIP kernels.
 """
 
 from __future__ import annotations
@@ -117,6 +120,7 @@ class LearnedPreconditioner:
         _lib.check(_lib.lib().tcp_learned_precond_create(mlp._ptr, plan._ptr, ctypes.byref(out)))
         self._ptr = out
         self.mlp = mlp
+        self._plan = plan  # the loss applies H through it: keep it alive
         self.dimension = int(plan.dimension)
         feats = ctypes.c_void_p()
         diag = ctypes.c_void_p()
@@ -137,8 +141,8 @@ class LearnedPreconditioner:
         return fn, self._ptr.value
 
     def log_multipliers(self, theta: float) -> int:
-        """Device pointer to float[dimension] multipliers for `theta`."""
-        ptr = _lib.lib().tcp_learned_precond_log_multiplier(self._ptr, theta)
+        """Device pointer to float[dimension] log diagonal for `theta` (default stream)."""
+        ptr = _lib.lib().tcp_learned_precond_log_multiplier(self._ptr, theta, None)
         if not ptr:
             raise _lib.DeviceError(_lib.last_error())
         return int(ptr)
@@ -150,9 +154,9 @@ class Snapshot:
 
     precond: LearnedPreconditioner
     theta: float
-    u: Any  # device complex128 [D]
-    r: Any
-    e: Any
+    u: Any  # device complex128 [D], normalised Ritz vector
+    r: Any  # H u - theta u
+    e0: float  # the lattice's ground energy
 
 
 def jacobi_snapshots(restricted: Any, mlp: MLP, iterations: Sequence[int] = (1, 2, 3, 5, 8, 12),
@@ -160,28 +164,34 @@ def jacobi_snapshots(restricted: Any, mlp: MLP, iterations: Sequence[int] = (1, 
     """Snapshots after `iterations` steps of plain Jacobi Davidson on `restricted`."""
     import torch
 
-    exact = lowest_eigenpair(restricted, tol=1e-11, seed=seed, return_vector=True)
-    x = exact.eigenvector
+    e0 = lowest_eigenpair(restricted, tol=1e-10, seed=seed).eigenvalue
     precond = LearnedPreconditioner(mlp, restricted)
     out = []
     for k in iterations:
         state = lowest_eigenpair(restricted, tol=1e-14, max_iterations=k, seed=seed, return_vector=True)
         u = state.eigenvector
         hu = restricted.apply(u)
-        r = hu - state.eigenvalue * u
-        overlap = torch.vdot(u, x)
-        if abs(overlap.item()) < 1e-3:
+        theta = torch.vdot(u, hu).real.item()
+        if theta - e0 <= 1e-9 * max(1.0, abs(e0)):
             continue
-        e = x / overlap - u
-        e = e - u * torch.vdot(u, e)
-        if torch.linalg.vector_norm(e).item() < 1e-10:
-            continue
-        out.append(Snapshot(precond, float(state.eigenvalue), u.contiguous(), r.contiguous(), e.contiguous()))
+        r = hu - theta * u
+        out.append(Snapshot(precond, theta, u.contiguous(), r.contiguous(), e0))
     return out
 
 
+def _ritz(snap: Snapshot, logm: Any, dlogm: Any) -> float:
+    loss = ctypes.c_double()
+    gain = ctypes.c_double()
+    _lib.check(_lib.lib().tcp_learned_precond_ritz_loss(
+        snap.precond._ptr, snap.theta, snap.u.data_ptr(), snap.r.data_ptr(),
+        None if logm is None else logm.data_ptr(), None if dlogm is None else dlogm.data_ptr(),
+        ctypes.byref(loss), ctypes.byref(gain), None))
+    return gain.value / (snap.theta - snap.e0)
+
+
 def snapshot_loss(mlp: MLP, snap: Snapshot, *, train: bool = False, lr: float = 1e-3) -> float:
-    """Correction-angle loss of one snapshot; with `train`, one Adam step on it."""
+    """-log of the fraction of the energy error one learned correction removes;
+    with `train`, one Adam step on it."""
     import torch
 
     D = snap.precond.dimension
@@ -190,28 +200,16 @@ def snapshot_loss(mlp: MLP, snap: Snapshot, *, train: bool = False, lr: float = 
     y = torch.empty(D, dtype=torch.float32, device="cuda")
     _lib.check(lib.tcp_mlp_forward(mlp._ptr, snap.precond.features_ptr, D, y.data_ptr(), None))
     dy = torch.empty(D, dtype=torch.float32, device="cuda")
-    loss = ctypes.c_double()
-    _lib.check(lib.tcp_precond_loss_gradient(y.data_ptr(), snap.r.data_ptr(), snap.e.data_ptr(), snap.u.data_ptr(),
-                                             snap.precond.diag_ptr, snap.theta, D, 1.0, dy.data_ptr(),
-                                             ctypes.byref(loss), None))
+    fraction = _ritz(snap, y, dy)
     if train:
         _lib.check(lib.tcp_mlp_gradient(mlp._ptr, snap.precond.features_ptr, D, dy.data_ptr(), None, None))
         mlp.adam_step(lr)
-    return float(loss.value)
+    return -float(np.log(fraction))
 
 
 def jacobi_loss(snap: Snapshot) -> float:
-    """The same loss for the plain Jacobi correction (all multipliers zero)."""
-    import torch
-
-    D = snap.precond.dimension
-    y = torch.zeros(D, dtype=torch.float32, device="cuda")
-    dy = torch.empty(D, dtype=torch.float32, device="cuda")
-    loss = ctypes.c_double()
-    _lib.check(_lib.lib().tcp_precond_loss_gradient(y.data_ptr(), snap.r.data_ptr(), snap.e.data_ptr(),
-                                                    snap.u.data_ptr(), snap.precond.diag_ptr, snap.theta, D, 1.0,
-                                                    dy.data_ptr(), ctypes.byref(loss), None))
-    return float(loss.value)
+    """The same loss for the plain Jacobi correction r_i / (theta - H_ii)."""
+    return -float(np.log(_ritz(snap, None, None)))
 
 
 def train(mlp: MLP, snapshots: Sequence[Snapshot], *, epochs: int = 30, lr: float = 2e-3,

@@ -238,15 +238,15 @@ class U1RestrictedOperator(NativePlanFacade):
 
 
 def lazy_plan_of(obj: Any) -> Any:
-    """The matrix-free plan behind a plan or restricted operator (what the
-    Davidson solver and the learned preconditioner run on); device CSR plans
-    are rejected."""
+    """The matrix-free plan behind a plan or restricted operator (the learned
+    preconditioner's features come from it); device CSR plans are rejected."""
     from .eager import CsrMvpPlan
 
     if hasattr(obj, "_ensure_lazy"):
         return obj._ensure_lazy(DEFAULT_MAX_BYTES)
     if isinstance(obj, CsrMvpPlan):
-        raise TypeError("this solver runs on full-space plans and lazy restricted plans, not on eager CSR plans")
+        raise TypeError("expected a full-space plan, a lazy restricted plan or a restricted operator, "
+                        "not an eager CSR plan")
     return obj
 
 

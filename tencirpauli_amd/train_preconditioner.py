@@ -4,8 +4,9 @@ The training family is the periodic spin-1/2 XXZ model of the 25-qubit target
 (Jxy = 1, Jz ~ U[0.8, 1.2] per bond, field h ~ U[-0.2, 0.2] on Z): 4x4, 3x6 and
 4x5 tori with several seeds, in their S_z = 0 and S_z = +-1 sectors.  Every
 lattice contributes snapshots of plain Jacobi Davidson (see
-``learned.jacobi_snapshots``); training runs entirely in the library's HIP
-kernels (MFMA forward and backward, Adam).
+``learned.jacobi_snapshots``); the loss is the one-step Ritz gain of the
+learned correction, and training runs entirely in the library's HIP kernels
+(MFMA forward and backward, loss gradient with an MVP, Adam).
 
     python -m tencirpauli_amd.train_preconditioner [--epochs 30] [--out PATH]
 """
@@ -28,7 +29,7 @@ TRAINING_LATTICES: Sequence[Tuple[int, int, int, int]] = (
     (3, 6, 201, 0), (3, 6, 202, -1),
     (4, 5, 301, 0), (4, 5, 302, 1),
 )
-SNAPSHOT_ITERATIONS = (1, 2, 3, 4, 6, 8, 11, 15, 20, 26)
+SNAPSHOT_ITERATIONS = (1, 2, 3, 4, 6, 8, 11, 15, 20, 26, 34, 44)
 
 
 def restricted_lattice(rows: int, cols: int, seed: int, offset: int = 0):

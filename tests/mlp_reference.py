@@ -3,7 +3,7 @@
 Mirrors the precision choices of the HIP kernels: bf16 weights, bf16 inputs
 to every matrix product (features, the first hidden layer, the backward
 pre-activation gradients), FP32/FP64 accumulation, tanh activations, and the
-correction-angle loss of the learned preconditioner.  Computed in FP64.
+one-step Ritz-gain loss of the learned preconditioner.  Computed in FP64.
 """
 
 from __future__ import annotations
@@ -59,16 +59,28 @@ def gradient(p: np.ndarray, X: np.ndarray, dy: np.ndarray, rounding: bool = True
     return np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gw3, [gb3]])
 
 
-def precond_loss(y, r, e, u, d, theta):
-    """L = 1 - <e,t>^2 / (|e|^2 |t - u<u,t>|^2), t = r e^y / (theta - d); returns (L, dL/dy)."""
-    den = theta - d
-    den = np.where(np.abs(den) < 1e-8, np.where(den > 0, 1e-8, -1e-8), den)
-    t = r * np.exp(y) / den
-    a = e @ t
-    ut = u @ t
-    c = t @ t - ut ** 2
-    b = e @ e
-    loss = 1.0 - a * a / (b * c)
-    tperp = t - ut * u
-    dt = -2.0 * a * e / (b * c) + 2.0 * a * a * tperp / (b * c * c)
-    return loss, dt * t
+def ritz_loss(scale, r, u, matvec, theta):
+    """One-step Ritz-gain loss of the correction t = scale * r for a Davidson
+    state (normalised u, r = H u - theta u).  q = t - u <u, t>; the lowest Ritz
+    value of span{u, q} is theta - gain; loss = -log(gain).  Returns
+    (loss, gain, dloss/dlog(scale)) -- the last is the gradient with respect to
+    the learned log diagonal m when scale = e^m."""
+    scale = np.asarray(scale, dtype=np.float64)
+    t = scale * r
+    q = t - u * np.vdot(u, t)
+    n2 = np.vdot(q, q).real
+    c = np.vdot(q, r)
+    hq = matvec(q)
+    gamma = np.vdot(q, hq).real / n2
+    c2 = abs(c) ** 2
+    rho2 = c2 / n2
+    a = 0.5 * (gamma - theta)
+    R = np.sqrt(a * a + rho2)
+    gain = rho2 / (R + a) if a > 0 else R - a
+    A1 = (1.0 - a / R) / (n2 * gain)
+    A2 = -1.0 / (n2 * R * gain)
+    A3 = c2 / (n2 * n2 * R * gain)
+    gq = A1 * (hq - gamma * q) + A2 * np.conj(c) * r + A3 * q
+    gt = gq - u * np.vdot(u, gq)
+    dm = (np.conj(gt) * t).real
+    return -np.log(gain), gain, dm

@@ -94,3 +94,65 @@ def test_davidson_block_sector_hubbard():
     res = davidson.lowest_eigenpair(restricted, tol=1e-9)
     exact = np.linalg.eigvalsh(dense.restricted_dense(8, *op.code_arrays(), dense.blocks_basis([(4, 2), (4, 2)])))[0]
     assert abs(res.eigenvalue - exact) <= 1e-9
+
+
+def test_davidson_fp32_phase_refinement_alone_meets_a_loose_tolerance():
+    # complex64 phase to ||r|| <= 1e-5, then one FP64 MVP: the FP64 residual of
+    # that Ritz vector already meets 1e-4, so no FP64 iterations follow, and
+    # the FP64 Rayleigh quotient is within (1e-4)^2 / gap of the FP64 energy
+    op = _heisenberg(4, 4)
+    restricted = op.restrict_charge(tcp.U1Sector(16, 8))
+    ref = davidson.lowest_eigenpair(restricted, tol=1e-10)
+    mixed = davidson.lowest_eigenpair(restricted, tol=1e-4, mvp_dtype="complex64", switch_tol=1e-5)
+    assert mixed.converged and mixed.final_fp64
+    assert mixed.mvps_c128 == 1 and mixed.restarts >= 1
+    assert abs(mixed.eigenvalue - ref.eigenvalue) <= 1e-8
+    assert mixed.trace[-1, 3] == 64 and mixed.trace[-2, 3] == 32
+
+
+def test_davidson_fp32_phase_stall_switches_to_fp64():
+    # a switch tolerance below the complex64 rounding floor cannot be met: the
+    # phase ends when its residual stops improving and FP64 finishes the solve
+    op = _heisenberg(3, 4)
+    restricted = op.restrict_charge(tcp.U1Sector(12, 6))
+    ref = davidson.lowest_eigenpair(restricted, tol=1e-10)
+    mixed = davidson.lowest_eigenpair(restricted, tol=1e-9, mvp_dtype="complex64", switch_tol=1e-13,
+                                      max_iterations=400)
+    assert mixed.converged and mixed.final_fp64 and mixed.mvps_c128 >= 1
+    assert abs(mixed.eigenvalue - ref.eigenvalue) <= 1e-9
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_davidson_on_eager_csr_matches_lazy_and_eigvalsh(dtype):
+    from tests.helpers import conserving_u1_terms
+
+    rng = np.random.default_rng(5)
+    n, k = 12, 6
+    codes, coef = conserving_u1_terms(rng, n, 40)  # complex Hermitian restricted matrix
+    op = tcp.PauliOperator.from_code_arrays(codes, coef)
+    eager = op.restrict_charge(tcp.U1Sector(n, k), storage="eager")
+    lazy = op.restrict_charge(tcp.U1Sector(n, k))
+    exact = np.linalg.eigvalsh(dense.restricted_dense(n, *op.code_arrays(), dense.u1_basis(n, k)))[0]
+    diag_e = davidson.diagonal(eager).cpu().numpy()
+    np.testing.assert_allclose(diag_e, davidson.diagonal(lazy).cpu().numpy(), atol=1e-13)
+    res = davidson.lowest_eigenpair(eager, tol=1e-9, mvp_dtype=dtype, switch_tol=1e-5, return_vector=True)
+    res_lazy = davidson.lowest_eigenpair(lazy, tol=1e-9)
+    assert res.converged and res.final_fp64
+    assert abs(res.eigenvalue - exact) <= 1e-9 and abs(res_lazy.eigenvalue - exact) <= 1e-9
+    vec = res.eigenvector
+    resid = eager.apply(vec) - res.eigenvalue * vec
+    import torch
+
+    assert torch.linalg.vector_norm(resid).item() <= 1e-8
+
+
+def test_davidson_small_subspace_restarts_and_limits():
+    op = _heisenberg(3, 4)
+    restricted = op.restrict_charge(tcp.U1Sector(12, 6))
+    exact = np.linalg.eigvalsh(dense.restricted_dense(12, *op.code_arrays(), dense.u1_basis(12, 6)))[0]
+    res = davidson.lowest_eigenpair(restricted, tol=1e-9, max_subspace=6, restart_keep=2)
+    assert res.converged and res.restarts > 0 and abs(res.eigenvalue - exact) <= 1e-9
+    res32 = davidson.lowest_eigenpair(restricted, tol=1e-9, max_subspace=32, restart_keep=8)
+    assert res32.converged and abs(res32.eigenvalue - exact) <= 1e-9
+    with pytest.raises(ValueError, match="max_subspace"):
+        davidson.lowest_eigenpair(restricted, max_subspace=33)
