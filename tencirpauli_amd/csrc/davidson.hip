@@ -151,15 +151,19 @@ __global__ __launch_bounds__(kThreads) void residual_kernel(Cols<ST> V, Cols<ST>
     for (int j = 0; j < MAXM + 2; ++j) acc[j] = dvec2{0.0, 0.0};
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
          i += static_cast<uint64_t>(gridDim.x) * kThreads) {
-        ST vv[MAXM];
+        // Every load is issued unconditionally (columns past m alias column 0
+        // with zero coefficients), so all of them are in flight together.
+        ST vv[MAXM], ww[MAXM];
+#pragma unroll
+        for (int j = 0; j < MAXM; ++j) {
+            vv[j] = V.p[j][i];
+            ww[j] = W.p[j][i];
+        }
         dvec2 u{0.0, 0.0}, hu{0.0, 0.0};
 #pragma unroll
         for (int j = 0; j < MAXM; ++j) {
-            if (j < static_cast<int>(m)) {
-                vv[j] = V.p[j][i];
-                u += cmul(s.c[j], widen(vv[j]));
-                hu += cmul(s.c[j], widen(W.p[j][i]));
-            }
+            u += cmul(s.c[j], widen(vv[j]));
+            hu += cmul(s.c[j], widen(ww[j]));
         }
         const dvec2 r = hu - u * theta;
         double scale;
@@ -176,8 +180,7 @@ __global__ __launch_bounds__(kThreads) void residual_kernel(Cols<ST> V, Cols<ST>
         acc[0].x += r.x * r.x + r.y * r.y;
         acc[1].x += tw.x * tw.x + tw.y * tw.y;
 #pragma unroll
-        for (int j = 0; j < MAXM; ++j)
-            if (j < static_cast<int>(m)) acc[2 + j] += cmulc(widen(vv[j]), tw);
+        for (int j = 0; j < MAXM; ++j) acc[2 + j] += cmulc(widen(vv[j]), tw);
     }
     block_partials<MAXM + 2>(acc, m + 2, partial);
 }
@@ -201,21 +204,17 @@ __global__ __launch_bounds__(kThreads) void project_kernel(Cols<ST> V, uint32_t 
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
          i += static_cast<uint64_t>(gridDim.x) * kThreads) {
         ST vv[MAXM];
+#pragma unroll
+        for (int j = 0; j < MAXM; ++j) vv[j] = V.p[j][i];
         dvec2 x = widen(t[i]);
 #pragma unroll
-        for (int j = 0; j < MAXM; ++j) {
-            if (j < static_cast<int>(m)) {
-                vv[j] = V.p[j][i];
-                x -= cmul(c[j], widen(vv[j]));
-            }
-        }
+        for (int j = 0; j < MAXM; ++j) x -= cmul(c[j], widen(vv[j]));
         const ST xs = narrow<ST>(x);
         t[i] = xs;
         const dvec2 xw = widen(xs);
         acc[0].x += xw.x * xw.x + xw.y * xw.y;
 #pragma unroll
-        for (int j = 0; j < MAXM; ++j)
-            if (j < static_cast<int>(m)) acc[1 + j] += cmulc(widen(vv[j]), xw);
+        for (int j = 0; j < MAXM; ++j) acc[1 + j] += cmulc(widen(vv[j]), xw);
     }
     block_partials<MAXM + 1>(acc, m + 1, partial);
 }
@@ -230,9 +229,11 @@ __global__ __launch_bounds__(kThreads) void column_kernel(Cols<ST> V, uint32_t n
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
          i += static_cast<uint64_t>(gridDim.x) * kThreads) {
         const dvec2 wi = widen(w[i]);
+        ST vv[MAXM + 1];
 #pragma unroll
-        for (int j = 0; j < MAXM + 1; ++j)
-            if (j < static_cast<int>(ncols)) acc[j] += cmulc(widen(V.p[j][i]), wi);
+        for (int j = 0; j < MAXM + 1; ++j) vv[j] = V.p[j][i];
+#pragma unroll
+        for (int j = 0; j < MAXM + 1; ++j) acc[j] += cmulc(widen(vv[j]), wi);
     }
     block_partials<MAXM + 1>(acc, ncols, partial);
 }
@@ -265,22 +266,20 @@ __global__ __launch_bounds__(kThreads) void pair_residual_kernel(const dvec2 *__
     block_partials<1>(acc, 1, partial);
 }
 
-// out_k = sum_j Y[k * m + j] in_j for k < K (the kept Ritz vectors of a
-// restart, or one Ritz vector formed in another precision).
+// out_k = sum_j Y[k * MAXM + j] in_j for k < K (the kept Ritz vectors of a
+// restart, or one Ritz vector formed in another precision); Y is zero past m.
 template <typename SI, typename SO, int MAXM>
 __global__ __launch_bounds__(kThreads) void combine_kernel(Cols<SI> in, uint32_t m, const dvec2 *__restrict__ Y,
                                                            uint32_t K, Cols<SO> out, uint64_t D) {
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
          i += static_cast<uint64_t>(gridDim.x) * kThreads) {
-        dvec2 x[MAXM];
+        SI x[MAXM];
 #pragma unroll
-        for (int j = 0; j < MAXM; ++j)
-            if (j < static_cast<int>(m)) x[j] = widen(in.p[j][i]);
+        for (int j = 0; j < MAXM; ++j) x[j] = in.p[j][i];
         for (uint32_t k = 0; k < K; ++k) {
             dvec2 acc{0.0, 0.0};
 #pragma unroll
-            for (int j = 0; j < MAXM; ++j)
-                if (j < static_cast<int>(m)) acc += cmul(Y[k * m + j], x[j]);
+            for (int j = 0; j < MAXM; ++j) acc += cmul(Y[k * MAXM + j], widen(x[j]));
             out.p[k][i] = narrow<SO>(acc);
         }
     }
@@ -429,9 +428,13 @@ void hermitian_lowest(int m, const std::vector<double> &g, int count, std::vecto
 // The smallest register-array bound that holds m columns.
 template <class F>
 int with_maxm(uint32_t m, F &&f) {
+    if (m <= 4) return f(std::integral_constant<int, 4>());
     if (m <= 8) return f(std::integral_constant<int, 8>());
+    if (m <= 12) return f(std::integral_constant<int, 12>());
     if (m <= 16) return f(std::integral_constant<int, 16>());
+    if (m <= 20) return f(std::integral_constant<int, 20>());
     if (m <= 24) return f(std::integral_constant<int, 24>());
+    if (m <= 28) return f(std::integral_constant<int, 28>());
     return f(std::integral_constant<int, 32>());
 }
 
@@ -459,16 +462,18 @@ struct Solver {
     dvec2 *scp(uint32_t k) { return static_cast<dvec2 *>(sc.ptr) + k; }
     dvec2 *part() { return static_cast<dvec2 *>(partial.ptr); }
 
+    // Column pointers; entries past n alias column 0 (the kernels load every
+    // entry and weight the extra ones by zero).
     template <typename ST>
     Cols<ST> vcols(uint32_t n) {
         Cols<ST> c{};
-        for (uint32_t j = 0; j < n; ++j) c.p[j] = static_cast<ST *>(vptr(vslot[j]));
+        for (uint32_t j = 0; j <= kMaxBasis; ++j) c.p[j] = static_cast<ST *>(vptr(vslot[j < n ? j : 0]));
         return c;
     }
     template <typename ST>
     Cols<ST> wcols(uint32_t n) {
         Cols<ST> c{};
-        for (uint32_t j = 0; j < n; ++j) c.p[j] = static_cast<ST *>(wptr(wslot[j]));
+        for (uint32_t j = 0; j <= kMaxBasis; ++j) c.p[j] = static_cast<ST *>(wptr(wslot[j < n ? j : 0]));
         return c;
     }
     uint32_t take(std::vector<uint32_t> &pool) {
@@ -506,23 +511,19 @@ struct Solver {
         return phase == TCP_C64 ? pair_dots<fvec2>(a, b) : pair_dots<dvec2>(a, b);
     }
     // out_k = sum_j Y[k][j] * (column j of the basis, normalised), k < K
+    // out_k = sum_j Y[k * m + j] * (column j of V, or of W with `products`), k < K
     template <typename SI, typename SO>
-    int combine(uint32_t m, const std::vector<dvec2> &Y, uint32_t K, const Cols<SO> &out) {
-        TCP_HIP_CHECK(hipMemcpyAsync(ybuf.ptr, Y.data(), sizeof(dvec2) * Y.size(), hipMemcpyHostToDevice, stream));
-        const Cols<SI> in = vcols<SI>(m);
-        return with_maxm(m, [&](auto MM) {
-            hipLaunchKernelGGL((combine_kernel<SI, SO, decltype(MM)::value>), dim3(nb), dim3(kThreads), 0, stream, in,
-                               m, static_cast<const dvec2 *>(ybuf.ptr), K, out, D);
-            return TCP_OK;
-        });
-    }
-    template <typename SI, typename SO>
-    int combine_w(uint32_t m, const std::vector<dvec2> &Y, uint32_t K, const Cols<SO> &out) {
-        TCP_HIP_CHECK(hipMemcpyAsync(ybuf.ptr, Y.data(), sizeof(dvec2) * Y.size(), hipMemcpyHostToDevice, stream));
-        const Cols<SI> in = wcols<SI>(m);
-        return with_maxm(m, [&](auto MM) {
-            hipLaunchKernelGGL((combine_kernel<SI, SO, decltype(MM)::value>), dim3(nb), dim3(kThreads), 0, stream, in,
-                               m, static_cast<const dvec2 *>(ybuf.ptr), K, out, D);
+    int combine(uint32_t m, const std::vector<dvec2> &Y, uint32_t K, const Cols<SO> &out, bool products = false) {
+        const Cols<SI> in = products ? wcols<SI>(m) : vcols<SI>(m);
+        return with_maxm(m, [&](auto MM) -> int {
+            constexpr int M = decltype(MM)::value;
+            std::vector<dvec2> Yp(static_cast<size_t>(K) * M, dvec2{0.0, 0.0});
+            for (uint32_t k = 0; k < K; ++k)
+                for (uint32_t j = 0; j < m; ++j) Yp[k * M + j] = Y[k * m + j];
+            TCP_HIP_CHECK(
+                hipMemcpyAsync(ybuf.ptr, Yp.data(), sizeof(dvec2) * Yp.size(), hipMemcpyHostToDevice, stream));
+            hipLaunchKernelGGL((combine_kernel<SI, SO, M>), dim3(nb), dim3(kThreads), 0, stream, in, m,
+                               static_cast<const dvec2 *>(ybuf.ptr), K, out, D);
             return TCP_OK;
         });
     }
@@ -531,8 +532,11 @@ struct Solver {
     // gated), MVP, K4.  Column m is the new direction.
     template <typename ST>
     int iterate(uint32_t m, const Coef &s, double theta, const float *logm) {
-        const Cols<ST> V = vcols<ST>(m + 1), W = wcols<ST>(m + 1);
-        ST *t = V.p[m];
+        // K1/K2 see the m basis columns only (the new slot m may hold stale
+        // data, which a zero weight would not cancel if it were not finite)
+        const Cols<ST> V = vcols<ST>(m), W = wcols<ST>(m), V1 = vcols<ST>(m + 1);
+        ST *t = static_cast<ST *>(vptr(vslot[m]));
+        ST *wm = static_cast<ST *>(wptr(wslot[m]));
         Sig2 sig2{};
         for (uint32_t j = 0; j < m; ++j) sig2.s[j] = sig[j] * sig[j];
         const double *dg = static_cast<const double *>(diag.ptr);
@@ -554,10 +558,10 @@ struct Solver {
             return TCP_OK;
         });
         if (st) return st;
-        if ((st = apply(t, W.p[m], *res))) return st;
+        if ((st = apply(t, wm, *res))) return st;
         return with_maxm(m + 1, [&](auto MM) {
             constexpr int M = decltype(MM)::value;
-            hipLaunchKernelGGL((column_kernel<ST, M>), dim3(nb), dim3(kThreads), 0, stream, V, m + 1, W.p[m], D,
+            hipLaunchKernelGGL((column_kernel<ST, M>), dim3(nb), dim3(kThreads), 0, stream, V1, m + 1, wm, D,
                                part());
             return reduce(m + 1, kScE, false);
         });
@@ -653,7 +657,7 @@ struct Solver {
                         ov.p[k] = static_cast<fvec2 *>(vptr(nv[k]));
                         ow.p[k] = static_cast<fvec2 *>(wptr(nw[k]));
                     }
-                    if ((st = combine<fvec2, fvec2>(m, Y, want, ov)) || (st = combine_w<fvec2, fvec2>(m, Y, want, ow)))
+                    if ((st = combine<fvec2, fvec2>(m, Y, want, ov)) || (st = combine<fvec2, fvec2>(m, Y, want, ow, true)))
                         return st;
                 } else {
                     Cols<dvec2> ov{}, ow{};
@@ -661,7 +665,7 @@ struct Solver {
                         ov.p[k] = static_cast<dvec2 *>(vptr(nv[k]));
                         ow.p[k] = static_cast<dvec2 *>(wptr(nw[k]));
                     }
-                    if ((st = combine<dvec2, dvec2>(m, Y, want, ov)) || (st = combine_w<dvec2, dvec2>(m, Y, want, ow)))
+                    if ((st = combine<dvec2, dvec2>(m, Y, want, ov)) || (st = combine<dvec2, dvec2>(m, Y, want, ow, true)))
                         return st;
                 }
                 for (uint32_t j = 0; j < m; ++j) {

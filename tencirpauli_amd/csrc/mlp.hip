@@ -27,6 +27,13 @@ constexpr uint32_t kP = kF * kH + kH + kH * kH + kH + kH + 1;
 constexpr uint32_t kOffW1 = 0, kOffB1 = kF * kH, kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH,
                    kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kH;
 constexpr uint32_t kThreads = 256;
+// The Davidson callback re-evaluates the network when theta has moved by more
+// than kRefresh times the coupling scale since its last evaluation.  A
+// preconditioner for a slightly different theta is as good a search
+// direction: on the 4x4 and 4x5 training-family tori the iteration counts are
+// the same for refresh thresholds from 0 to 0.5 energy units (26 and 42), while
+// the evaluations fall from 25 and 41 to 7 and 13.
+constexpr double kRefresh = 0.25;
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -38,8 +45,13 @@ __host__ __device__ __forceinline__ uint16_t f2bf(float f) {
     u += 0x7fffu + ((u >> 16) & 1u);
     return static_cast<uint16_t>(u >> 16);
 }
-__host__ __device__ __forceinline__ float bf2f(uint16_t h) {
-    return __builtin_bit_cast(float, static_cast<uint32_t>(h) << 16);
+
+// tanh x = 1 - 2 / (e^{2x} + 1) from the hardware base-2 exponential and
+// reciprocal (absolute error ~1e-7; the result is rounded to bf16 or feeds an
+// FP32 dot product).  Saturates correctly: e^{2x} -> inf gives 1, -> 0 gives -1.
+__device__ __forceinline__ float fast_tanh(float x) {
+    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // 2 / ln 2
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
 }
 
 __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) {
@@ -80,10 +92,13 @@ __global__ void fill_feature0_kernel(uint16_t *__restrict__ X, const double *__r
 }
 
 // y[r] = MLP(X[r]); X is bf16 [B x 64].  With h1/h2, the post-activation
-// hidden layers are stored (FP32 [B x 128]) for the backward pass.
+// hidden layers are stored (FP32 [B x 128]) for the backward pass.  With diag,
+// feature 0 is taken as (diag[r] - theta) * inv_scale instead of X[r][0].
 __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const uint16_t *__restrict__ X,
                                                                uint64_t B, float *__restrict__ y,
-                                                               float *__restrict__ h1, float *__restrict__ h2) {
+                                                               float *__restrict__ h1, float *__restrict__ h2,
+                                                               const double *__restrict__ diag, double theta,
+                                                               double inv_scale) {
     __shared__ s4 w1s[8 * 4 * 64];
     __shared__ s4 w2s[8 * 8 * 64];
     __shared__ float bs1[kH], bs2[kH], ws3[kH];
@@ -108,6 +123,8 @@ __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const u
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks)
             a[ks] = valid ? *reinterpret_cast<const s4 *>(X + row * kF + ks * 16 + lg * 4) : s4{0, 0, 0, 0};
+        if (diag && valid && lg == 0)
+            a[0][0] = static_cast<short>(f2bf(static_cast<float>((diag[row] - theta) * inv_scale)));
         f4 acc[8];
 #pragma unroll
         for (int n = 0; n < 8; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
@@ -119,7 +136,7 @@ __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const u
         for (int n = 0; n < 8; ++n)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float v = tanhf(acc[n][j] + bs1[n * 16 + li]);
+                const float v = fast_tanh(acc[n][j] + bs1[n * 16 + li]);
                 hst[wave][(lg * 4 + j) * kH + n * 16 + li] = f2bf(v);
                 const uint64_t r = row0 + lg * 4 + j;
                 if (h1 && r < B) h1[r * kH + n * 16 + li] = v;
@@ -139,7 +156,7 @@ __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const u
         for (int n = 0; n < 8; ++n)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float v = tanhf(acc2[n][j] + bs2[n * 16 + li]);
+                const float v = fast_tanh(acc2[n][j] + bs2[n * 16 + li]);
                 const uint64_t r = row0 + lg * 4 + j;
                 if (h2 && r < B) h2[r * kH + n * 16 + li] = v;
                 part[j] += v * ws3[n * 16 + li];
@@ -483,9 +500,9 @@ struct tcp_learned_precond {
     double scale = 1.0;
     tcp::DeviceBuffer features, diag, logm;
     tcp::DeviceBuffer t, q, hq, partial;  // Ritz-gain loss workspace (allocated on first use)
-    // the callback's last evaluation: reused while theta moves less than
-    // 1e-3 * scale (below the bf16 resolution of feature 0) and the weights
-    // are unchanged
+    // the callback's last evaluation: reused while theta stays within
+    // kRefresh * scale of the theta it was made for and the weights are
+    // unchanged
     bool cached = false;
     double cached_theta = 0.0;
     uint32_t cached_step = 0;
@@ -512,11 +529,13 @@ int ensure(tcp::DeviceBuffer &b, size_t bytes) {
     return b.allocate(bytes);
 }
 
-int forward(const tcp_mlp *mlp, const uint16_t *X, uint64_t B, float *y, float *h1, float *h2, hipStream_t s) {
+int forward(const tcp_mlp *mlp, const uint16_t *X, uint64_t B, float *y, float *h1, float *h2, hipStream_t s,
+            const double *diag = nullptr, double theta = 0.0, double inv_scale = 0.0) {
     if (B == 0) return TCP_OK;
     const uint64_t tiles = (B + 63) / 64;
     const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(tiles, 256 * 4));
-    hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks), dim3(kThreads), 0, s, mlp->dev(), X, B, y, h1, h2);
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks), dim3(kThreads), 0, s, mlp->dev(), X, B, y, h1, h2, diag,
+                       theta, inv_scale);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TCP_OK : tcp::hip_fail(e, "mlp forward");
 }
@@ -762,12 +781,12 @@ int tcp_learned_precond_prepare(tcp_learned_precond *p, double theta, void *stre
 const void *tcp_learned_precond_log_multiplier(void *ctx, double theta, void *stream) {
     auto *p = static_cast<tcp_learned_precond *>(ctx);
     if (!p) return nullptr;
-    if (p->cached && p->cached_step == p->mlp->version && std::fabs(theta - p->cached_theta) < 1e-3 * p->scale)
+    if (p->cached && p->cached_step == p->mlp->version && std::fabs(theta - p->cached_theta) < kRefresh * p->scale)
         return p->logm.ptr;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (tcp_learned_precond_prepare(p, theta, s) != TCP_OK) return nullptr;
     int st = forward(p->mlp, static_cast<const uint16_t *>(p->features.ptr), p->dim,
-                     static_cast<float *>(p->logm.ptr), nullptr, nullptr, s);
+                     static_cast<float *>(p->logm.ptr), nullptr, nullptr, s,
+                     static_cast<const double *>(p->diag.ptr), theta, 1.0 / p->scale);
     if (st) return nullptr;
     p->cached = true;
     p->cached_theta = theta;
