@@ -236,7 +236,10 @@ def extra_davidson() -> dict:
     judge = davidson.lowest_eigenpair(targets["lazy"], tol=1e-9)
     e_ref = judge.eigenvalue
     out = {"energy_fp64": e_ref}
-    fp32 = dict(mvp_dtype="complex64", tol=1e-4, switch_tol=1e-5)
+    # The FP32 phase stops at ||r|| <= 8e-5 and one FP64 MVP gives the FP64 Rayleigh quotient, accepted at
+    # ||r_64|| <= 2e-4: late in the Jacobi run dE ~ ||r||^2 / 15 on this lattice, so that lands near 3e-10
+    # (abs_dE below reports the achieved value against the FP64 energy).
+    fp32 = dict(mvp_dtype="complex64", tol=2e-4, switch_tol=8e-5)
     for name, target in targets.items():
         runs = {
             "jacobi_fp64": davidson.lowest_eigenpair(target, tol=1e-9),

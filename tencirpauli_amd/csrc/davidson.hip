@@ -49,6 +49,7 @@ constexpr uint32_t kMaxBasis = 32;  // basis columns
 constexpr uint32_t kMaxKeep = 8;    // Ritz vectors kept at a restart
 constexpr double kEta2 = 0.5;  // second projection when |t'|^2 < kEta2 |t|^2
 constexpr uint32_t kStallIterations = 20;  // complex64 phase: iterations without a new best residual
+constexpr double kStallFloor = 1e-5;       // ... counted as a stall only below kStallFloor |theta|
 
 // Device scalar slots (dvec2 each): reductions land here, kernels read their
 // coefficients from here, and one copy per iteration brings them to the host.
@@ -701,10 +702,12 @@ struct Solver {
                 best32_it = it;
             }
             const bool done64 = phase == TCP_C128 && rnorm <= tol;
-            // the complex64 phase also ends when its residual stops improving
-            // (the storage rounding floor, ~1e-7 |theta|, lies above switch_tol)
-            const bool switch32 =
-                phase == TCP_C64 && (rnorm <= switch_tol || it - best32_it >= kStallIterations);
+            // the complex64 phase also ends when its residual has stopped
+            // improving near the storage rounding floor (~1e-7 |theta|, which
+            // may lie above switch_tol); a stall far above it is ordinary
+            // Davidson behaviour and the phase goes on
+            const bool stalled = it - best32_it >= kStallIterations && best32 <= kStallFloor * std::fabs(theta);
+            const bool switch32 = phase == TCP_C64 && (rnorm <= switch_tol || stalled);
             if (done64 || switch32) {
                 // drop the direction just built; form the Ritz vector in FP64
                 vfree.push_back(vslot.back());

@@ -94,6 +94,14 @@ __global__ void fill_feature0_kernel(uint16_t *__restrict__ X, const double *__r
 // y[r] = MLP(X[r]); X is bf16 [B x 64].  With h1/h2, the post-activation
 // hidden layers are stored (FP32 [B x 128]) for the backward pass.  With diag,
 // feature 0 is taken as (diag[r] - theta) * inv_scale instead of X[r][0].
+//
+// Both layers run transposed, H^T = W^T X^T: the weight fragments are the A
+// operand and 16 rows of X (or of H1) the B operand.  The MFMA result then
+// holds, in each lane, hidden units 4 (lane / 16) + j of one 16-block for row
+// lane % 16 -- exactly the B-operand layout of the next layer, so the first
+// hidden layer goes from accumulator to bf16 fragment in registers (no LDS
+// staging, no barrier).  The output layer is a dot product over the lanes that
+// share a row.
 __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const uint16_t *__restrict__ X,
                                                                uint64_t B, float *__restrict__ y,
                                                                float *__restrict__ h1, float *__restrict__ h2,
@@ -102,7 +110,6 @@ __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const u
     __shared__ s4 w1s[8 * 4 * 64];
     __shared__ s4 w2s[8 * 8 * 64];
     __shared__ float bs1[kH], bs2[kH], ws3[kH];
-    __shared__ uint16_t hst[4][16 * kH];
     for (uint32_t i = threadIdx.x; i < 8 * 4 * 64; i += kThreads) w1s[i] = p.w1f[i];
     for (uint32_t i = threadIdx.x; i < 8 * 8 * 64; i += kThreads) w2s[i] = p.w2f[i];
     for (uint32_t i = threadIdx.x; i < kH; i += kThreads) {
@@ -116,60 +123,56 @@ __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const u
     const uint32_t li = lane & 15, lg = lane >> 4;
     const uint64_t ntile = (B + 63) / 64;
     for (uint64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-        const uint64_t row0 = tile * 64 + wave * 16;
-        const uint64_t row = row0 + li;
+        // keeps the weight fragments in LDS: without it the compiler hoists all
+        // 96 loop-invariant fragment loads into registers (occupancy 1)
+        asm volatile("" ::: "memory");
+        const uint64_t row = tile * 64 + wave * 16 + li;
         const bool valid = row < B;
-        s4 a[4];
+        s4 x[4];
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks)
-            a[ks] = valid ? *reinterpret_cast<const s4 *>(X + row * kF + ks * 16 + lg * 4) : s4{0, 0, 0, 0};
+            x[ks] = valid ? *reinterpret_cast<const s4 *>(X + row * kF + ks * 16 + lg * 4) : s4{0, 0, 0, 0};
         if (diag && valid && lg == 0)
-            a[0][0] = static_cast<short>(f2bf(static_cast<float>((diag[row] - theta) * inv_scale)));
+            x[0][0] = static_cast<short>(f2bf(static_cast<float>((diag[row] - theta) * inv_scale)));
         f4 acc[8];
 #pragma unroll
         for (int n = 0; n < 8; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
-            for (int n = 0; n < 8; ++n) acc[n] = mfma(a[ks], w1s[(n * 4 + ks) * 64 + lane], acc[n]);
+            for (int n = 0; n < 8; ++n) acc[n] = mfma(w1s[(n * 4 + ks) * 64 + lane], x[ks], acc[n]);
+        s4 hb[8];  // layer-1 activations of block n as the layer-2 B fragment
 #pragma unroll
-        for (int n = 0; n < 8; ++n)
+        for (int n = 0; n < 8; ++n) {
+            f4 v;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float v = fast_tanh(acc[n][j] + bs1[n * 16 + li]);
-                hst[wave][(lg * 4 + j) * kH + n * 16 + li] = f2bf(v);
-                const uint64_t r = row0 + lg * 4 + j;
-                if (h1 && r < B) h1[r * kH + n * 16 + li] = v;
+                v[j] = fast_tanh(acc[n][j] + bs1[n * 16 + lg * 4 + j]);
+                hb[n][j] = static_cast<short>(f2bf(v[j]));
             }
-        __syncthreads();
+            if (h1 && valid) *reinterpret_cast<f4 *>(h1 + row * kH + n * 16 + lg * 4) = v;
+        }
         f4 acc2[8];
 #pragma unroll
         for (int n = 0; n < 8; ++n) acc2[n] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < 8; ++ks) {
-            const s4 a2 = *reinterpret_cast<const s4 *>(&hst[wave][li * kH + ks * 16 + lg * 4]);
+        for (int ks = 0; ks < 8; ++ks)
 #pragma unroll
-            for (int n = 0; n < 8; ++n) acc2[n] = mfma(a2, w2s[(n * 8 + ks) * 64 + lane], acc2[n]);
+            for (int n = 0; n < 8; ++n) acc2[n] = mfma(w2s[(n * 8 + ks) * 64 + lane], hb[ks], acc2[n]);
+        float part = 0.f;
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            f4 v;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                v[j] = fast_tanh(acc2[n][j] + bs2[n * 16 + lg * 4 + j]);
+                part += v[j] * ws3[n * 16 + lg * 4 + j];
+            }
+            if (h2 && valid) *reinterpret_cast<f4 *>(h2 + row * kH + n * 16 + lg * 4) = v;
         }
-        float part[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int n = 0; n < 8; ++n)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float v = fast_tanh(acc2[n][j] + bs2[n * 16 + li]);
-                const uint64_t r = row0 + lg * 4 + j;
-                if (h2 && r < B) h2[r * kH + n * 16 + li] = v;
-                part[j] += v * ws3[n * 16 + li];
-            }
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            for (int off = 1; off < 16; off <<= 1) part[j] += __shfl_xor(part[j], off);
-        if (li == 0)
-            for (int j = 0; j < 4; ++j) {
-                const uint64_t r = row0 + lg * 4 + j;
-                if (r < B) y[r] = part[j] + b3;
-            }
-        __syncthreads();
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        if (lg == 0 && valid) y[row] = part + b3;
     }
 }
 
