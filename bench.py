@@ -46,8 +46,9 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run one plain device copy (y <- v, 2^n amplitudes each way) before the MVPs, "
                         "the known-size reference for HBM counter scaling")
-    p.add_argument("--extras", default="fp32,sector,eager,davidson",
-                   help="comma list of extra sections: fp32, sector, eager, davidson, batched ('' for none)")
+    p.add_argument("--extras", default="fp32,sector,eager,hubbard_half,davidson",
+                   help="comma list of extra sections: fp32, sector, eager, hubbard_half, davidson, batched "
+                        "('' for none)")
     return p.parse_args()
 
 
@@ -137,8 +138,26 @@ def extra_fp32(plan, v, dim) -> dict:
             "tile_model": tile_model(plan, "complex64", dim, ms)}
 
 
-def extra_sector() -> dict:
-    """Restricted MVPs: 4x4 Hubbard (32 qubits, N_up = N_dn = 4) and the 5x5 XXZ S_z = 1/2 sector."""
+def sample_cpu(run, total: int, seconds: float) -> dict:
+    """Time run((begin, end)) on a prefix of `total` units sized to about `seconds` and
+    extrapolate to all of them (the probe is 1/1000 of the units, at least 64)."""
+    probe = min(total, max(64, total // 1000))
+    t0 = time.perf_counter()
+    run((0, probe))
+    rate = probe / max(time.perf_counter() - t0, 1e-6)
+    count = int(min(total, max(probe, rate * seconds)))
+    t0 = time.perf_counter()
+    run((0, count))
+    elapsed = time.perf_counter() - t0
+    return {"units": count, "of": total, "elapsed_s": round(elapsed, 3),
+            "full_apply_s": round(elapsed * total / count, 3)}
+
+
+def extra_sector(cpu_seconds: float = 0.0) -> dict:
+    """Restricted MVPs: 4x4 Hubbard (32 qubits, N_up = N_dn = 4) and the 5x5 XXZ S_z = 1/2 sector.
+    With cpu_seconds, the oracle's restatement of the reference's serial loops (the generic charge
+    loop, core/charge/generic.rs:949-1045; the U(1) lazy scatter, core/sector.rs:624-640) is timed
+    on a prefix of the source ranks and extrapolated to one full apply (1 core: serial by design)."""
     import torch
 
     import tencirpauli_amd as tcp
@@ -149,12 +168,12 @@ def extra_sector() -> dict:
     op = tcp.PauliOperator.from_code_arrays(codes, coef)
     up = tcp.AdditiveCharge.number(32, range(16))
     dn = tcp.AdditiveCharge.number(32, range(16, 32))
-    cases = [("hubbard_4x4_jw32_nup4_ndn4", op.restrict_charge(tcp.ChargeSector(((up, 4), (dn, 4))))),
-             ("xxz_5x5_periodic_sz_half", None)]
-    codes, coef = lattice.heisenberg_periodic(5, 5, seed=SEED)
-    op5 = tcp.PauliOperator.from_code_arrays(codes, coef)
-    cases[1] = (cases[1][0], op5.restrict_charge(tcp.U1Sector(25, 12)))
-    for name, restricted in cases:
+    codes5, coef5 = lattice.heisenberg_periodic(5, 5, seed=SEED)
+    op5 = tcp.PauliOperator.from_code_arrays(codes5, coef5)
+    cases = [("hubbard_4x4_jw32_nup4_ndn4", op.restrict_charge(tcp.ChargeSector(((up, 4), (dn, 4)))), op,
+              "blocks"),
+             ("xxz_5x5_periodic_sz_half", op5.restrict_charge(tcp.U1Sector(25, 12)), op5, "u1")]
+    for name, restricted, operator, kind in cases:
         gen = torch.Generator(device="cuda").manual_seed(SEED + 7)
         v = torch.randn(restricted.dimension, dtype=torch.complex128, device="cuda", generator=gen)
         rec = {"dimension": restricted.dimension, "terms": restricted.plan.term_count,
@@ -165,11 +184,62 @@ def extra_sector() -> dict:
             ms = time_applies(lambda: restricted.apply_into(vv, yy), 10)
             rec[f"ms_per_mvp_{tag}"] = round(ms, 4)
             rec[f"amp_per_s_{tag}"] = round(restricted.dimension / (ms * 1e-3), 1)
+        if cpu_seconds > 0:
+            import oracle
+
+            cc, cf = operator.code_arrays()
+            vh = v.cpu().numpy()
+            if kind == "u1":
+                run = lambda rng: oracle.u1_lazy_apply(25, 12, cc, cf, vh, sources=rng)  # noqa: E731
+                algo = "U(1) lazy serial scatter (core/sector.rs:624-640)"
+            else:
+                run = lambda rng: oracle.blocks_generic_apply([(16, 4), (16, 4)], cc, cf, vh,  # noqa: E731
+                                                              sources=rng)
+                algo = "generic charge loop (core/charge/generic.rs:949-1045)"
+            smp = sample_cpu(run, restricted.dimension, cpu_seconds)
+            rec["cpu_baseline"] = {"value_ms_per_mvp": round(1e3 * smp["full_apply_s"], 1), "cores": 1,
+                                   "kind": "port", "algorithm": algo,
+                                   "sample": f"source ranks 0..{smp['units'] - 1} of {smp['of']} "
+                                             f"({smp['elapsed_s']} s, extrapolated)",
+                                   "gpu_fp64_speedup": round(1e3 * smp["full_apply_s"] / rec["ms_per_mvp_fp64"], 1)}
         out[name] = rec
     return out
 
 
-def extra_eager() -> dict:
+def extra_hubbard_half() -> dict:
+    """4x4 Fermi-Hubbard at half filling (N_up = N_dn = 8 of 16; D = C(16,8)^2 = 165,636,900; 2.65 GB
+    complex128 vectors), Jordan-Wigner Pauli form on the lazy block-sector plan.  The reference's
+    one recorded restricted-MVP timing is this sector: lazy apply() at 54.004 s on an arm64/macOS CPU
+    through its structured fermion path (docs/vibe/reports/phase-8.5-review-2026-08-04.md:191)."""
+    import torch
+
+    import tencirpauli_amd as tcp
+    from tencirpauli_amd import lattice
+
+    codes, coef = lattice.fermi_hubbard_jw(4, 4, 1.0, 4.0, seed=SEED + 3)
+    op = tcp.PauliOperator.from_code_arrays(codes, coef)
+    up = tcp.AdditiveCharge.number(32, range(16))
+    dn = tcp.AdditiveCharge.number(32, range(16, 32))
+    t0 = time.perf_counter()
+    restricted = op.restrict_charge(tcp.ChargeSector(((up, 8), (dn, 8))))
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    gen = torch.Generator(device="cuda").manual_seed(SEED + 11)
+    v = torch.randn(restricted.dimension, dtype=torch.complex128, device="cuda", generator=gen)
+    rec = {"dimension": restricted.dimension, "plan_build_s": round(build_s, 3)}
+    for tag, dt in (("fp64", torch.complex128), ("fp32", torch.complex64)):
+        vv = v.to(dt)
+        yy = torch.empty_like(vv)
+        ms = time_applies(lambda: restricted.apply_into(vv, yy), 5)
+        rec[f"ms_per_mvp_{tag}"] = round(ms, 3)
+        del vv, yy
+    rec["reference_published_s"] = 54.004
+    rec["reference_hardware"] = "arm64/macOS CPU (structured fermion lazy path)"
+    rec["ratio_vs_published_fp64"] = round(54.004 / (rec["ms_per_mvp_fp64"] * 1e-3), 1)
+    return rec
+
+
+def extra_eager(cpu_seconds: float = 0.0) -> dict:
     """5x5 periodic XXZ, S_z = 1/2 (dim 5,200,300): the eager restricted operator (device CSR)
     against the lazy sector gather.  CSR bytes per apply: indptr 8 (D+1) + columns 4 nnz +
     values 16|8 nnz + y 16|8 D, plus one read of v (gathers are assumed to hit in cache)."""
@@ -194,6 +264,18 @@ def extra_eager() -> dict:
            "retained_device_bytes": eager.mvp_plan(storage="eager").device_bytes}
     y_ref = torch.empty_like(v)
     lazy.apply_into(v, y_ref)
+    if cpu_seconds > 0:
+        import oracle
+
+        indptr, indices, values = eager.mvp_plan(storage="eager").csr_arrays()
+        vh = v.cpu().numpy()
+        threads = max(1, min(16, os.cpu_count() or 1))
+        smp = sample_cpu(lambda rng: oracle.csr_apply(indptr, indices, values, vh, rows=rng, nthreads=threads),
+                         dim, cpu_seconds)
+        rec["cpu_baseline"] = {"value_ms_per_mvp": round(1e3 * smp["full_apply_s"], 2), "cores": threads,
+                               "kind": "port", "algorithm": "CsrGraph::apply_into row-parallel (core/csr.rs:194-238)",
+                               "sample": f"rows 0..{smp['units'] - 1} of {dim} ({smp['elapsed_s']} s, extrapolated)"}
+        del indptr, indices, values, vh
     for tag, dt, s in (("fp64", torch.complex128, 16), ("fp32", torch.complex64, 8)):
         vv = v.to(dt)
         yy = torch.empty_like(vv)
@@ -206,6 +288,9 @@ def extra_eager() -> dict:
                     "bytes_alg_GBps": round(alg / (ms * 1e-3) / 1e9, 2),
                     "frac_of_hbm_peak": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     "rel_err_vs_lazy_fp64": err}
+    if "cpu_baseline" in rec:
+        rec["cpu_baseline"]["gpu_fp64_speedup"] = round(rec["cpu_baseline"]["value_ms_per_mvp"] /
+                                                        rec["fp64"]["ms_per_mvp"], 1)
     return rec
 
 
@@ -402,13 +487,17 @@ def main() -> int:
     if "fp32" in extras and tag == "complex128":
         record["fp32_storage"] = extra_fp32(plan, v, dim)
     del y
-    if any(e in extras for e in ("sector", "eager", "davidson", "batched")):
+    if any(e in extras for e in ("sector", "eager", "hubbard_half", "davidson", "batched")):
         del v, plan
         torch.cuda.empty_cache()
+    sector_cpu = 0.0 if args.no_cpu_baseline else args.cpu_seconds / 3
     if "sector" in extras:
-        record["sector_mvp"] = extra_sector()
+        record["sector_mvp"] = extra_sector(sector_cpu)
     if "eager" in extras:
-        record["eager_csr_5x5_xxz"] = extra_eager()
+        record["eager_csr_5x5_xxz"] = extra_eager(sector_cpu)
+    if "hubbard_half" in extras:
+        torch.cuda.empty_cache()
+        record["hubbard_4x4_half_filling"] = extra_hubbard_half()
     if "davidson" in extras:
         record["davidson_5x5_xxz"] = extra_davidson()
     if "batched" in extras:

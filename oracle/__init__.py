@@ -11,8 +11,9 @@ Two independent oracles live here:
   TermDirect apply (core/hamiltonian.rs:204-270), U(1) rank/unrank and the lazy
   restricted scatter (core/sector.rs:79-342, 579-642, 1051-1127), contiguous
   fixed-population blocks with the generic charge loop
-  (core/charge_sector.rs:973-1038, core/charge/generic.rs:949-1045) and
-  ``analyze_charge`` (core/operator.rs:396-445).
+  (core/charge_sector.rs:973-1038, core/charge/generic.rs:949-1045), the eager
+  CSR apply (core/csr.rs:194-238) and ``analyze_charge``
+  (core/operator.rs:396-445).
 * ``dense.py``: a dense Kronecker-sum oracle and brute-force sector bases built
   from textbook definitions only (the pattern of the reference's
   tests/reference.py:15-115 and tests/test_charge.py:63-83).
@@ -91,11 +92,16 @@ def _declare(L: ctypes.CDLL) -> None:
     L.orc_u1_unrank_many.argtypes = [c.c_uint32, c.c_uint32, _u64p, c.c_uint64, _u64p]
     L.orc_u1_lazy_apply.argtypes = [
         c.c_uint32, c.c_uint32, c.c_uint64, _u8p, _dp, _dp, _dp, _u64p, _u32p]
+    L.orc_u1_lazy_apply_range.argtypes = [
+        c.c_uint32, c.c_uint32, c.c_uint64, _u8p, _dp, _dp, _dp, c.c_uint64, c.c_uint64, _u64p, _u32p]
     L.orc_blocks_dimension.argtypes = [c.c_uint32, _u32p, _u32p, _u64p]
     L.orc_blocks_rank_many.argtypes = [c.c_uint32, _u32p, _u32p, _u64p, c.c_uint64, _u64p]
     L.orc_blocks_unrank_many.argtypes = [c.c_uint32, _u32p, _u32p, _u64p, c.c_uint64, _u64p]
     L.orc_blocks_generic_apply.argtypes = [
         c.c_uint32, _u32p, _u32p, c.c_uint64, _u8p, _dp, _dp, _dp, _u64p]
+    L.orc_blocks_generic_apply_range.argtypes = [
+        c.c_uint32, _u32p, _u32p, c.c_uint64, _u8p, _dp, _dp, _dp, c.c_uint64, c.c_uint64, _u64p]
+    L.orc_csr_apply.argtypes = [c.c_uint64, _u64p, _u64p, _dp, _dp, _dp, c.c_uint64, c.c_uint64, c.c_int]
     L.orc_analyze_charge.argtypes = [c.c_uint32, c.c_uint64, _u8p, _dp, _dp, _u64p]
 
 
@@ -194,17 +200,20 @@ def u1_unrank_many(nqubits: int, k: int, indices: np.ndarray) -> np.ndarray:
 
 
 def u1_lazy_apply(nqubits: int, k: int, codes: np.ndarray, coef: np.ndarray,
-                  state: np.ndarray) -> np.ndarray:
-    """U1LazyMvpPlan::apply_into on canonical terms (raises OracleError on leakage)."""
+                  state: np.ndarray, sources: Optional[Tuple[int, int]] = None) -> np.ndarray:
+    """U1LazyMvpPlan::apply_into on canonical terms (raises OracleError on leakage).
+    With ``sources=(begin, end)`` only those source ranks scatter (a bounded
+    timing sample; the full apply is every source)."""
     codes, cf = _terms(nqubits, codes, coef)
     T = codes.shape[0]
     v = np.ascontiguousarray(state, dtype=np.complex128)
     y = np.empty_like(v)
     leak = ctypes.c_uint64(0)
     weight = ctypes.c_uint32(0)
-    st = lib().orc_u1_lazy_apply(nqubits, k, T, _p(codes, _u8p), _p(cf.view(np.float64), _dp),
-                                 _p(v.view(np.float64), _dp), _p(y.view(np.float64), _dp),
-                                 ctypes.byref(leak), ctypes.byref(weight))
+    lo, hi = sources if sources is not None else (0, 2**64 - 1)
+    st = lib().orc_u1_lazy_apply_range(nqubits, k, T, _p(codes, _u8p), _p(cf.view(np.float64), _dp),
+                                       _p(v.view(np.float64), _dp), _p(y.view(np.float64), _dp), lo, hi,
+                                       ctypes.byref(leak), ctypes.byref(weight))
     _check(st, f"source_index={leak.value} actual={weight.value}")
     return y
 
@@ -239,8 +248,9 @@ def blocks_unrank_many(blocks: Sequence[Tuple[int, int]], indices: np.ndarray) -
 
 
 def blocks_generic_apply(blocks: Sequence[Tuple[int, int]], codes: np.ndarray, coef: np.ndarray,
-                         state: np.ndarray) -> np.ndarray:
-    """Generic charge loop over a contiguous fixed-population block sector."""
+                         state: np.ndarray, sources: Optional[Tuple[int, int]] = None) -> np.ndarray:
+    """Generic charge loop over a contiguous fixed-population block sector
+    (``sources=(begin, end)``: those source ranks only)."""
     s, p = _blocks_arrays(blocks)
     n = int(s.sum())
     codes, cf = _terms(n, codes, coef)
@@ -248,10 +258,28 @@ def blocks_generic_apply(blocks: Sequence[Tuple[int, int]], codes: np.ndarray, c
     v = np.ascontiguousarray(state, dtype=np.complex128)
     y = np.empty_like(v)
     leak = ctypes.c_uint64(0)
-    st = lib().orc_blocks_generic_apply(len(s), _p(s, _u32p), _p(p, _u32p), T, _p(codes, _u8p),
-                                        _p(cf.view(np.float64), _dp), _p(v.view(np.float64), _dp),
-                                        _p(y.view(np.float64), _dp), ctypes.byref(leak))
+    lo, hi = sources if sources is not None else (0, 2**64 - 1)
+    st = lib().orc_blocks_generic_apply_range(len(s), _p(s, _u32p), _p(p, _u32p), T, _p(codes, _u8p),
+                                              _p(cf.view(np.float64), _dp), _p(v.view(np.float64), _dp),
+                                              _p(y.view(np.float64), _dp), lo, hi, ctypes.byref(leak))
     _check(st, f"source_index={leak.value}")
+    return y
+
+
+def csr_apply(indptr: np.ndarray, indices: np.ndarray, values: np.ndarray, state: np.ndarray,
+              rows: Optional[Tuple[int, int]] = None, nthreads: int = 1) -> np.ndarray:
+    """CsrGraph::apply_into (core/csr.rs:194-238): row-parallel gather in stored
+    column order.  With ``rows=(begin, end)`` only those rows are computed (the
+    others are zero)."""
+    ip = np.ascontiguousarray(indptr, dtype=np.uint64)
+    ix = np.ascontiguousarray(indices, dtype=np.uint64)
+    va = np.ascontiguousarray(values, dtype=np.complex128)
+    v = np.ascontiguousarray(state, dtype=np.complex128)
+    dim = len(ip) - 1
+    y = np.zeros(dim, dtype=np.complex128)
+    lo, hi = rows if rows is not None else (0, dim)
+    _check(lib().orc_csr_apply(dim, _p(ip, _u64p), _p(ix, _u64p), _p(va.view(np.float64), _dp),
+                               _p(v.view(np.float64), _dp), _p(y.view(np.float64), _dp), lo, hi, nthreads))
     return y
 
 

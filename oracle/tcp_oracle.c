@@ -456,11 +456,15 @@ static void packed_terms(uint32_t n, uint64_t T, const uint8_t *codes, const dou
     }
 }
 
-int orc_u1_lazy_apply(uint32_t n, uint32_t k, uint64_t T, const uint8_t *codes, const double *coef,
-                      const double *state, double *output, uint64_t *leak_source, uint32_t *leak_weight) {
+/* Sources [src_begin, src_end) only (a bounded sample for timing; the full
+   apply is the range [0, dim)).  The output is zero-filled in full. */
+int orc_u1_lazy_apply_range(uint32_t n, uint32_t k, uint64_t T, const uint8_t *codes, const double *coef,
+                            const double *state, double *output, uint64_t src_begin, uint64_t src_end,
+                            uint64_t *leak_source, uint32_t *leak_weight) {
     u1_sector s;
     int st = u1_new(n, k, &s);
     if (st) return st;
+    if (src_end > s.dim) src_end = s.dim;
     uint64_t *xq = (uint64_t *)malloc((T + 1) * sizeof(uint64_t));
     uint64_t *zq = (uint64_t *)malloc((T + 1) * sizeof(uint64_t));
     cplx *w = (cplx *)malloc((T + 1) * sizeof(cplx));
@@ -501,7 +505,7 @@ int orc_u1_lazy_apply(uint32_t n, uint32_t k, uint64_t T, const uint8_t *codes, 
     }
     for (uint64_t i = 0; i < 2 * s.dim; ++i) output[i] = 0.0;
     uint64_t src_word = 0;
-    for (uint64_t src = 0; src < s.dim && !st; ++src) {
+    for (uint64_t src = src_begin; src < src_end && !st; ++src) {
         st = u1_unrank_packed(&s, src, &src_word);
         if (st) break;
         cplx sv = {state[2 * src], state[2 * src + 1]};
@@ -533,6 +537,11 @@ int orc_u1_lazy_apply(uint32_t n, uint32_t k, uint64_t T, const uint8_t *codes, 
     free(xq); free(zq); free(w); free(groups); free(tg); free(next); free(gz); free(gw);
     u1_free(&s);
     return st;
+}
+
+int orc_u1_lazy_apply(uint32_t n, uint32_t k, uint64_t T, const uint8_t *codes, const double *coef,
+                      const double *state, double *output, uint64_t *leak_source, uint32_t *leak_weight) {
+    return orc_u1_lazy_apply_range(n, k, T, codes, coef, state, output, 0, UINT64_MAX, leak_source, leak_weight);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -688,17 +697,20 @@ int orc_blocks_unrank_many(uint32_t nblocks, const uint32_t *sizes, const uint32
 /* same way).  Exact-zero sums are skipped; a nonzero value whose            */
 /* destination is outside the sector is InvalidSector (leakage).            */
 /* ------------------------------------------------------------------------ */
-int orc_blocks_generic_apply(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops,
-                             uint64_t T, const uint8_t *codes, const double *coef,
-                             const double *state, double *output, uint64_t *leak_source) {
+/* Sources [col_begin, col_end) only (a bounded sample for timing). */
+int orc_blocks_generic_apply_range(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops,
+                                   uint64_t T, const uint8_t *codes, const double *coef,
+                                   const double *state, double *output, uint64_t col_begin, uint64_t col_end,
+                                   uint64_t *leak_source) {
     blocks_sector b;
     int st = blocks_new(nblocks, sizes, pops, &b);
     if (st) return st;
+    if (col_end > b.total_dim) col_end = b.total_dim;
     uint32_t n = b.n;
     uint64_t *keys = (uint64_t *)malloc((T + 1) * sizeof(uint64_t));
     cplx *vals = (cplx *)malloc((T + 1) * sizeof(cplx));
     for (uint64_t i = 0; i < 2 * b.total_dim; ++i) output[i] = 0.0;
-    for (uint64_t col = 0; col < b.total_dim && !st; ++col) {
+    for (uint64_t col = col_begin; col < col_end && !st; ++col) {
         uint64_t src;
         st = blocks_unrank(&b, col, &src);
         if (st) break;
@@ -753,6 +765,64 @@ int orc_blocks_generic_apply(uint32_t nblocks, const uint32_t *sizes, const uint
     free(keys);
     free(vals);
     return st;
+}
+
+int orc_blocks_generic_apply(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops,
+                             uint64_t T, const uint8_t *codes, const double *coef,
+                             const double *state, double *output, uint64_t *leak_source) {
+    return orc_blocks_generic_apply_range(nblocks, sizes, pops, T, codes, coef, state, output, 0, UINT64_MAX,
+                                          leak_source);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Eager restricted apply: CsrGraph::apply_into (core/csr.rs:194-238).  Each */
+/* output row is the row's entries in stored order, accumulated with the     */
+/* ManualRealImag arithmetic (re += a.re b.re - a.im b.im, im += a.re b.im + */
+/* a.im b.re); rows split over threads as rayon's par_windows does.  Rows    */
+/* [row_begin, row_end) only (a bounded sample; the others are untouched).   */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const uint64_t *indptr, *cols;
+    const double *vals, *state;
+    double *out;
+    uint64_t lo, hi;
+} csr_job;
+
+static void *csr_rows(void *arg) {
+    const csr_job *j = (const csr_job *)arg;
+    for (uint64_t r = j->lo; r < j->hi; ++r) {
+        double re = 0.0, im = 0.0;
+        for (uint64_t k = j->indptr[r]; k < j->indptr[r + 1]; ++k) {
+            const double cr = j->vals[2 * k], ci = j->vals[2 * k + 1];
+            const uint64_t c = j->cols[k];
+            const double sr = j->state[2 * c], si = j->state[2 * c + 1];
+            re += cr * sr - ci * si;
+            im += cr * si + ci * sr;
+        }
+        j->out[2 * r] = re;
+        j->out[2 * r + 1] = im;
+    }
+    return NULL;
+}
+
+int orc_csr_apply(uint64_t dim, const uint64_t *indptr, const uint64_t *cols, const double *vals,
+                  const double *state, double *output, uint64_t row_begin, uint64_t row_end, int nthreads) {
+    if (row_end > dim) row_end = dim;
+    if (row_begin > row_end) row_begin = row_end;
+    const uint64_t rows = row_end - row_begin;
+    if (nthreads < 1) nthreads = 1;
+    if ((uint64_t)nthreads > rows) nthreads = rows ? (int)rows : 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    csr_job *jobs = (csr_job *)malloc(sizeof(csr_job) * nthreads);
+    for (int i = 0; i < nthreads; ++i) {
+        jobs[i] = (csr_job){indptr, cols, vals, state, output, row_begin + rows * i / nthreads,
+                            row_begin + rows * (i + 1) / nthreads};
+        pthread_create(&th[i], NULL, csr_rows, &jobs[i]);
+    }
+    for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    free(th);
+    free(jobs);
+    return ORC_OK;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -251,3 +251,49 @@ def test_analyze_charge():
     assert not oracle.analyze_charge(3, pc, pf, levels)[0]
     # same letters on a qubit whose levels are equal are ignored
     assert oracle.analyze_charge(3, lc, lf, [(0.0, 0.0), (0.0, 1.0), (0.0, 1.0)])[0]
+
+
+def test_sector_applies_by_source_range_sum_to_the_full_apply():
+    # the bounded-sample forms used for CPU timing: disjoint source ranges are
+    # separate scatters whose outputs add up to the full apply
+    from tests.helpers import conserving_u1_terms
+
+    rng = np.random.default_rng(11)
+    n, k = 9, 4
+    codes, coef = conserving_u1_terms(rng, n, 20)
+    dim = oracle.u1_dimension(n, k)
+    v = rng.normal(size=dim) + 1j * rng.normal(size=dim)
+    full = oracle.u1_lazy_apply(n, k, codes, coef, v)
+    parts = sum(oracle.u1_lazy_apply(n, k, codes, coef, v, sources=(lo, hi))
+                for lo, hi in ((0, 17), (17, 90), (90, dim)))
+    np.testing.assert_allclose(parts, full, rtol=0, atol=1e-12)
+    from tencirpauli_amd import lattice
+
+    blocks = [(4, 2), (4, 2)]
+    bcodes, bcoef = lattice.fermi_hubbard_jw(2, 2, 1.0, 4.0, seed=5)  # conserves each spin block
+    bdim = oracle.blocks_dimension(blocks)
+    bv = rng.normal(size=bdim) + 1j * rng.normal(size=bdim)
+    bfull = oracle.blocks_generic_apply(blocks, bcodes, bcoef, bv)
+    bparts = sum(oracle.blocks_generic_apply(blocks, bcodes, bcoef, bv, sources=(lo, hi))
+                 for lo, hi in ((0, 50), (50, bdim)))
+    np.testing.assert_allclose(bparts, bfull, rtol=0, atol=1e-12)
+
+
+def test_csr_apply_matches_dense_product_and_row_ranges():
+    from scipy.sparse import csr_matrix
+
+    from tests.helpers import conserving_u1_terms
+
+    rng = np.random.default_rng(12)
+    n, k = 8, 3
+    codes, coef = conserving_u1_terms(rng, n, 18)
+    H = dense.restricted_dense(n, codes, coef, dense.u1_basis(n, k))
+    m = csr_matrix(H)
+    m.sort_indices()
+    v = rng.normal(size=H.shape[0]) + 1j * rng.normal(size=H.shape[0])
+    for threads in (1, 3):
+        y = oracle.csr_apply(m.indptr, m.indices, m.data, v, nthreads=threads)
+        np.testing.assert_allclose(y, H @ v, rtol=0, atol=1e-12)
+    part = oracle.csr_apply(m.indptr, m.indices, m.data, v, rows=(5, 20), nthreads=2)
+    np.testing.assert_allclose(part[5:20], (H @ v)[5:20], rtol=0, atol=1e-12)
+    assert not part[:5].any() and not part[20:].any()
