@@ -202,6 +202,20 @@ int tcp_u1_dimension(uint32_t nqubits, uint32_t particle_number, uint64_t *dim);
 int tcp_u1_rank(uint32_t nqubits, uint32_t particle_number, uint64_t bitstring, uint64_t *rank);
 int tcp_u1_unrank(uint32_t nqubits, uint32_t particle_number, uint64_t index,
                   uint64_t *bitstring);
+/* U1Sector::rank_words / unrank_into (core/sector.rs:127-238, wide sectors):
+ * the same order on packed words, ceil(n / 64) of them with qubit q in word
+ * q / 64, bit q % 64 (core/word.rs:53-110); n <= 1024.  A wrong weight or a
+ * set padding bit is TCP_ERR_INVALID_INDEX.  Sector plans of 64 or more
+ * qubits (U(1) only) run the same lazy, eager and Davidson paths on these
+ * words; their device rank/unrank/walk entry points are TCP_ERR_UNSUPPORTED. */
+int tcp_u1_rank_words(uint32_t nqubits, uint32_t particle_number, const uint64_t *words,
+                      uint64_t *rank);
+int tcp_u1_unrank_words(uint32_t nqubits, uint32_t particle_number, uint64_t index,
+                        uint64_t *words);
+/* U1Sector.basis_words_packed (py/symmetry.py:230-240): the whole basis in
+ * restricted order, `rows` = dimension rows of one integer bitstring (n <= 64)
+ * or of ceil(n / 64) packed words. */
+int tcp_u1_basis_words(uint32_t nqubits, uint32_t particle_number, uint64_t *out, uint64_t rows);
 
 /* Contiguous fixed-population blocks, the factorized ChargeSector layout
  * (core/charge_sector.rs:193-217, 419-456, 973-1038): block j covers the next

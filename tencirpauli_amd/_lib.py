@@ -92,6 +92,9 @@ SIGNATURES = {
     "tcp_u1_dimension": (_c.c_int, [_c.c_uint32, _c.c_uint32, _u64p]),
     "tcp_u1_rank": (_c.c_int, [_c.c_uint32, _c.c_uint32, _c.c_uint64, _u64p]),
     "tcp_u1_unrank": (_c.c_int, [_c.c_uint32, _c.c_uint32, _c.c_uint64, _u64p]),
+    "tcp_u1_rank_words": (_c.c_int, [_c.c_uint32, _c.c_uint32, _u64p, _u64p]),
+    "tcp_u1_unrank_words": (_c.c_int, [_c.c_uint32, _c.c_uint32, _c.c_uint64, _u64p]),
+    "tcp_u1_basis_words": (_c.c_int, [_c.c_uint32, _c.c_uint32, _u64p, _c.c_uint64]),
     "tcp_blocks_dimension": (_c.c_int, [_c.c_uint32, _u32p, _u32p, _u64p]),
     "tcp_blocks_rank": (_c.c_int, [_c.c_uint32, _u32p, _u32p, _c.c_uint64, _u64p]),
     "tcp_blocks_unrank": (_c.c_int, [_c.c_uint32, _u32p, _u32p, _c.c_uint64, _u64p]),

@@ -33,6 +33,7 @@
 #include <mutex>
 
 #include "tcp_internal.hpp"
+#include "wide_sector.hpp"
 
 namespace {
 
@@ -603,6 +604,7 @@ struct tcp_sector_plan {
     uint32_t leak_weight = 0;  // its destination weight (U(1) message)
     SectorArgs host_args{};
     tcp::DeviceBuffer gx, goff, tz, tw64, tw32, lut, choose;
+    std::unique_ptr<tcp::WideSector> wide;  // U(1) sectors of 64 or more qubits
     uint64_t retained_bytes = 0;
     std::mutex staging_mutex;
     tcp::DeviceBuffer stage_in, stage_out;
@@ -728,6 +730,32 @@ int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, u
         return fail(TCP_ERR_INCOMPATIBLE_QUBIT_COUNTS, "incompatible qubit counts: " +
                                                             std::to_string(op.nqubits) + " and " +
                                                             std::to_string(n));
+    if (u1 && spec.nblocks == 1 && n >= 64) {
+        // wide U(1) sector: packed-word bitstrings (see wide_sector.hpp)
+        auto plan = new tcp_sector_plan();
+        std::unique_ptr<tcp_sector_plan> guard(plan);
+        plan->wide.reset(new tcp::WideSector());
+        int st = tcp::wide_build(op, spec.pops[0], max_bytes, require_conserved, *plan->wide);
+        if (st) return st;
+        const tcp::WideSector &w = *plan->wide;
+        plan->n = n;
+        plan->nblocks = 1;
+        plan->sizes[0] = n;
+        plan->pops[0] = spec.pops[0];
+        plan->dim = w.geo.dim;
+        plan->T = w.T;
+        plan->G = w.G;
+        plan->diag_group = w.diag_group;
+        plan->energy_scale = w.energy_scale;
+        plan->leaks = w.leaks;
+        plan->leak_source = w.leak_source;
+        plan->leak_weight = w.leak_weight;
+        plan->u1 = true;
+        plan->retained_bytes = sizeof(tcp_sector_plan) + w.retained_bytes;
+        TCP_HIP_CHECK(hipGetDevice(&plan->device));
+        *out = guard.release();
+        return tcp::ok();
+    }
     SectorArgs a;
     uint64_t dim = 0;
     std::vector<uint32_t> lut;
@@ -890,6 +918,8 @@ int plan_apply(const tcp_sector_plan *plan, const void *in, void *out, int dtype
 }
 double plan_energy_scale(const tcp_sector_plan *plan) { return plan->energy_scale; }
 int plan_features(const tcp_sector_plan *plan, double scale, uint16_t *features, double *diag, hipStream_t stream) {
+    if (plan->wide)
+        return fail(TCP_ERR_UNSUPPORTED, "learned-preconditioner features support sectors of at most 63 qubits");
     if (plan->dim == 0) return TCP_OK;
     const uint32_t blocks = static_cast<uint32_t>((plan->dim + 255) / 256);
     hipLaunchKernelGGL(sector_features_kernel, dim3(blocks), dim3(256), 0, stream, plan->args(TCP_C128),
@@ -898,6 +928,7 @@ int plan_features(const tcp_sector_plan *plan, double scale, uint16_t *features,
     return e == hipSuccess ? TCP_OK : hip_fail(e, "sector features kernel");
 }
 int plan_diagonal(const tcp_sector_plan *plan, double *diag, hipStream_t stream) {
+    if (plan->wide) return wide_diagonal(*plan->wide, diag, stream);
     if (plan->dim == 0) return TCP_OK;
     const uint64_t threads = (plan->dim + kRanksPerThread - 1) / kRanksPerThread;
     const uint32_t blocks = static_cast<uint32_t>((threads + 255) / 256);
@@ -947,6 +978,7 @@ int apply_device(const tcp_sector_plan *plan, const void *in, void *out, int dty
     int st = dg.enter(plan->device);
     if (st) return st;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (plan->wide) return (st = tcp::wide_apply(*plan->wide, in, out, dtype, s)) ? st : tcp::ok();
     hipError_t e = dtype == TCP_C128 ? launch_gather<dvec2>(plan, in, out, dtype, s)
                                      : launch_gather<fvec2>(plan, in, out, dtype, s);
     if (e != hipSuccess) return tcp::hip_fail(e, "sector kernel launch");
@@ -1066,6 +1098,23 @@ int tcp_u1_unrank(uint32_t nqubits, uint32_t particle_number, uint64_t index, ui
     return st ? st : tcp::ok();
 }
 
+int tcp_u1_basis_words(uint32_t nqubits, uint32_t particle_number, uint64_t *out, uint64_t rows) {
+    tcp::WideGeometry g;
+    int st = tcp::wide_geometry(nqubits, particle_number, g);
+    if (st) return st;
+    if (!out && g.dim) return fail(TCP_ERR_INVALID_ARGUMENT, "null output");
+    if (rows != g.dim) return check_len(g.dim, rows);
+    if (nqubits <= 64) {
+        const BlockSpec spec = u1_spec(nqubits, particle_number);
+        for (uint64_t i = 0; i < g.dim; ++i)
+            if ((st = host_unrank(spec, i, out[i]))) return st;
+    } else {
+        for (uint64_t i = 0; i < g.dim; ++i)
+            if ((st = tcp::wide_unrank(g, i, out + i * g.words))) return st;
+    }
+    return tcp::ok();
+}
+
 int tcp_blocks_dimension(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops, uint64_t *dim) {
     if (!dim) return fail(TCP_ERR_INVALID_ARGUMENT, "null output");
     BlockSpec s;
@@ -1156,6 +1205,9 @@ int tcp_sector_plan_apply_host(const tcp_sector_plan *plan, const double *input,
 int tcp_sector_plan_rank_device(const tcp_sector_plan *plan, const uint64_t *bitstrings, uint64_t *ranks,
                                 uint64_t count, void *stream) {
     if (!plan || (count && (!bitstrings || !ranks))) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    if (plan->wide)
+        return fail(TCP_ERR_UNSUPPORTED, "device rank/unrank of sectors of 64 or more qubits is not supported; "
+                                         "use tcp_u1_rank_words / tcp_u1_unrank_words");
     if (count == 0) return tcp::ok();
     tcp::DeviceGuard dg;
     int st = dg.enter(plan->device);
@@ -1170,6 +1222,9 @@ int tcp_sector_plan_rank_device(const tcp_sector_plan *plan, const uint64_t *bit
 int tcp_sector_plan_unrank_device(const tcp_sector_plan *plan, const uint64_t *ranks, uint64_t *bitstrings,
                                   uint64_t count, void *stream) {
     if (!plan || (count && (!bitstrings || !ranks))) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    if (plan->wide)
+        return fail(TCP_ERR_UNSUPPORTED, "device rank/unrank of sectors of 64 or more qubits is not supported; "
+                                         "use tcp_u1_rank_words / tcp_u1_unrank_words");
     if (count == 0) return tcp::ok();
     tcp::DeviceGuard dg;
     int st = dg.enter(plan->device);
@@ -1184,6 +1239,9 @@ int tcp_sector_plan_unrank_device(const tcp_sector_plan *plan, const uint64_t *r
 int tcp_sector_plan_walk_device(const tcp_sector_plan *plan, uint64_t first, uint64_t *bitstrings,
                                 uint64_t count, uint32_t run, void *stream) {
     if (!plan || (count && !bitstrings) || run == 0) return fail(TCP_ERR_INVALID_ARGUMENT, "bad argument");
+    if (plan->wide)
+        return fail(TCP_ERR_UNSUPPORTED, "device rank/unrank of sectors of 64 or more qubits is not supported; "
+                                         "use tcp_u1_rank_words / tcp_u1_unrank_words");
     if (count == 0) return tcp::ok();
     if (first > plan->dim || count > plan->dim - first)
         return fail(TCP_ERR_INVALID_INDEX, "invalid index: restricted basis index is out of range");
@@ -1234,8 +1292,10 @@ int build_csr(const tcp_sector_plan *sp, uint64_t max_bytes, tcp_csr_plan **out)
     // pass 1: entries per row; exclusive scan on the host
     tcp::DeviceBuffer counts;
     int st;
-    if ((st = counts.allocate(4 * dim))) return st;
-    if (sp->lut_in_lds)
+    if ((st = counts.allocate(4 * std::max<uint64_t>(dim, 1)))) return st;
+    if (sp->wide) {
+        if ((st = tcp::wide_csr_counts(*sp->wide, static_cast<uint32_t *>(counts.ptr), nullptr))) return st;
+    } else if (sp->lut_in_lds)
         hipLaunchKernelGGL(csr_count_kernel<true>, dim3(blocks), dim3(kSectorThreads), lds, nullptr, a,
                            static_cast<uint32_t *>(counts.ptr));
     else
@@ -1257,7 +1317,12 @@ int build_csr(const tcp_sector_plan *sp, uint64_t max_bytes, tcp_csr_plan **out)
     if ((st = plan->cols.allocate(4 * plan->nnz))) return st;
     if ((st = plan->vals64.allocate(16 * plan->nnz))) return st;
     if ((st = plan->vals32.allocate(8 * plan->nnz))) return st;
-    if (sp->lut_in_lds)
+    if (sp->wide) {
+        if ((st = tcp::wide_csr_fill(*sp->wide, static_cast<const uint64_t *>(plan->indptr.ptr),
+                                     static_cast<uint32_t *>(plan->cols.ptr), plan->vals64.ptr, plan->vals32.ptr,
+                                     nullptr)))
+            return st;
+    } else if (sp->lut_in_lds)
         hipLaunchKernelGGL(csr_fill_kernel<true>, dim3(blocks), dim3(kSectorThreads), lds, nullptr, a,
                            static_cast<const uint64_t *>(plan->indptr.ptr), static_cast<uint32_t *>(plan->cols.ptr),
                            static_cast<dvec2 *>(plan->vals64.ptr), static_cast<fvec2 *>(plan->vals32.ptr));

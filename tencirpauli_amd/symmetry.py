@@ -42,6 +42,24 @@ def _coerce_bitstring(nqubits: int, bitstring: Any) -> int:
     return value
 
 
+def _word_count(nqubits: int) -> int:
+    return max(1, (nqubits + 63) // 64)
+
+
+def _packed_words(nqubits: int, value: int) -> np.ndarray:
+    """Integer bitstring (qubit 0 = MSB) -> packed words, qubit q in word q // 64,
+    bit q % 64 (core/word.rs:53-110)."""
+    words = np.zeros(_word_count(nqubits), dtype=np.uint64)
+    for q in range(nqubits):
+        if (value >> (nqubits - 1 - q)) & 1:
+            words[q >> 6] |= np.uint64(1 << (q & 63))
+    return words
+
+
+def _bits_of_words(nqubits: int, words: np.ndarray) -> Tuple[int, ...]:
+    return tuple(int((int(words[q >> 6]) >> (q & 63)) & 1) for q in range(nqubits))
+
+
 @dataclass(frozen=True)
 class U1Sector:
     """Fixed-Hamming-weight basis with TensorCircuit integer ordering."""
@@ -66,24 +84,69 @@ class U1Sector:
         value = _coerce_bitstring(self.nqubits, bitstring)
         if bin(value).count("1") != self.particle_number:
             raise ValueError("bitstring has the wrong Hamming weight")
-        if self.nqubits > 64:
-            raise OverflowError("ranking is limited to 64 qubits in this build")
         out = ctypes.c_uint64()
-        _lib.check(_lib.lib().tcp_u1_rank(self.nqubits, self.particle_number, value, ctypes.byref(out)))
+        if self.nqubits > 64:
+            words = _packed_words(self.nqubits, value)
+            _lib.check(_lib.lib().tcp_u1_rank_words(self.nqubits, self.particle_number,
+                                                    words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                    ctypes.byref(out)))
+        else:
+            _lib.check(_lib.lib().tcp_u1_rank(self.nqubits, self.particle_number, value, ctypes.byref(out)))
         return int(out.value)
+
+    def rank_words(self, words: Any) -> int:
+        """Rank of packed words (qubit q in word q // 64, bit q % 64), the form
+        of U1Sector::rank_words (core/sector.rs:127-238)."""
+        w = np.ascontiguousarray(words, dtype=np.uint64)
+        if w.shape != (_word_count(self.nqubits),):
+            raise ValueError(f"expected {_word_count(self.nqubits)} packed words, got shape {w.shape}")
+        out = ctypes.c_uint64()
+        _lib.check(_lib.lib().tcp_u1_rank_words(self.nqubits, self.particle_number,
+                                                w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(out)))
+        return int(out.value)
+
+    def unrank_words(self, index: int) -> np.ndarray:
+        """Packed words of a restricted basis index (U1Sector::unrank_into)."""
+        if not isinstance(index, (int, np.integer)) or isinstance(index, bool) or not 0 <= index < self.dimension:
+            raise IndexError("restricted basis index is out of range")
+        words = np.zeros(_word_count(self.nqubits), dtype=np.uint64)
+        _lib.check(_lib.lib().tcp_u1_unrank_words(self.nqubits, self.particle_number, int(index),
+                                                  words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return words
 
     def unrank(self, index: int) -> Tuple[int, ...]:
         if not isinstance(index, (int, np.integer)) or isinstance(index, bool) or not 0 <= index < self.dimension:
             raise IndexError("restricted basis index is out of range")
+        if self.nqubits > 64:
+            return _bits_of_words(self.nqubits, self.unrank_words(index))
         out = ctypes.c_uint64()
         _lib.check(_lib.lib().tcp_u1_unrank(self.nqubits, self.particle_number, int(index), ctypes.byref(out)))
         value = int(out.value)
         return tuple((value >> (self.nqubits - 1 - q)) & 1 for q in range(self.nqubits))
 
     def unrank_int(self, index: int) -> int:
+        """Integer bitstring (qubit 0 = MSB) of a restricted basis index."""
+        if self.nqubits > 64:
+            value = 0
+            for bit in self.unrank(index):
+                value = (value << 1) | bit
+            return value
         out = ctypes.c_uint64()
         _lib.check(_lib.lib().tcp_u1_unrank(self.nqubits, self.particle_number, int(index), ctypes.byref(out)))
         return int(out.value)
+
+    def basis_words_packed(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> np.ndarray:
+        """Read-only uint64 rows in restricted order: the integer bitstring for
+        n <= 64, else the packed words (py/symmetry.py:230-240)."""
+        validate_max_bytes(max_bytes)
+        width = _word_count(self.nqubits)
+        check_allocation(self.dimension * width * 8, max_bytes, "U1 packed basis")
+        out = np.zeros((self.dimension, width), dtype=np.uint64)
+        _lib.check(_lib.lib().tcp_u1_basis_words(self.nqubits, self.particle_number,
+                                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                 self.dimension))
+        out.flags.writeable = False
+        return out
 
     def basis_states(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> np.ndarray:
         """Read-only uint8 rows (qubit 0 first) in restricted order."""
