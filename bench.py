@@ -46,9 +46,9 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run one plain device copy (y <- v, 2^n amplitudes each way) before the MVPs, "
                         "the known-size reference for HBM counter scaling")
-    p.add_argument("--extras", default="fp32,sector,eager,hubbard_half,davidson",
-                   help="comma list of extra sections: fp32, sector, eager, hubbard_half, davidson, batched "
-                        "('' for none)")
+    p.add_argument("--extras", default="fp32,sector,eager,wide,hubbard_half,davidson",
+                   help="comma list of extra sections: fp32, sector, eager, wide, hubbard_half, davidson, "
+                        "batched ('' for none)")
     return p.parse_args()
 
 
@@ -294,6 +294,50 @@ def extra_eager(cpu_seconds: float = 0.0) -> dict:
     return rec
 
 
+def extra_wide_chain() -> dict:
+    """U(1) sectors of 128, 256 and 512 qubits with k = 2 (dims 8,128 / 32,640 / 130,816), open
+    hopping chain 0.5 (X_q X_q+1 + Y_q Y_q+1): eager CSR build and apply, lazy apply (complex128,
+    device-resident).  The reference records, for the same sizes on an arm64/macOS CPU, eager
+    setup of about 14.7 / 109 / 841 ms and a steady MVP of about 79 / 173 / 370 us
+    (docs/vibe/reports/phase-5-and-5.5-review-2026-08-02.md:76)."""
+    import torch
+
+    import tencirpauli_amd as tcp
+
+    published = {128: (14.7, 79.0), 256: (109.0, 173.0), 512: (841.0, 370.0)}
+    out = {}
+    warm = np.zeros((2, 70), dtype=np.uint8)
+    warm[0, :2] = 1
+    warm[1, :2] = 2
+    tcp.PauliOperator.from_code_arrays(warm, np.full(2, 0.5)).restrict_charge(tcp.U1Sector(70, 2), storage="eager")
+    for n in (128, 256, 512):
+        codes = []
+        for q in range(n - 1):
+            for c in (1, 2):
+                row = [0] * n
+                row[q] = row[q + 1] = c
+                codes.append(row)
+        op = tcp.PauliOperator.from_code_arrays(np.asarray(codes, dtype=np.uint8), np.full(len(codes), 0.5))
+        sector = tcp.U1Sector(n, 2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eager = op.restrict_charge(sector, storage="eager")
+        torch.cuda.synchronize()
+        build_ms = 1e3 * (time.perf_counter() - t0)
+        lazy = op.restrict_charge(sector)
+        gen = torch.Generator(device="cuda").manual_seed(SEED + n)
+        v = torch.randn(sector.dimension, dtype=torch.complex128, device="cuda", generator=gen)
+        y = torch.empty_like(v)
+        eager_us = 1e3 * time_applies(lambda: eager.apply_into(v, y), 200, warmup=5)
+        lazy_us = 1e3 * time_applies(lambda: lazy.apply_into(v, y), 200, warmup=5)
+        out[f"n{n}_k2"] = {"dimension": sector.dimension, "nnz": eager.transition_count,
+                           "eager_build_ms": round(build_ms, 3), "eager_mvp_us": round(eager_us, 2),
+                           "lazy_mvp_us": round(lazy_us, 2),
+                           "reference_published_setup_ms": published[n][0],
+                           "reference_published_mvp_us": published[n][1]}
+    return out
+
+
 def extra_davidson() -> dict:
     """5x5 periodic XXZ (25 qubits, S_z = 1/2, dim 5,200,300): seconds until the Ritz value stays
     within 1e-8 of the converged FP64 energy.  Plain FP64 Jacobi Davidson against the learned
@@ -487,7 +531,7 @@ def main() -> int:
     if "fp32" in extras and tag == "complex128":
         record["fp32_storage"] = extra_fp32(plan, v, dim)
     del y
-    if any(e in extras for e in ("sector", "eager", "hubbard_half", "davidson", "batched")):
+    if any(e in extras for e in ("sector", "eager", "wide", "hubbard_half", "davidson", "batched")):
         del v, plan
         torch.cuda.empty_cache()
     sector_cpu = 0.0 if args.no_cpu_baseline else args.cpu_seconds / 3
@@ -495,6 +539,8 @@ def main() -> int:
         record["sector_mvp"] = extra_sector(sector_cpu)
     if "eager" in extras:
         record["eager_csr_5x5_xxz"] = extra_eager(sector_cpu)
+    if "wide" in extras:
+        record["u1_wide_chain_k2"] = extra_wide_chain()
     if "hubbard_half" in extras:
         torch.cuda.empty_cache()
         record["hubbard_4x4_half_filling"] = extra_hubbard_half()
