@@ -240,6 +240,35 @@ def test_large_state_properties_n24():
     assert rel_err(hu_n[rows.astype(np.int64)], expected) <= FP64_TOL
 
 
+def test_config1_28q_full_size_rows_and_properties():
+    # the 28-qubit workload itself (4x7 XXZ + field + 1,885 random terms, 2^28
+    # complex128 amplitudes): linearity and Hermiticity on the device, and a
+    # row sample against the oracle's TermDirect rows
+    import torch
+
+    n = 28
+    codes, coef = lattice.heisenberg_4x7_with_random_terms(lattice.BASE_SEED, 1885)
+    op = tcp.PauliOperator.from_code_arrays(codes, coef)  # real coefficients: Hermitian
+    plan = op.native_mvp_plan()
+    g = torch.Generator(device="cuda").manual_seed(281)
+    u = torch.randn(1 << n, dtype=torch.complex128, device="cuda", generator=g)
+    w = torch.randn(1 << n, dtype=torch.complex128, device="cuda", generator=g)
+    hu, hw = plan.apply(u), plan.apply(w)
+    a, b = 0.3 - 1.1j, -0.7 + 0.2j
+    hlin = plan.apply(a * u + b * w)
+    lin_err = (torch.linalg.vector_norm(hlin - (a * hu + b * hw)) / torch.linalg.vector_norm(hlin)).item()
+    assert lin_err <= 1e-13
+    herm = abs((torch.vdot(w, hu) - torch.vdot(hw, u)).item())
+    assert herm <= 1e-12 * torch.linalg.vector_norm(hu).item() * torch.linalg.vector_norm(w).item()
+    del hw, hlin, w
+    rows = np.random.default_rng(282).choice(1 << n, size=2000, replace=False).astype(np.uint64)
+    u_host = u.cpu().numpy()
+    codes_c, coef_c = op.code_arrays()
+    expected = oracle.termdirect_rows(n, codes_c, coef_c, u_host, rows, nthreads=THREADS)
+    got = hu[torch.from_numpy(rows.astype(np.int64)).cuda()].cpu().numpy()
+    assert rel_err(got, expected) <= FP64_TOL
+
+
 def test_concurrent_host_applies_from_threads():
     codes, coef = lattice.random_pauli_sum(12, 100, seed=71)
     plan = tcp.PauliOperator.from_code_arrays(codes, coef).native_mvp_plan()
