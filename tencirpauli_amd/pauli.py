@@ -218,6 +218,23 @@ class PauliOperator:
             return restrict_charge(self, sector, storage=storage, max_bytes=max_bytes)
         raise TypeError("sector must be a U1Sector or a ChargeSector")
 
+    def restrict_u1(self, sector: Any, *, storage: str = "lazy",
+                    max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        """Deprecated alias of ``restrict_charge`` for a ``U1Sector``
+        (py/pauli.py:803-828): same checks, same DeprecationWarning."""
+        import warnings
+
+        from .symmetry import U1Sector
+
+        warnings.warn("PauliOperator.restrict_u1() is deprecated; use restrict_charge(U1Sector(...)) instead",
+                      DeprecationWarning, stacklevel=2)
+        validate_max_bytes(max_bytes)
+        if not isinstance(sector, U1Sector):
+            raise TypeError(f"expected U1Sector, got {type(sector).__name__}")
+        if sector.nqubits != self.nqubits:
+            raise ValueError(f"expected sector for {self.nqubits} qubits, got {sector.nqubits}")
+        return self.restrict_charge(sector, storage=storage, max_bytes=max_bytes)
+
     # -- MVP --------------------------------------------------------------
     def mvp(self, state: Any, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
         """One-shot H @ state (core/hamiltonian.rs:481-483); output is new."""

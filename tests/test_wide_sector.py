@@ -98,3 +98,13 @@ def test_wide_oracle_sparse_matches_dense_on_a_narrow_case():
     for (r, c), v in wide.restricted_sparse(9, 4, codes, coef).items():
         got[r, c] = v
     np.testing.assert_allclose(got, H, rtol=0, atol=1e-13)
+
+
+def test_restrict_u1_is_a_deprecated_alias_with_checks():
+    op = tcp.PauliOperator.from_terms(3, [("XXI", 1.0), ("YYI", 1.0)])
+    with pytest.warns(DeprecationWarning, match="restrict_charge"):
+        with pytest.raises(TypeError, match="expected U1Sector"):
+            op.restrict_u1("not a sector")
+    with pytest.warns(DeprecationWarning):
+        with pytest.raises(ValueError, match="expected sector for 3 qubits, got 4"):
+            op.restrict_u1(tcp.U1Sector(4, 1))
