@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes
 import sys
+from dataclasses import dataclass
 from typing import Any, Optional, Sequence
 
 import numpy as np
@@ -118,6 +119,42 @@ def _dptr(array: np.ndarray) -> Any:
     return array.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+@dataclass(frozen=True)
+class NativePlanMetadata:
+    """The metadata every native MVP plan exposes (py/_native_mvp.py:84-150)."""
+
+    dimension: int
+    term_count: int
+    storage: str
+    basis_ordering: str
+    target: str
+    execution_family: str
+    native_variant: str
+    retained_format: str
+    source_term_count: int
+    schedule_count: int
+    estimated_retained_bytes: int
+    estimated_construction_peak_bytes: int
+    transition_count: Optional[int]
+
+    def __post_init__(self) -> None:
+        for name in ("dimension", "term_count", "source_term_count", "schedule_count", "estimated_retained_bytes",
+                     "estimated_construction_peak_bytes"):
+            value = getattr(self, name)
+            if not isinstance(value, int) or isinstance(value, bool) or value < 0:
+                raise ValueError(f"{name} must be a non-negative integer")
+        if self.storage not in {"lazy", "eager"}:
+            raise ValueError("storage must be either 'eager' or 'lazy'")
+        if self.target != "native_mvp":
+            raise ValueError("native MVP plans must have target='native_mvp'")
+        for name in ("basis_ordering", "execution_family", "native_variant", "retained_format"):
+            if not isinstance(getattr(self, name), str) or not getattr(self, name):
+                raise ValueError(f"{name} must be a non-empty string")
+        if self.transition_count is not None and (not isinstance(self.transition_count, int)
+                                                  or self.transition_count < 0):
+            raise ValueError("transition_count must be None or a non-negative integer")
+
+
 class NativePlanFacade:
     """apply / apply_into / __call__ shared by full-space and sector plans.
 
@@ -126,6 +163,25 @@ class NativePlanFacade:
     """
 
     dimension: int
+
+    @property
+    def metadata(self) -> NativePlanMetadata:
+        """The plan's metadata record (the fields of the reference's facades)."""
+        term_count = int(getattr(self, "term_count", 0))
+        retained = int(getattr(self, "estimated_retained_bytes", getattr(self, "estimated_bytes", 0)))
+        transition = getattr(self, "transition_count", None)
+        return NativePlanMetadata(
+            dimension=int(self.dimension), term_count=term_count, storage=str(getattr(self, "storage", "lazy")),
+            basis_ordering=str(getattr(self, "basis_ordering", "qubit0_msb_matrix")),
+            target=str(getattr(self, "target", "native_mvp")),
+            execution_family=str(getattr(self, "execution_family", "")),
+            native_variant=str(getattr(self, "native_variant", "")),
+            retained_format=str(getattr(self, "retained_format", "compiled_actions")),
+            source_term_count=int(getattr(self, "source_term_count", term_count)),
+            schedule_count=int(getattr(self, "schedule_count", term_count)),
+            estimated_retained_bytes=retained,
+            estimated_construction_peak_bytes=int(getattr(self, "estimated_construction_peak_bytes", retained)),
+            transition_count=None if transition is None else int(transition))
 
     def _native_apply_host(self, inp: np.ndarray, out: np.ndarray) -> int:  # pragma: no cover
         raise NotImplementedError

@@ -74,6 +74,7 @@ class CsrMvpPlan(NativePlanFacade):
     storage = "eager"
     retained_format = "csr"
     basis_ordering = "qubit0_msb_matrix"
+    target = "native_mvp"
 
     def __init__(self, ptr: ctypes.c_void_p, native_variant: str, execution_family: str, term_count: int) -> None:
         self._ptr = ptr
@@ -88,6 +89,7 @@ class CsrMvpPlan(NativePlanFacade):
         # the reference's retained-bytes formula for eager U1 plans (py/symmetry.py:565-569)
         self.estimated_bytes = self.dimension * 16 + (self.dimension + 1) * 8 + self.transition_count * 32
         self.estimated_retained_bytes = self.estimated_bytes
+        self.estimated_construction_peak_bytes = self.estimated_bytes
 
     def __del__(self) -> None:
         if getattr(self, "_ptr", None) and _lib._lib is not None:

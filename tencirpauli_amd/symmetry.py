@@ -172,6 +172,12 @@ class SectorMvpPlan(NativePlanFacade):
         self.native_variant = native_variant
         self.execution_family = execution_family
         self.storage = "lazy"
+        self.retained_format = "compiled_actions"
+        self.basis_ordering = "qubit0_msb_matrix"
+        self.target = "native_mvp"
+        self.schedule_count = self.term_count
+        self.estimated_retained_bytes = self.estimated_bytes
+        self.estimated_construction_peak_bytes = self.estimated_bytes
         self.transition_count: Optional[int] = None
 
     def __del__(self) -> None:
@@ -332,7 +338,7 @@ def _lazy_u1_plan(operator: Any, sector: U1Sector, max_bytes: Optional[int]) -> 
     out = ctypes.c_void_p()
     _lib.check(_lib.lib().tcp_sector_plan_create_u1(operator._handle.ptr, sector.particle_number,
                                                      effective_max_bytes(max_bytes), ctypes.byref(out)))
-    return SectorMvpPlan(out, "qubit", "u1")
+    return SectorMvpPlan(out, "qubit", "product")
 
 
 def restrict_u1(operator: Any, sector: U1Sector, max_bytes: Optional[int], *,

@@ -148,3 +148,26 @@ def test_materialization_budget_and_errors():
         big = lattice.heisenberg_periodic(4, 4, seed=1)
         tcp.PauliOperator.from_code_arrays(*big).restrict_charge(tcp.U1Sector(16, 8), storage="eager").dense(
             max_bytes=1 << 30)
+
+
+def test_plan_metadata_records():
+    # the fields and values of the reference's plan facades (py/_native_mvp.py:84-150,
+    # py/symmetry.py:541-590 for U(1): execution_family "product", variant "qubit")
+    op = tcp.PauliOperator.from_terms(4, [("XXII", 1.0), ("YYII", 1.0), ("IIZZ", 0.3), ("ZIII", -0.2)])
+    full = op.native_mvp_plan().metadata
+    assert (full.execution_family, full.native_variant, full.retained_format, full.storage) == \
+        ("pauli", "pauli", "compiled_actions", "lazy")
+    assert full.target == "native_mvp" and full.basis_ordering == "qubit0_msb_matrix"
+    assert full.dimension == 16 and full.term_count == 4 and full.transition_count is None
+    sector = tcp.U1Sector(4, 2)
+    lazy = op.restrict_charge(sector).mvp_plan().metadata
+    assert (lazy.execution_family, lazy.native_variant, lazy.retained_format, lazy.storage) == \
+        ("product", "qubit", "compiled_actions", "lazy")
+    assert lazy.transition_count is None and lazy.schedule_count == lazy.term_count == 4
+    eager = op.restrict_charge(sector, storage="eager").mvp_plan(storage="eager").metadata
+    assert (eager.execution_family, eager.native_variant, eager.retained_format, eager.storage) == \
+        ("product", "qubit", "csr", "eager")
+    D, nnz = 6, eager.transition_count
+    assert nnz == int(np.count_nonzero(op.restrict_charge(sector, storage="eager").dense()))
+    assert eager.estimated_retained_bytes == D * 16 + (D + 1) * 8 + nnz * 32
+    assert eager.estimated_construction_peak_bytes == eager.estimated_retained_bytes
