@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import threading
 from dataclasses import dataclass, field
 from typing import Any, Dict, Optional, Sequence, Tuple
 
@@ -163,6 +164,7 @@ class ChargeRestrictedOperator(NativePlanFacade):
         self.term_count = operator.term_count
         self._plan = plan
         self._csr = csr_plan
+        self._lock = threading.Lock()
         self.native_variant = "factorized"
         self.execution_family = "factorized"
 
@@ -175,17 +177,19 @@ class ChargeRestrictedOperator(NativePlanFacade):
         return None if self._csr is None else self._csr.transition_count
 
     def _ensure_lazy(self, max_bytes: Optional[int]) -> Any:
-        if self._plan is None:
-            self._plan = _lazy_blocks_plan(self.operator, self.sector, max_bytes)
-        return self._plan
+        with self._lock:  # a concurrent first use builds one plan
+            if self._plan is None:
+                self._plan = _lazy_blocks_plan(self.operator, self.sector, max_bytes)
+            return self._plan
 
     def _ensure_eager(self, max_bytes: Optional[int]) -> Any:
-        if self._csr is None:
-            from .eager import csr_plan_blocks
+        with self._lock:
+            if self._csr is None:
+                from .eager import csr_plan_blocks
 
-            sizes, pops = self.sector._arrays()
-            self._csr = csr_plan_blocks(self.operator, sizes, pops, max_bytes)
-        return self._csr
+                sizes, pops = self.sector._arrays()
+                self._csr = csr_plan_blocks(self.operator, sizes, pops, max_bytes)
+            return self._csr
 
     def mvp_plan(self, *, storage: str = "lazy", max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
         validate_max_bytes(max_bytes)

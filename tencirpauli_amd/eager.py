@@ -57,6 +57,11 @@ class CSRMatrix:
         return csr_matrix((self.data, self.indices, self.indptr), shape=self.shape)
 
 
+def _frozen(array: np.ndarray) -> np.ndarray:
+    array.flags.writeable = False
+    return array
+
+
 def materialization_bytes(dimension: int, transitions: int, target: str) -> int:
     """py/symmetry.py:646-655."""
     if target == "dense":
@@ -126,13 +131,27 @@ class CsrMvpPlan(NativePlanFacade):
                          f"U1 {target} materialization")
         indptr, indices, values = self.csr_arrays()
         if target == "csr":
-            return CSRMatrix(indptr, indices, values, (n, n))
+            return CSRMatrix(_frozen(indptr), _frozen(indices), _frozen(values), (n, n))
         rows = np.repeat(np.arange(n, dtype=np.uint64), np.diff(indptr).astype(np.int64))
         if target == "coo":
-            return COOMatrix(rows, indices, values, (n, n))
+            return COOMatrix(_frozen(rows), _frozen(indices), _frozen(values), (n, n))
         dense = np.zeros((n, n), dtype=np.complex128)
         dense[rows.astype(np.int64), indices.astype(np.int64)] = values
         return dense
+
+    # the retained arrays, read back on demand and read-only (py/symmetry.py
+    # eager plans expose indptr / columns / coefficients this way)
+    @property
+    def indptr(self) -> np.ndarray:
+        return _frozen(self.csr_arrays()[0])
+
+    @property
+    def columns(self) -> np.ndarray:
+        return _frozen(self.csr_arrays()[1])
+
+    @property
+    def coefficients(self) -> np.ndarray:
+        return _frozen(self.csr_arrays()[2])
 
 
 def csr_plan_u1(operator: Any, particle_number: int, max_bytes: Optional[int]) -> CsrMvpPlan:

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import threading
 from dataclasses import dataclass
 from typing import Any, Optional, Sequence, Tuple
 
@@ -248,6 +249,7 @@ class U1RestrictedOperator(NativePlanFacade):
         self.term_count = operator.term_count
         self._plan = plan
         self._csr = csr_plan
+        self._lock = threading.Lock()
         self.native_variant = "qubit"
 
     @property
@@ -264,16 +266,18 @@ class U1RestrictedOperator(NativePlanFacade):
         return lazy + (0 if self._csr is None else self._csr.transition_count * 32)
 
     def _ensure_lazy(self, max_bytes: Optional[int]) -> SectorMvpPlan:
-        if self._plan is None:
-            self._plan = _lazy_u1_plan(self.operator, self.sector, max_bytes)
-        return self._plan
+        with self._lock:  # a concurrent first use builds one plan
+            if self._plan is None:
+                self._plan = _lazy_u1_plan(self.operator, self.sector, max_bytes)
+            return self._plan
 
     def _ensure_eager(self, max_bytes: Optional[int]) -> Any:
-        if self._csr is None:
-            from .eager import csr_plan_u1
+        with self._lock:
+            if self._csr is None:
+                from .eager import csr_plan_u1
 
-            self._csr = csr_plan_u1(self.operator, self.sector.particle_number, max_bytes)
-        return self._csr
+                self._csr = csr_plan_u1(self.operator, self.sector.particle_number, max_bytes)
+            return self._csr
 
     def mvp_plan(self, *, storage: str = "lazy", max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
         """Reusable restricted plan (py/symmetry.py:344-382)."""

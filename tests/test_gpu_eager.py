@@ -171,3 +171,27 @@ def test_plan_metadata_records():
     assert nnz == int(np.count_nonzero(op.restrict_charge(sector, storage="eager").dense()))
     assert eager.estimated_retained_bytes == D * 16 + (D + 1) * 8 + nnz * 32
     assert eager.estimated_construction_peak_bytes == eager.estimated_retained_bytes
+
+
+def test_materialization_is_cached_read_only_and_leaves_lazy_plans_lazy():
+    # tests/test_native_mvp_resources.py:47-88 of the reference, on a Pauli operator
+    from concurrent.futures import ThreadPoolExecutor
+
+    op = tcp.PauliOperator.from_terms(6, [("XXIIII", 1.0), ("YYIIII", 1.0), ("IIXXII", 0.5), ("IIYYII", 0.5),
+                                          ("ZIIIIZ", 0.2)])
+    restricted = op.restrict_charge(tcp.U1Sector(6, 3))
+    lazy_plan = restricted.mvp_plan()
+    before = restricted.estimated_bytes
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        matrices = list(pool.map(lambda _: restricted.csr(max_bytes=None), range(4)))
+    assert all(m.shape == (20, 20) for m in matrices)
+    assert restricted.estimated_bytes > before
+    assert lazy_plan.storage == "lazy" and restricted.mvp_plan() is lazy_plan
+    eager = restricted.mvp_plan(storage="eager")
+    assert restricted.mvp_plan(storage="eager") is eager
+    first = restricted.csr()
+    assert first.data.flags.writeable is False and first.indices.flags.writeable is False
+    assert eager.indptr.flags.writeable is False
+    assert eager.columns.flags.writeable is False and eager.coefficients.flags.writeable is False
+    np.testing.assert_array_equal(eager.indptr, first.indptr)
+    np.testing.assert_array_equal(eager.coefficients, first.data)

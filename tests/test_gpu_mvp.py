@@ -300,3 +300,51 @@ def test_scipy_eigsh_matches_dense_spectrum():
     values = eigsh(linear, k=3, which="SA", return_eigenvectors=False, tol=1e-13, v0=np.ones(256))
     exact = np.linalg.eigvalsh(dense.dense_operator(8, *op.code_arrays()))[:3]
     np.testing.assert_allclose(np.sort(values), exact, atol=1e-10)
+
+
+def test_scipy_linear_operator_contracts():
+    # tests/test_scipy_linear_operator.py:26-101 of the reference, on this API
+    op = tcp.PauliOperator.from_terms(2, (("XX", 0.5), ("ZI", -1.25j)))
+    plan = op.native_mvp_plan()
+    linear = plan.to_scipy_linear_operator()
+    rng = np.random.default_rng(20260807)
+    vector = rng.normal(size=4) + 1j * rng.normal(size=4)
+    np.testing.assert_allclose(linear.matvec(vector), plan.apply(vector))
+    np.testing.assert_allclose(linear.matvec(vector[:, None]), plan.apply(vector)[:, None])
+    columns = np.column_stack((vector, 2.0 * vector))
+    np.testing.assert_allclose(linear.matmat(columns), np.column_stack((plan.apply(vector), plan.apply(2 * vector))))
+    assert linear.shape == (4, 4) and linear.dtype == np.dtype(np.complex128)
+    for bad in (np.ones(3), np.ones((3, 1)), np.ones((1, 4)), np.ones((4, 1, 1))):
+        with pytest.raises(ValueError, match=r"dimension mismatch|shape"):
+            linear.matvec(bad)
+    # the convenience form compiles one plan and reuses it
+    calls = []
+    original = tcp.PauliOperator.native_mvp_plan
+
+    def counted(self, **kwargs):
+        calls.append(1)
+        return original(self, **kwargs)
+
+    op2 = tcp.PauliOperator.from_terms(2, (("XX", 1.0), ("IZ", 0.25)))
+    expected = op2.native_mvp_plan().apply(vector)
+    tcp.PauliOperator.native_mvp_plan = counted
+    try:
+        lin2 = op2.to_scipy_linear_operator()
+        np.testing.assert_allclose(lin2 @ vector, expected)
+        np.testing.assert_allclose(lin2 @ (2.0 * vector), 2.0 * expected)
+    finally:
+        tcp.PauliOperator.native_mvp_plan = original
+    assert len(calls) == 1
+    with pytest.raises(TypeError):
+        op2.to_scipy_linear_operator(mapping="jordan_wigner")
+    np.testing.assert_allclose(op2.native_mvp_plan().to_scipy_linear_operator(max_bytes=1).matvec(vector), expected)
+    assert not hasattr(op2.backend_mvp_plan(), "to_scipy_linear_operator")
+    # restricted plans, both storages; no adjoint product
+    hop = tcp.PauliOperator.from_terms(2, (("XX", 0.5), ("YY", 0.5), ("ZI", 0.3)))
+    for storage in ("lazy", "eager"):
+        rplan = hop.restrict_charge(tcp.U1Sector(2, 1), storage=storage).mvp_plan(storage=storage)
+        lin = rplan.to_scipy_linear_operator()
+        v2 = np.array([1.0 + 2.0j, -0.5 + 0.25j])
+        np.testing.assert_allclose(lin.matvec(v2), rplan.apply(v2))
+        with pytest.raises(NotImplementedError):
+            lin.rmatvec(v2)
