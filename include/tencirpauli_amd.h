@@ -292,6 +292,14 @@ int tcp_csr_plan_create_u1(const tcp_operator *op, uint32_t particle_number, uin
                            tcp_csr_plan **out);
 int tcp_csr_plan_create_blocks(const tcp_operator *op, uint32_t nblocks, const uint32_t *sizes,
                                const uint32_t *pops, uint64_t max_bytes, tcp_csr_plan **out);
+/* The full-space matrix as the same device CSR: replaces
+ * PauliOperator::coo_matrix / csr_matrix (core/hamiltonian.rs:360-460) and the
+ * pauli_coo_handle / pauli_csr_handle bindings behind py/pauli.py:884-928.
+ * Row r holds, for each distinct X mask x (ascending), column r ^ x with value
+ * sum_t w_t (-1)^popcount(z_t & column) over the group's terms in canonical
+ * order; exact zeros are dropped and columns ascend.  nqubits <= 32
+ * (TCP_ERR_UNSUPPORTED beyond: columns are 32-bit). */
+int tcp_csr_plan_create_full(const tcp_operator *op, uint64_t max_bytes, tcp_csr_plan **out);
 uint64_t tcp_csr_plan_dimension(const tcp_csr_plan *plan);
 uint64_t tcp_csr_plan_transition_count(const tcp_csr_plan *plan); /* stored entries */
 uint64_t tcp_csr_plan_estimated_bytes(const tcp_csr_plan *plan);

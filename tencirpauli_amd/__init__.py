@@ -10,7 +10,7 @@ include/tencirpauli_amd.h); there is no CPU fallback.
 from ._lib import DeviceError, NativeLibraryError, UnsupportedOnDeviceError, device_count
 from ._plan import DEFAULT_MAX_BYTES
 from .charge import AdditiveCharge, ChargeRestrictedOperator, ChargeSector
-from .pauli import BackendPlanArrays, ChargeAnalysis, NativeMVPPlan, PauliOperator, PauliTerm
+from .pauli import BackendMVPPlan, BackendPlanArrays, ChargeAnalysis, NativeMVPPlan, PauliOperator, PauliTerm
 from .symmetry import SectorMvpPlan, U1RestrictedOperator, U1Sector
 
 __version__ = "0.1.0"
@@ -18,6 +18,7 @@ __version__ = "0.1.0"
 __all__ = [
     "DEFAULT_MAX_BYTES",
     "AdditiveCharge",
+    "BackendMVPPlan",
     "BackendPlanArrays",
     "ChargeAnalysis",
     "ChargeRestrictedOperator",

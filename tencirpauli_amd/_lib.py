@@ -115,6 +115,7 @@ SIGNATURES = {
     "tcp_csr_plan_create_u1": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint64, _c.POINTER(_vp)]),
     "tcp_csr_plan_create_blocks": (_c.c_int, [_vp, _c.c_uint32, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint32),
                                                _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_csr_plan_create_full": (_c.c_int, [_vp, _c.c_uint64, _c.POINTER(_vp)]),
     "tcp_csr_plan_dimension": (_c.c_uint64, [_vp]),
     "tcp_csr_plan_transition_count": (_c.c_uint64, [_vp]),
     "tcp_csr_plan_estimated_bytes": (_c.c_uint64, [_vp]),

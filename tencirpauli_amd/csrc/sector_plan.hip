@@ -32,6 +32,7 @@
 #include <memory>
 #include <mutex>
 
+#include "full_matrix.hpp"
 #include "tcp_internal.hpp"
 #include "wide_sector.hpp"
 
@@ -1279,32 +1280,23 @@ namespace {
 
 constexpr uint32_t kCsrLanes = 8;  // lanes per row in the SpMV (rows average ~27 entries on lattice models)
 
-int build_csr(const tcp_sector_plan *sp, uint64_t max_bytes, tcp_csr_plan **out) {
-    if (sp->leaks) return leak_error(sp);
+// Two-pass CSR assembly: `count` writes the entries of every row, the host
+// scans them into indptr, `fill` writes each row's columns (ascending) and
+// values in FP64 and FP32.
+template <class Count, class Fill>
+int assemble_csr(uint32_t n, uint64_t dim, int device, uint64_t max_bytes, Count count, Fill fill,
+                 tcp_csr_plan **out) {
     auto plan = new tcp_csr_plan();
     std::unique_ptr<tcp_csr_plan> guard(plan);
-    plan->n = sp->n;
-    plan->dim = sp->dim;
-    plan->device = sp->device;
-    const uint64_t dim = sp->dim;
-    const SectorArgs a = sp->args(TCP_C128);
-    const uint64_t per_block = static_cast<uint64_t>(kSectorThreads) * kRanksPerThread;
-    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((dim + per_block - 1) / per_block, 256 * 16));
-    const size_t lds = sp->lut_in_lds ? a.lut_words * 4 : 0;
+    plan->n = n;
+    plan->dim = dim;
+    plan->device = device;
 
     // pass 1: entries per row; exclusive scan on the host
     tcp::DeviceBuffer counts;
     int st;
     if ((st = counts.allocate(4 * std::max<uint64_t>(dim, 1)))) return st;
-    if (sp->wide) {
-        if ((st = tcp::wide_csr_counts(*sp->wide, static_cast<uint32_t *>(counts.ptr), nullptr))) return st;
-    } else if (sp->lut_in_lds)
-        hipLaunchKernelGGL(csr_count_kernel<true>, dim3(blocks), dim3(kSectorThreads), lds, nullptr, a,
-                           static_cast<uint32_t *>(counts.ptr));
-    else
-        hipLaunchKernelGGL(csr_count_kernel<false>, dim3(blocks), dim3(kSectorThreads), 0, nullptr, a,
-                           static_cast<uint32_t *>(counts.ptr));
-    TCP_HIP_CHECK(hipGetLastError());
+    if ((st = count(static_cast<uint32_t *>(counts.ptr)))) return st;
     std::vector<uint32_t> hcounts(dim);
     TCP_HIP_CHECK(hipMemcpy(hcounts.data(), counts.ptr, 4 * dim, hipMemcpyDeviceToHost));
     counts.release();
@@ -1317,26 +1309,45 @@ int build_csr(const tcp_sector_plan *sp, uint64_t max_bytes, tcp_csr_plan **out)
 
     // pass 2: fill rows, columns ascending
     if ((st = plan->indptr.upload(hptr.data(), 8 * (dim + 1)))) return st;
-    if ((st = plan->cols.allocate(4 * plan->nnz))) return st;
-    if ((st = plan->vals64.allocate(16 * plan->nnz))) return st;
-    if ((st = plan->vals32.allocate(8 * plan->nnz))) return st;
-    if (sp->wide) {
-        if ((st = tcp::wide_csr_fill(*sp->wide, static_cast<const uint64_t *>(plan->indptr.ptr),
-                                     static_cast<uint32_t *>(plan->cols.ptr), plan->vals64.ptr, plan->vals32.ptr,
-                                     nullptr)))
-            return st;
-    } else if (sp->lut_in_lds)
-        hipLaunchKernelGGL(csr_fill_kernel<true>, dim3(blocks), dim3(kSectorThreads), lds, nullptr, a,
-                           static_cast<const uint64_t *>(plan->indptr.ptr), static_cast<uint32_t *>(plan->cols.ptr),
-                           static_cast<dvec2 *>(plan->vals64.ptr), static_cast<fvec2 *>(plan->vals32.ptr));
-    else
-        hipLaunchKernelGGL(csr_fill_kernel<false>, dim3(blocks), dim3(kSectorThreads), 0, nullptr, a,
-                           static_cast<const uint64_t *>(plan->indptr.ptr), static_cast<uint32_t *>(plan->cols.ptr),
-                           static_cast<dvec2 *>(plan->vals64.ptr), static_cast<fvec2 *>(plan->vals32.ptr));
-    TCP_HIP_CHECK(hipGetLastError());
+    if ((st = plan->cols.allocate(4 * std::max<uint64_t>(plan->nnz, 1)))) return st;
+    if ((st = plan->vals64.allocate(16 * std::max<uint64_t>(plan->nnz, 1)))) return st;
+    if ((st = plan->vals32.allocate(8 * std::max<uint64_t>(plan->nnz, 1)))) return st;
+    if ((st = fill(static_cast<const uint64_t *>(plan->indptr.ptr), static_cast<uint32_t *>(plan->cols.ptr),
+                   plan->vals64.ptr, plan->vals32.ptr)))
+        return st;
     TCP_HIP_CHECK(hipDeviceSynchronize());
     *out = guard.release();
     return TCP_OK;
+}
+
+int build_csr(const tcp_sector_plan *sp, uint64_t max_bytes, tcp_csr_plan **out) {
+    if (sp->leaks) return leak_error(sp);
+    const uint64_t dim = sp->dim;
+    const SectorArgs a = sp->args(TCP_C128);
+    const uint64_t per_block = static_cast<uint64_t>(kSectorThreads) * kRanksPerThread;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((dim + per_block - 1) / per_block, 256 * 16));
+    const size_t lds = sp->lut_in_lds ? a.lut_words * 4 : 0;
+    auto count = [&](uint32_t *counts) -> int {
+        if (sp->wide) return tcp::wide_csr_counts(*sp->wide, counts, nullptr);
+        if (sp->lut_in_lds)
+            hipLaunchKernelGGL(csr_count_kernel<true>, dim3(blocks), dim3(kSectorThreads), lds, nullptr, a, counts);
+        else
+            hipLaunchKernelGGL(csr_count_kernel<false>, dim3(blocks), dim3(kSectorThreads), 0, nullptr, a, counts);
+        TCP_HIP_CHECK(hipGetLastError());
+        return TCP_OK;
+    };
+    auto fill = [&](const uint64_t *indptr, uint32_t *cols, void *v64, void *v32) -> int {
+        if (sp->wide) return tcp::wide_csr_fill(*sp->wide, indptr, cols, v64, v32, nullptr);
+        if (sp->lut_in_lds)
+            hipLaunchKernelGGL(csr_fill_kernel<true>, dim3(blocks), dim3(kSectorThreads), lds, nullptr, a, indptr,
+                               cols, static_cast<dvec2 *>(v64), static_cast<fvec2 *>(v32));
+        else
+            hipLaunchKernelGGL(csr_fill_kernel<false>, dim3(blocks), dim3(kSectorThreads), 0, nullptr, a, indptr,
+                               cols, static_cast<dvec2 *>(v64), static_cast<fvec2 *>(v32));
+        TCP_HIP_CHECK(hipGetLastError());
+        return TCP_OK;
+    };
+    return assemble_csr(sp->n, dim, sp->device, max_bytes, count, fill, out);
 }
 
 int csr_apply_device(const tcp_csr_plan *plan, const void *in, void *out, int dtype, void *stream) {
@@ -1430,6 +1441,24 @@ int tcp_csr_plan_create_blocks(const tcp_operator *op, uint32_t nblocks, const u
     BlockSpec s;
     int st = blocks_spec(nblocks, sizes, pops, s);
     if (!st) st = csr_from_spec(tcp::operator_of(op), s, false, max_bytes, out);
+    return st ? st : tcp::ok();
+}
+
+int tcp_csr_plan_create_full(const tcp_operator *op, uint64_t max_bytes, tcp_csr_plan **out) {
+    if (!op) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
+    if (!out) return fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    const tcp::Operator &o = tcp::operator_of(op);
+    tcp::FullGroups f;
+    int st = tcp::full_groups_build(o, f);
+    if (st) return st;
+    int device = 0;
+    TCP_HIP_CHECK(hipGetDevice(&device));
+    auto count = [&](uint32_t *counts) { return tcp::full_csr_counts(f, counts, nullptr); };
+    auto fill = [&](const uint64_t *indptr, uint32_t *cols, void *v64, void *v32) {
+        return tcp::full_csr_fill(f, indptr, cols, v64, v32, nullptr);
+    };
+    st = assemble_csr(o.nqubits, f.dim, device, max_bytes, count, fill, out);
     return st ? st : tcp::ok();
 }
 

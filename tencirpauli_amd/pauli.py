@@ -17,6 +17,7 @@ from typing import Any, Iterable, Optional, Sequence, Tuple, Union
 import numpy as np
 
 from . import _lib
+from .backend import BackendMVPPlan
 from ._plan import (
     DEFAULT_MAX_BYTES,
     NativePlanFacade,
@@ -128,6 +129,31 @@ class PauliOperator:
         return instance
 
     @classmethod
+    def from_words(cls, nqubits: int, x_words: Any, z_words: Any, coefficients: Any) -> "PauliOperator":
+        """From packed [T, W] X/Z words (qubit q at word q // 64, bit q % 64), as
+        in the BackendMVPPlan arrays; canonicalised like every constructor."""
+        if not isinstance(nqubits, int) or isinstance(nqubits, bool) or nqubits < 0:
+            raise ValueError("nqubits must be a non-negative integer")
+        W = (nqubits + 63) // 64
+        coef = np.ascontiguousarray(np.asarray(coefficients, dtype=np.complex128).reshape(-1))
+        x = np.ascontiguousarray(np.asarray(x_words, dtype=np.uint64))
+        z = np.ascontiguousarray(np.asarray(z_words, dtype=np.uint64))
+        if x.shape != (coef.shape[0], W) or z.shape != (coef.shape[0], W):
+            raise ValueError(f"word arrays must have shape ({coef.shape[0]}, {W})")
+        out = ctypes.c_void_p()
+        u64 = ctypes.POINTER(ctypes.c_uint64)
+        _lib.check(_lib.lib().tcp_operator_from_words(nqubits, coef.shape[0], x.ctypes.data_as(u64),
+                                                      z.ctypes.data_as(u64),
+                                                      coef.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                      ctypes.byref(out)))
+        instance = object.__new__(cls)
+        instance._handle = _OperatorHandle(out)
+        instance.nqubits = nqubits
+        instance._terms = None
+        instance._arrays = None
+        return instance
+
+    @classmethod
     def empty(cls, nqubits: int) -> "PauliOperator":
         return cls(nqubits, ())
 
@@ -162,20 +188,11 @@ class PauliOperator:
         """Number of distinct X-flip masks (native/operator.rs:50-59)."""
         return int(_lib.lib().tcp_operator_distinct_x_mask_count(self._handle.ptr))
 
-    def backend_mvp_plan(self) -> "BackendPlanArrays":
+    def backend_mvp_plan(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "BackendMVPPlan":
         """Schema-v1 flat arrays (core/hamiltonian.rs:501-524, py/pauli.py:954-983)."""
-        T, n = self.term_count, self.nqubits
-        W = (n + 63) // 64
-        x = np.zeros((T, W), dtype=np.uint64)
-        z = np.zeros((T, W), dtype=np.uint64)
-        coef = np.zeros(T, dtype=np.complex128)
-        _lib.check(_lib.lib().tcp_operator_export_words(
-            self._handle.ptr, x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
-            z.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
-            coef.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
-        for a in (x, z, coef):
-            a.flags.writeable = False
-        return BackendPlanArrays(1, n, W, x, z, coef, "qubit0_msb_matrix", int(T * (W * 16 + 16)))
+        from .backend import plan_from_operator
+
+        return plan_from_operator(self, max_bytes)
 
     # -- charges ----------------------------------------------------------
     def analyze_charge(self, charge: Any, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "ChargeAnalysis":
@@ -281,6 +298,78 @@ class PauliOperator:
     def to_scipy_linear_operator(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
         return self.native_mvp_plan(max_bytes=max_bytes).to_scipy_linear_operator(max_bytes=max_bytes)
 
+    # -- materialisation (py/pauli.py:866-928, core/hamiltonian.rs:313-460) --
+    def _matrix_dimension(self) -> int:
+        if self.nqubits >= 64:
+            raise OverflowError("integer overflow while computing matrix dimension")
+        return 1 << self.nqubits
+
+    def _sparse_estimate(self, target: str) -> int:
+        """Working bytes the reference reserves for COO/CSR (core/hamiltonian.rs:360-460):
+        G * 2^n candidate entries, with a candidate buffer and row counts unless
+        every X group holds one term."""
+        dim = self._matrix_dimension()
+        groups = self.distinct_x_mask_count()
+        upper = groups * dim
+        direct = groups == self.term_count
+        if target == "coo":
+            return upper * 32 if direct else upper * (24 + 32) + dim * 8
+        output = 8 * (dim + 1) + 24 * upper
+        return output if direct else upper * 24 + output + dim * 8
+
+    def _full_csr_plan(self, max_bytes: Optional[int]) -> Any:
+        from .eager import CsrMvpPlan
+
+        _lib.require_device()
+        out = ctypes.c_void_p()
+        _lib.check(_lib.lib().tcp_csr_plan_create_full(self._handle.ptr, effective_max_bytes(max_bytes),
+                                                        ctypes.byref(out)))
+        return CsrMvpPlan(out, "pauli", "pauli", self.term_count)
+
+    def dense(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> np.ndarray:
+        """Dense complex128 matrix, qubit 0 the most significant index bit.
+
+        Built from the device CSR of the operator; the budget is the dense
+        array's 16 * 4^n bytes (core/hamiltonian.rs:313-320)."""
+        validate_max_bytes(max_bytes)
+        dim = self._matrix_dimension()
+        check_allocation(dim * dim * 16, max_bytes, "dense matrix")
+        return self._full_csr_plan(max_bytes).materialize("dense", None)
+
+    def coo(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        """Deterministic aggregated COO: row-major, columns ascending, exact zeros dropped."""
+        validate_max_bytes(max_bytes)
+        check_allocation(self._sparse_estimate("coo"), max_bytes, "COO working memory")
+        return self._full_csr_plan(max_bytes).materialize("coo", None)
+
+    def csr(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        """Deterministic CSR (uint64 indptr/indices, complex128 data)."""
+        validate_max_bytes(max_bytes)
+        check_allocation(self._sparse_estimate("csr"), max_bytes, "CSR working memory")
+        return self._full_csr_plan(max_bytes).materialize("csr", None)
+
+    def csr_mvp_plan(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        """The full-space matrix retained as device CSR, applied by the row-parallel
+        SpMV (the same plan object as the eager restricted operators)."""
+        validate_max_bytes(max_bytes)
+        check_allocation(self._sparse_estimate("csr"), max_bytes, "CSR working memory")
+        return self._full_csr_plan(max_bytes)
+
+    def compile(self, target: str, *, storage: str = "lazy", max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
+        """One named target (py/pauli.py:1047-1071): "dense", "coo", "csr",
+        "native_mvp" or "backend_mvp"."""
+        if target == "dense":
+            return self.dense(max_bytes=max_bytes)
+        if target == "coo":
+            return self.coo(max_bytes=max_bytes)
+        if target == "csr":
+            return self.csr(max_bytes=max_bytes)
+        if target == "backend_mvp":
+            return self.backend_mvp_plan(max_bytes=max_bytes)
+        if target == "native_mvp":
+            return self.native_mvp_plan(storage=storage, max_bytes=max_bytes)
+        raise ValueError("target must be one of 'dense', 'coo', 'csr', 'native_mvp', or 'backend_mvp'")
+
     def __repr__(self) -> str:
         return f"PauliOperator(nqubits={self.nqubits}, term_count={self.term_count})"
 
@@ -291,18 +380,8 @@ class ChargeAnalysis:
     nonzero_commutator_terms: int
 
 
-@dataclass(frozen=True)
-class BackendPlanArrays:
-    """BackendMvpPlan arrays (core/hamiltonian.rs:297-309)."""
-
-    schema_version: int
-    nqubits: int
-    word_count: int
-    x_words: np.ndarray
-    z_words: np.ndarray
-    coefficients: np.ndarray
-    ordering: str
-    estimated_bytes: int
+# the earlier name of the schema-v1 plan class
+BackendPlanArrays = BackendMVPPlan
 
 
 class NativeMVPPlan(NativePlanFacade):
@@ -394,6 +473,7 @@ class NativeMVPPlan(NativePlanFacade):
 
 __all__ = [
     "DEFAULT_MAX_BYTES",
+    "BackendMVPPlan",
     "BackendPlanArrays",
     "ChargeAnalysis",
     "NativeMVPPlan",
