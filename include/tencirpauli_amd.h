@@ -402,6 +402,50 @@ int tcp_davidson_full(const tcp_mvp_plan *plan, const tcp_davidson_options *opt,
 int tcp_davidson_csr(const tcp_csr_plan *plan, const tcp_davidson_options *opt, void *eigenvector,
                      tcp_davidson_result *result);
 
+/* ---- thick-restart Lanczos (no reference counterpart: replaces the
+ *      scipy.sparse.linalg.eigsh call the reference makes over its
+ *      LinearOperator, examples/research/fermi_hubbard/run_tencirpauli.py:
+ *      194-211, py/integrations/scipy.py:13-47) ------------------------------ */
+
+enum {
+    TCP_WHICH_SA = 0, /* smallest algebraic */
+    TCP_WHICH_LA = 1, /* largest algebraic */
+    TCP_WHICH_LM = 2  /* largest magnitude */
+};
+
+typedef struct tcp_lanczos_options {
+    uint32_t nev;          /* wanted eigenpairs: 1 <= nev < dimension */
+    uint32_t ncv;          /* basis size: nev < ncv <= min(dimension, 32) (default 20) */
+    uint32_t which;        /* TCP_WHICH_* */
+    uint32_t max_restarts; /* default 1000 */
+    double tol;            /* residual estimate <= tol * max(|theta|, eps^(2/3)); 0 = machine epsilon */
+    const void *v0;        /* optional device complex128 start vector (NULL: deterministic hash) */
+    uint64_t seed;
+    void *stream;          /* hipStream_t (NULL = default stream) */
+} tcp_lanczos_options;
+
+typedef struct tcp_lanczos_result {
+    uint32_t converged;    /* wanted pairs that met tol (== nev on success) */
+    uint32_t restarts;
+    uint64_t mvps;
+    double seconds;        /* wall time, host clock */
+} tcp_lanczos_result;
+
+void tcp_lanczos_default_options(tcp_lanczos_options *opt);
+
+/* The `nev` wanted eigenvalues in ascending order into host double[nev];
+ * optional outputs: `eigenvectors` a device complex128 [nev][dimension] array
+ * (row k the normalised Ritz vector of eigenvalue k) and `residual_norms` a
+ * host double[nev] of explicit ||A x_k - lambda_k x_k|| (one MVP each).  A
+ * solve that stops at max_restarts returns TCP_OK with converged < nev.
+ * Device memory: ncv + 3 complex128 vectors. */
+int tcp_lanczos_full(const tcp_mvp_plan *plan, const tcp_lanczos_options *opt, double *eigenvalues,
+                     void *eigenvectors, double *residual_norms, tcp_lanczos_result *result);
+int tcp_lanczos_sector(const tcp_sector_plan *plan, const tcp_lanczos_options *opt, double *eigenvalues,
+                       void *eigenvectors, double *residual_norms, tcp_lanczos_result *result);
+int tcp_lanczos_csr(const tcp_csr_plan *plan, const tcp_lanczos_options *opt, double *eigenvalues,
+                    void *eigenvectors, double *residual_norms, tcp_lanczos_result *result);
+
 /* Real diagonal H_ii of a plan into a device double[dimension] array. */
 int tcp_sector_plan_diagonal(const tcp_sector_plan *plan, double *diag, void *stream);
 int tcp_mvp_plan_diagonal(const tcp_mvp_plan *plan, double *diag, void *stream);

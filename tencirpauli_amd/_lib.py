@@ -141,6 +141,10 @@ SIGNATURES = {
     "tcp_davidson_sector": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "tcp_davidson_full": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "tcp_davidson_csr": (_c.c_int, [_vp, _vp, _vp, _vp]),
+    "tcp_lanczos_default_options": (None, [_vp]),
+    "tcp_lanczos_full": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "tcp_lanczos_sector": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "tcp_lanczos_csr": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "tcp_sector_plan_diagonal": (_c.c_int, [_vp, _vp, _vp]),
     "tcp_mvp_plan_diagonal": (_c.c_int, [_vp, _vp, _vp]),
     "tcp_csr_plan_diagonal": (_c.c_int, [_vp, _vp, _vp]),
