@@ -46,9 +46,9 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run one plain device copy (y <- v, 2^n amplitudes each way) before the MVPs, "
                         "the known-size reference for HBM counter scaling")
-    p.add_argument("--extras", default="fp32,sector,eager,wide,hubbard_half,davidson",
+    p.add_argument("--extras", default="fp32,sector,eager,wide,hubbard_half,davidson,eigsh",
                    help="comma list of extra sections: fp32, sector, eager, wide, hubbard_half, davidson, "
-                        "batched ('' for none)")
+                        "eigsh, batched ('' for none)")
     return p.parse_args()
 
 
@@ -233,6 +233,14 @@ def extra_hubbard_half() -> dict:
         ms = time_applies(lambda: restricted.apply_into(vv, yy), 5)
         rec[f"ms_per_mvp_{tag}"] = round(ms, 3)
         del vv, yy
+    # the example's eigen call on this sector, every step on the GPU
+    from tencirpauli_amd import lanczos
+
+    res = lanczos.lanczos(restricted, 1, v0=v, tol=1e-8, max_restarts=300, ncv=20, residuals=True)
+    rec["eigsh_k1_sa"] = {"seconds": round(res.seconds, 2), "mvps": res.mvps, "restarts": res.restarts,
+                          "converged": res.converged, "energy": float(res.eigenvalues[0]),
+                          "residual": float(res.residual_norms[0])}
+    del res
     rec["reference_published_s"] = 54.004
     rec["reference_hardware"] = "arm64/macOS CPU (structured fermion lazy path)"
     rec["ratio_vs_published_fp64"] = round(54.004 / (rec["ms_per_mvp_fp64"] * 1e-3), 1)
@@ -392,6 +400,55 @@ def extra_davidson() -> dict:
     return out
 
 
+def extra_eigsh() -> dict:
+    """The reference's eigen call, eigsh(linear, k=1, which="SA", v0=state, tol=1e-8, maxiter=300,
+    ncv=20) (examples/research/fermi_hubbard/run_tencirpauli.py:118-128, 194-211), on the 4x5
+    periodic XXZ sector (20 qubits, S_z = 0, D = 184,756): SciPy's host ARPACK over the plan's
+    LinearOperator (each product staged through host memory) against the whole iteration on the
+    GPU (tcp.eigsh, lazy and eager plans), plus k = 4 on the GPU."""
+    from scipy.sparse.linalg import eigsh as scipy_eigsh
+
+    import tencirpauli_amd as tcp
+    from tencirpauli_amd import lanczos, lattice
+
+    codes, coef = lattice.heisenberg_periodic(4, 5, seed=SEED)
+    op = tcp.PauliOperator.from_code_arrays(codes, coef)
+    restricted = op.restrict_charge(tcp.U1Sector(20, 10))
+    plan = restricted.mvp_plan()
+    eager = restricted.mvp_plan(storage="eager")
+    state = lattice.gaussian_state(plan.dimension, 5)
+    args = dict(k=1, which="SA", tol=1e-8, maxiter=300, ncv=20)
+    lanczos.lanczos(plan, 1, max_restarts=1)  # warm-up
+    linear = plan.to_scipy_linear_operator()
+    calls = [0]
+    matvec = linear.matvec
+
+    def counted(x):
+        calls[0] += 1
+        return matvec(x)
+
+    from scipy.sparse.linalg import LinearOperator
+
+    counted_linear = LinearOperator(linear.shape, matvec=counted, dtype=linear.dtype)
+    t0 = time.perf_counter()
+    w_host, v_host = scipy_eigsh(counted_linear, v0=state, **args)
+    host_s = time.perf_counter() - t0
+    rec = {"dimension": plan.dimension, "call": "k=1 SA tol=1e-8 maxiter=300 ncv=20",
+           "scipy_arpack_over_linear_operator": {"seconds": round(host_s, 3), "mvps": calls[0],
+                                                 "energy": float(w_host[0])}}
+    for name, p in (("gpu_lazy", plan), ("gpu_eager_csr", eager)):
+        res = lanczos.lanczos(p, 1, v0=state, tol=1e-8, max_restarts=300, ncv=20, residuals=True)
+        rec[name] = {"seconds": round(res.seconds, 4), "mvps": res.mvps, "restarts": res.restarts,
+                     "energy": float(res.eigenvalues[0]), "residual": float(res.residual_norms[0]),
+                     "abs_dE_vs_scipy": abs(float(res.eigenvalues[0]) - float(w_host[0]))}
+    rec["speedup_lazy_vs_scipy"] = round(host_s / rec["gpu_lazy"]["seconds"], 1)
+    four = lanczos.lanczos(plan, 4, tol=1e-8, max_restarts=300, residuals=True)
+    rec["gpu_lazy_k4"] = {"seconds": round(four.seconds, 4), "mvps": four.mvps,
+                          "energies": [float(x) for x in four.eigenvalues],
+                          "max_residual": float(four.residual_norms.max())}
+    return rec
+
+
 def extra_batched() -> dict:
     """512 random Pauli sums (n ~ U{10..22}, T ~ U{64..4096}), 8 right-hand sides each,
     complex64 storage with FP64 accumulation, one persistent launch replayed as a hipGraph."""
@@ -531,7 +588,7 @@ def main() -> int:
     if "fp32" in extras and tag == "complex128":
         record["fp32_storage"] = extra_fp32(plan, v, dim)
     del y
-    if any(e in extras for e in ("sector", "eager", "wide", "hubbard_half", "davidson", "batched")):
+    if any(e in extras for e in ("sector", "eager", "wide", "hubbard_half", "davidson", "eigsh", "batched")):
         del v, plan
         torch.cuda.empty_cache()
     sector_cpu = 0.0 if args.no_cpu_baseline else args.cpu_seconds / 3
@@ -546,6 +603,8 @@ def main() -> int:
         record["hubbard_4x4_half_filling"] = extra_hubbard_half()
     if "davidson" in extras:
         record["davidson_5x5_xxz"] = extra_davidson()
+    if "eigsh" in extras:
+        record["eigsh_4x5_xxz"] = extra_eigsh()
     if "batched" in extras:
         record["batched_512x8"] = extra_batched()
     print(json.dumps(record), flush=True)
