@@ -111,6 +111,35 @@ int tcp_operator_analyze_charge(const tcp_operator *op, const double *qubit_leve
 
 void tcp_operator_destroy(tcp_operator *op);
 
+/* ---- fermion operators: FermionOperator.from_terms / map_fermions
+ *      (py/_structured/fermion.py:51-244) over structured_fermion_canonicalize
+ *      and jordan_wigner (core/structured/integrals.rs:2-54,
+ *      core/structured/fermion.rs:141-183, 261-283) ------------------------- */
+
+typedef struct tcp_fermion tcp_fermion;
+
+/* CAR-canonicalise raw ladder products: term t is the product of factors
+ * offsets[t] .. offsets[t+1]-1, factor k acting on modes[k] with actions[k]
+ * 0 = create, 1 = annihilate; coef interleaved (re, im).  Canonical words put
+ * creations (increasing modes) before annihilations (decreasing modes); equal
+ * words are summed in IEEE-bit-pattern order, sorted by (creation modes,
+ * annihilation modes), exact zeros dropped.  Errors: mode >= n_modes
+ * (TCP_ERR_INVALID_INDEX), action > 1 (TCP_ERR_INVALID_CODE), non-finite
+ * coefficient, and 192 bytes per expansion item against max_bytes
+ * (TCP_ERR_MEMORY_LIMIT). */
+int tcp_fermion_from_ladder(uint32_t n_modes, uint64_t nterms, const uint64_t *offsets, const uint32_t *modes,
+                            const uint8_t *actions, const double *coef, uint64_t max_bytes, tcp_fermion **out);
+uint32_t tcp_fermion_n_modes(const tcp_fermion *f);
+uint64_t tcp_fermion_term_count(const tcp_fermion *f);
+uint64_t tcp_fermion_factor_count(const tcp_fermion *f); /* factors over all canonical words */
+/* Canonical words as factor lists (offsets[term_count + 1], modes/actions
+ * [factor_count], coef[2 term_count]); any output may be NULL. */
+int tcp_fermion_export(const tcp_fermion *f, uint64_t *offsets, uint32_t *modes, uint8_t *actions, double *coef);
+/* Jordan-Wigner image (mode m = qubit m, a_m = Z_<m (X_m + iY_m)/2) as a
+ * canonical Pauli operator: the input of every MVP plan and sector plan. */
+int tcp_fermion_jordan_wigner(const tcp_fermion *f, uint64_t max_bytes, tcp_operator **out);
+void tcp_fermion_destroy(tcp_fermion *f);
+
 /* ---- full-space plan: MvpPlan (core/hamiltonian.rs:52-270) via
  *      pauli_mvp_plan_handle (native/hamiltonian.rs:270-288) ------------- */
 
@@ -242,6 +271,16 @@ int tcp_sector_plan_create_u1(const tcp_operator *op, uint32_t particle_number,
 int tcp_sector_plan_create_blocks(const tcp_operator *op, uint32_t nblocks, const uint32_t *sizes,
                                   const uint32_t *pops, uint64_t max_bytes,
                                   tcp_sector_plan **out);
+/* The same block sectors for a fermion operator, gathered from its ladder
+ * words directly (the raw-fermion restricted MVP of the reference,
+ * FermionMaskDescriptor gathers, core/fermion_mask.rs and
+ * core/charge/generic.rs:642-809, behind FermionOperator.restrict_charge,
+ * py/_structured/fermion.py:230-244): a word acts on a source whose
+ * annihilated modes are occupied and other created modes empty, with its
+ * Jordan-Wigner sign; mode m is bit n-1-m of the occupation index.  Every word
+ * must conserve every block's number (TCP_ERR_INVALID_SECTOR); n <= 63. */
+int tcp_sector_plan_create_fermion(const tcp_fermion *f, uint32_t nblocks, const uint32_t *sizes,
+                                   const uint32_t *pops, uint64_t max_bytes, tcp_sector_plan **out);
 
 uint64_t tcp_sector_plan_dimension(const tcp_sector_plan *plan);
 uint32_t tcp_sector_plan_nqubits(const tcp_sector_plan *plan);
@@ -300,6 +339,11 @@ int tcp_csr_plan_create_blocks(const tcp_operator *op, uint32_t nblocks, const u
  * order; exact zeros are dropped and columns ascend.  nqubits <= 32
  * (TCP_ERR_UNSUPPORTED beyond: columns are 32-bit). */
 int tcp_csr_plan_create_full(const tcp_operator *op, uint64_t max_bytes, tcp_csr_plan **out);
+/* Eager CSR of a fermion operator on block sectors (as
+ * tcp_sector_plan_create_fermion; no conservation precondition, a word that
+ * leaves the sector fails the build with TCP_ERR_INVALID_SECTOR). */
+int tcp_csr_plan_create_fermion(const tcp_fermion *f, uint32_t nblocks, const uint32_t *sizes,
+                                const uint32_t *pops, uint64_t max_bytes, tcp_csr_plan **out);
 uint64_t tcp_csr_plan_dimension(const tcp_csr_plan *plan);
 uint64_t tcp_csr_plan_transition_count(const tcp_csr_plan *plan); /* stored entries */
 uint64_t tcp_csr_plan_estimated_bytes(const tcp_csr_plan *plan);

@@ -10,6 +10,7 @@ include/tencirpauli_amd.h); there is no CPU fallback.
 from ._lib import DeviceError, NativeLibraryError, UnsupportedOnDeviceError, device_count
 from ._plan import DEFAULT_MAX_BYTES
 from .charge import AdditiveCharge, ChargeRestrictedOperator, ChargeSector
+from .fermion import FermionOperator, FermionTerm, FermionWord, OperatorSpace
 from .lanczos import eigsh
 from .pauli import BackendMVPPlan, BackendPlanArrays, ChargeAnalysis, NativeMVPPlan, PauliOperator, PauliTerm
 from .symmetry import SectorMvpPlan, U1RestrictedOperator, U1Sector
@@ -25,8 +26,12 @@ __all__ = [
     "ChargeRestrictedOperator",
     "ChargeSector",
     "DeviceError",
+    "FermionOperator",
+    "FermionTerm",
+    "FermionWord",
     "NativeLibraryError",
     "NativeMVPPlan",
+    "OperatorSpace",
     "PauliOperator",
     "PauliTerm",
     "SectorMvpPlan",

@@ -59,6 +59,14 @@ SIGNATURES = {
     "tcp_last_error": (_c.c_size_t, [_c.c_char_p, _c.c_size_t]),
     "tcp_device_count": (_c.c_int, [_ip]),
     "tcp_operator_from_codes": (_c.c_int, [_c.c_uint32, _c.c_uint64, _u8p, _dp, _c.POINTER(_vp)]),
+    "tcp_fermion_from_ladder": (_c.c_int, [_c.c_uint32, _c.c_uint64, _u64p, _c.POINTER(_c.c_uint32),
+                                            _c.POINTER(_c.c_uint8), _dp, _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_fermion_n_modes": (_c.c_uint32, [_vp]),
+    "tcp_fermion_term_count": (_c.c_uint64, [_vp]),
+    "tcp_fermion_factor_count": (_c.c_uint64, [_vp]),
+    "tcp_fermion_export": (_c.c_int, [_vp, _u64p, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint8), _dp]),
+    "tcp_fermion_jordan_wigner": (_c.c_int, [_vp, _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_fermion_destroy": (None, [_vp]),
     "tcp_operator_from_words": (_c.c_int, [_c.c_uint32, _c.c_uint64, _u64p, _u64p, _dp, _c.POINTER(_vp)]),
     "tcp_operator_nqubits": (_c.c_uint32, [_vp]),
     "tcp_operator_term_count": (_c.c_uint64, [_vp]),
@@ -116,6 +124,10 @@ SIGNATURES = {
     "tcp_csr_plan_create_blocks": (_c.c_int, [_vp, _c.c_uint32, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint32),
                                                _c.c_uint64, _c.POINTER(_vp)]),
     "tcp_csr_plan_create_full": (_c.c_int, [_vp, _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_csr_plan_create_fermion": (_c.c_int, [_vp, _c.c_uint32, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint32),
+                                                _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_sector_plan_create_fermion": (_c.c_int, [_vp, _c.c_uint32, _c.POINTER(_c.c_uint32),
+                                                   _c.POINTER(_c.c_uint32), _c.c_uint64, _c.POINTER(_vp)]),
     "tcp_csr_plan_dimension": (_c.c_uint64, [_vp]),
     "tcp_csr_plan_transition_count": (_c.c_uint64, [_vp]),
     "tcp_csr_plan_estimated_bytes": (_c.c_uint64, [_vp]),

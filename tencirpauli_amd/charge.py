@@ -1,10 +1,12 @@
-"""Additive qubit charges and charge sectors (py/charge.py:162-483, 608-794, 1075-1502).
+"""Additive charges and charge sectors (py/charge.py:162-483, 608-794, 1075-1502).
 
-Only qubit charges are modelled (the Pauli path); a sector whose constraints
-are particle numbers of contiguous, disjoint qubit blocks covering qubits
-0..n-1 is the factorized layout (core/charge_sector.rs:150-217, 1786-1868) and
-runs on the GPU block kernel.  Rank order is lexicographic over qubits with
-qubit 0 most significant (``operator_space_axis0_msb_mixed_radix``).
+Charges act on binary axes: qubits, or fermion modes of an ``OperatorSpace``
+(occupation n_m in {0, 1}).  A sector whose constraints fix the particle
+number of every block of a partition of the axes into at most four
+contiguous blocks -- e.g. total number plus 2 S_z fixing N_up and N_down --
+is the factorized layout (core/charge_sector.rs:150-217, 1786-1868) and runs
+on the GPU block kernel.  Rank order is lexicographic over axes with axis 0
+most significant (``operator_space_axis0_msb_mixed_radix``).
 """
 
 from __future__ import annotations
@@ -13,7 +15,7 @@ import ctypes
 import math
 import threading
 from dataclasses import dataclass, field
-from typing import Any, Dict, Optional, Sequence, Tuple
+from typing import Any, Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -23,27 +25,59 @@ from ._plan import DEFAULT_MAX_BYTES, NativePlanFacade, effective_max_bytes, val
 
 @dataclass(frozen=True, eq=True)
 class AdditiveCharge:
-    """Q = offset + sum_q level_q(bit_q) over qubits; levels default to (0, 0)."""
+    """Q = offset + sum_a level_a(bit_a) over the binary axes (py/charge.py:162-260).
+
+    Constructed as the reference's ``AdditiveCharge(space, name=...,
+    fermions={mode: weight}, qubits={index: (level_0, level_1)}, offset=...)``
+    on an ``OperatorSpace`` (a fermion weight w is the level pair (0, w)), or as
+    ``AdditiveCharge(nqubits, {index: (level_0, level_1)})`` on qubits alone.
+    Boson and qudit axes are not on the GPU path.
+    """
 
     nqubits: int
     qubits: Tuple[Tuple[int, Tuple[float, float]], ...] = ()
     offset: float = 0
     name: str = field(default="charge", compare=False)
+    space: Any = field(default=None, compare=False)
 
-    def __init__(self, nqubits: int, qubits: Optional[Dict[int, Tuple[float, float]]] = None, *,
-                 offset: float = 0, name: str = "charge") -> None:
-        if not isinstance(nqubits, int) or nqubits < 0:
-            raise ValueError("nqubits must be a non-negative integer")
+    def __init__(self, space: Any, qubits: Optional[Dict[int, Tuple[float, float]]] = None, *,
+                 name: str = "", fermions: Optional[Dict[int, float]] = None, bosons: Any = None,
+                 qudits: Any = None, offset: float = 0) -> None:
+        from .fermion import OperatorSpace
+
+        if not isinstance(name, str):
+            raise TypeError("name must be a string")
+        if isinstance(space, OperatorSpace):
+            if bosons or qudits or not space.is_binary:
+                raise NotImplementedError("boson and qudit charges are not on the GPU path")
+            n_f, n = space.fermions, space.binary_axes
+            levels: Dict[int, Tuple[float, float]] = {}
+            for mode, weight in (fermions or {}).items():
+                if not isinstance(mode, int) or isinstance(mode, bool) or not 0 <= mode < n_f:
+                    raise ValueError(f"fermions index {mode} is outside 0..{n_f}")
+                levels[mode] = (0.0, float(weight))
+            for index, pair in (qubits or {}).items():
+                if not isinstance(index, int) or isinstance(index, bool) or not 0 <= index < space.qubits:
+                    raise ValueError(f"qubits index {index} is outside 0..{space.qubits}")
+                levels[n_f + index] = pair
+            nqubits, table = n, levels
+        else:
+            if fermions or bosons or qudits:
+                raise TypeError("space must be an OperatorSpace")
+            if not isinstance(space, int) or isinstance(space, bool) or space < 0:
+                raise ValueError("nqubits must be a non-negative integer")
+            nqubits, table, space = space, dict(qubits or {}), None
         items = []
-        for index, levels in sorted((qubits or {}).items()):
+        for index, pair in sorted(table.items()):
             if not 0 <= int(index) < nqubits:
                 raise ValueError(f"qubit index {index} is outside 0..{nqubits - 1}")
-            l0, l1 = levels
+            l0, l1 = pair
             items.append((int(index), (float(l0), float(l1))))
         object.__setattr__(self, "nqubits", nqubits)
         object.__setattr__(self, "qubits", tuple(items))
         object.__setattr__(self, "offset", offset)
-        object.__setattr__(self, "name", name)
+        object.__setattr__(self, "name", name or "charge")
+        object.__setattr__(self, "space", space)
 
     @classmethod
     def number(cls, nqubits: int, qubits: Sequence[int], *, name: str = "N") -> "AdditiveCharge":
@@ -57,8 +91,8 @@ class AdditiveCharge:
     def level_array(self) -> np.ndarray:
         return np.asarray(self.qubit_levels, dtype=np.float64).reshape(self.nqubits, 2)
 
-    def sector(self, target: int) -> "ChargeSector":
-        return ChargeSector(((self, target),))
+    def sector(self, target: int, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "ChargeSector":
+        return ChargeSector(((self, target),), max_bytes=max_bytes)
 
 
 class ChargeSector:
@@ -73,8 +107,12 @@ class ChargeSector:
         n = constraints[0][0].nqubits
         if any(c.nqubits != n for c, _ in constraints):
             raise ValueError("all charges must act on the same qubits")
+        spaces = {c.space for c, _ in constraints if c.space is not None}
+        if len(spaces) > 1:
+            raise ValueError("all charges must share one operator space")
         self.constraints = constraints
         self.nqubits = n
+        self.space = spaces.pop() if spaces else None
         self.blocks = _factorized_blocks(constraints)
 
     @property
@@ -123,30 +161,62 @@ class ChargeSector:
 
 
 def _factorized_blocks(constraints: Sequence[Tuple[AdditiveCharge, int]]) -> Optional[Tuple[Tuple[int, int], ...]]:
-    """(size, population) per block when every constraint is a 0/1 number charge
-    on a contiguous block and the blocks tile qubits 0..n-1; otherwise None."""
+    """(size, population) of each contiguous fixed-population block, or None.
+
+    Every charge is const_c + sum_a w_{c,a} b_a with w = level_1 - level_0.
+    Maximal runs of axes with equal weight columns form the blocks; the
+    constraints then fix the block populations n_B through
+    sum_B w_{c,B} n_B = target_c - const_c.  The sector is the factorized
+    layout (core/charge_sector.rs:150-217, 1786-1868) when exactly one
+    population vector within the block sizes solves them; a block that no
+    charge weighs, or several solutions, make a union of layouts (None).
+    """
     n = constraints[0][0].nqubits
-    spans = []
-    for charge, target in constraints:
-        if charge.offset != 0:
-            return None
-        support = [q for q, levels in enumerate(charge.qubit_levels) if levels != (0.0, 0.0)]
-        if any(charge.qubit_levels[q] != (0.0, 1.0) for q in support):
-            return None
-        if not support or support != list(range(support[0], support[-1] + 1)):
-            return None
-        if not 0 <= target <= len(support):
-            raise ValueError("charge target is unreachable")
-        spans.append((support[0], len(support), target))
-    spans.sort()
-    position = 0
-    for start, size, _ in spans:
-        if start != position:
-            return None
-        position += size
-    if position != n:
+    weights = [[l1 - l0 for l0, l1 in c.qubit_levels] for c, _ in constraints]
+    rhs = [t - (c.offset + sum(l0 for l0, _ in c.qubit_levels)) for c, t in constraints]
+    runs: List[List[Any]] = []  # [start, size, column]
+    for a in range(n):
+        column = tuple(w[a] for w in weights)
+        if runs and runs[-1][2] == column:
+            runs[-1][1] += 1
+        else:
+            runs.append([a, 1, column])
+    if any(all(x == 0 for x in run[2]) for run in runs):
         return None
-    return tuple((size, target) for _, size, target in spans)
+    if len(runs) > 4:
+        return None
+    solutions = _block_populations([r[2] for r in runs], [r[1] for r in runs], rhs)
+    if not solutions:
+        raise ValueError("charge target is unreachable")
+    if len(solutions) > 1:
+        return None
+    return tuple((run[1], pop) for run, pop in zip(runs, solutions[0]))
+
+
+def _block_populations(columns: Sequence[Tuple[float, ...]], sizes: Sequence[int],
+                       rhs: Sequence[float]) -> List[Tuple[int, ...]]:
+    """Population vectors 0 <= n_B <= size_B with sum_B columns[B] n_B == rhs (at most two)."""
+    found: List[Tuple[int, ...]] = []
+
+    def search(b: int, partial: List[float], chosen: List[int]) -> None:
+        if len(found) > 1:
+            return
+        if b == len(columns):
+            if all(abs(p - r) <= 1e-9 * max(1.0, abs(r)) for p, r in zip(partial, rhs)):
+                found.append(tuple(chosen))
+            return
+        if b == len(columns) - 1:  # the last population follows from any charge that weighs it
+            c = next(i for i, x in enumerate(columns[b]) if x != 0)
+            pop_f = (rhs[c] - partial[c]) / columns[b][c]
+            pop = int(round(pop_f))
+            if abs(pop - pop_f) <= 1e-9 * max(1.0, abs(pop_f)) and 0 <= pop <= sizes[b]:
+                search(b + 1, [p + x * pop for p, x in zip(partial, columns[b])], chosen + [pop])
+            return
+        for pop in range(sizes[b] + 1):
+            search(b + 1, [p + c * pop for p, c in zip(partial, columns[b])], chosen + [pop])
+
+    search(0, [0.0] * len(rhs), [])
+    return found
 
 
 class ChargeRestrictedOperator(NativePlanFacade):
@@ -211,6 +281,60 @@ class ChargeRestrictedOperator(NativePlanFacade):
 
     def _native_apply_device(self, inp: Any, out: Any, dtype: int, stream: int) -> int:
         return self.plan._native_apply_device(inp, out, dtype, stream)
+
+
+class FermionRestrictedOperator(ChargeRestrictedOperator):
+    """FermionOperator restricted to a factorized fermion charge sector.
+
+    The plans gather from the canonical ladder words directly
+    (``tcp_sector_plan_create_fermion`` / ``tcp_csr_plan_create_fermion``): a
+    word acts where its annihilated modes are occupied and its other created
+    modes empty, with its Jordan-Wigner sign -- the reference's raw-fermion
+    restricted path (core/fermion_mask.rs, core/charge/generic.rs:642-809).
+    """
+
+    def _ensure_lazy(self, max_bytes: Optional[int]) -> Any:
+        with self._lock:
+            if self._plan is None:
+                self._plan = _fermion_plan(self.operator, self.sector, max_bytes, eager=False)
+            return self._plan
+
+    def _ensure_eager(self, max_bytes: Optional[int]) -> Any:
+        with self._lock:
+            if self._csr is None:
+                self._csr = _fermion_plan(self.operator, self.sector, max_bytes, eager=True)
+            return self._csr
+
+
+def _fermion_plan(operator: Any, sector: ChargeSector, max_bytes: Optional[int], *, eager: bool) -> Any:
+    from .eager import CsrMvpPlan
+    from .symmetry import SectorMvpPlan
+
+    _lib.require_device()
+    sizes, pops = sector._arrays()
+    out = ctypes.c_void_p()
+    u32 = ctypes.POINTER(ctypes.c_uint32)
+    create = _lib.lib().tcp_csr_plan_create_fermion if eager else _lib.lib().tcp_sector_plan_create_fermion
+    _lib.check(create(operator._handle.ptr, len(sizes), sizes.ctypes.data_as(u32), pops.ctypes.data_as(u32),
+                      effective_max_bytes(max_bytes), ctypes.byref(out)))
+    if eager:
+        return CsrMvpPlan(out, "factorized", "fermion", operator.term_count)
+    return SectorMvpPlan(out, "factorized", "fermion")
+
+
+def restrict_fermion(operator: Any, sector: ChargeSector, *, storage: str = "lazy",
+                     max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> FermionRestrictedOperator:
+    if storage not in ("lazy", "eager"):
+        raise ValueError("storage must be either 'eager' or 'lazy'")
+    validate_max_bytes(max_bytes)
+    if sector.blocks is None:
+        raise NotImplementedError("only sectors that fix the particle number of contiguous mode blocks are "
+                                  "implemented on the GPU")
+    plan = _fermion_plan(operator, sector, max_bytes, eager=storage == "eager")
+    restricted = FermionRestrictedOperator(operator, sector, None if storage == "eager" else plan, storage,
+                                           plan if storage == "eager" else None)
+    restricted.execution_family = "fermion"
+    return restricted
 
 
 def _lazy_blocks_plan(operator: Any, sector: ChargeSector, max_bytes: Optional[int]) -> Any:

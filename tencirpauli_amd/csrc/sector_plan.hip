@@ -67,12 +67,24 @@ struct SectorArgs {
     const uint32_t *goff;    // [G+1]
     const uint64_t *tz;      // [T] index-space z-masks
     const void *tw;          // [T] (re, im) of w = c * i^(#Y mod 4)
+    const uint64_t *tc1;     // [T] fermion terms: modes that must be occupied in the source (null: Pauli)
+    const uint64_t *tc0;     // [T] fermion terms: modes that must be empty in the source
     const uint32_t *lut;     // rank tables
     const uint64_t *choose;  // [64 * 64] C(p, j), p, j < 64
 };
 
 __device__ __forceinline__ double flip_sign(double x, uint32_t parity) {
     return __longlong_as_double(__double_as_longlong(x) ^ (static_cast<long long>(parity) << 63));
+}
+
+// A fermion ladder word acts on source s only when its annihilated modes are
+// occupied and its other created modes empty; then its value is
+// w (-1)^popc(z & s) like a Pauli term's.  Pauli terms (COND false) always act.
+template <bool COND>
+__device__ __forceinline__ bool term_acts(const SectorArgs &p, uint32_t k, uint64_t s) {
+    if (!COND) return true;
+    const uint64_t need = ((cptr<uint64_t>)p.tc1)[k];
+    return (s & need) == need && (s & ((cptr<uint64_t>)p.tc0)[k]) == 0;
 }
 
 // The rank/unrank/successor helpers are __host__ __device__ so the same code
@@ -163,7 +175,7 @@ struct Amp<fvec2> {
 // 4x4 Hubbard one (whose groups mostly leave the sector); 8 adds 19% there.
 constexpr uint32_t kGatherChunk = 4;
 
-template <typename VT, bool LDS>
+template <typename VT, bool LDS, bool COND>
 __global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT *__restrict__ v,
                                                                        VT *__restrict__ y,
                                                                        SectorArgs p) {
@@ -206,6 +218,7 @@ __global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT 
                     if (ok) {
                         const uint32_t k1 = goff[g + 1];
                         for (uint32_t k = goff[g]; k < k1; ++k) {
+                            if (!term_acts<COND>(p, k, s)) continue;
                             const uint32_t par = __popcll(tz[k] & s) & 1u;
                             const dvec2 w = tw[k];
                             dr += flip_sign(w.x, par);
@@ -283,6 +296,7 @@ __global__ void sector_walk_kernel(SectorArgs p, uint64_t first, uint64_t *__res
 // zero.  The smallest (source rank, group) pair that is not is recorded as
 // rank * G + group, which is the first error the reference's serial loop
 // raises.  State-independent, so it runs once per plan.
+template <bool COND>
 __global__ __launch_bounds__(kSectorThreads) void sector_leak_kernel(SectorArgs p,
                                                                      unsigned long long *first) {
     cptr<uint64_t> gx = (cptr<uint64_t>)p.gx;
@@ -302,6 +316,7 @@ __global__ __launch_bounds__(kSectorThreads) void sector_leak_kernel(SectorArgs 
                 double dr = 0.0, di = 0.0;
                 const uint32_t k1 = goff[g + 1];
                 for (uint32_t k = goff[g]; k < k1; ++k) {
+                    if (!term_acts<COND>(p, k, b)) continue;
                     const uint32_t par = __popcll(tz[k] & b) & 1u;
                     const dvec2 w = tw[k];
                     dr += flip_sign(w.x, par);
@@ -318,6 +333,7 @@ __global__ __launch_bounds__(kSectorThreads) void sector_leak_kernel(SectorArgs 
 }
 
 // Real part of the diagonal H_rr: the x = 0 group evaluated on each state.
+template <bool COND>
 __global__ void sector_diagonal_kernel(SectorArgs p, uint32_t diag_group, double *__restrict__ diag) {
     const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     const uint64_t start = t * kRanksPerThread;
@@ -328,7 +344,7 @@ __global__ void sector_diagonal_kernel(SectorArgs p, uint32_t diag_group, double
         if (diag_group < p.G) {
             const dvec2 *tw = static_cast<const dvec2 *>(p.tw);
             for (uint32_t k = p.goff[diag_group]; k < p.goff[diag_group + 1]; ++k)
-                d += flip_sign(tw[k].x, __popcll(p.tz[k] & b) & 1u);
+                if (term_acts<COND>(p, k, b)) d += flip_sign(tw[k].x, __popcll(p.tz[k] & b) & 1u);
         }
         diag[start + j] = d;
         b = sector_next(p, b);
@@ -353,22 +369,24 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
     return static_cast<uint16_t>(u >> 16);
 }
 
+template <bool COND>
 __device__ double diag_at(const SectorArgs &p, uint32_t diag_group, uint64_t s) {
     double d = 0.0;
     if (diag_group < p.G) {
         const dvec2 *tw = static_cast<const dvec2 *>(p.tw);
         for (uint32_t k = p.goff[diag_group]; k < p.goff[diag_group + 1]; ++k)
-            d += flip_sign(tw[k].x, __popcll(p.tz[k] & s) & 1u);
+            if (term_acts<COND>(p, k, s)) d += flip_sign(tw[k].x, __popcll(p.tz[k] & s) & 1u);
     }
     return d;
 }
 
+template <bool COND>
 __global__ void sector_features_kernel(SectorArgs p, uint32_t diag_group, double inv_scale,
                                        uint16_t *__restrict__ features, double *__restrict__ diag) {
     const uint64_t r = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (r >= p.dim) return;
     const uint64_t b = sector_unrank(p, r);
-    const double db = diag_at(p, diag_group, b);
+    const double db = diag_at<COND>(p, diag_group, b);
     float f[kFeatures];
     for (uint32_t i = 0; i < kFeatures; ++i) f[i] = 0.0f;
     double ncoup = 0.0, c2sum = 0.0, desum = 0.0, demin = 1e300, demax = -1e300, shift = 0.0;
@@ -379,13 +397,14 @@ __global__ void sector_features_kernel(SectorArgs p, uint32_t diag_group, double
         if (!in_sector(p, s)) continue;
         double vr = 0.0, vi = 0.0;
         for (uint32_t k = p.goff[g]; k < p.goff[g + 1]; ++k) {
+            if (!term_acts<COND>(p, k, s)) continue;
             const uint32_t par = __popcll(p.tz[k] & s) & 1u;
             vr += flip_sign(tw[k].x, par);
             vi += flip_sign(tw[k].y, par);
         }
         const double c2 = (vr * vr + vi * vi) * inv_scale * inv_scale;
         if (c2 == 0.0) continue;
-        const double de = (diag_at(p, diag_group, s) - db) * inv_scale;
+        const double de = (diag_at<COND>(p, diag_group, s) - db) * inv_scale;
         ncoup += 1.0;
         c2sum += c2;
         desum += de;
@@ -420,7 +439,7 @@ __global__ void sector_features_kernel(SectorArgs p, uint32_t diag_group, double
 
 constexpr uint32_t kCsrSortCap = 64;  // rows up to this length are sorted in registers/scratch
 
-template <bool LDS>
+template <bool LDS, bool COND>
 __global__ __launch_bounds__(kSectorThreads) void csr_count_kernel(SectorArgs p, uint32_t *__restrict__ counts) {
     extern __shared__ uint32_t lds_lut[];
     if (LDS) {
@@ -444,6 +463,7 @@ __global__ __launch_bounds__(kSectorThreads) void csr_count_kernel(SectorArgs p,
                 if (!in_sector(p, s)) continue;
                 double dr = 0.0, di = 0.0;
                 for (uint32_t k = goff[g]; k < goff[g + 1]; ++k) {
+                    if (!term_acts<COND>(p, k, s)) continue;
                     const uint32_t par = __popcll(tz[k] & s) & 1u;
                     dr += flip_sign(tw[k].x, par);
                     di += flip_sign(tw[k].y, par);
@@ -456,7 +476,7 @@ __global__ __launch_bounds__(kSectorThreads) void csr_count_kernel(SectorArgs p,
     }
 }
 
-template <bool LDS>
+template <bool LDS, bool COND>
 __global__ __launch_bounds__(kSectorThreads) void csr_fill_kernel(SectorArgs p, const uint64_t *__restrict__ indptr,
                                                                   uint32_t *__restrict__ cols, dvec2 *__restrict__ vals,
                                                                   fvec2 *__restrict__ vals32) {
@@ -490,6 +510,7 @@ __global__ __launch_bounds__(kSectorThreads) void csr_fill_kernel(SectorArgs p, 
                 if (!in_sector(p, s)) continue;
                 double dr = 0.0, di = 0.0;
                 for (uint32_t k = goff[g]; k < goff[g + 1]; ++k) {
+                    if (!term_acts<COND>(p, k, s)) continue;
                     const uint32_t par = __popcll(tz[k] & s) & 1u;
                     dr += flip_sign(tw[k].x, par);
                     di += flip_sign(tw[k].y, par);
@@ -607,7 +628,8 @@ struct tcp_sector_plan {
     uint64_t leak_source = 0;  // first leaking source rank
     uint32_t leak_weight = 0;  // its destination weight (U(1) message)
     SectorArgs host_args{};
-    tcp::DeviceBuffer gx, goff, tz, tw64, tw32, lut, choose;
+    bool cond = false;  // terms are fermion ladder words with occupancy conditions
+    tcp::DeviceBuffer gx, goff, tz, tw64, tw32, lut, choose, tc1, tc0;
     std::unique_ptr<tcp::WideSector> wide;  // U(1) sectors of 64 or more qubits
     uint64_t retained_bytes = 0;
     std::mutex staging_mutex;
@@ -619,6 +641,8 @@ struct tcp_sector_plan {
         a.goff = static_cast<const uint32_t *>(goff.ptr);
         a.tz = static_cast<const uint64_t *>(tz.ptr);
         a.tw = dtype == TCP_C64 ? tw32.ptr : tw64.ptr;
+        a.tc1 = cond ? static_cast<const uint64_t *>(tc1.ptr) : nullptr;
+        a.tc0 = cond ? static_cast<const uint64_t *>(tc0.ptr) : nullptr;
         a.lut = static_cast<const uint32_t *>(lut.ptr);
         a.choose = static_cast<const uint64_t *>(choose.ptr);
         return a;
@@ -724,22 +748,27 @@ int scan_leakage(tcp_sector_plan *plan, const std::vector<uint64_t> &gx);
 // every block's particle number).  The eager builders skip it, as the
 // reference's do (core/sector.rs:345-432 checks each aggregated transition
 // instead), and rely on the leakage scan.
-int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, uint64_t max_bytes,
-                      tcp_sector_plan **out, bool require_conserved = true) {
+// A sector plan over the terms of a Pauli operator (`op`) or of a fermion
+// operator (`fermion`, ladder words with occupancy conditions); exactly one is
+// given.
+int build_sector_plan_impl(const tcp::Operator *op, const tcp_fermion *fermion, const BlockSpec &spec, bool u1,
+                           uint64_t max_bytes, tcp_sector_plan **out, bool require_conserved) {
     if (!out) return fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
     *out = nullptr;
     uint32_t n = 0;
     for (uint32_t j = 0; j < spec.nblocks && j < kMaxBlocks; ++j) n += spec.sizes[j];
-    if (op.nqubits != n)
-        return fail(TCP_ERR_INCOMPATIBLE_QUBIT_COUNTS, "incompatible qubit counts: " +
-                                                            std::to_string(op.nqubits) + " and " +
-                                                            std::to_string(n));
+    const uint32_t width = op ? op->nqubits : fermion->n_modes;
+    if (width != n)
+        return fail(TCP_ERR_INCOMPATIBLE_QUBIT_COUNTS, "incompatible qubit counts: " + std::to_string(width) +
+                                                            " and " + std::to_string(n));
+    if (fermion && n > 63) return fail(TCP_ERR_UNSUPPORTED, "GPU fermion sector kernels support at most 63 modes");
     if (u1 && spec.nblocks == 1 && n >= 64) {
+        const tcp::Operator &opr = *op;
         // wide U(1) sector: packed-word bitstrings (see wide_sector.hpp)
         auto plan = new tcp_sector_plan();
         std::unique_ptr<tcp_sector_plan> guard(plan);
         plan->wide.reset(new tcp::WideSector());
-        int st = tcp::wide_build(op, spec.pops[0], max_bytes, require_conserved, *plan->wide);
+        int st = tcp::wide_build(opr, spec.pops[0], max_bytes, require_conserved, *plan->wide);
         if (st) return st;
         const tcp::WideSector &w = *plan->wide;
         plan->n = n;
@@ -772,13 +801,23 @@ int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, u
 
     // Exact conservation of each block's particle number: the precondition
     // of lazy restriction (py/symmetry.py:675-681, py/charge.py:1128-1132).
+    // Fermion words conserve a block's number when they create as many
+    // particles in it as they annihilate.
+    std::vector<tcp::CondTerm> terms;
+    if (fermion) terms = tcp::fermion_index_terms(*fermion);
     for (uint32_t j = 0; require_conserved && j < spec.nblocks; ++j) {
-        std::vector<double> levels(2 * n, 0.0);
-        const uint32_t first = n - a.shift[j] - spec.sizes[j];
-        for (uint32_t q = first; q < first + spec.sizes[j]; ++q) levels[2 * q + 1] = 1.0;
-        bool conserved = false;
-        uint64_t nonzero = 0;
-        if ((st = tcp::analyze_charge(op, levels.data(), conserved, nonzero))) return st;
+        bool conserved = true;
+        if (op) {
+            std::vector<double> levels(2 * n, 0.0);
+            const uint32_t first = n - a.shift[j] - spec.sizes[j];
+            for (uint32_t q = first; q < first + spec.sizes[j]; ++q) levels[2 * q + 1] = 1.0;
+            uint64_t nonzero = 0;
+            if ((st = tcp::analyze_charge(*op, levels.data(), conserved, nonzero))) return st;
+        } else {
+            const uint64_t block = a.bmask[j] << a.shift[j];
+            for (const tcp::CondTerm &t : terms)
+                conserved &= __builtin_popcountll(t.c1 & block) == __builtin_popcountll((t.c0 | (t.c1 & ~t.x)) & block);
+        }
         if (!conserved)
             return fail(TCP_ERR_SECTOR_LEAKAGE,
                         u1 ? "selected U1 sector requires an exactly conserved operator; nonzero "
@@ -798,7 +837,8 @@ int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, u
 
     // x-mask groups in first-appearance order over the canonical terms; a
     // group's terms keep canonical order (compile_terms, core/sector.rs:926-1021).
-    std::vector<tcp::IndexTerm> terms = tcp::index_terms(op);
+    if (op)
+        for (const tcp::IndexTerm &t : tcp::index_terms(*op)) terms.push_back(tcp::CondTerm{t.x, t.z, 0, 0, t.w});
     std::vector<uint64_t> gx;
     std::vector<std::vector<uint64_t>> members;
     for (uint64_t k = 0; k < terms.size(); ++k) {
@@ -810,12 +850,14 @@ int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, u
         members[g].push_back(k);
     }
     std::vector<uint32_t> goff;
-    std::vector<uint64_t> tz;
+    std::vector<uint64_t> tz, tc1, tc0;
     std::vector<dvec2> tw64, tw32;
     for (size_t g = 0; g < gx.size(); ++g) {
         goff.push_back(static_cast<uint32_t>(tz.size()));
         for (uint64_t k : members[g]) {
             tz.push_back(terms[k].z);
+            tc1.push_back(terms[k].c1);
+            tc0.push_back(terms[k].c0);
             tw64.push_back(dvec2{terms[k].w.re, terms[k].w.im});
             tw32.push_back(dvec2{static_cast<double>(static_cast<float>(terms[k].w.re)),
                                  static_cast<double>(static_cast<float>(terms[k].w.im))});
@@ -839,13 +881,15 @@ int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, u
     if (gx.empty()) gx.push_back(0);
     if (tz.empty()) {
         tz.push_back(0);
+        tc1.push_back(0);
+        tc0.push_back(0);
         tw64.push_back(dvec2{0, 0});
         tw32.push_back(dvec2{0, 0});
     }
     plan->lut_in_lds = lut.size() * 4 <= kLdsLutLimit;
     plan->host_args = a;
     plan->retained_bytes = sizeof(tcp_sector_plan) + gx.size() * 8 + goff.size() * 4 +
-                           tz.size() * (8 + 32) + lut.size() * 4 + choose_table.size() * 8;
+                           tz.size() * (fermion ? 8 + 32 + 16 : 8 + 32) + lut.size() * 4 + choose_table.size() * 8;
     if (tcp::exceeds_budget(plan->retained_bytes, max_bytes))
         return tcp::memory_limit(plan->retained_bytes, max_bytes);
 
@@ -857,10 +901,20 @@ int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, u
     if ((st = plan->tw32.upload(tw32.data(), tw32.size() * 16))) return st;
     if ((st = plan->lut.upload(lut.data(), lut.size() * 4))) return st;
     if ((st = plan->choose.upload(choose_table.data(), choose_table.size() * 8))) return st;
+    if (fermion) {
+        if ((st = plan->tc1.upload(tc1.data(), tc1.size() * 8))) return st;
+        if ((st = plan->tc0.upload(tc0.data(), tc0.size() * 8))) return st;
+        plan->cond = true;
+    }
     plan->u1 = u1;
     if ((st = scan_leakage(plan, gx))) return st;
     *out = guard.release();
     return tcp::ok();
+}
+
+int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, uint64_t max_bytes,
+                      tcp_sector_plan **out, bool require_conserved = true) {
+    return build_sector_plan_impl(&op, nullptr, spec, u1, max_bytes, out, require_conserved);
 }
 
 int host_unrank(const BlockSpec &spec, uint64_t index, uint64_t &bits);
@@ -874,8 +928,12 @@ int scan_leakage(tcp_sector_plan *plan, const std::vector<uint64_t> &gx) {
     if (e == hipSuccess) {
         const uint64_t per_block = static_cast<uint64_t>(kSectorThreads) * kRanksPerThread;
         const uint64_t blocks = std::min<uint64_t>((plan->dim + per_block - 1) / per_block, 256 * 16);
-        hipLaunchKernelGGL(sector_leak_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(kSectorThreads), 0,
-                           nullptr, plan->args(TCP_C128), dfirst);
+        if (plan->cond)
+            hipLaunchKernelGGL(sector_leak_kernel<true>, dim3(static_cast<uint32_t>(blocks)), dim3(kSectorThreads),
+                               0, nullptr, plan->args(TCP_C128), dfirst);
+        else
+            hipLaunchKernelGGL(sector_leak_kernel<false>, dim3(static_cast<uint32_t>(blocks)), dim3(kSectorThreads),
+                               0, nullptr, plan->args(TCP_C128), dfirst);
         e = hipGetLastError();
     }
     unsigned long long first = none;
@@ -926,8 +984,12 @@ int plan_features(const tcp_sector_plan *plan, double scale, uint16_t *features,
         return fail(TCP_ERR_UNSUPPORTED, "learned-preconditioner features support sectors of at most 63 qubits");
     if (plan->dim == 0) return TCP_OK;
     const uint32_t blocks = static_cast<uint32_t>((plan->dim + 255) / 256);
-    hipLaunchKernelGGL(sector_features_kernel, dim3(blocks), dim3(256), 0, stream, plan->args(TCP_C128),
-                       plan->diag_group, 1.0 / scale, features, diag);
+    if (plan->cond)
+        hipLaunchKernelGGL(sector_features_kernel<true>, dim3(blocks), dim3(256), 0, stream, plan->args(TCP_C128),
+                           plan->diag_group, 1.0 / scale, features, diag);
+    else
+        hipLaunchKernelGGL(sector_features_kernel<false>, dim3(blocks), dim3(256), 0, stream, plan->args(TCP_C128),
+                           plan->diag_group, 1.0 / scale, features, diag);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TCP_OK : hip_fail(e, "sector features kernel");
 }
@@ -936,8 +998,12 @@ int plan_diagonal(const tcp_sector_plan *plan, double *diag, hipStream_t stream)
     if (plan->dim == 0) return TCP_OK;
     const uint64_t threads = (plan->dim + kRanksPerThread - 1) / kRanksPerThread;
     const uint32_t blocks = static_cast<uint32_t>((threads + 255) / 256);
-    hipLaunchKernelGGL(sector_diagonal_kernel, dim3(blocks), dim3(256), 0, stream, plan->args(TCP_C128),
-                       plan->diag_group, diag);
+    if (plan->cond)
+        hipLaunchKernelGGL(sector_diagonal_kernel<true>, dim3(blocks), dim3(256), 0, stream, plan->args(TCP_C128),
+                           plan->diag_group, diag);
+    else
+        hipLaunchKernelGGL(sector_diagonal_kernel<false>, dim3(blocks), dim3(256), 0, stream, plan->args(TCP_C128),
+                           plan->diag_group, diag);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TCP_OK : hip_fail(e, "sector diagonal kernel");
 }
@@ -953,18 +1019,18 @@ int check_len(uint64_t expected, uint64_t actual) {
     return TCP_OK;
 }
 
-template <typename VT>
+template <typename VT, bool COND>
 hipError_t launch_gather(const tcp_sector_plan *plan, const void *in, void *out, int dtype,
                          hipStream_t stream) {
     SectorArgs a = plan->args(dtype);
     const uint64_t per_block = static_cast<uint64_t>(kSectorThreads) * kRanksPerThread;
     const uint64_t blocks = std::min<uint64_t>((plan->dim + per_block - 1) / per_block, 256 * 16);
     if (plan->lut_in_lds)
-        hipLaunchKernelGGL((sector_gather_kernel<VT, true>), dim3(static_cast<uint32_t>(blocks)),
+        hipLaunchKernelGGL((sector_gather_kernel<VT, true, COND>), dim3(static_cast<uint32_t>(blocks)),
                            dim3(kSectorThreads), a.lut_words * 4, stream, static_cast<const VT *>(in),
                            static_cast<VT *>(out), a);
     else
-        hipLaunchKernelGGL((sector_gather_kernel<VT, false>), dim3(static_cast<uint32_t>(blocks)),
+        hipLaunchKernelGGL((sector_gather_kernel<VT, false, COND>), dim3(static_cast<uint32_t>(blocks)),
                            dim3(kSectorThreads), 0, stream, static_cast<const VT *>(in),
                            static_cast<VT *>(out), a);
     return hipGetLastError();
@@ -983,8 +1049,13 @@ int apply_device(const tcp_sector_plan *plan, const void *in, void *out, int dty
     if (st) return st;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (plan->wide) return (st = tcp::wide_apply(*plan->wide, in, out, dtype, s)) ? st : tcp::ok();
-    hipError_t e = dtype == TCP_C128 ? launch_gather<dvec2>(plan, in, out, dtype, s)
-                                     : launch_gather<fvec2>(plan, in, out, dtype, s);
+    hipError_t e;
+    if (plan->cond)
+        e = dtype == TCP_C128 ? launch_gather<dvec2, true>(plan, in, out, dtype, s)
+                              : launch_gather<fvec2, true>(plan, in, out, dtype, s);
+    else
+        e = dtype == TCP_C128 ? launch_gather<dvec2, false>(plan, in, out, dtype, s)
+                              : launch_gather<fvec2, false>(plan, in, out, dtype, s);
     if (e != hipSuccess) return tcp::hip_fail(e, "sector kernel launch");
     return tcp::ok();
 }
@@ -1167,6 +1238,15 @@ int tcp_sector_plan_create_blocks(const tcp_operator *op, uint32_t nblocks, cons
     return build_sector_plan(tcp::operator_of(op), s, false, max_bytes, out);
 }
 
+int tcp_sector_plan_create_fermion(const tcp_fermion *f, uint32_t nblocks, const uint32_t *sizes,
+                                   const uint32_t *pops, uint64_t max_bytes, tcp_sector_plan **out) {
+    if (!f) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
+    BlockSpec s;
+    int st = blocks_spec(nblocks, sizes, pops, s);
+    if (!st) st = build_sector_plan_impl(nullptr, f, s, false, max_bytes, out, true);
+    return st ? st : tcp::ok();
+}
+
 uint64_t tcp_sector_plan_dimension(const tcp_sector_plan *plan) { return plan ? plan->dim : 0; }
 uint32_t tcp_sector_plan_nqubits(const tcp_sector_plan *plan) { return plan ? plan->n : 0; }
 uint64_t tcp_sector_plan_term_count(const tcp_sector_plan *plan) { return plan ? plan->T : 0; }
@@ -1327,25 +1407,36 @@ int build_csr(const tcp_sector_plan *sp, uint64_t max_bytes, tcp_csr_plan **out)
     const uint64_t per_block = static_cast<uint64_t>(kSectorThreads) * kRanksPerThread;
     const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((dim + per_block - 1) / per_block, 256 * 16));
     const size_t lds = sp->lut_in_lds ? a.lut_words * 4 : 0;
-    auto count = [&](uint32_t *counts) -> int {
-        if (sp->wide) return tcp::wide_csr_counts(*sp->wide, counts, nullptr);
+    auto count_with = [&](auto cond, uint32_t *counts) -> int {
+        constexpr bool C = decltype(cond)::value;
         if (sp->lut_in_lds)
-            hipLaunchKernelGGL(csr_count_kernel<true>, dim3(blocks), dim3(kSectorThreads), lds, nullptr, a, counts);
+            hipLaunchKernelGGL((csr_count_kernel<true, C>), dim3(blocks), dim3(kSectorThreads), lds, nullptr, a,
+                               counts);
         else
-            hipLaunchKernelGGL(csr_count_kernel<false>, dim3(blocks), dim3(kSectorThreads), 0, nullptr, a, counts);
+            hipLaunchKernelGGL((csr_count_kernel<false, C>), dim3(blocks), dim3(kSectorThreads), 0, nullptr, a,
+                               counts);
         TCP_HIP_CHECK(hipGetLastError());
         return TCP_OK;
     };
-    auto fill = [&](const uint64_t *indptr, uint32_t *cols, void *v64, void *v32) -> int {
-        if (sp->wide) return tcp::wide_csr_fill(*sp->wide, indptr, cols, v64, v32, nullptr);
+    auto fill_with = [&](auto cond, const uint64_t *indptr, uint32_t *cols, void *v64, void *v32) -> int {
+        constexpr bool C = decltype(cond)::value;
         if (sp->lut_in_lds)
-            hipLaunchKernelGGL(csr_fill_kernel<true>, dim3(blocks), dim3(kSectorThreads), lds, nullptr, a, indptr,
-                               cols, static_cast<dvec2 *>(v64), static_cast<fvec2 *>(v32));
+            hipLaunchKernelGGL((csr_fill_kernel<true, C>), dim3(blocks), dim3(kSectorThreads), lds, nullptr, a,
+                               indptr, cols, static_cast<dvec2 *>(v64), static_cast<fvec2 *>(v32));
         else
-            hipLaunchKernelGGL(csr_fill_kernel<false>, dim3(blocks), dim3(kSectorThreads), 0, nullptr, a, indptr,
-                               cols, static_cast<dvec2 *>(v64), static_cast<fvec2 *>(v32));
+            hipLaunchKernelGGL((csr_fill_kernel<false, C>), dim3(blocks), dim3(kSectorThreads), 0, nullptr, a,
+                               indptr, cols, static_cast<dvec2 *>(v64), static_cast<fvec2 *>(v32));
         TCP_HIP_CHECK(hipGetLastError());
         return TCP_OK;
+    };
+    auto count = [&](uint32_t *counts) -> int {
+        if (sp->wide) return tcp::wide_csr_counts(*sp->wide, counts, nullptr);
+        return sp->cond ? count_with(std::true_type(), counts) : count_with(std::false_type(), counts);
+    };
+    auto fill = [&](const uint64_t *indptr, uint32_t *cols, void *v64, void *v32) -> int {
+        if (sp->wide) return tcp::wide_csr_fill(*sp->wide, indptr, cols, v64, v32, nullptr);
+        return sp->cond ? fill_with(std::true_type(), indptr, cols, v64, v32)
+                        : fill_with(std::false_type(), indptr, cols, v64, v32);
     };
     return assemble_csr(sp->n, dim, sp->device, max_bytes, count, fill, out);
 }
@@ -1459,6 +1550,23 @@ int tcp_csr_plan_create_full(const tcp_operator *op, uint64_t max_bytes, tcp_csr
         return tcp::full_csr_fill(f, indptr, cols, v64, v32, nullptr);
     };
     st = assemble_csr(o.nqubits, f.dim, device, max_bytes, count, fill, out);
+    return st ? st : tcp::ok();
+}
+
+int tcp_csr_plan_create_fermion(const tcp_fermion *f, uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops,
+                                uint64_t max_bytes, tcp_csr_plan **out) {
+    if (!f) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
+    if (!out) return fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    BlockSpec s;
+    int st = blocks_spec(nblocks, sizes, pops, s);
+    if (st) return st;
+    tcp_sector_plan *sp = nullptr;
+    if ((st = build_sector_plan_impl(nullptr, f, s, false, max_bytes, &sp, false))) return st;
+    std::unique_ptr<tcp_sector_plan> guard(sp);
+    tcp::DeviceGuard dg;
+    if ((st = dg.enter(sp->device))) return st;
+    st = build_csr(sp, max_bytes, out);
     return st ? st : tcp::ok();
 }
 

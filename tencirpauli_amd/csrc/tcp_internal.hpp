@@ -82,6 +82,16 @@ struct IndexTerm {
 };
 std::vector<IndexTerm> index_terms(const Operator &op);
 
+// Index-space form of a canonical fermion ladder word (mode m -> bit n-1-m):
+// on a source s with every bit of c1 set and every bit of c0 clear, the word
+// maps s to s ^ x with value w (-1)^popcount(z & s); elsewhere it gives zero.
+// z is the Jordan-Wigner parity mask of the word and w carries its constant
+// sign.  Pauli terms are the special case c1 = c0 = 0.
+struct CondTerm {
+    uint64_t x, z, c1, c0;
+    Complex w;
+};
+
 // Exact conservation analysis of a diagonal qubit charge (core/operator.rs:396-445).
 int analyze_charge(const Operator &op, const double *levels, bool &conserved, uint64_t &nonzero);
 
@@ -128,4 +138,18 @@ inline bool ranges_overlap(const void *a, size_t abytes, const void *b, size_t b
     return pa < pb + bbytes && pb < pa + abytes;
 }
 
+}  // namespace tcp
+
+// A canonical fermion operator: word t is creation[t] (increasing modes)
+// followed by annihilation[t] (decreasing modes), with coefficient coef[t].
+struct tcp_fermion {
+    uint32_t n_modes = 0;
+    std::vector<std::vector<uint32_t>> creation, annihilation;
+    std::vector<tcp::Complex> coef;
+};
+
+namespace tcp {
+// The words of a fermion operator on at most 64 modes as CondTerms, in
+// canonical order.
+std::vector<CondTerm> fermion_index_terms(const tcp_fermion &f);
 }  // namespace tcp

@@ -223,6 +223,14 @@ class PauliOperator:
         a ``ChargeSector`` of contiguous fixed-population qubit blocks uses
         the factorized block path.
         """
+        from .charge import ChargeSector
+
+        if isinstance(sector, ChargeSector) and sector.space is not None and sector.space.fermions:
+            raise ValueError("Pauli operator and charge sector layouts are incompatible")
+        return self._restrict(sector, storage, max_bytes)
+
+    def _restrict(self, sector: Any, storage: str, max_bytes: Optional[int]) -> Any:
+        """Sector dispatch shared with the Jordan-Wigner image of a FermionOperator."""
         from .charge import ChargeSector, restrict_charge
         from .symmetry import U1Sector, canonical_u1_sector, restrict_u1
 
