@@ -208,34 +208,40 @@ def extra_sector(cpu_seconds: float = 0.0) -> dict:
 
 def extra_hubbard_half() -> dict:
     """4x4 Fermi-Hubbard at half filling (N_up = N_dn = 8 of 16; D = C(16,8)^2 = 165,636,900; 2.65 GB
-    complex128 vectors), Jordan-Wigner Pauli form on the lazy block-sector plan.  The reference's
-    one recorded restricted-MVP timing is this sector: lazy apply() at 54.004 s on an arm64/macOS CPU
-    through its structured fermion path (docs/vibe/reports/phase-8.5-review-2026-08-04.md:191)."""
+    complex128 vectors), built exactly as the reference's example builds it -- FermionOperator from
+    raw ladder terms, ChargeSector of total number and 2 S_z on an OperatorSpace, restrict_charge,
+    apply (examples/research/fermi_hubbard/run_tencirpauli.py:22-98, 146-160) -- and run on the
+    ladder-word gather; the Jordan-Wigner Pauli image on the Pauli block path beside it.  The
+    reference's one recorded restricted-MVP timing is this sector: lazy apply() at 54.004 s on an
+    arm64/macOS CPU (docs/vibe/reports/phase-8.5-review-2026-08-04.md:191)."""
     import torch
 
     import tencirpauli_amd as tcp
-    from tencirpauli_amd import lattice
+    from tencirpauli_amd import lanczos, lattice
 
-    codes, coef = lattice.fermi_hubbard_jw(4, 4, 1.0, 4.0, seed=SEED + 3)
-    op = tcp.PauliOperator.from_code_arrays(codes, coef)
-    up = tcp.AdditiveCharge.number(32, range(16))
-    dn = tcp.AdditiveCharge.number(32, range(16, 32))
+    sites = 16
+    terms = lattice.hubbard_ladder_terms(4, 4, 1.0, 4.0)
     t0 = time.perf_counter()
-    restricted = op.restrict_charge(tcp.ChargeSector(((up, 8), (dn, 8))))
+    operator = tcp.FermionOperator.from_terms(2 * sites, terms)
+    space = tcp.OperatorSpace(fermions=2 * sites)
+    total = tcp.AdditiveCharge(space, name="particle_number", fermions={m: 1 for m in range(2 * sites)})
+    spin = tcp.AdditiveCharge(space, name="two_Sz", fermions={m: (1 if m < sites else -1) for m in range(2 * sites)})
+    sector = tcp.ChargeSector(((total, sites), (spin, 0)), max_bytes=16 * 1024 ** 3)
+    restricted = operator.restrict_charge(sector, storage="lazy")
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
+    pauli = operator.map_fermions()._restrict(sector, "lazy", None)
     gen = torch.Generator(device="cuda").manual_seed(SEED + 11)
     v = torch.randn(restricted.dimension, dtype=torch.complex128, device="cuda", generator=gen)
-    rec = {"dimension": restricted.dimension, "plan_build_s": round(build_s, 3)}
+    rec = {"dimension": restricted.dimension, "fermion_words": operator.term_count,
+           "pauli_terms": operator.map_fermions().term_count, "plan_build_s": round(build_s, 3)}
     for tag, dt in (("fp64", torch.complex128), ("fp32", torch.complex64)):
         vv = v.to(dt)
         yy = torch.empty_like(vv)
-        ms = time_applies(lambda: restricted.apply_into(vv, yy), 5)
-        rec[f"ms_per_mvp_{tag}"] = round(ms, 3)
+        rec[f"ms_per_mvp_{tag}"] = round(time_applies(lambda: restricted.apply_into(vv, yy), 5), 3)
+        rec[f"ms_per_mvp_jw_pauli_{tag}"] = round(time_applies(lambda: pauli.apply_into(vv, yy), 5), 3)
         del vv, yy
     # the example's eigen call on this sector, every step on the GPU
-    from tencirpauli_amd import lanczos
-
     res = lanczos.lanczos(restricted, 1, v0=v, tol=1e-8, max_restarts=300, ncv=20, residuals=True)
     rec["eigsh_k1_sa"] = {"seconds": round(res.seconds, 2), "mvps": res.mvps, "restarts": res.restarts,
                           "converged": res.converged, "energy": float(res.eigenvalues[0]),

@@ -50,31 +50,6 @@ def operator_matrix(n_modes: int, terms: Iterable[Tuple[Sequence[Factor], comple
     return out
 
 
-def hubbard_terms(rows: int, cols: int, hopping: float, interaction: float):
-    """Raw ladder terms of the spinful Fermi-Hubbard model in the reference
-    example's layout: up modes 0..S-1, down modes S..2S-1, open bonds
-    (examples/research/fermi_hubbard/run_tencirpauli.py:22-80)."""
-    sites = rows * cols
-    bonds = []
-    for r in range(rows):
-        for c in range(cols):
-            s = r * cols + c
-            if c + 1 < cols:
-                bonds.append((s, s + 1))
-            if r + 1 < rows:
-                bonds.append((s, s + cols))
-    terms = []
-    for s in range(sites):
-        up, dn = s, sites + s
-        terms.append((((up, "create"), (up, "annihilate"), (dn, "create"), (dn, "annihilate")), complex(interaction)))
-    for left, right in bonds:
-        for spin in (0, 1):
-            a, b = left + spin * sites, right + spin * sites
-            terms.append((((a, "create"), (b, "annihilate")), -hopping))
-            terms.append((((b, "create"), (a, "annihilate")), -hopping))
-    return terms
-
-
 def spinful_terms(sites: int):
     """Complex hopping, on-site and pair-hopping terms with exact cancellations
     (the shape of the reference's tests/test_charge.py:652-690)."""

@@ -189,3 +189,29 @@ def gaussian_state(dimension: int, seed: int, dtype=np.complex128) -> np.ndarray
     v = (re + 1j * im).astype(dtype)
     v /= np.linalg.norm(v)
     return v
+
+
+def hubbard_ladder_terms(rows: int, cols: int, hopping: float, interaction: float):
+    """Raw ladder terms of the spinful Fermi-Hubbard model, up modes 0..S-1 and
+    down modes S..2S-1, open bonds, U n_up n_dn on every site and -t on every
+    bond and spin in both directions: the FermionOperator input of the
+    reference's study (examples/research/fermi_hubbard/run_tencirpauli.py:22-80)."""
+    sites = rows * cols
+    bonds = []
+    for r in range(rows):
+        for c in range(cols):
+            s = r * cols + c
+            if c + 1 < cols:
+                bonds.append((s, s + 1))
+            if r + 1 < rows:
+                bonds.append((s, s + cols))
+    terms = []
+    for s in range(sites):
+        up, dn = s, sites + s
+        terms.append((((up, "create"), (up, "annihilate"), (dn, "create"), (dn, "annihilate")), complex(interaction)))
+    for left, right in bonds:
+        for spin in (0, 1):
+            a, b = left + spin * sites, right + spin * sites
+            terms.append((((a, "create"), (b, "annihilate")), -hopping))
+            terms.append((((b, "create"), (a, "annihilate")), -hopping))
+    return terms

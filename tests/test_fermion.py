@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import tencirpauli_amd as tcp
+from tencirpauli_amd import lattice
 from oracle import dense
 from oracle import fermion as foracle
 
@@ -88,7 +89,7 @@ def test_car_identities_across_modes():
 
 def test_hubbard_quartic_terms_and_adjoint():
     # tests/test_structured_algebra.py:513-530: two-site spinful Hubbard
-    terms = foracle.hubbard_terms(1, 2, 0.7, 1.25)
+    terms = lattice.hubbard_ladder_terms(1, 2, 0.7, 1.25)
     op = tcp.FermionOperator.from_terms(4, terms)
     assert max(len(t.word.factors) for t in op.terms) == 4
     expected = foracle.operator_matrix(4, terms)

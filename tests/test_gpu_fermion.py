@@ -18,6 +18,7 @@ import pytest
 
 import oracle
 import tencirpauli_amd as tcp
+from tencirpauli_amd import lattice
 from oracle import dense
 from oracle import fermion as foracle
 from tests.helpers import rel_err
@@ -67,7 +68,7 @@ def test_spinful_lazy_and_eager_match_the_occupation_oracle(sites, particles):
 def test_reference_example_api_on_a_2x2_hubbard_at_half_filling():
     # examples/research/fermi_hubbard/run_tencirpauli.py:82-160 at rows = cols = 2
     rows, cols, sites = 2, 2, 4
-    terms = foracle.hubbard_terms(rows, cols, 1.0, 4.0)
+    terms = lattice.hubbard_ladder_terms(rows, cols, 1.0, 4.0)
     operator = tcp.FermionOperator.from_terms(2 * sites, terms)
     space = tcp.OperatorSpace(fermions=2 * sites)
     total = tcp.AdditiveCharge(space, name="particle_number", fermions={m: 1 for m in range(2 * sites)})
@@ -95,7 +96,7 @@ def test_compile_targets_follow_the_occupation_basis():
     aa, ad = annihilate.compile("dense"), create.compile("dense")
     np.testing.assert_allclose(aa @ ad + ad @ aa, np.eye(4))
     np.testing.assert_allclose(aa, foracle.product_matrix(2, ((0, "annihilate"),)))
-    terms = foracle.hubbard_terms(1, 2, 0.7, 1.25)
+    terms = lattice.hubbard_ladder_terms(1, 2, 0.7, 1.25)
     op = tcp.FermionOperator.from_terms(4, terms)
     expected = foracle.operator_matrix(4, terms)
     np.testing.assert_allclose(op.compile("dense"), expected, atol=1e-12)
@@ -108,7 +109,7 @@ def test_compile_targets_follow_the_occupation_basis():
 def test_3x4_hubbard_quarter_filling_against_the_generic_charge_loop():
     # 24 modes, N_up = N_dn = 3: D = C(12,3)^2 = 48,400; the C oracle's serial
     # generic loop (core/charge/generic.rs:949-1045) over the same Pauli image
-    terms = foracle.hubbard_terms(3, 4, 1.0, 4.0)
+    terms = lattice.hubbard_ladder_terms(3, 4, 1.0, 4.0)
     op = tcp.FermionOperator.from_terms(24, terms)
     sector = _spinful_sector(tcp.OperatorSpace(fermions=24), 12, 3)
     restricted = op.restrict_charge(sector)
