@@ -69,6 +69,7 @@ struct SectorArgs {
     const void *tw;          // [T] (re, im) of w = c * i^(#Y mod 4)
     const uint64_t *tc1;     // [T] fermion terms: modes that must be occupied in the source (null: Pauli)
     const uint64_t *tc0;     // [T] fermion terms: modes that must be empty in the source
+    const uint32_t *gblk;    // [G] bit j set when group g's x-mask touches block j
     const uint32_t *lut;     // rank tables
     const uint64_t *choose;  // [64 * 64] C(p, j), p, j < 64
 };
@@ -96,20 +97,24 @@ __host__ __device__ __forceinline__ bool in_sector(const SectorArgs &p, uint64_t
     return ok;
 }
 
+// Rank of block j's bits of s among the block's fixed-population patterns.
+__host__ __device__ __forceinline__ uint32_t block_rank(const SectorArgs &p, const uint32_t *lut, uint32_t j,
+                                                         uint64_t s) {
+    const uint64_t w = (s >> p.shift[j]) & p.bmask[j];
+    const uint32_t width = (p.pop[j] + 1) * 256;
+    const uint32_t *table = lut + p.lut_off[j];
+    uint32_t local = 0, below = 0;
+    for (uint32_t c = 0; c < p.nchunk[j]; ++c) {
+        const uint32_t byte = static_cast<uint32_t>(w >> (8 * c)) & 0xffu;
+        local += table[c * width + below * 256 + byte];
+        below += __builtin_popcount(byte);
+    }
+    return local;
+}
+
 __host__ __device__ __forceinline__ uint32_t sector_rank(const SectorArgs &p, const uint32_t *lut, uint64_t s) {
     uint32_t rank = 0;
-    for (uint32_t j = 0; j < p.nblocks; ++j) {
-        const uint64_t w = (s >> p.shift[j]) & p.bmask[j];
-        const uint32_t width = (p.pop[j] + 1) * 256;
-        const uint32_t *table = lut + p.lut_off[j];
-        uint32_t local = 0, below = 0;
-        for (uint32_t c = 0; c < p.nchunk[j]; ++c) {
-            const uint32_t byte = static_cast<uint32_t>(w >> (8 * c)) & 0xffu;
-            local += table[c * width + below * 256 + byte];
-            below += __builtin_popcount(byte);
-        }
-        rank += local * p.stride[j];
-    }
+    for (uint32_t j = 0; j < p.nblocks; ++j) rank += block_rank(p, lut, j, s) * p.stride[j];
     return rank;
 }
 
@@ -190,6 +195,7 @@ __global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT 
     cptr<uint32_t> goff = (cptr<uint32_t>)p.goff;
     cptr<uint64_t> tz = (cptr<uint64_t>)p.tz;
     cptr<dvec2> tw = (cptr<dvec2>)p.tw;
+    cptr<uint32_t> gblk = (cptr<uint32_t>)p.gblk;
 
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * kRanksPerThread;
     for (uint64_t r0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kRanksPerThread;
@@ -198,6 +204,11 @@ __global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT 
         for (uint32_t j = 0; j < kRanksPerThread; ++j) {
             const uint64_t r = r0 + j;
             if (r >= p.dim) break;
+            // a group leaves the blocks its x-mask does not touch as they are in
+            // b: only touched blocks are tested and re-ranked
+            uint32_t lr[kMaxBlocks];
+#pragma unroll
+            for (uint32_t q = 0; q < kMaxBlocks; ++q) lr[q] = q < p.nblocks ? block_rank(p, lut, q, b) : 0u;
             double ar = 0.0, ai = 0.0;
             // kGatherChunk groups at a time: their source ranks and weights
             // first, then all their gathers in flight together (a group that
@@ -212,9 +223,14 @@ __global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT 
                     const uint32_t g = g0 + u;
                     const bool live = g < p.G;
                     const uint64_t x = live ? gx[g] : 0;
+                    const uint32_t touched = live ? gblk[g] : 0u;
                     const uint64_t s = b ^ x;
                     double dr = 0.0, di = 0.0;
-                    bool ok = live && in_sector(p, s);
+                    bool ok = live;
+#pragma unroll
+                    for (uint32_t q = 0; q < kMaxBlocks; ++q)
+                        if (q < p.nblocks && ((touched >> q) & 1u))
+                            ok &= static_cast<uint32_t>(__popcll((s >> p.shift[q]) & p.bmask[q])) == p.pop[q];
                     if (ok) {
                         const uint32_t k1 = goff[g + 1];
                         for (uint32_t k = goff[g]; k < k1; ++k) {
@@ -228,7 +244,12 @@ __global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT 
                     }
                     uint64_t src = r;
                     if (ok && x != 0) {
-                        src = sector_rank(p, lut, s);
+                        uint32_t rank = 0;
+#pragma unroll
+                        for (uint32_t q = 0; q < kMaxBlocks; ++q)
+                            if (q < p.nblocks)
+                                rank += (((touched >> q) & 1u) ? block_rank(p, lut, q, s) : lr[q]) * p.stride[q];
+                        src = rank;
                         ok = src < p.dim;  // unreachable for in-sector s; keeps the gather in bounds
                         if (!ok) src = r;
                     }
@@ -629,7 +650,7 @@ struct tcp_sector_plan {
     uint32_t leak_weight = 0;  // its destination weight (U(1) message)
     SectorArgs host_args{};
     bool cond = false;  // terms are fermion ladder words with occupancy conditions
-    tcp::DeviceBuffer gx, goff, tz, tw64, tw32, lut, choose, tc1, tc0;
+    tcp::DeviceBuffer gx, goff, tz, tw64, tw32, lut, choose, tc1, tc0, gblk;
     std::unique_ptr<tcp::WideSector> wide;  // U(1) sectors of 64 or more qubits
     uint64_t retained_bytes = 0;
     std::mutex staging_mutex;
@@ -643,6 +664,7 @@ struct tcp_sector_plan {
         a.tw = dtype == TCP_C64 ? tw32.ptr : tw64.ptr;
         a.tc1 = cond ? static_cast<const uint64_t *>(tc1.ptr) : nullptr;
         a.tc0 = cond ? static_cast<const uint64_t *>(tc0.ptr) : nullptr;
+        a.gblk = static_cast<const uint32_t *>(gblk.ptr);
         a.lut = static_cast<const uint32_t *>(lut.ptr);
         a.choose = static_cast<const uint64_t *>(choose.ptr);
         return a;
@@ -901,6 +923,11 @@ int build_sector_plan_impl(const tcp::Operator *op, const tcp_fermion *fermion, 
     if ((st = plan->tw32.upload(tw32.data(), tw32.size() * 16))) return st;
     if ((st = plan->lut.upload(lut.data(), lut.size() * 4))) return st;
     if ((st = plan->choose.upload(choose_table.data(), choose_table.size() * 8))) return st;
+    std::vector<uint32_t> gblk(gx.size(), 0);
+    for (size_t g = 0; g < gx.size(); ++g)
+        for (uint32_t j = 0; j < spec.nblocks; ++j)
+            if ((gx[g] >> a.shift[j]) & a.bmask[j]) gblk[g] |= 1u << j;
+    if ((st = plan->gblk.upload(gblk.data(), gblk.size() * 4))) return st;
     if (fermion) {
         if ((st = plan->tc1.upload(tc1.data(), tc1.size() * 8))) return st;
         if ((st = plan->tc0.upload(tc0.data(), tc0.size() * 8))) return st;
