@@ -91,16 +91,19 @@ class AdditiveCharge:
     def level_array(self) -> np.ndarray:
         return np.asarray(self.qubit_levels, dtype=np.float64).reshape(self.nqubits, 2)
 
-    def sector(self, target: int, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "ChargeSector":
-        return ChargeSector(((self, target),), max_bytes=max_bytes)
+    def sector(self, target: int, *, boson_cutoffs: Any = None,
+               max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "ChargeSector":
+        return ChargeSector(((self, target),), boson_cutoffs=boson_cutoffs, max_bytes=max_bytes)
 
 
 class ChargeSector:
     """Computational states satisfying every (charge, target) constraint."""
 
     def __init__(self, constraints: Sequence[Tuple[AdditiveCharge, int]], *,
-                 max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> None:
+                 boson_cutoffs: Any = None, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> None:
         validate_max_bytes(max_bytes)
+        if boson_cutoffs:
+            raise NotImplementedError("boson cutoffs: boson axes are not on the GPU path")
         constraints = tuple((c, int(t)) for c, t in constraints)
         if not constraints:
             raise ValueError("a charge sector needs at least one constraint")
