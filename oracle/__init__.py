@@ -17,6 +17,12 @@ Two independent oracles live here:
 * ``dense.py``: a dense Kronecker-sum oracle and brute-force sector bases built
   from textbook definitions only (the pattern of the reference's
   tests/reference.py:15-115 and tests/test_charge.py:63-83).
+* ``wide.py``: the U(1) restricted operator over Python big integers for
+  sectors of more than 64 qubits.
+* ``fermion.py``: ladder operators on the occupation basis from their
+  definition (tests/test_structured_algebra.py:256-277), the checker of the
+  fermion canonicalisation, the Jordan-Wigner map and the fermion sector
+  plans.
 
 Parity of each function is pinned in ``tests/test_oracle.py`` against the
 reference's known-answer vectors (``tests/golden/``) and the dense oracle.

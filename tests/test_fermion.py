@@ -152,3 +152,23 @@ def test_pauli_operator_rejects_a_fermion_sector():
     number = tcp.AdditiveCharge(space, fermions={0: 1, 1: 1})
     with pytest.raises(ValueError, match="incompatible"):
         tcp.PauliOperator.from_terms(2, [("ZZ", 1.0)]).restrict_charge(number.sector(1))
+
+
+def test_occupation_oracle_is_pinned_by_the_car_and_known_matrices():
+    # The oracle itself: {a_i, a+_j} = delta_ij, {a_i, a_j} = 0, a+ = a^H, the
+    # single-mode a = |0><1|, and the sign of a_1 on |11> (mode 0 occupied below).
+    n = 3
+    eye = np.eye(1 << n)
+    a = [foracle.ladder_matrix(n, m, "annihilate") for m in range(n)]
+    c = [foracle.ladder_matrix(n, m, "create") for m in range(n)]
+    for i in range(n):
+        np.testing.assert_array_equal(c[i], a[i].conj().T)
+        for j in range(n):
+            np.testing.assert_array_equal(a[i] @ c[j] + c[j] @ a[i], eye if i == j else 0 * eye)
+            np.testing.assert_array_equal(a[i] @ a[j] + a[j] @ a[i], 0 * eye)
+    np.testing.assert_array_equal(foracle.ladder_matrix(1, 0, "annihilate"), [[0, 1], [0, 0]])
+    a1 = foracle.ladder_matrix(2, 1, "annihilate")
+    assert a1[0b10, 0b11] == -1.0 and a1[0b00, 0b01] == 1.0
+    # products apply right to left: a+_0 a_0 is the number operator of mode 0
+    np.testing.assert_array_equal(foracle.product_matrix(2, ((0, "create"), (0, "annihilate"))),
+                                  np.diag([0, 0, 1, 1]))
