@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <utility>
@@ -53,10 +54,42 @@ void add_scaled(IntegerMap &target, const IntegerMap &source, __int128 factor) {
     for (const auto &kv : source) target[kv.first] += kv.second * factor;
 }
 
+// A product in which no annihilation precedes a creation of the same mode
+// never contracts: it is zero when a mode repeats among its creations or its
+// annihilations, and otherwise the canonical word with the sign of the
+// permutation that sorts it (the one-key result of
+// canonical_without_contractions, core/structured/boson.rs:241-300).
+bool rewrite_without_contractions(const std::vector<Factor> &seq, IntegerMap &out) {
+    for (size_t i = 0; i < seq.size(); ++i)
+        if (!seq[i].second)
+            for (size_t j = i + 1; j < seq.size(); ++j)
+                if (seq[j].second && seq[j].first == seq[i].first) return false;
+    Word w;
+    for (const auto &[mode, creation] : seq) (creation ? w.first : w.second).push_back(mode);
+    // canonical position key: creations by increasing mode, then annihilations by decreasing mode
+    auto key = [](const Factor &f) { return f.second ? std::make_pair(0, static_cast<int64_t>(f.first))
+                                                     : std::make_pair(1, -static_cast<int64_t>(f.first)); };
+    uint64_t inversions = 0;
+    for (size_t i = 0; i < seq.size(); ++i)
+        for (size_t j = i + 1; j < seq.size(); ++j) {
+            if (key(seq[i]) == key(seq[j])) {  // a repeated mode of one kind: nilpotent
+                out.clear();
+                return true;
+            }
+            inversions += key(seq[j]) < key(seq[i]);
+        }
+    std::sort(w.first.begin(), w.first.end());
+    std::sort(w.second.begin(), w.second.end(), std::greater<uint32_t>());
+    out.clear();
+    out.emplace(std::move(w), (inversions & 1) ? -1 : 1);
+    return true;
+}
+
 // Normal-orders one product: the first adjacent pair that is nilpotent makes
 // the word vanish; the first adjacent inversion is swapped (weight -1) and,
 // for a_m a+_m, also contracted (weight +1); a word with neither is canonical.
 int rewrite(const std::vector<Factor> &seq, uint64_t max_bytes, IntegerMap &out) {
+    if (rewrite_without_contractions(seq, out)) return TCP_OK;
     out.clear();
     for (size_t i = 0; i + 1 < seq.size(); ++i) {
         const auto [lm, lc] = seq[i];

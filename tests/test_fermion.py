@@ -172,3 +172,17 @@ def test_occupation_oracle_is_pinned_by_the_car_and_known_matrices():
     # products apply right to left: a+_0 a_0 is the number operator of mode 0
     np.testing.assert_array_equal(foracle.product_matrix(2, ((0, "create"), (0, "annihilate"))),
                                   np.diag([0, 0, 1, 1]))
+
+
+def test_long_products_without_contractions_take_the_sign_of_their_sort():
+    # 200 annihilations in increasing order reverse into canonical order:
+    # 19,900 transpositions, an even permutation
+    n = 200
+    op = tcp.FermionOperator.from_terms(n, [(tuple((m, "annihilate") for m in range(n)), 1.0)])
+    assert op.term_count == 1 and op.terms[0].coefficient == 1.0
+    assert op.terms[0].word.annihilation_modes == tuple(range(n - 1, -1, -1))
+    # a_0 a_1 a+_0 = -a_1 - a+_0 a_1 a_0 needs the contraction path
+    op = tcp.FermionOperator.from_terms(2, [(((0, "annihilate"), (1, "annihilate"), (0, "create")), 1.0)])
+    assert op.to_dict() == {((1, "annihilate"),): -1.0, ((0, "create"), (1, "annihilate"), (0, "annihilate")): -1.0}
+    np.testing.assert_allclose(_jw_dense(op), foracle.product_matrix(2, ((0, "annihilate"), (1, "annihilate"),
+                                                                         (0, "create"))), atol=1e-15)
