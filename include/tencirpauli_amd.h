@@ -109,6 +109,18 @@ int tcp_operator_export_words(const tcp_operator *op, uint64_t *x_words, uint64_
 int tcp_operator_analyze_charge(const tcp_operator *op, const double *qubit_levels,
                                 int *is_conserved, uint64_t *nonzero);
 
+/* Operator algebra of PauliOperator (core/operator.rs:217-393; py/pauli.py:
+ * 597-668): add merges equal words (left + right, exact zeros dropped); scale
+ * multiplies every coefficient (zero scalar -> empty operator); multiply forms
+ * every product word with its exact Pauli phase and sums equal words in
+ * IEEE-bit-pattern order (max_bytes bounds term pairs x (nqubits + 16) bytes);
+ * adjoint conjugates the coefficients.  Operands must have equal qubit
+ * counts (TCP_ERR_INCOMPATIBLE_QUBIT_COUNTS). */
+int tcp_operator_add(const tcp_operator *a, const tcp_operator *b, tcp_operator **out);
+int tcp_operator_scale(const tcp_operator *a, double re, double im, tcp_operator **out);
+int tcp_operator_multiply(const tcp_operator *a, const tcp_operator *b, uint64_t max_bytes, tcp_operator **out);
+int tcp_operator_adjoint(const tcp_operator *a, tcp_operator **out);
+
 void tcp_operator_destroy(tcp_operator *op);
 
 /* ---- fermion operators: FermionOperator.from_terms / map_fermions

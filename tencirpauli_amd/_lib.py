@@ -67,6 +67,10 @@ SIGNATURES = {
     "tcp_fermion_export": (_c.c_int, [_vp, _u64p, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint8), _dp]),
     "tcp_fermion_jordan_wigner": (_c.c_int, [_vp, _c.c_uint64, _c.POINTER(_vp)]),
     "tcp_fermion_destroy": (None, [_vp]),
+    "tcp_operator_add": (_c.c_int, [_vp, _vp, _c.POINTER(_vp)]),
+    "tcp_operator_scale": (_c.c_int, [_vp, _c.c_double, _c.c_double, _c.POINTER(_vp)]),
+    "tcp_operator_multiply": (_c.c_int, [_vp, _vp, _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_operator_adjoint": (_c.c_int, [_vp, _c.POINTER(_vp)]),
     "tcp_operator_from_words": (_c.c_int, [_c.c_uint32, _c.c_uint64, _u64p, _u64p, _dp, _c.POINTER(_vp)]),
     "tcp_operator_nqubits": (_c.c_uint32, [_vp]),
     "tcp_operator_term_count": (_c.c_uint64, [_vp]),

@@ -306,6 +306,121 @@ int tcp_operator_analyze_charge(const tcp_operator *op, const double *qubit_leve
     return tcp::ok();
 }
 
+// ---- operator algebra (core/operator.rs:217-393) --------------------------
+
+namespace {
+
+int compatible(const tcp_operator *a, const tcp_operator *b) {
+    if (!a || !b) return tcp::fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
+    if (a->op.nqubits != b->op.nqubits)
+        return tcp::fail(TCP_ERR_INCOMPATIBLE_QUBIT_COUNTS, "incompatible qubit counts: " +
+                                                                std::to_string(a->op.nqubits) + " and " +
+                                                                std::to_string(b->op.nqubits));
+    return TCP_OK;
+}
+
+tcp::Complex cmul(tcp::Complex a, tcp::Complex b) {
+    return tcp::Complex{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+int wrap(const std::vector<uint8_t> &codes, const std::vector<double> &coef, uint32_t n, tcp_operator **out) {
+    auto *h = new tcp_operator();
+    const int st = tcp::canonicalize(n, coef.size() / 2, codes.data(), coef.data(), h->op);
+    if (st) {
+        delete h;
+        return st;
+    }
+    *out = h;
+    return TCP_OK;
+}
+
+}  // namespace
+
+int tcp_operator_add(const tcp_operator *a, const tcp_operator *b, tcp_operator **out) {
+    if (!out) return tcp::fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    int st = compatible(a, b);
+    if (st) return st;
+    // equal words add their two coefficients (an IEEE sum is commutative, so
+    // the canonical bit-pattern order gives the reference's left + right)
+    std::vector<uint8_t> codes(a->op.codes);
+    codes.insert(codes.end(), b->op.codes.begin(), b->op.codes.end());
+    std::vector<double> coef;
+    for (const auto *o : {&a->op, &b->op})
+        for (const tcp::Complex &c : o->coef) {
+            coef.push_back(c.re);
+            coef.push_back(c.im);
+        }
+    st = wrap(codes, coef, a->op.nqubits, out);
+    return st ? st : tcp::ok();
+}
+
+int tcp_operator_scale(const tcp_operator *a, double re, double im, tcp_operator **out) {
+    if (!a || !out) return tcp::fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (!std::isfinite(re) || !std::isfinite(im))
+        return tcp::fail(TCP_ERR_NON_FINITE_COEFFICIENT, "coefficient at index 0 is not finite");
+    auto *h = new tcp_operator();
+    h->op.nqubits = a->op.nqubits;
+    if (re != 0.0 || im != 0.0) {
+        const uint32_t n = a->op.nqubits;
+        for (uint64_t t = 0; t < a->op.term_count(); ++t) {
+            const tcp::Complex c = cmul(a->op.coef[t], tcp::Complex{re, im});
+            if (c.re == 0.0 && c.im == 0.0) continue;
+            h->op.codes.insert(h->op.codes.end(), a->op.codes.begin() + t * n, a->op.codes.begin() + (t + 1) * n);
+            h->op.coef.push_back(c);
+        }
+    }
+    *out = h;
+    return tcp::ok();
+}
+
+int tcp_operator_multiply(const tcp_operator *a, const tcp_operator *b, uint64_t max_bytes, tcp_operator **out) {
+    if (!out) return tcp::fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    int st = compatible(a, b);
+    if (st) return st;
+    const uint32_t n = a->op.nqubits;
+    const uint64_t Ta = a->op.term_count(), Tb = b->op.term_count();
+    const unsigned __int128 pairs = static_cast<unsigned __int128>(Ta) * Tb;
+    const unsigned __int128 bytes = pairs * (static_cast<unsigned __int128>(n) + 16);
+    if (bytes > static_cast<unsigned __int128>(UINT64_MAX))
+        return tcp::fail(TCP_ERR_OVERFLOW, "integer overflow while estimating Pauli operator product");
+    if (tcp::exceeds_budget(static_cast<uint64_t>(bytes), max_bytes))
+        return tcp::memory_limit(static_cast<uint64_t>(bytes), max_bytes);
+    // product code and phase exponent (powers of i) of two single-qubit Paulis:
+    // XY = iZ, YZ = iX, ZX = iY, the reversed orders -i
+    static const uint8_t code[4][4] = {{0, 1, 2, 3}, {1, 0, 3, 2}, {2, 3, 0, 1}, {3, 2, 1, 0}};
+    static const uint8_t power[4][4] = {{0, 0, 0, 0}, {0, 0, 1, 3}, {0, 3, 0, 1}, {0, 1, 3, 0}};
+    static const tcp::Complex phase[4] = {{1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}, {0.0, -1.0}};
+    std::vector<uint8_t> codes(static_cast<size_t>(pairs) * n);
+    std::vector<double> coef(2 * static_cast<size_t>(pairs));
+    size_t k = 0;
+    for (uint64_t i = 0; i < Ta; ++i)
+        for (uint64_t j = 0; j < Tb; ++j, ++k) {
+            const uint8_t *x = a->op.codes.data() + i * n, *y = b->op.codes.data() + j * n;
+            uint32_t p = 0;
+            for (uint32_t q = 0; q < n; ++q) {
+                codes[k * n + q] = code[x[q]][y[q]];
+                p += power[x[q]][y[q]];
+            }
+            const tcp::Complex c = cmul(cmul(a->op.coef[i], b->op.coef[j]), phase[p & 3]);
+            coef[2 * k] = c.re;
+            coef[2 * k + 1] = c.im;
+        }
+    st = wrap(codes, coef, n, out);
+    return st ? st : tcp::ok();
+}
+
+int tcp_operator_adjoint(const tcp_operator *a, tcp_operator **out) {
+    if (!a || !out) return tcp::fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    auto *h = new tcp_operator();
+    h->op = a->op;
+    for (tcp::Complex &c : h->op.coef) c.im = -c.im;
+    *out = h;
+    return tcp::ok();
+}
+
 void tcp_operator_destroy(tcp_operator *op) { delete op; }
 
 }  // extern "C"

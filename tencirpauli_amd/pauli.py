@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass
-from typing import Any, Iterable, Optional, Sequence, Tuple, Union
+from typing import Any, Dict, Iterable, Optional, Sequence, Tuple, Union
 
 import numpy as np
 
@@ -194,6 +194,117 @@ class PauliOperator:
 
         return plan_from_operator(self, max_bytes)
 
+    def to_dict(self) -> Dict[str, complex]:
+        """Canonical Pauli strings and coefficients (py/pauli.py:585-595)."""
+        codes, coef = self.code_arrays()
+        letters = np.frombuffer(b"IXYZ", dtype=np.uint8)
+        return {letters[row].tobytes().decode(): complex(c) for row, c in zip(codes, coef)}
+
+    @classmethod
+    def from_strings(cls, terms: Iterable[Tuple[str, complex]], *,
+                     max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "PauliOperator":
+        """From ("IXYZ" string, coefficient) pairs, inferring the qubit count (py/pauli.py:514-540)."""
+        items = [(str(w), c) for w, c in terms]
+        widths = {len(w) for w, _ in items}
+        if len(widths) > 1:
+            raise ValueError("all Pauli strings must have the same length")
+        return cls(widths.pop() if widths else 0, items, max_bytes=max_bytes)
+
+    # -- algebra (core/operator.rs:217-393, py/pauli.py:597-704) ------------
+    def _wrap(self, ptr: ctypes.c_void_p) -> "PauliOperator":
+        out = object.__new__(PauliOperator)
+        out._handle = _OperatorHandle(ptr)
+        out.nqubits = self.nqubits
+        out._terms = None
+        out._arrays = None
+        return out
+
+    def _check_compatible(self, other: object) -> "PauliOperator":
+        if not isinstance(other, PauliOperator):
+            raise TypeError("expected a PauliOperator")
+        return other
+
+    def add(self, other: "PauliOperator", *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "PauliOperator":
+        """Sum; equal words add their coefficients and exact zeros are dropped."""
+        validate_max_bytes(max_bytes)
+        out = ctypes.c_void_p()
+        _lib.check(_lib.lib().tcp_operator_add(self._handle.ptr, self._check_compatible(other)._handle.ptr,
+                                               ctypes.byref(out)))
+        return self._wrap(out)
+
+    def scale(self, scalar: complex, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "PauliOperator":
+        validate_max_bytes(max_bytes)
+        value = complex(scalar)
+        if not (np.isfinite(value.real) and np.isfinite(value.imag)):
+            raise ValueError("scale must be a finite complex128 value")
+        out = ctypes.c_void_p()
+        _lib.check(_lib.lib().tcp_operator_scale(self._handle.ptr, value.real, value.imag, ctypes.byref(out)))
+        return self._wrap(out)
+
+    def multiply(self, other: "PauliOperator", *,
+                 max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "PauliOperator":
+        """Operator product with exact Pauli phases absorbed into the coefficients."""
+        validate_max_bytes(max_bytes)
+        out = ctypes.c_void_p()
+        _lib.check(_lib.lib().tcp_operator_multiply(self._handle.ptr, self._check_compatible(other)._handle.ptr,
+                                                    effective_max_bytes(max_bytes), ctypes.byref(out)))
+        return self._wrap(out)
+
+    def commutator(self, other: "PauliOperator", *,
+                   max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "PauliOperator":
+        """self * other - other * self."""
+        return self.multiply(other, max_bytes=max_bytes).add(
+            other.multiply(self, max_bytes=max_bytes).scale(-1.0), max_bytes=max_bytes)
+
+    def anticommutator(self, other: "PauliOperator", *,
+                       max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "PauliOperator":
+        """self * other + other * self."""
+        return self.multiply(other, max_bytes=max_bytes).add(other.multiply(self, max_bytes=max_bytes),
+                                                             max_bytes=max_bytes)
+
+    def adjoint(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "PauliOperator":
+        validate_max_bytes(max_bytes)
+        out = ctypes.c_void_p()
+        _lib.check(_lib.lib().tcp_operator_adjoint(self._handle.ptr, ctypes.byref(out)))
+        return self._wrap(out)
+
+    def is_hermitian(self, tolerance: float = 0.0) -> bool:
+        """Every coefficient real within ``tolerance`` (|c - conj(c)| <= tolerance)."""
+        if not np.isfinite(tolerance) or tolerance < 0.0:
+            raise ValueError("Hermiticity tolerance must be finite and non-negative")
+        _, coef = self.code_arrays()
+        return bool(np.all(np.abs(2.0 * coef.imag) <= tolerance))
+
+    def __add__(self, other: object) -> "PauliOperator":
+        return self.add(other) if isinstance(other, PauliOperator) else NotImplemented
+
+    def __sub__(self, other: object) -> "PauliOperator":
+        return self.add(other.scale(-1.0)) if isinstance(other, PauliOperator) else NotImplemented
+
+    def __neg__(self) -> "PauliOperator":
+        return self.scale(-1.0)
+
+    def __mul__(self, other: object) -> "PauliOperator":
+        if isinstance(other, PauliOperator):
+            return self.multiply(other)
+        if isinstance(other, (int, float, complex, np.number)) and not isinstance(other, bool):
+            return self.scale(complex(other))
+        return NotImplemented
+
+    def __rmul__(self, other: object) -> "PauliOperator":
+        if isinstance(other, (int, float, complex, np.number)) and not isinstance(other, bool):
+            return self.scale(complex(other))
+        return NotImplemented
+
+    def __eq__(self, other: object) -> bool:
+        if not isinstance(other, PauliOperator):
+            return NotImplemented
+        a, b = self.code_arrays(), other.code_arrays()
+        return (self.nqubits == other.nqubits and a[0].shape == b[0].shape and bool(np.array_equal(a[0], b[0]))
+                and bool(np.array_equal(a[1], b[1])))
+
+    __hash__ = None  # type: ignore[assignment]
+
     # -- charges ----------------------------------------------------------
     def analyze_charge(self, charge: Any, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> "ChargeAnalysis":
         """Exact conservation analysis of a diagonal qubit charge (core/operator.rs:396-445)."""
@@ -211,6 +322,10 @@ class PauliOperator:
             self._handle.ptr, levels.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
             ctypes.byref(conserved), ctypes.byref(nonzero)))
         return ChargeAnalysis(bool(conserved.value), int(nonzero.value))
+
+    def conserves(self, charge: Any, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> bool:
+        """Whether this operator exactly conserves ``charge`` (py/pauli.py:696-703)."""
+        return self.conserves_charge(charge, max_bytes=max_bytes)
 
     def conserves_charge(self, charge: Any, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> bool:
         return self.analyze_charge(charge, max_bytes=max_bytes).is_conserved
