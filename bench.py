@@ -452,6 +452,13 @@ def extra_eigsh() -> dict:
     rec["gpu_lazy_k4"] = {"seconds": round(four.seconds, 4), "mvps": four.mvps,
                           "energies": [float(x) for x in four.eigenvalues],
                           "max_residual": float(four.residual_norms.max())}
+    # the Davidson workload (5x5 periodic XXZ, S_z = 1/2, D = 5,200,300) through the same call
+    codes, coef = lattice.heisenberg_periodic(5, 5, seed=SEED)
+    big = tcp.PauliOperator.from_code_arrays(codes, coef).restrict_charge(tcp.U1Sector(25, 12))
+    for name, p in (("gpu_5x5_xxz_lazy", big.mvp_plan()), ("gpu_5x5_xxz_eager_csr", big.mvp_plan(storage="eager"))):
+        res = lanczos.lanczos(p, 1, tol=1e-10, max_restarts=300, ncv=20, residuals=True)
+        rec[name] = {"seconds": round(res.seconds, 4), "mvps": res.mvps, "energy": float(res.eigenvalues[0]),
+                     "residual": float(res.residual_norms[0])}
     return rec
 
 
