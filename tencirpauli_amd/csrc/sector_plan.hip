@@ -39,6 +39,7 @@
 namespace {
 
 constexpr uint32_t kMaxBlocks = 4;
+constexpr uint32_t kMaxChunksPerBlock = 8;  // byte chunks of a block of up to 64 qubits
 constexpr uint32_t kSectorThreads = 256;
 // Consecutive destination ranks per thread (one unrank, then successor
 // steps): 2 against 1 and 4 is 3-6% faster on the 5x5 XXZ and 4x4 Hubbard
@@ -71,6 +72,7 @@ struct SectorArgs {
     const uint64_t *tc0;     // [T] fermion terms: modes that must be empty in the source
     const uint32_t *gblk;    // [G] bit j set when group g's x-mask touches block j
     const uint32_t *lut;     // rank tables
+    const uint32_t *cbase;   // [kMaxBlocks][8] table base of (block, chunk), offset by its lowest row
     const uint64_t *choose;  // [64 * 64] C(p, j), p, j < 64
 };
 
@@ -101,12 +103,11 @@ __host__ __device__ __forceinline__ bool in_sector(const SectorArgs &p, uint64_t
 __host__ __device__ __forceinline__ uint32_t block_rank(const SectorArgs &p, const uint32_t *lut, uint32_t j,
                                                          uint64_t s) {
     const uint64_t w = (s >> p.shift[j]) & p.bmask[j];
-    const uint32_t width = (p.pop[j] + 1) * 256;
-    const uint32_t *table = lut + p.lut_off[j];
+    const uint32_t *base = p.cbase + j * kMaxChunksPerBlock;
     uint32_t local = 0, below = 0;
     for (uint32_t c = 0; c < p.nchunk[j]; ++c) {
         const uint32_t byte = static_cast<uint32_t>(w >> (8 * c)) & 0xffu;
-        local += table[c * width + below * 256 + byte];
+        local += lut[base[c] + below * 256 + byte];
         below += __builtin_popcount(byte);
     }
     return local;
@@ -650,7 +651,7 @@ struct tcp_sector_plan {
     uint32_t leak_weight = 0;  // its destination weight (U(1) message)
     SectorArgs host_args{};
     bool cond = false;  // terms are fermion ladder words with occupancy conditions
-    tcp::DeviceBuffer gx, goff, tz, tw64, tw32, lut, choose, tc1, tc0, gblk;
+    tcp::DeviceBuffer gx, goff, tz, tw64, tw32, lut, choose, tc1, tc0, gblk, cbase;
     std::unique_ptr<tcp::WideSector> wide;  // U(1) sectors of 64 or more qubits
     uint64_t retained_bytes = 0;
     std::mutex staging_mutex;
@@ -667,6 +668,7 @@ struct tcp_sector_plan {
         a.gblk = static_cast<const uint32_t *>(gblk.ptr);
         a.lut = static_cast<const uint32_t *>(lut.ptr);
         a.choose = static_cast<const uint64_t *>(choose.ptr);
+        a.cbase = static_cast<const uint32_t *>(cbase.ptr);
         return a;
     }
 };
@@ -698,7 +700,7 @@ struct BlockSpec {
 
 // Host-side sector geometry shared by rank/unrank entry points and plans.
 int sector_geometry(const BlockSpec &spec, SectorArgs &a, uint64_t &dim, std::vector<uint32_t> &lut,
-                    std::vector<uint64_t> &choose_table) {
+                    std::vector<uint64_t> &choose_table, std::vector<uint32_t> &cbase) {
     if (spec.nblocks == 0) return fail(TCP_ERR_INVALID_SECTOR, "invalid sector: no blocks");
     if (spec.nblocks > kMaxBlocks)
         return fail(TCP_ERR_UNSUPPORTED, "at most 4 contiguous blocks are supported on the GPU");
@@ -741,12 +743,21 @@ int sector_geometry(const BlockSpec &spec, SectorArgs &a, uint64_t &dim, std::ve
             if (!choose_checked(q, k, c)) c = UINT64_MAX;
             choose_table[q * 64 + k] = c;
         }
+    // Rows only for the ones-below counts an in-sector pattern can have in
+    // front of chunk c: max(0, k - (size - 8c)) .. min(k, 8c).  cbase[j][c]
+    // is the row base minus lo * 256 (unsigned wrap-around), so the entry of
+    // (below, byte) sits at cbase + below * 256 + byte.
     lut.clear();
+    cbase.assign(kMaxBlocks * kMaxChunksPerBlock, 0);
     for (uint32_t j = 0; j < spec.nblocks; ++j) {
         a.lut_off[j] = static_cast<uint32_t>(lut.size());
         const uint32_t k = spec.pops[j];
-        for (uint32_t c = 0; c < a.nchunk[j]; ++c)
-            for (uint32_t below = 0; below <= k; ++below)
+        for (uint32_t c = 0; c < a.nchunk[j]; ++c) {
+            const uint32_t rest = spec.sizes[j] - 8 * c;  // positions from chunk c upwards
+            const uint32_t lo = k > rest ? k - rest : 0;
+            const uint32_t hi = std::min(k, 8 * c);
+            cbase[j * kMaxChunksPerBlock + c] = static_cast<uint32_t>(lut.size()) - lo * 256u;
+            for (uint32_t below = lo; below <= hi; ++below)
                 for (uint32_t byte = 0; byte < 256; ++byte) {
                     uint64_t value = 0;
                     uint32_t seen = 0;
@@ -759,6 +770,7 @@ int sector_geometry(const BlockSpec &spec, SectorArgs &a, uint64_t &dim, std::ve
                     }
                     lut.push_back(static_cast<uint32_t>(value));
                 }
+        }
     }
     a.lut_words = static_cast<uint32_t>(lut.size());
     return TCP_OK;
@@ -815,7 +827,8 @@ int build_sector_plan_impl(const tcp::Operator *op, const tcp_fermion *fermion, 
     uint64_t dim = 0;
     std::vector<uint32_t> lut;
     std::vector<uint64_t> choose_table;
-    int st = sector_geometry(spec, a, dim, lut, choose_table);
+    std::vector<uint32_t> cbase;
+    int st = sector_geometry(spec, a, dim, lut, choose_table, cbase);
     if (st) return st;
     if (n > 63) return fail(TCP_ERR_UNSUPPORTED, "GPU sector kernels support at most 63 qubits");
     if (dim >= (1ull << 32))
@@ -923,6 +936,7 @@ int build_sector_plan_impl(const tcp::Operator *op, const tcp_fermion *fermion, 
     if ((st = plan->tw32.upload(tw32.data(), tw32.size() * 16))) return st;
     if ((st = plan->lut.upload(lut.data(), lut.size() * 4))) return st;
     if ((st = plan->choose.upload(choose_table.data(), choose_table.size() * 8))) return st;
+    if ((st = plan->cbase.upload(cbase.data(), cbase.size() * 4))) return st;
     std::vector<uint32_t> gblk(gx.size(), 0);
     for (size_t g = 0; g < gx.size(); ++g)
         for (uint32_t j = 0; j < spec.nblocks; ++j)
@@ -1115,7 +1129,8 @@ int host_rank(const BlockSpec &spec, uint64_t bits, uint64_t &rank) {
     uint64_t dim;
     std::vector<uint32_t> lut;
     std::vector<uint64_t> ch;
-    int st = sector_geometry(spec, a, dim, lut, ch);
+    std::vector<uint32_t> cb;
+    int st = sector_geometry(spec, a, dim, lut, ch, cb);
     if (st) return st;
     uint32_t n = 0;
     for (uint32_t j = 0; j < spec.nblocks; ++j) n += spec.sizes[j];
@@ -1147,7 +1162,8 @@ int host_unrank(const BlockSpec &spec, uint64_t index, uint64_t &bits) {
     uint64_t dim;
     std::vector<uint32_t> lut;
     std::vector<uint64_t> ch;
-    int st = sector_geometry(spec, a, dim, lut, ch);
+    std::vector<uint32_t> cb;
+    int st = sector_geometry(spec, a, dim, lut, ch, cb);
     if (st) return st;
     if (index >= dim) return fail(TCP_ERR_INVALID_INDEX, "invalid index: restricted basis index is out of range");
     bits = 0;
@@ -1225,7 +1241,8 @@ int tcp_blocks_dimension(uint32_t nblocks, const uint32_t *sizes, const uint32_t
     SectorArgs a;
     std::vector<uint32_t> lut;
     std::vector<uint64_t> ch;
-    st = sector_geometry(s, a, *dim, lut, ch);
+    std::vector<uint32_t> cb;
+    st = sector_geometry(s, a, *dim, lut, ch, cb);
     return st ? st : tcp::ok();
 }
 
