@@ -478,6 +478,7 @@ typedef struct tcp_lanczos_options {
     const void *v0;        /* optional device complex128 start vector (NULL: deterministic hash) */
     uint64_t seed;
     void *stream;          /* hipStream_t (NULL = default stream) */
+    uint32_t keep;         /* Ritz vectors kept at a restart, nev <= keep < ncv (0: nev + (ncv - nev) / 2) */
 } tcp_lanczos_options;
 
 typedef struct tcp_lanczos_result {

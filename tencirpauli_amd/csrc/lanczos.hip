@@ -222,7 +222,7 @@ struct Lanczos {
         const uint32_t ncv = opt.ncv;
         const double tol = opt.tol > 0 ? opt.tol : 2.220446049250313e-16;
         const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
-        const uint32_t keep = std::min(ncv - 1, nev + (ncv - nev) / 2);
+        const uint32_t keep = opt.keep ? opt.keep : std::min(ncv - 1, nev + (ncv - nev) / 2);
         int st;
         if ((st = pool.allocate(static_cast<size_t>(16) * D * (ncv + 3))) ||
             (st = partial.allocate(sizeof(dvec2) * static_cast<size_t>(nb) * (kMaxBasis + 1))) ||
@@ -408,6 +408,8 @@ int lanczos(const Plan *plan, const tcp_lanczos_options *opt, double *evals, voi
         return fail(TCP_ERR_INVALID_ARGUMENT,
                     "ncv must satisfy nev < ncv <= min(dimension, " + std::to_string(kMaxBasis) + ")");
     if (!(opt->tol >= 0.0)) return fail(TCP_ERR_INVALID_ARGUMENT, "tol must be non-negative");
+    if (opt->keep && (opt->keep < opt->nev || opt->keep >= opt->ncv))
+        return fail(TCP_ERR_INVALID_ARGUMENT, "keep must satisfy nev <= keep < ncv");
     tcp::DeviceGuard dg;
     int st = dg.enter(tcp::plan_device(plan));
     if (st) return st;

@@ -39,6 +39,7 @@ class LanczosOptions(ctypes.Structure):
         ("v0", ctypes.c_void_p),
         ("seed", ctypes.c_uint64),
         ("stream", ctypes.c_void_p),
+        ("keep", ctypes.c_uint32),
     ]
 
 
@@ -76,7 +77,8 @@ def _entry(plan: Any):
 
 def lanczos(plan: Any, k: int = 1, *, which: str = "SA", v0: Any = None, ncv: Optional[int] = None,
             max_restarts: Optional[int] = None, tol: float = 0.0, seed: int = 20260801,
-            return_eigenvectors: bool = True, residuals: bool = False) -> LanczosResult:
+            return_eigenvectors: bool = True, residuals: bool = False,
+            keep: Optional[int] = None) -> LanczosResult:
     """Lowest / highest / largest-magnitude eigenpairs with every step on the GPU."""
     import torch
 
@@ -95,6 +97,10 @@ def lanczos(plan: Any, k: int = 1, *, which: str = "SA", v0: Any = None, ncv: Op
     _lib.lib().tcp_lanczos_default_options(ctypes.byref(opt))
     opt.nev, opt.ncv, opt.which, opt.tol, opt.seed = k, ncv, _WHICH[which], float(tol), seed
     opt.max_restarts = min(int(max_restarts if max_restarts is not None else 10 * dim), 2 ** 32 - 1)
+    if keep is not None:
+        if not k <= keep < ncv:
+            raise ValueError(f"keep must satisfy k <= keep < ncv, got keep={keep}")
+        opt.keep = keep
     start = None
     if v0 is not None:
         if is_device_tensor(v0):

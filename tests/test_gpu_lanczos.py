@@ -141,6 +141,11 @@ def test_argument_errors_and_no_convergence():
         tcp.eigsh(plan, k=2, sigma=0.5)
     with pytest.raises(ValueError, match="v0"):
         tcp.eigsh(plan, k=1, v0=np.ones(10))
+    with pytest.raises(ValueError, match="keep"):
+        lanczos.lanczos(plan, 2, keep=1)
+    kept = lanczos.lanczos(plan, 2, ncv=12, keep=6, tol=1e-10)
+    np.testing.assert_allclose(kept.eigenvalues, np.linalg.eigvalsh(dense.dense_operator(6, *op.code_arrays()))[:2],
+                               atol=1e-9)
     big = _random_operator(12, 80, seed=9).native_mvp_plan()
     with pytest.raises(ArpackNoConvergence) as info:
         tcp.eigsh(big, k=3, which="SA", ncv=8, maxiter=0, tol=1e-14)
