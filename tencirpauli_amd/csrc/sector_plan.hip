@@ -31,6 +31,7 @@
 #include <cmath>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 
 #include "full_matrix.hpp"
 #include "tcp_internal.hpp"
@@ -181,7 +182,9 @@ struct Amp<fvec2> {
 // 4x4 Hubbard one (whose groups mostly leave the sector); 8 adds 19% there.
 constexpr uint32_t kGatherChunk = 4;
 
-template <typename VT, bool LDS, bool COND>
+// NB = number of blocks, fixed at compile time so the per-group block loops
+// keep their block fields in registers without spilling kernel arguments.
+template <typename VT, bool LDS, bool COND, uint32_t NB>
 __global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT *__restrict__ v,
                                                                        VT *__restrict__ y,
                                                                        SectorArgs p) {
@@ -209,7 +212,7 @@ __global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT 
             // b: only touched blocks are tested and re-ranked
             uint32_t lr[kMaxBlocks];
 #pragma unroll
-            for (uint32_t q = 0; q < kMaxBlocks; ++q) lr[q] = q < p.nblocks ? block_rank(p, lut, q, b) : 0u;
+            for (uint32_t q = 0; q < kMaxBlocks; ++q) lr[q] = q < NB ? block_rank(p, lut, q, b) : 0u;
             double ar = 0.0, ai = 0.0;
             // kGatherChunk groups at a time: their source ranks and weights
             // first, then all their gathers in flight together (a group that
@@ -229,8 +232,8 @@ __global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT 
                     double dr = 0.0, di = 0.0;
                     bool ok = live;
 #pragma unroll
-                    for (uint32_t q = 0; q < kMaxBlocks; ++q)
-                        if (q < p.nblocks && ((touched >> q) & 1u))
+                    for (uint32_t q = 0; q < NB; ++q)
+                        if (NB == 1 || ((touched >> q) & 1u))
                             ok &= static_cast<uint32_t>(__popcll((s >> p.shift[q]) & p.bmask[q])) == p.pop[q];
                     if (ok) {
                         const uint32_t k1 = goff[g + 1];
@@ -247,9 +250,9 @@ __global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT 
                     if (ok && x != 0) {
                         uint32_t rank = 0;
 #pragma unroll
-                        for (uint32_t q = 0; q < kMaxBlocks; ++q)
-                            if (q < p.nblocks)
-                                rank += (((touched >> q) & 1u) ? block_rank(p, lut, q, s) : lr[q]) * p.stride[q];
+                        for (uint32_t q = 0; q < NB; ++q)
+                            rank += ((NB == 1 || ((touched >> q) & 1u)) ? block_rank(p, lut, q, s) : lr[q]) *
+                                    p.stride[q];
                         src = rank;
                         ok = src < p.dim;  // unreachable for in-sector s; keeps the gather in bounds
                         if (!ok) src = r;
@@ -1066,14 +1069,23 @@ hipError_t launch_gather(const tcp_sector_plan *plan, const void *in, void *out,
     SectorArgs a = plan->args(dtype);
     const uint64_t per_block = static_cast<uint64_t>(kSectorThreads) * kRanksPerThread;
     const uint64_t blocks = std::min<uint64_t>((plan->dim + per_block - 1) / per_block, 256 * 16);
-    if (plan->lut_in_lds)
-        hipLaunchKernelGGL((sector_gather_kernel<VT, true, COND>), dim3(static_cast<uint32_t>(blocks)),
-                           dim3(kSectorThreads), a.lut_words * 4, stream, static_cast<const VT *>(in),
-                           static_cast<VT *>(out), a);
-    else
-        hipLaunchKernelGGL((sector_gather_kernel<VT, false, COND>), dim3(static_cast<uint32_t>(blocks)),
-                           dim3(kSectorThreads), 0, stream, static_cast<const VT *>(in),
-                           static_cast<VT *>(out), a);
+    auto launch = [&](auto nb) {
+        constexpr uint32_t NB = decltype(nb)::value;
+        if (plan->lut_in_lds)
+            hipLaunchKernelGGL((sector_gather_kernel<VT, true, COND, NB>), dim3(static_cast<uint32_t>(blocks)),
+                               dim3(kSectorThreads), a.lut_words * 4, stream, static_cast<const VT *>(in),
+                               static_cast<VT *>(out), a);
+        else
+            hipLaunchKernelGGL((sector_gather_kernel<VT, false, COND, NB>), dim3(static_cast<uint32_t>(blocks)),
+                               dim3(kSectorThreads), 0, stream, static_cast<const VT *>(in),
+                               static_cast<VT *>(out), a);
+    };
+    switch (a.nblocks) {
+        case 1: launch(std::integral_constant<uint32_t, 1>()); break;
+        case 2: launch(std::integral_constant<uint32_t, 2>()); break;
+        case 3: launch(std::integral_constant<uint32_t, 3>()); break;
+        default: launch(std::integral_constant<uint32_t, 4>()); break;
+    }
     return hipGetLastError();
 }
 
