@@ -120,10 +120,13 @@ __host__ __device__ __forceinline__ uint32_t sector_rank(const SectorArgs &p, co
     return rank;
 }
 
+// NB > 0 fixes the block count at compile time (0: p.nblocks).
+template <uint32_t NB = 0>
 __host__ __device__ uint64_t sector_unrank(const SectorArgs &p, uint64_t rank) {
     const uint64_t *choose = p.choose;
     uint64_t bits = 0;
-    for (uint32_t j = 0; j < p.nblocks; ++j) {
+    const uint32_t nb = NB ? NB : p.nblocks;
+    for (uint32_t j = 0; j < nb; ++j) {
         uint64_t local = rank / p.stride[j];
         rank -= local * p.stride[j];
         uint32_t needed = p.pop[j];
@@ -143,8 +146,9 @@ __host__ __device__ uint64_t sector_unrank(const SectorArgs &p, uint64_t rank) {
 
 // Next sector state in ascending rank order: advance the lowest block to its
 // next combination (Gosper's step); on overflow reset it and carry upwards.
+template <uint32_t NB = 0>
 __host__ __device__ __forceinline__ uint64_t sector_next(const SectorArgs &p, uint64_t bits) {
-    for (int j = static_cast<int>(p.nblocks) - 1; j >= 0; --j) {
+    for (int j = static_cast<int>(NB ? NB : p.nblocks) - 1; j >= 0; --j) {
         const uint64_t w = (bits >> p.shift[j]) & p.bmask[j];
         uint64_t nw = ~0ull;
         if (p.pop[j] > 0) {
@@ -204,7 +208,7 @@ __global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT 
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * kRanksPerThread;
     for (uint64_t r0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kRanksPerThread;
          r0 < p.dim; r0 += stride) {
-        uint64_t b = sector_unrank(p, r0);
+        uint64_t b = sector_unrank<NB>(p, r0);
         for (uint32_t j = 0; j < kRanksPerThread; ++j) {
             const uint64_t r = r0 + j;
             if (r >= p.dim) break;
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT 
                 }
             }
             y[r] = Amp<VT>::make(ar, ai);
-            b = sector_next(p, b);
+            b = sector_next<NB>(p, b);
         }
     }
 }
