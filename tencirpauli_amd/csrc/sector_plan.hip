@@ -42,6 +42,9 @@ namespace {
 constexpr uint32_t kMaxBlocks = 4;
 constexpr uint32_t kMaxChunksPerBlock = 8;  // byte chunks of a block of up to 64 qubits
 constexpr uint32_t kSectorThreads = 256;
+// The gather kernel's workgroups hold 8 waves, so the LDS copy of the rank
+// tables (one per workgroup) no longer caps the waves per CU below 8 per SIMD.
+constexpr uint32_t kGatherThreads = 512;
 // Consecutive destination ranks per thread (one unrank, then successor
 // steps): 2 against 1 and 4 is 3-6% faster on the 5x5 XXZ and 4x4 Hubbard
 // sectors.
@@ -189,7 +192,7 @@ constexpr uint32_t kGatherChunk = 4;
 // NB = number of blocks, fixed at compile time so the per-group block loops
 // keep their block fields in registers without spilling kernel arguments.
 template <typename VT, bool LDS, bool COND, uint32_t NB>
-__global__ __launch_bounds__(kSectorThreads) void sector_gather_kernel(const VT *__restrict__ v,
+__global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT *__restrict__ v,
                                                                        VT *__restrict__ y,
                                                                        SectorArgs p) {
     extern __shared__ uint32_t lds_lut[];
@@ -1071,17 +1074,17 @@ template <typename VT, bool COND>
 hipError_t launch_gather(const tcp_sector_plan *plan, const void *in, void *out, int dtype,
                          hipStream_t stream) {
     SectorArgs a = plan->args(dtype);
-    const uint64_t per_block = static_cast<uint64_t>(kSectorThreads) * kRanksPerThread;
+    const uint64_t per_block = static_cast<uint64_t>(kGatherThreads) * kRanksPerThread;
     const uint64_t blocks = std::min<uint64_t>((plan->dim + per_block - 1) / per_block, 256 * 16);
     auto launch = [&](auto nb) {
         constexpr uint32_t NB = decltype(nb)::value;
         if (plan->lut_in_lds)
             hipLaunchKernelGGL((sector_gather_kernel<VT, true, COND, NB>), dim3(static_cast<uint32_t>(blocks)),
-                               dim3(kSectorThreads), a.lut_words * 4, stream, static_cast<const VT *>(in),
+                               dim3(kGatherThreads), a.lut_words * 4, stream, static_cast<const VT *>(in),
                                static_cast<VT *>(out), a);
         else
             hipLaunchKernelGGL((sector_gather_kernel<VT, false, COND, NB>), dim3(static_cast<uint32_t>(blocks)),
-                               dim3(kSectorThreads), 0, stream, static_cast<const VT *>(in),
+                               dim3(kGatherThreads), 0, stream, static_cast<const VT *>(in),
                                static_cast<VT *>(out), a);
     };
     switch (a.nblocks) {
