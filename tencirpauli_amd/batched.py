@@ -43,7 +43,7 @@ class BatchedMVP:
         self.work = float(L.tcp_batch_plan_work(out))  # term * amp * rhs per apply
 
     def __del__(self) -> None:
-        if getattr(self, "_ptr", None) and _lib._lib is not None:
+        if getattr(self, "_ptr", None) and getattr(_lib, "_lib", None) is not None:
             _lib.lib().tcp_batch_plan_destroy(self._ptr)
             self._ptr = None
 

@@ -97,7 +97,7 @@ class CsrMvpPlan(NativePlanFacade):
         self.estimated_construction_peak_bytes = self.estimated_bytes
 
     def __del__(self) -> None:
-        if getattr(self, "_ptr", None) and _lib._lib is not None:
+        if getattr(self, "_ptr", None) and getattr(_lib, "_lib", None) is not None:
             _lib.lib().tcp_csr_plan_destroy(self._ptr)
             self._ptr = None
 

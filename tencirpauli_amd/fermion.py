@@ -131,7 +131,7 @@ class _FermionHandle:
         self.ptr = ptr
 
     def __del__(self) -> None:
-        if self.ptr and _lib._lib is not None:
+        if self.ptr and getattr(_lib, "_lib", None) is not None:
             _lib.lib().tcp_fermion_destroy(self.ptr)
             self.ptr = None
 

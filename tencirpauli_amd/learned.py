@@ -41,7 +41,7 @@ class MLP:
         self._ptr = out
 
     def __del__(self) -> None:
-        if getattr(self, "_ptr", None) and _lib._lib is not None:
+        if getattr(self, "_ptr", None) and getattr(_lib, "_lib", None) is not None:
             _lib.lib().tcp_mlp_destroy(self._ptr)
             self._ptr = None
 
@@ -132,7 +132,7 @@ class LearnedPreconditioner:
         self.scale = float(scale.value)
 
     def __del__(self) -> None:
-        if getattr(self, "_ptr", None) and _lib._lib is not None:
+        if getattr(self, "_ptr", None) and getattr(_lib, "_lib", None) is not None:
             _lib.lib().tcp_learned_precond_destroy(self._ptr)
             self._ptr = None
 

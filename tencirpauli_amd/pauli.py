@@ -54,7 +54,7 @@ class _OperatorHandle:
         self.ptr = ptr
 
     def __del__(self) -> None:
-        if getattr(self, "ptr", None) and _lib._lib is not None:
+        if getattr(self, "ptr", None) and getattr(_lib, "_lib", None) is not None:
             _lib.lib().tcp_operator_destroy(self.ptr)
             self.ptr = None
 
@@ -550,7 +550,7 @@ class NativeMVPPlan(NativePlanFacade):
         self.device_bytes = int(L.tcp_mvp_plan_estimated_bytes(ptr))
 
     def __del__(self) -> None:
-        if getattr(self, "_ptr", None) and _lib._lib is not None:
+        if getattr(self, "_ptr", None) and getattr(_lib, "_lib", None) is not None:
             _lib.lib().tcp_mvp_plan_destroy(self._ptr)
             self._ptr = None
 

@@ -182,7 +182,7 @@ class SectorMvpPlan(NativePlanFacade):
         self.transition_count: Optional[int] = None
 
     def __del__(self) -> None:
-        if getattr(self, "_ptr", None) and _lib._lib is not None:
+        if getattr(self, "_ptr", None) and getattr(_lib, "_lib", None) is not None:
             _lib.lib().tcp_sector_plan_destroy(self._ptr)
             self._ptr = None
 
