@@ -174,6 +174,7 @@ SIGNATURES = {
     "tcp_mlp_destroy": (None, [_vp]),
     "tcp_learned_precond_create": (_c.c_int, [_vp, _vp, _c.POINTER(_vp)]),
     "tcp_learned_precond_log_multiplier": (_vp, [_vp, _c.c_double, _vp]),
+    "tcp_learned_precond_set_refresh": (_c.c_int, [_vp, _c.c_double]),
     "tcp_learned_precond_ritz_loss": (_c.c_int, [_vp, _c.c_double, _vp, _vp, _vp, _vp, _dp, _dp, _vp]),
     "tcp_learned_precond_prepare": (_c.c_int, [_vp, _c.c_double, _vp]),
     "tcp_learned_precond_buffers": (_c.c_int, [_vp, _c.POINTER(_vp), _c.POINTER(_vp), _dp]),

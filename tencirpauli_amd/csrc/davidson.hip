@@ -63,6 +63,42 @@ struct Sig2 {
     double s[kMaxBasis];
 };
 
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+// Elements per thread of the streaming passes K1, K2 and K4: complex64
+// columns go in pairs (one 16-byte access, the size of a complex128 element),
+// so a complex64 pass keeps as many bytes in flight per wave as a complex128
+// one instead of half as many.  Above 24 columns the pairs would spill.
+template <typename ST, int MAXM>
+struct Run {
+    static constexpr int E = std::is_same<ST, fvec2>::value && MAXM <= 24 ? 2 : 1;
+};
+
+// x[0..E) = p[i..i+E) for i a multiple of E.  The pair that starts at the
+// last index of an odd-length column reads one element past its end: every
+// column these passes touch is a solver slot of 16 D bytes whose complex64
+// data fills the first half, so the read stays inside the slot, and the extra
+// element is never used.
+template <typename ST>
+__device__ __forceinline__ void load_run(const ST *p, uint64_t i, ST (&x)[1]) {
+    x[0] = p[i];
+}
+__device__ __forceinline__ void load_run(const fvec2 *p, uint64_t i, fvec2 (&x)[2]) {
+    const fvec4 v = *reinterpret_cast<const fvec4 *>(p + i);
+    x[0] = fvec2{v.x, v.y};
+    x[1] = fvec2{v.z, v.w};
+}
+template <typename ST>
+__device__ __forceinline__ void store_run(ST *p, uint64_t i, const ST (&x)[1], bool) {
+    p[i] = x[0];
+}
+__device__ __forceinline__ void store_run(fvec2 *p, uint64_t i, const fvec2 (&x)[2], bool pair) {
+    if (pair)
+        *reinterpret_cast<fvec4 *>(p + i) = fvec4{x[0].x, x[0].y, x[1].x, x[1].y};
+    else
+        p[i] = x[0];
+}
+
 // Second projection gate: true when |t'|^2 >= kEta2 |t|^2 (no second pass).
 __device__ __forceinline__ bool gate_skips(const dvec2 *__restrict__ sc) {
     return sc[kScP1].x >= kEta2 * sc[kScT2].x;
@@ -93,41 +129,50 @@ __global__ __launch_bounds__(kThreads) void residual_kernel(Cols<ST> V, Cols<ST>
                                                             double theta, const double *__restrict__ diag,
                                                             const float *__restrict__ logm, ST *__restrict__ t,
                                                             uint64_t D, dvec2 *__restrict__ partial) {
+    constexpr int E = Run<ST, MAXM>::E;
     dvec2 acc[MAXM + 2];
 #pragma unroll
     for (int j = 0; j < MAXM + 2; ++j) acc[j] = dvec2{0.0, 0.0};
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
-         i += static_cast<uint64_t>(gridDim.x) * kThreads) {
+    for (uint64_t i0 = E * (blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x); i0 < D;
+         i0 += E * static_cast<uint64_t>(gridDim.x) * kThreads) {
         // Every load is issued unconditionally (columns past m alias column 0
         // with zero coefficients), so all of them are in flight together.
-        ST vv[MAXM], ww[MAXM];
+        ST vv[MAXM][E], ww[MAXM][E];
 #pragma unroll
         for (int j = 0; j < MAXM; ++j) {
-            vv[j] = V.p[j][i];
-            ww[j] = W.p[j][i];
+            load_run(V.p[j], i0, vv[j]);
+            load_run(W.p[j], i0, ww[j]);
         }
-        dvec2 u{0.0, 0.0}, hu{0.0, 0.0};
+        ST ts[E];
 #pragma unroll
-        for (int j = 0; j < MAXM; ++j) {
-            u += cmul(s.c[j], widen(vv[j]));
-            hu += cmul(s.c[j], widen(ww[j]));
-        }
-        const dvec2 r = hu - u * theta;
-        double scale;
-        if (LEARNED) {
-            scale = exp(static_cast<double>(logm[i]));
-        } else {
-            double den = theta - diag[i];
-            if (fabs(den) < 1e-8) den = den < 0.0 ? -1e-8 : 1e-8;
-            scale = 1.0 / den;
-        }
-        const ST ts = narrow<ST>(r * scale);
-        t[i] = ts;
-        const dvec2 tw = widen(ts);
-        acc[0].x += r.x * r.x + r.y * r.y;
-        acc[1].x += tw.x * tw.x + tw.y * tw.y;
+        for (int e = 0; e < E; ++e) {
+            const uint64_t i = i0 + e;
+            const bool in = e == 0 || i < D;
+            dvec2 u{0.0, 0.0}, hu{0.0, 0.0};
 #pragma unroll
-        for (int j = 0; j < MAXM; ++j) acc[2 + j] += cmulc(widen(vv[j]), tw);
+            for (int j = 0; j < MAXM; ++j) {
+                u += cmul(s.c[j], widen(vv[j][e]));
+                hu += cmul(s.c[j], widen(ww[j][e]));
+            }
+            const dvec2 r = hu - u * theta;
+            double scale;
+            if (LEARNED) {
+                scale = in ? exp(static_cast<double>(logm[i])) : 0.0;
+            } else {
+                double den = theta - (in ? diag[i] : 0.0);
+                if (fabs(den) < 1e-8) den = den < 0.0 ? -1e-8 : 1e-8;
+                scale = 1.0 / den;
+            }
+            ts[e] = narrow<ST>(r * scale);
+            if (in) {
+                const dvec2 tw = widen(ts[e]);
+                acc[0].x += r.x * r.x + r.y * r.y;
+                acc[1].x += tw.x * tw.x + tw.y * tw.y;
+#pragma unroll
+                for (int j = 0; j < MAXM; ++j) acc[2 + j] += cmulc(widen(vv[j][e]), tw);
+            }
+        }
+        store_run(t, i0, ts, E == 1 || i0 + 1 < D);
     }
     block_partials<MAXM + 2>(acc, m + 2, partial);
 }
@@ -141,6 +186,7 @@ __global__ __launch_bounds__(kThreads) void project_kernel(Cols<ST> V, uint32_t 
                                                            ST *__restrict__ t, uint64_t D,
                                                            dvec2 *__restrict__ partial, bool gate) {
     if (gate && gate_skips(sc)) return;
+    constexpr int E = Run<ST, MAXM>::E;
     __shared__ dvec2 c[MAXM];  // uniform coefficients: LDS broadcasts, not per-lane registers
     if (threadIdx.x < MAXM)
         c[threadIdx.x] = threadIdx.x < m ? sc[coef + threadIdx.x] * sig2.s[threadIdx.x] : dvec2{0.0, 0.0};
@@ -148,20 +194,27 @@ __global__ __launch_bounds__(kThreads) void project_kernel(Cols<ST> V, uint32_t 
     dvec2 acc[MAXM + 1];
 #pragma unroll
     for (int j = 0; j < MAXM + 1; ++j) acc[j] = dvec2{0.0, 0.0};
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
-         i += static_cast<uint64_t>(gridDim.x) * kThreads) {
-        ST vv[MAXM];
+    for (uint64_t i0 = E * (blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x); i0 < D;
+         i0 += E * static_cast<uint64_t>(gridDim.x) * kThreads) {
+        ST vv[MAXM][E], tt[E];
 #pragma unroll
-        for (int j = 0; j < MAXM; ++j) vv[j] = V.p[j][i];
-        dvec2 x = widen(t[i]);
+        for (int j = 0; j < MAXM; ++j) load_run(V.p[j], i0, vv[j]);
+        load_run(t, i0, tt);
 #pragma unroll
-        for (int j = 0; j < MAXM; ++j) x -= cmul(c[j], widen(vv[j]));
-        const ST xs = narrow<ST>(x);
-        t[i] = xs;
-        const dvec2 xw = widen(xs);
-        acc[0].x += xw.x * xw.x + xw.y * xw.y;
+        for (int e = 0; e < E; ++e) {
+            const bool in = e == 0 || i0 + e < D;
+            dvec2 x = widen(tt[e]);
 #pragma unroll
-        for (int j = 0; j < MAXM; ++j) acc[1 + j] += cmulc(widen(vv[j]), xw);
+            for (int j = 0; j < MAXM; ++j) x -= cmul(c[j], widen(vv[j][e]));
+            tt[e] = narrow<ST>(x);
+            if (in) {
+                const dvec2 xw = widen(tt[e]);
+                acc[0].x += xw.x * xw.x + xw.y * xw.y;
+#pragma unroll
+                for (int j = 0; j < MAXM; ++j) acc[1 + j] += cmulc(widen(vv[j][e]), xw);
+            }
+        }
+        store_run(t, i0, tt, E == 1 || i0 + 1 < D);
     }
     block_partials<MAXM + 1>(acc, m + 1, partial);
 }
@@ -170,17 +223,23 @@ __global__ __launch_bounds__(kThreads) void project_kernel(Cols<ST> V, uint32_t 
 template <typename ST, int MAXM>
 __global__ __launch_bounds__(kThreads) void column_kernel(Cols<ST> V, uint32_t ncols, const ST *__restrict__ w,
                                                           uint64_t D, dvec2 *__restrict__ partial) {
+    constexpr int E = Run<ST, MAXM>::E;
     dvec2 acc[MAXM + 1];
 #pragma unroll
     for (int j = 0; j < MAXM + 1; ++j) acc[j] = dvec2{0.0, 0.0};
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x; i < D;
-         i += static_cast<uint64_t>(gridDim.x) * kThreads) {
-        const dvec2 wi = widen(w[i]);
-        ST vv[MAXM + 1];
+    for (uint64_t i0 = E * (blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x); i0 < D;
+         i0 += E * static_cast<uint64_t>(gridDim.x) * kThreads) {
+        ST ww[E], vv[MAXM + 1][E];
+        load_run(w, i0, ww);
 #pragma unroll
-        for (int j = 0; j < MAXM + 1; ++j) vv[j] = V.p[j][i];
+        for (int j = 0; j < MAXM + 1; ++j) load_run(V.p[j], i0, vv[j]);
 #pragma unroll
-        for (int j = 0; j < MAXM + 1; ++j) acc[j] += cmulc(widen(vv[j]), wi);
+        for (int e = 0; e < E; ++e) {
+            if (e > 0 && i0 + e >= D) break;
+            const dvec2 wi = widen(ww[e]);
+#pragma unroll
+            for (int j = 0; j < MAXM + 1; ++j) acc[j] += cmulc(widen(vv[j][e]), wi);
+        }
     }
     block_partials<MAXM + 1>(acc, ncols, partial);
 }

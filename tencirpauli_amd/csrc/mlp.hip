@@ -28,12 +28,14 @@ constexpr uint32_t kOffW1 = 0, kOffB1 = kF * kH, kOffW2 = kOffB1 + kH, kOffB2 = 
                    kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kH;
 constexpr uint32_t kThreads = 256;
 // The Davidson callback re-evaluates the network when theta has moved by more
-// than kRefresh times the coupling scale since its last evaluation.  A
-// preconditioner for a slightly different theta is as good a search
-// direction: on the 4x4 and 4x5 training-family tori the iteration counts are
-// the same for refresh thresholds from 0 to 0.5 energy units (26 and 42), while
-// the evaluations fall from 25 and 41 to 7 and 13.
-constexpr double kRefresh = 0.25;
+// than kRefresh times the coupling scale since its last evaluation (default;
+// tcp_learned_precond_set_refresh changes it).  A preconditioner for a
+// slightly different theta is as good a search direction: on the 4x4 and 4x5
+// training-family tori the iteration counts are the same for refresh
+// thresholds from 0 to 0.5 energy units (26 and 42), while the evaluations
+// fall from 25 and 41 to 7 and 13; on the 5x5 torus (D = 5,200,300) they are
+// the same from 0.25 to 1.0.
+constexpr double kRefresh = 0.5;
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -49,9 +51,24 @@ __host__ __device__ __forceinline__ uint16_t f2bf(float f) {
 // tanh x = 1 - 2 / (e^{2x} + 1) from the hardware base-2 exponential and
 // reciprocal (absolute error ~1e-7; the result is rounded to bf16 or feeds an
 // FP32 dot product).  Saturates correctly: e^{2x} -> inf gives 1, -> 0 gives -1.
-__device__ __forceinline__ float fast_tanh(float x) {
-    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // 2 / ln 2
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+// The forward kernel applies it as 1 - 2 rcp(2^{a k + b k} + 1) with k =
+// kTwoLog2e and the bias b pre-scaled by k, one FMA in place of add and mul.
+constexpr float kTwoLog2e = 2.8853900817779268f;  // 2 / ln 2
+__device__ __forceinline__ float tanh_denominator_rcp(float scaled) {  // 1 / (2^scaled + 1)
+    return __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(scaled) + 1.0f);
+}
+
+typedef unsigned u2 __attribute__((ext_vector_type(2)));
+// bf16(lo) | bf16(hi) << 16, round to nearest even: one v_cvt_pk_bf16_f32
+// (written out because the compiler splits a converted float pair into two
+// half-used conversions and a permute).
+__device__ __forceinline__ unsigned cvt_pk_bf16(float lo, float hi) {
+    unsigned r;
+    asm("v_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(r) : "v"(lo), "v"(hi));
+    return r;
+}
+__device__ __forceinline__ s4 pack_bf16(f4 v) {
+    return __builtin_bit_cast(s4, (u2{cvt_pk_bf16(v.x, v.y), cvt_pk_bf16(v.z, v.w)}));
 }
 
 __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) {
@@ -102,6 +119,10 @@ __global__ void fill_feature0_kernel(uint16_t *__restrict__ X, const double *__r
 // hidden layer goes from accumulator to bf16 fragment in registers (no LDS
 // staging, no barrier).  The output layer is a dot product over the lanes that
 // share a row.
+// Without STORE (inference) the output needs only sum_u w3_u tanh z_u =
+// sum_u w3_u - 2 sum_u w3_u / (e^{2 z_u} + 1): each lane starts its partial
+// sum at its share of sum_u w3_u and adds one FMA per hidden unit.
+template <bool STORE>
 __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const uint16_t *__restrict__ X,
                                                                uint64_t B, float *__restrict__ y,
                                                                float *__restrict__ h1, float *__restrict__ h2,
@@ -109,18 +130,22 @@ __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const u
                                                                double inv_scale) {
     __shared__ s4 w1s[8 * 4 * 64];
     __shared__ s4 w2s[8 * 8 * 64];
-    __shared__ float bs1[kH], bs2[kH], ws3[kH];
+    __shared__ float bs1[kH], bs2[kH], ws3[kH];  // biases times kTwoLog2e; w3 (times -2 without STORE)
     for (uint32_t i = threadIdx.x; i < 8 * 4 * 64; i += kThreads) w1s[i] = p.w1f[i];
     for (uint32_t i = threadIdx.x; i < 8 * 8 * 64; i += kThreads) w2s[i] = p.w2f[i];
     for (uint32_t i = threadIdx.x; i < kH; i += kThreads) {
-        bs1[i] = p.params[kOffB1 + i];
-        bs2[i] = p.params[kOffB2 + i];
-        ws3[i] = p.params[kOffW3 + i];
+        bs1[i] = p.params[kOffB1 + i] * kTwoLog2e;
+        bs2[i] = p.params[kOffB2 + i] * kTwoLog2e;
+        ws3[i] = STORE ? p.params[kOffW3 + i] : -2.0f * p.params[kOffW3 + i];
     }
     const float b3 = p.params[kOffB3];
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t li = lane & 15, lg = lane >> 4;
+    float base = 0.f;
+    if (!STORE)
+        for (int n = 0; n < 8; ++n)
+            for (int j = 0; j < 4; ++j) base += p.params[kOffW3 + n * 16 + lg * 4 + j];
     const uint64_t ntile = (B + 63) / 64;
     for (uint64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
         // keeps the weight fragments in LDS: without it the compiler hoists all
@@ -146,11 +171,10 @@ __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const u
         for (int n = 0; n < 8; ++n) {
             f4 v;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                v[j] = fast_tanh(acc[n][j] + bs1[n * 16 + lg * 4 + j]);
-                hb[n][j] = static_cast<short>(f2bf(v[j]));
-            }
-            if (h1 && valid) *reinterpret_cast<f4 *>(h1 + row * kH + n * 16 + lg * 4) = v;
+            for (int j = 0; j < 4; ++j)
+                v[j] = fmaf(-2.0f, tanh_denominator_rcp(fmaf(acc[n][j], kTwoLog2e, bs1[n * 16 + lg * 4 + j])), 1.0f);
+            hb[n] = pack_bf16(v);
+            if (STORE && valid) *reinterpret_cast<f4 *>(h1 + row * kH + n * 16 + lg * 4) = v;
         }
         f4 acc2[8];
 #pragma unroll
@@ -159,16 +183,22 @@ __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const u
         for (int ks = 0; ks < 8; ++ks)
 #pragma unroll
             for (int n = 0; n < 8; ++n) acc2[n] = mfma(w2s[(n * 8 + ks) * 64 + lane], hb[ks], acc2[n]);
-        float part = 0.f;
+        float part = base;
 #pragma unroll
         for (int n = 0; n < 8; ++n) {
             f4 v;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                v[j] = fast_tanh(acc2[n][j] + bs2[n * 16 + lg * 4 + j]);
-                part += v[j] * ws3[n * 16 + lg * 4 + j];
+                const uint32_t u = n * 16 + lg * 4 + j;
+                const float r = tanh_denominator_rcp(fmaf(acc2[n][j], kTwoLog2e, bs2[u]));
+                if (STORE) {
+                    v[j] = fmaf(-2.0f, r, 1.0f);
+                    part = fmaf(v[j], ws3[u], part);
+                } else {
+                    part = fmaf(ws3[u], r, part);
+                }
             }
-            if (h2 && valid) *reinterpret_cast<f4 *>(h2 + row * kH + n * 16 + lg * 4) = v;
+            if (STORE && valid) *reinterpret_cast<f4 *>(h2 + row * kH + n * 16 + lg * 4) = v;
         }
         part += __shfl_xor(part, 16);
         part += __shfl_xor(part, 32);
@@ -504,8 +534,9 @@ struct tcp_learned_precond {
     tcp::DeviceBuffer features, diag, logm;
     tcp::DeviceBuffer t, q, hq, partial;  // Ritz-gain loss workspace (allocated on first use)
     // the callback's last evaluation: reused while theta stays within
-    // kRefresh * scale of the theta it was made for and the weights are
+    // refresh * scale of the theta it was made for and the weights are
     // unchanged
+    double refresh = kRefresh;
     bool cached = false;
     double cached_theta = 0.0;
     uint32_t cached_step = 0;
@@ -537,8 +568,14 @@ int forward(const tcp_mlp *mlp, const uint16_t *X, uint64_t B, float *y, float *
     if (B == 0) return TCP_OK;
     const uint64_t tiles = (B + 63) / 64;
     const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(tiles, 256 * 4));
-    hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks), dim3(kThreads), 0, s, mlp->dev(), X, B, y, h1, h2, diag,
-                       theta, inv_scale);
+    if (h1 || h2) {
+        if (!h1 || !h2) return fail(TCP_ERR_INVALID_ARGUMENT, "mlp forward stores both hidden layers or neither");
+        hipLaunchKernelGGL(mlp_forward_kernel<true>, dim3(blocks), dim3(kThreads), 0, s, mlp->dev(), X, B, y, h1, h2,
+                           diag, theta, inv_scale);
+    } else {
+        hipLaunchKernelGGL(mlp_forward_kernel<false>, dim3(blocks), dim3(kThreads), 0, s, mlp->dev(), X, B, y, h1,
+                           h2, diag, theta, inv_scale);
+    }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TCP_OK : tcp::hip_fail(e, "mlp forward");
 }
@@ -781,10 +818,19 @@ int tcp_learned_precond_prepare(tcp_learned_precond *p, double theta, void *stre
 
 // tcp_log_multiplier_fn: log multipliers for the current theta (feature 0 =
 // (H_ii - theta) / scale, then MLP inference over every state).
+int tcp_learned_precond_set_refresh(tcp_learned_precond *precond, double threshold) {
+    if (!precond) return fail(TCP_ERR_INVALID_ARGUMENT, "null preconditioner");
+    if (!(threshold >= 0.0) || !std::isfinite(threshold))
+        return fail(TCP_ERR_INVALID_ARGUMENT, "refresh threshold must be finite and >= 0");
+    precond->refresh = threshold;
+    precond->cached = false;
+    return TCP_OK;
+}
+
 const void *tcp_learned_precond_log_multiplier(void *ctx, double theta, void *stream) {
     auto *p = static_cast<tcp_learned_precond *>(ctx);
     if (!p) return nullptr;
-    if (p->cached && p->cached_step == p->mlp->version && std::fabs(theta - p->cached_theta) < kRefresh * p->scale)
+    if (p->cached && p->cached_step == p->mlp->version && std::fabs(theta - p->cached_theta) < p->refresh * p->scale)
         return p->logm.ptr;
     hipStream_t s = static_cast<hipStream_t>(stream);
     int st = forward(p->mlp, static_cast<const uint16_t *>(p->features.ptr), p->dim,

@@ -108,9 +108,14 @@ class MLP:
 
 
 class LearnedPreconditioner:
-    """Features of one restricted operator plus the trained MLP, as a Davidson hook."""
+    """Features of one restricted operator plus the trained MLP, as a Davidson hook.
 
-    def __init__(self, mlp: MLP, restricted: Any) -> None:
+    The hook re-evaluates the network only when theta has moved by more than
+    ``refresh`` times the operator's coupling scale since its last evaluation
+    (0: on every call).
+    """
+
+    def __init__(self, mlp: MLP, restricted: Any, refresh: float = 0.5) -> None:
         from .symmetry import SectorMvpPlan, lazy_plan_of
 
         plan = lazy_plan_of(restricted)
@@ -119,6 +124,7 @@ class LearnedPreconditioner:
         out = ctypes.c_void_p()
         _lib.check(_lib.lib().tcp_learned_precond_create(mlp._ptr, plan._ptr, ctypes.byref(out)))
         self._ptr = out
+        _lib.check(_lib.lib().tcp_learned_precond_set_refresh(out, ctypes.c_double(refresh)))
         self.mlp = mlp
         self._plan = plan  # the loss applies H through it: keep it alive
         self.dimension = int(plan.dimension)

@@ -143,3 +143,35 @@ def test_learned_preconditioner_in_davidson():
     assert smart.converged and abs(smart.eigenvalue - ref_run.eigenvalue) <= 1e-8
     print(f"4x5 Davidson iterations: jacobi {plain.iterations}, learned {smart.iterations}", flush=True)
     assert smart.iterations < plain.iterations
+
+
+class _DeviceFloats:
+    """A float32 device array at a raw address, readable by torch.as_tensor."""
+
+    def __init__(self, ptr, n):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<f4", "data": (ptr, False), "version": 2}
+
+
+def _device_floats(ptr, n):
+    import torch
+
+    return torch.as_tensor(_DeviceFloats(ptr, n), device="cuda").cpu().numpy().copy()
+
+
+def test_refresh_threshold_controls_network_reevaluation():
+    target = _lattice_sector(3, 4, 5)
+    mlp = learned.MLP(seed=3)
+    dim = target.dimension
+    always = learned.LearnedPreconditioner(mlp, target, refresh=0.0)
+    never = learned.LearnedPreconditioner(mlp, target, refresh=1e9)
+    theta = -5.0
+    a0 = _device_floats(always.log_multipliers(theta), dim)
+    n0 = _device_floats(never.log_multipliers(theta), dim)
+    np.testing.assert_array_equal(a0, n0)
+    shifted = theta - 0.05 * always.scale
+    a1 = _device_floats(always.log_multipliers(shifted), dim)
+    n1 = _device_floats(never.log_multipliers(shifted), dim)
+    assert not np.array_equal(a1, a0)  # feature 0 moved with theta: a new evaluation
+    np.testing.assert_array_equal(n1, n0)  # within the threshold: the cached evaluation
+    with pytest.raises(ValueError, match="refresh"):
+        learned.LearnedPreconditioner(mlp, target, refresh=-1.0)
