@@ -359,6 +359,13 @@ int tcp_csr_plan_create_fermion(const tcp_fermion *f, uint32_t nblocks, const ui
 uint64_t tcp_csr_plan_dimension(const tcp_csr_plan *plan);
 uint64_t tcp_csr_plan_transition_count(const tcp_csr_plan *plan); /* stored entries */
 uint64_t tcp_csr_plan_estimated_bytes(const tcp_csr_plan *plan);
+/* 1 when the plan stores its entries in the compact form: a 24-bit column and
+ * an 8-bit index into a table of at most 255 distinct off-diagonal values per
+ * entry, the diagonal in a dense array (chosen when the matrix has fewer than
+ * 2^24 rows and at most 255 distinct off-diagonal values); 0 for plain CSR.
+ * No reference counterpart (the reference's CsrGraph, core/csr.rs, stores a
+ * value per entry); results are the same either way. */
+int tcp_csr_plan_is_compact(const tcp_csr_plan *plan);
 /* y = A v: row-parallel CSR gather (CsrGraph::apply_into), FP64 accumulation;
  * TCP_C64 uses complex64 vectors and FP32-rounded values. */
 int tcp_csr_plan_apply_into(const tcp_csr_plan *plan, const void *input, void *output, uint64_t input_len,

@@ -644,6 +644,184 @@ __global__ __launch_bounds__(256) void csr_spmv_kernel(const uint64_t *__restric
     }
 }
 
+// ---------------------------------------------------------------- compact CSR
+// Lattice operators have few distinct off-diagonal values (the 5x5 XXZ sector
+// has one), so the off-diagonal values become an 8-bit index into a table of
+// at most kDictMax values, packed with a 24-bit column into one word per
+// entry (4 bytes instead of 4 + 16); index kDiagIndex marks the row's diagonal
+// entry, whose value sits in a dense per-row array.  The table is found by a
+// hash set of kDictSlots slots (hash of the value's bits; 0 marks an empty
+// slot), filled on the device.
+constexpr uint32_t kDictSlots = 1024;
+constexpr uint32_t kDictMax = 255;
+constexpr uint32_t kDiagIndex = 255;
+constexpr uint32_t kColBits = 24;
+constexpr uint32_t kColMask = (1u << kColBits) - 1;
+
+__device__ __forceinline__ unsigned long long value_hash(dvec2 v) {
+    unsigned long long z = __builtin_bit_cast(unsigned long long, v.x) ^
+                           (__builtin_bit_cast(unsigned long long, v.y) * 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return (z ^ (z >> 31)) | 1ull;  // never 0
+}
+__device__ __forceinline__ uint32_t slot_of(unsigned long long h) {
+    return static_cast<uint32_t>(h >> 40) & (kDictSlots - 1);
+}
+
+// Inserts hash h (value v) into the set; state[0] counts distinct values,
+// state[1] is raised when they exceed kDictMax or the set is full.
+__device__ void dict_insert(unsigned long long h, dvec2 v, unsigned long long *slots, dvec2 *slot_vals,
+                            uint32_t *state) {
+    uint32_t s = slot_of(h);
+    for (uint32_t probe = 0; probe < kDictSlots; ++probe, s = (s + 1) & (kDictSlots - 1)) {
+        const unsigned long long cur = __hip_atomic_load(&slots[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == h) return;
+        if (cur == 0) {
+            const unsigned long long prev = atomicCAS(&slots[s], 0ull, h);
+            if (prev == 0) {
+                slot_vals[s] = v;
+                if (atomicAdd(&state[0], 1u) + 1 > kDictMax) atomicExch(&state[1], 1u);
+                return;
+            }
+            if (prev == h) return;
+        }
+    }
+    atomicExch(&state[1], 1u);
+}
+
+// One thread per row over its off-diagonal entries; lanes of a wave holding
+// the same hash are served by one insertion (the wave's first such lane).
+__global__ __launch_bounds__(256) void csr_dict_insert_kernel(const uint64_t *__restrict__ indptr,
+                                                              const uint32_t *__restrict__ cols,
+                                                              const dvec2 *__restrict__ vals, uint64_t dim,
+                                                              unsigned long long *slots, dvec2 *slot_vals,
+                                                              uint32_t *state) {
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < dim;
+         base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t row = base + threadIdx.x;
+        uint64_t k = row < dim ? indptr[row] : 0;
+        const uint64_t k1 = row < dim ? indptr[row + 1] : 0;
+        for (;;) {  // wave-uniform: every lane stays until no lane has entries left
+            if (k < k1 && cols[k] == row) ++k;  // columns ascend: at most one diagonal entry
+            const bool has = k < k1;
+            if (!__any(has)) break;
+            if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(&state[1], __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_AGENT)))
+                return;  // too many distinct values: the caller keeps plain CSR
+            const dvec2 v = has ? vals[k] : dvec2{0.0, 0.0};
+            const unsigned long long h = has ? value_hash(v) : 0ull;
+            bool todo = has;
+            for (;;) {
+                const unsigned long long m = __ballot(todo);
+                if (!m) break;
+                const int first = __ffsll(static_cast<long long>(m)) - 1;
+                const unsigned long long lead = __shfl(h, first);
+                if (static_cast<int>(lane) == first) dict_insert(h, v, slots, slot_vals, state);
+                if (todo && h == lead) todo = false;
+            }
+            if (has) ++k;
+        }
+    }
+}
+
+// Packs every entry as column | index << kColBits and writes the diagonal
+// values; raises state[1] if an entry's value is not bit-identical to its
+// table value (a hash collision) in FP64 or FP32.
+__global__ __launch_bounds__(256) void csr_pack_kernel(const uint64_t *__restrict__ indptr,
+                                                       const uint32_t *__restrict__ cols,
+                                                       const dvec2 *__restrict__ vals,
+                                                       const fvec2 *__restrict__ vals32, uint64_t dim,
+                                                       const unsigned long long *__restrict__ slots,
+                                                       const uint32_t *__restrict__ slot_index,
+                                                       const dvec2 *__restrict__ table64,
+                                                       const fvec2 *__restrict__ table32, uint32_t *__restrict__ packed,
+                                                       dvec2 *__restrict__ diag64, fvec2 *__restrict__ diag32,
+                                                       uint32_t *state) {
+    for (uint64_t row = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; row < dim;
+         row += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        for (uint64_t k = indptr[row]; k < indptr[row + 1]; ++k) {
+            const uint32_t c = cols[k];
+            const dvec2 v = vals[k];
+            if (c == row) {
+                packed[k] = c | (kDiagIndex << kColBits);
+                diag64[row] = v;
+                diag32[row] = vals32[k];
+                continue;
+            }
+            const unsigned long long h = value_hash(v);
+            uint32_t s = slot_of(h), probe = 0;
+            while (slots[s] != h && ++probe < kDictSlots) s = (s + 1) & (kDictSlots - 1);
+            const uint32_t idx = probe < kDictSlots ? slot_index[s] : kDiagIndex;
+            const bool same = idx < kDictMax &&
+                              __builtin_bit_cast(unsigned long long, table64[idx].x) ==
+                                  __builtin_bit_cast(unsigned long long, v.x) &&
+                              __builtin_bit_cast(unsigned long long, table64[idx].y) ==
+                                  __builtin_bit_cast(unsigned long long, v.y) &&
+                              __builtin_bit_cast(uint32_t, table32[idx].x) == __builtin_bit_cast(uint32_t, vals32[k].x) &&
+                              __builtin_bit_cast(uint32_t, table32[idx].y) == __builtin_bit_cast(uint32_t, vals32[k].y);
+            if (!same) {
+                atomicExch(&state[1], 1u);
+                return;
+            }
+            packed[k] = c | (idx << kColBits);
+        }
+    }
+}
+
+// The SpMV of csr_spmv_kernel over the packed entries: the value table sits
+// in LDS, the diagonal entry reads the per-row array.
+template <typename VT, uint32_t LANES, uint32_t kUnroll = 4>
+__global__ __launch_bounds__(256) void csr_compact_spmv_kernel(const uint64_t *__restrict__ indptr,
+                                                               const uint32_t *__restrict__ packed,
+                                                               const VT *__restrict__ table, uint32_t ntable,
+                                                               const VT *__restrict__ diag,
+                                                               const VT *__restrict__ v, VT *__restrict__ y,
+                                                               uint64_t dim) {
+    __shared__ VT tab[kDictMax];
+    for (uint32_t i = threadIdx.x; i < kDictMax; i += blockDim.x) tab[i] = i < ntable ? table[i] : VT{0, 0};
+    __syncthreads();
+    const uint32_t lane = threadIdx.x % LANES;
+    const uint64_t rows_per_step = static_cast<uint64_t>(gridDim.x) * (blockDim.x / LANES);
+    for (uint64_t row = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / LANES) + threadIdx.x / LANES; row < dim;
+         row += rows_per_step) {
+        double ar = 0.0, ai = 0.0;
+        const uint64_t k0 = indptr[row], k1 = indptr[row + 1];
+        for (uint64_t kb = k0 + lane; kb < k1; kb += kUnroll * LANES) {
+            uint32_t c[kUnroll];
+            VT w[kUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                const uint64_t k = kb + u * LANES;
+                const uint32_t p = k < k1 ? packed[k] : 0u;
+                c[u] = p & kColMask;
+                const uint32_t idx = p >> kColBits;
+                w[u] = idx == kDiagIndex ? diag[c[u]] : tab[idx];
+            }
+            VT x[kUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) x[u] = v[c[u]];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                const bool in = kb + u * LANES < k1;  // padded slots add an exact zero
+                const double cr = in ? static_cast<double>(w[u].x) : 0.0, ci = in ? static_cast<double>(w[u].y) : 0.0;
+                const double sr = in ? static_cast<double>(x[u].x) : 0.0, si = in ? static_cast<double>(x[u].y) : 0.0;
+                ar = fma(cr, sr, ar);
+                ar = fma(-ci, si, ar);
+                ai = fma(cr, si, ai);
+                ai = fma(ci, sr, ai);
+            }
+        }
+#pragma unroll
+        for (uint32_t off = LANES / 2; off > 0; off >>= 1) {
+            ar += __shfl_xor(ar, off, LANES);
+            ai += __shfl_xor(ai, off, LANES);
+        }
+        if (lane == 0) y[row] = Amp<VT>::make(ar, ai);
+    }
+}
+
 }  // namespace
 
 struct tcp_sector_plan {
@@ -1414,6 +1592,11 @@ struct tcp_csr_plan {
     uint64_t dim = 0, nnz = 0;
     int device = 0;
     tcp::DeviceBuffer indptr, cols, vals64, vals32;
+    // compact form (used by the SpMV when `compact`): packed column | value
+    // index per entry, the value table and the dense diagonal, FP64 and FP32
+    bool compact = false;
+    uint32_t ntable = 0;
+    tcp::DeviceBuffer packed, table64, table32, diag64, diag32;
     uint64_t retained_bytes = 0;
     std::mutex staging_mutex;
     tcp::DeviceBuffer stage_in, stage_out;
@@ -1422,6 +1605,80 @@ struct tcp_csr_plan {
 namespace {
 
 constexpr uint32_t kCsrLanes = 8;  // lanes per row in the SpMV (rows average ~27 entries on lattice models)
+
+// Builds the compact form of a filled plan when it applies: fewer than 2^24
+// rows, at most kDictMax distinct off-diagonal values, and room in the budget
+// while both forms exist.  Otherwise the plan keeps plain CSR (not an error).
+// On success the FP32 values are dropped (the compact form carries them).
+int compact_csr(tcp_csr_plan *plan, uint64_t max_bytes) {
+    const uint64_t dim = plan->dim, nnz = plan->nnz;
+    if (dim == 0 || nnz == 0 || dim > kColMask + 1ull) return TCP_OK;
+    const uint64_t extra = 4 * nnz + 24 * dim + 64 * kDictSlots;
+    if (tcp::exceeds_budget(plan->retained_bytes + extra, max_bytes)) return TCP_OK;
+    tcp::DeviceBuffer slots, slot_vals, state, slot_index;
+    int st;
+    if ((st = slots.allocate(8 * kDictSlots)) || (st = slot_vals.allocate(16 * kDictSlots)) ||
+        (st = state.allocate(8)))
+        return st;
+    TCP_HIP_CHECK(hipMemset(slots.ptr, 0, 8 * kDictSlots));
+    TCP_HIP_CHECK(hipMemset(state.ptr, 0, 8));
+    const auto *ip = static_cast<const uint64_t *>(plan->indptr.ptr);
+    const auto *cp = static_cast<const uint32_t *>(plan->cols.ptr);
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((dim + 255) / 256, 256 * 16));
+    hipLaunchKernelGGL(csr_dict_insert_kernel, dim3(blocks), dim3(256), 0, nullptr, ip, cp,
+                       static_cast<const dvec2 *>(plan->vals64.ptr), dim,
+                       static_cast<unsigned long long *>(slots.ptr), static_cast<dvec2 *>(slot_vals.ptr),
+                       static_cast<uint32_t *>(state.ptr));
+    TCP_HIP_CHECK(hipGetLastError());
+    uint32_t hstate[2];
+    TCP_HIP_CHECK(hipMemcpy(hstate, state.ptr, 8, hipMemcpyDeviceToHost));
+    if (hstate[1]) return TCP_OK;
+    std::vector<unsigned long long> hslots(kDictSlots);
+    std::vector<dvec2> hvals(kDictSlots);
+    TCP_HIP_CHECK(hipMemcpy(hslots.data(), slots.ptr, 8 * kDictSlots, hipMemcpyDeviceToHost));
+    TCP_HIP_CHECK(hipMemcpy(hvals.data(), slot_vals.ptr, 16 * kDictSlots, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> index(kDictSlots, kDiagIndex);
+    std::vector<dvec2> t64;
+    std::vector<fvec2> t32;
+    for (uint32_t k = 0; k < kDictSlots; ++k)
+        if (hslots[k]) {
+            index[k] = static_cast<uint32_t>(t64.size());
+            t64.push_back(hvals[k]);
+            t32.push_back(fvec2{static_cast<float>(hvals[k].x), static_cast<float>(hvals[k].y)});
+        }
+    if (t64.size() > kDictMax) return TCP_OK;
+    const uint32_t ntable = static_cast<uint32_t>(t64.size());
+    const size_t tb = std::max<size_t>(ntable, 1);
+    t64.resize(tb, dvec2{0.0, 0.0});
+    t32.resize(tb, fvec2{0.f, 0.f});
+    if ((st = slot_index.upload(index.data(), 4 * kDictSlots)) || (st = plan->table64.upload(t64.data(), 16 * tb)) ||
+        (st = plan->table32.upload(t32.data(), 8 * tb)) || (st = plan->packed.allocate(4 * nnz)) ||
+        (st = plan->diag64.allocate(16 * dim)) || (st = plan->diag32.allocate(8 * dim)))
+        return st;
+    TCP_HIP_CHECK(hipMemset(plan->diag64.ptr, 0, 16 * dim));
+    TCP_HIP_CHECK(hipMemset(plan->diag32.ptr, 0, 8 * dim));
+    hipLaunchKernelGGL(csr_pack_kernel, dim3(blocks), dim3(256), 0, nullptr, ip, cp,
+                       static_cast<const dvec2 *>(plan->vals64.ptr), static_cast<const fvec2 *>(plan->vals32.ptr), dim,
+                       static_cast<const unsigned long long *>(slots.ptr), static_cast<const uint32_t *>(slot_index.ptr),
+                       static_cast<const dvec2 *>(plan->table64.ptr), static_cast<const fvec2 *>(plan->table32.ptr),
+                       static_cast<uint32_t *>(plan->packed.ptr), static_cast<dvec2 *>(plan->diag64.ptr),
+                       static_cast<fvec2 *>(plan->diag32.ptr), static_cast<uint32_t *>(state.ptr));
+    TCP_HIP_CHECK(hipGetLastError());
+    TCP_HIP_CHECK(hipMemcpy(hstate, state.ptr, 8, hipMemcpyDeviceToHost));
+    if (hstate[1]) {  // a hash collision: keep plain CSR
+        plan->packed.release();
+        plan->diag64.release();
+        plan->diag32.release();
+        plan->table64.release();
+        plan->table32.release();
+        return TCP_OK;
+    }
+    plan->compact = true;
+    plan->ntable = ntable;
+    plan->vals32.release();
+    plan->retained_bytes = plan->retained_bytes - 8 * nnz + 4 * nnz + 24 * dim + 24 * tb;
+    return TCP_OK;
+}
 
 // Two-pass CSR assembly: `count` writes the entries of every row, the host
 // scans them into indptr, `fill` writes each row's columns (ascending) and
@@ -1459,6 +1716,7 @@ int assemble_csr(uint32_t n, uint64_t dim, int device, uint64_t max_bytes, Count
                    plan->vals64.ptr, plan->vals32.ptr)))
         return st;
     TCP_HIP_CHECK(hipDeviceSynchronize());
+    if ((st = compact_csr(plan, max_bytes))) return st;
     *out = guard.release();
     return TCP_OK;
 }
@@ -1520,7 +1778,19 @@ int csr_apply_device(const tcp_csr_plan *plan, const void *in, void *out, int dt
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint64_t *ip = static_cast<const uint64_t *>(plan->indptr.ptr);
     const uint32_t *cp = static_cast<const uint32_t *>(plan->cols.ptr);
-    if (dtype == TCP_C128)
+    if (plan->compact) {
+        const auto *pk = static_cast<const uint32_t *>(plan->packed.ptr);
+        if (dtype == TCP_C128)
+            hipLaunchKernelGGL((csr_compact_spmv_kernel<dvec2, kCsrLanes>), dim3(blocks), dim3(256), 0, s, ip, pk,
+                               static_cast<const dvec2 *>(plan->table64.ptr), plan->ntable,
+                               static_cast<const dvec2 *>(plan->diag64.ptr), static_cast<const dvec2 *>(in),
+                               static_cast<dvec2 *>(out), plan->dim);
+        else
+            hipLaunchKernelGGL((csr_compact_spmv_kernel<fvec2, kCsrLanes>), dim3(blocks), dim3(256), 0, s, ip, pk,
+                               static_cast<const fvec2 *>(plan->table32.ptr), plan->ntable,
+                               static_cast<const fvec2 *>(plan->diag32.ptr), static_cast<const fvec2 *>(in),
+                               static_cast<fvec2 *>(out), plan->dim);
+    } else if (dtype == TCP_C128)
         hipLaunchKernelGGL((csr_spmv_kernel<dvec2, dvec2, kCsrLanes>), dim3(blocks), dim3(256), 0, s, ip, cp,
                            static_cast<const dvec2 *>(plan->vals64.ptr), static_cast<const dvec2 *>(in),
                            static_cast<dvec2 *>(out), plan->dim);
@@ -1636,6 +1906,7 @@ int tcp_csr_plan_create_fermion(const tcp_fermion *f, uint32_t nblocks, const ui
 uint64_t tcp_csr_plan_dimension(const tcp_csr_plan *plan) { return plan ? plan->dim : 0; }
 uint64_t tcp_csr_plan_transition_count(const tcp_csr_plan *plan) { return plan ? plan->nnz : 0; }
 uint64_t tcp_csr_plan_estimated_bytes(const tcp_csr_plan *plan) { return plan ? plan->retained_bytes : 0; }
+int tcp_csr_plan_is_compact(const tcp_csr_plan *plan) { return plan && plan->compact ? 1 : 0; }
 
 int tcp_csr_plan_apply_into(const tcp_csr_plan *plan, const void *input, void *output, uint64_t input_len,
                             uint64_t output_len, int dtype, void *stream) {

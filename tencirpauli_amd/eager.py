@@ -87,6 +87,10 @@ class CsrMvpPlan(NativePlanFacade):
         self.dimension = int(L.tcp_csr_plan_dimension(ptr))
         self.transition_count = int(L.tcp_csr_plan_transition_count(ptr))
         self.device_bytes = int(L.tcp_csr_plan_estimated_bytes(ptr))
+        # "compact": 24-bit column + 8-bit index into a table of the distinct
+        # off-diagonal values per entry, diagonal in a dense array; "plain": a
+        # value per entry.  The SpMV reads whichever form the plan holds.
+        self.storage_format = "compact" if L.tcp_csr_plan_is_compact(ptr) else "plain"
         self.native_variant = native_variant
         self.execution_family = execution_family
         self.term_count = term_count

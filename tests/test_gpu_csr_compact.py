@@ -1,0 +1,90 @@
+"""GPU checks of the compact CSR form against the Kronecker-product oracle.
+
+An eager plan whose matrix has fewer than 2^24 rows and at most 255 distinct
+off-diagonal values stores each entry as a 24-bit column plus an 8-bit index
+into a value table, with the diagonal in a dense array; otherwise it keeps a
+value per entry.  Both forms must give the oracle's product (FP64 within
+1e-12 relative, complex64 within FP32 rounding) and export the same CSR.
+The cases pin the table boundary (255 distinct values compact, 256 plain),
+rows without a diagonal entry, complex table values, and the choice made for
+the U(1) lattice sectors.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import tencirpauli_amd as tcp
+from oracle import dense
+from tencirpauli_amd import lattice
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-6
+
+
+def _x_word(n, mask):
+    return tuple(1 if (mask >> (n - 1 - q)) & 1 else 0 for q in range(n))
+
+
+def _distinct_value_operator(n, distinct, phase=1.0):
+    """`distinct` X-only words with coefficients phase * (k + 1): every row
+    holds one entry per word with exactly that value, so the matrix has
+    `distinct` distinct off-diagonal values; plus a diagonal that vanishes on
+    half of the rows (0.5 Z0 Z1 + 0.5 Z0)."""
+    terms = [(_x_word(n, m), phase * (k + 1)) for k, m in enumerate(range(1, distinct + 1))]
+    terms.append(((3, 3) + (0,) * (n - 2), 0.5))
+    terms.append(((3,) + (0,) * (n - 1), 0.5))
+    return tcp.PauliOperator.from_terms(n, terms)
+
+
+def _check(plan, op_dense, seed):
+    import torch
+
+    rng = np.random.default_rng(seed)
+    state = rng.normal(size=plan.dimension) + 1j * rng.normal(size=plan.dimension)
+    expected = op_dense @ state
+    assert rel_err(plan.apply(state), expected) <= FP64_TOL
+    got32 = torch.empty(plan.dimension, dtype=torch.complex64, device="cuda")
+    plan.apply_into(torch.from_numpy(state.astype(np.complex64)).cuda(), got32)
+    assert rel_err(got32.cpu().numpy(), expected) <= FP32_TOL
+
+
+@pytest.mark.parametrize("distinct,form", [(1, "compact"), (255, "compact"), (256, "plain")])
+def test_table_boundary_and_rows_without_diagonal(distinct, form):
+    n = 9
+    op = _distinct_value_operator(n, distinct, phase=(0.6 - 0.8j) if distinct == 255 else 1.0)
+    plan = op.csr_mvp_plan()
+    assert plan.storage_format == form
+    full = dense.dense_operator(n, *op.code_arrays())
+    _check(plan, full, distinct)
+    # the exported CSR is the same in either form
+    csr = op.csr()
+    np.testing.assert_allclose(csr.to_scipy().toarray(), full, atol=1e-13)
+
+
+def test_u1_lattice_sector_is_compact_and_matches_lazy_and_oracle():
+    codes, coef = lattice.heisenberg_periodic(3, 4, seed=5)
+    op = tcp.PauliOperator.from_code_arrays(codes, coef)
+    sector = tcp.U1Sector(12, 6)
+    plan = op.restrict_charge(sector).mvp_plan(storage="eager")
+    assert plan.storage_format == "compact"
+    basis = np.asarray(dense.blocks_basis([(12, 6)]), dtype=np.int64)
+    full = dense.dense_operator(12, codes, coef)[np.ix_(basis, basis)]
+    _check(plan, full, 11)
+    state = np.random.default_rng(2).normal(size=plan.dimension) + 0j
+    assert rel_err(plan.apply(state), op.restrict_charge(sector).apply(state)) <= FP64_TOL
+
+
+def test_random_coefficients_keep_plain_csr():
+    rng = np.random.default_rng(9)
+    n = 10
+    codes = dense.random_codes(rng, n, 600, p_identity=0.3)
+    coef = rng.normal(size=600) + 1j * rng.normal(size=600)
+    op = tcp.PauliOperator.from_code_arrays(codes, coef)
+    plan = op.csr_mvp_plan()
+    assert plan.storage_format == "plain"
+    _check(plan, dense.dense_operator(n, codes, coef), 4)
