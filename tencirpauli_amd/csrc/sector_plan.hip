@@ -191,7 +191,11 @@ constexpr uint32_t kGatherChunk = 4;
 
 // NB = number of blocks, fixed at compile time so the per-group block loops
 // keep their block fields in registers without spilling kernel arguments.
-template <typename VT, bool LDS, bool COND, uint32_t NB>
+// RPT = consecutive destination ranks per thread (one unrank, then successor
+// steps): 1 for one-block (U(1)) sectors, 2 for block sectors -- on the 5x5
+// XXZ sector one rank per thread is 8% faster than two, on the 4x4 Hubbard
+// sectors two are 4-5% faster than one.
+template <typename VT, bool LDS, bool COND, uint32_t NB, uint32_t RPT>
 __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT *__restrict__ v,
                                                                        VT *__restrict__ y,
                                                                        SectorArgs p) {
@@ -208,11 +212,11 @@ __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT 
     cptr<dvec2> tw = (cptr<dvec2>)p.tw;
     cptr<uint32_t> gblk = (cptr<uint32_t>)p.gblk;
 
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * kRanksPerThread;
-    for (uint64_t r0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kRanksPerThread;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * RPT;
+    for (uint64_t r0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * RPT;
          r0 < p.dim; r0 += stride) {
         uint64_t b = sector_unrank<NB>(p, r0);
-        for (uint32_t j = 0; j < kRanksPerThread; ++j) {
+        for (uint32_t j = 0; j < RPT; ++j) {
             const uint64_t r = r0 + j;
             if (r >= p.dim) break;
             // a group leaves the blocks its x-mask does not touch as they are in
@@ -1252,16 +1256,17 @@ template <typename VT, bool COND>
 hipError_t launch_gather(const tcp_sector_plan *plan, const void *in, void *out, int dtype,
                          hipStream_t stream) {
     SectorArgs a = plan->args(dtype);
-    const uint64_t per_block = static_cast<uint64_t>(kGatherThreads) * kRanksPerThread;
-    const uint64_t blocks = std::min<uint64_t>((plan->dim + per_block - 1) / per_block, 256 * 16);
     auto launch = [&](auto nb) {
         constexpr uint32_t NB = decltype(nb)::value;
+        constexpr uint32_t RPT = NB == 1 ? 1 : 2;
+        const uint64_t per_block = static_cast<uint64_t>(kGatherThreads) * RPT;
+        const uint64_t blocks = std::min<uint64_t>((plan->dim + per_block - 1) / per_block, 256 * 16);
         if (plan->lut_in_lds)
-            hipLaunchKernelGGL((sector_gather_kernel<VT, true, COND, NB>), dim3(static_cast<uint32_t>(blocks)),
+            hipLaunchKernelGGL((sector_gather_kernel<VT, true, COND, NB, RPT>), dim3(static_cast<uint32_t>(blocks)),
                                dim3(kGatherThreads), a.lut_words * 4, stream, static_cast<const VT *>(in),
                                static_cast<VT *>(out), a);
         else
-            hipLaunchKernelGGL((sector_gather_kernel<VT, false, COND, NB>), dim3(static_cast<uint32_t>(blocks)),
+            hipLaunchKernelGGL((sector_gather_kernel<VT, false, COND, NB, RPT>), dim3(static_cast<uint32_t>(blocks)),
                                dim3(kGatherThreads), 0, stream, static_cast<const VT *>(in),
                                static_cast<VT *>(out), a);
     };
