@@ -88,3 +88,22 @@ def test_random_coefficients_keep_plain_csr():
     plan = op.csr_mvp_plan()
     assert plan.storage_format == "plain"
     _check(plan, dense.dense_operator(n, codes, coef), 4)
+
+
+def test_degenerate_matrices():
+    # diagonal only (no off-diagonal values: an empty table), a zero operator
+    # (no entries: plain CSR) and a one-state sector
+    n = 6
+    diag_only = tcp.PauliOperator.from_terms(n, [((3,) + (0,) * (n - 1), 0.5), ((0, 3) + (0,) * (n - 2), -1.25)])
+    plan = diag_only.csr_mvp_plan()
+    assert plan.storage_format == "compact"
+    _check(plan, dense.dense_operator(n, *diag_only.code_arrays()), 1)
+    zero = tcp.PauliOperator.from_terms(n, [((1,) + (0,) * (n - 1), 0.0)])
+    plan = zero.csr_mvp_plan()
+    assert plan.transition_count == 0 and plan.storage_format == "plain"
+    state = np.arange(1 << n, dtype=np.complex128)
+    np.testing.assert_array_equal(plan.apply(state), np.zeros(1 << n))
+    hop = tcp.PauliOperator.from_terms(2, [("XX", 0.5), ("YY", 0.5), ("ZZ", 0.25)])
+    one = hop.restrict_charge(tcp.U1Sector(2, 2)).mvp_plan(storage="eager")
+    assert one.dimension == 1
+    np.testing.assert_allclose(one.apply(np.array([2.0 + 1.0j])), [0.25 * (2.0 + 1.0j)], rtol=1e-15)
