@@ -178,12 +178,19 @@ def extra_sector(cpu_seconds: float = 0.0) -> dict:
         v = torch.randn(restricted.dimension, dtype=torch.complex128, device="cuda", generator=gen)
         rec = {"dimension": restricted.dimension, "terms": restricted.plan.term_count,
                "x_groups": restricted.plan.group_count}
+        # stored transitions (the eager form's nonzeros, diagonal included) for the gather model's
+        # bytes, (nnz + D) * 16 per FP64 apply, and transitions/s
+        nnz = restricted.mvp_plan(storage="eager").transition_count
+        rec["transitions"] = nnz
         for tag, dt in (("fp64", torch.complex128), ("fp32", torch.complex64)):
             vv = v.to(dt)
             yy = torch.empty_like(vv)
             ms = time_applies(lambda: restricted.apply_into(vv, yy), 10)
             rec[f"ms_per_mvp_{tag}"] = round(ms, 4)
             rec[f"amp_per_s_{tag}"] = round(restricted.dimension / (ms * 1e-3), 1)
+            rec[f"transitions_per_s_{tag}"] = round(nnz / (ms * 1e-3), 1)
+            rec[f"gather_model_GBps_{tag}"] = round((nnz + restricted.dimension) * (16 if tag == "fp64" else 8)
+                                                    / (ms * 1e-3) / 1e9, 1)
         if cpu_seconds > 0:
             import oracle
 
