@@ -218,3 +218,17 @@ def test_apply_into_buffer_protocol():
     assert _plan.enforced_max_bytes(1) is None
     with pytest.raises(MemoryError):
         _plan.check_allocation(2 << 30, 1 << 30, "output")
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the measured path is one GPU's share; --gpus above 1 exits with a message and status 2
+    import json
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    proc = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                          text=True, cwd=root, timeout=120)
+    assert proc.returncode == 2
+    assert "one GPU" in json.loads(proc.stdout.strip().splitlines()[-1])["error"]
