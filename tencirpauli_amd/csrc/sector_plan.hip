@@ -473,7 +473,7 @@ __global__ void sector_features_kernel(SectorArgs p, uint32_t diag_group, double
 // aggregated value, (rank(s), value); columns ascend inside a row, the order
 // the reference's source-major two-pass build produces.
 
-constexpr uint32_t kCsrSortCap = 64;  // rows up to this length are sorted in registers/scratch
+constexpr uint32_t kCsrSortCap = 64;  // rows up to this length are sorted in registers
 
 template <bool LDS, bool COND>
 __global__ __launch_bounds__(kSectorThreads) void csr_count_kernel(SectorArgs p, uint32_t *__restrict__ counts) {
@@ -527,8 +527,17 @@ __global__ __launch_bounds__(kSectorThreads) void csr_fill_kernel(SectorArgs p, 
     cptr<uint32_t> goff = (cptr<uint32_t>)p.goff;
     cptr<uint64_t> tz = (cptr<uint64_t>)p.tz;
     cptr<dvec2> tw = (cptr<dvec2>)p.tw;
-    uint32_t lc[kCsrSortCap];
-    dvec2 lv[kCsrSortCap];
+    // Sum of group g's terms at source s (zero when none acts).
+    auto group_value = [&](uint32_t g, uint64_t s) -> dvec2 {
+        double dr = 0.0, di = 0.0;
+        for (uint32_t k = goff[g]; k < goff[g + 1]; ++k) {
+            if (!term_acts<COND>(p, k, s)) continue;
+            const uint32_t par = __popcll(tz[k] & s) & 1u;
+            dr += flip_sign(tw[k].x, par);
+            di += flip_sign(tw[k].y, par);
+        }
+        return dvec2{dr, di};
+    };
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * kRanksPerThread;
     for (uint64_t r0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kRanksPerThread;
          r0 < p.dim; r0 += stride) {
@@ -538,42 +547,53 @@ __global__ __launch_bounds__(kSectorThreads) void csr_fill_kernel(SectorArgs p, 
             if (r >= p.dim) break;
             const uint64_t pos = indptr[r];
             const uint64_t len = indptr[r + 1] - pos;
-            const bool local = len <= kCsrSortCap;
-            uint32_t n = 0;
-            for (uint32_t g = 0; g < p.G && n < len; ++g) {
-                const uint64_t x = gx[g];
-                const uint64_t s = b ^ x;
-                if (!in_sector(p, s)) continue;
-                double dr = 0.0, di = 0.0;
-                for (uint32_t k = goff[g]; k < goff[g + 1]; ++k) {
-                    if (!term_acts<COND>(p, k, s)) continue;
-                    const uint32_t par = __popcll(tz[k] & s) & 1u;
-                    dr += flip_sign(tw[k].x, par);
-                    di += flip_sign(tw[k].y, par);
-                }
-                if (dr == 0.0 && di == 0.0) continue;
-                const uint32_t c = x == 0 ? static_cast<uint32_t>(r) : sector_rank(p, lut, s);
-                if (local) {  // insertion into the sorted prefix
-                    uint32_t q = n;
-                    while (q > 0 && lc[q - 1] > c) {
-                        lc[q] = lc[q - 1];
-                        lv[q] = lv[q - 1];
-                        --q;
+            if (len <= kCsrSortCap) {
+                // Rows up to kCsrSortCap entries: keys column << 32 | group
+                // kept sorted in registers (every index below is static, so
+                // the array stays out of private memory), then emitted in
+                // order with each group's value recomputed.
+                uint64_t keys[kCsrSortCap];
+#pragma unroll
+                for (uint32_t q = 0; q < kCsrSortCap; ++q) keys[q] = ~0ull;
+                uint32_t n = 0;
+                for (uint32_t g = 0; g < p.G && n < len; ++g) {
+                    const uint64_t x = gx[g];
+                    const uint64_t s = b ^ x;
+                    if (!in_sector(p, s)) continue;
+                    const dvec2 v = group_value(g, s);
+                    if (v.x == 0.0 && v.y == 0.0) continue;
+                    const uint32_t c = x == 0 ? static_cast<uint32_t>(r) : sector_rank(p, lut, s);
+                    uint64_t k = (static_cast<uint64_t>(c) << 32) | g;
+#pragma unroll
+                    for (uint32_t q = 0; q < kCsrSortCap; ++q) {  // insertion by compare-exchange
+                        const uint64_t lo = keys[q] < k ? keys[q] : k;
+                        k = keys[q] < k ? k : keys[q];
+                        keys[q] = lo;
                     }
-                    lc[q] = c;
-                    lv[q] = dvec2{dr, di};
-                } else {
-                    cols[pos + n] = c;
-                    vals[pos + n] = dvec2{dr, di};
+                    ++n;
                 }
-                ++n;
-            }
-            if (local) {
                 for (uint32_t q = 0; q < n; ++q) {
-                    cols[pos + q] = lc[q];
-                    vals[pos + q] = lv[q];
+                    const uint32_t c = static_cast<uint32_t>(keys[0] >> 32);
+                    const uint32_t g = static_cast<uint32_t>(keys[0]);
+#pragma unroll
+                    for (uint32_t i = 0; i + 1 < kCsrSortCap; ++i) keys[i] = keys[i + 1];
+                    const dvec2 v = group_value(g, b ^ gx[g]);
+                    cols[pos + q] = c;
+                    vals[pos + q] = v;
+                    vals32[pos + q] = fvec2{static_cast<float>(v.x), static_cast<float>(v.y)};
                 }
-            } else {  // long row: insertion sort in place
+            } else {  // long row: written in group order, then insertion-sorted in place
+                uint32_t n = 0;
+                for (uint32_t g = 0; g < p.G && n < len; ++g) {
+                    const uint64_t x = gx[g];
+                    const uint64_t s = b ^ x;
+                    if (!in_sector(p, s)) continue;
+                    const dvec2 v = group_value(g, s);
+                    if (v.x == 0.0 && v.y == 0.0) continue;
+                    cols[pos + n] = x == 0 ? static_cast<uint32_t>(r) : sector_rank(p, lut, s);
+                    vals[pos + n] = v;
+                    ++n;
+                }
                 for (uint32_t q = 1; q < n; ++q) {
                     const uint32_t c = cols[pos + q];
                     const dvec2 v = vals[pos + q];
@@ -586,10 +606,10 @@ __global__ __launch_bounds__(kSectorThreads) void csr_fill_kernel(SectorArgs p, 
                     cols[pos + i] = c;
                     vals[pos + i] = v;
                 }
-            }
-            for (uint32_t q = 0; q < n; ++q) {
-                const dvec2 v = vals[pos + q];
-                vals32[pos + q] = fvec2{static_cast<float>(v.x), static_cast<float>(v.y)};
+                for (uint32_t q = 0; q < n; ++q) {
+                    const dvec2 v = vals[pos + q];
+                    vals32[pos + q] = fvec2{static_cast<float>(v.x), static_cast<float>(v.y)};
+                }
             }
             b = sector_next(p, b);
         }
