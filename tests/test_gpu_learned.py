@@ -175,3 +175,33 @@ def test_refresh_threshold_controls_network_reevaluation():
     np.testing.assert_array_equal(n1, n0)  # within the threshold: the cached evaluation
     with pytest.raises(ValueError, match="refresh"):
         learned.LearnedPreconditioner(mlp, target, refresh=-1.0)
+
+
+class _DeviceArray:
+    """A device array of any dtype at a raw address, readable by torch.as_tensor."""
+
+    def __init__(self, ptr, shape, typestr):
+        self.__cuda_array_interface__ = {"shape": shape, "typestr": typestr, "data": (ptr, False), "version": 2}
+
+
+def test_preconditioner_log_multipliers_match_the_reference_network():
+    # The Davidson callback evaluates the network with feature 0 replaced by
+    # bf16((H_ii - theta) / scale) from the diagonal, in the inference variant
+    # of the forward kernel: it must equal the NumPy network on those inputs.
+    import torch
+
+    target = _lattice_sector(3, 4, 8)
+    rng = np.random.default_rng(21)
+    mlp = learned.MLP()
+    p = _random_params(rng)
+    mlp.set_parameters(p)
+    hook = learned.LearnedPreconditioner(mlp, target, refresh=0.0)
+    dim = hook.dimension
+    theta = -4.5
+    got = _device_floats(hook.log_multipliers(theta), dim)
+    raw = torch.as_tensor(_DeviceArray(hook.features_ptr, (dim, 64), "<i2"), device="cuda").cpu().numpy()
+    diag = torch.as_tensor(_DeviceArray(hook.diag_ptr, (dim,), "<f8"), device="cuda").cpu().numpy()
+    X = (raw.astype(np.uint16).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    X[:, 0] = ref.bf16(((diag - theta) * (1.0 / hook.scale)).astype(np.float32))
+    expected = ref.forward(p, X)[0]
+    np.testing.assert_allclose(got, expected, atol=2e-3 * max(1.0, np.abs(expected).max()))
