@@ -172,17 +172,26 @@ int tcp_mvp_plan_create_from_words(uint32_t nqubits, uint64_t nterms, const uint
  * TCP_TILE_XCD_ORDER walks the tiles XCD by XCD (the default of
  * TCP_TILE_AUTO; TCP_TILE_LINEAR_ORDER turns it off).  Spaces smaller than the
  * geometry's tile fall back to a smaller one.  tcp_mvp_plan_create uses
- * TCP_TILE_AUTO: 512x8 for complex128 and 512x4 for complex64 storage, both
- * with the XCD order. */
+ * TCP_TILE_AUTO: from 24 qubits 1024x8 for complex128 and 512x8 for complex64
+ * storage, below that 512x8 and 512x4, all with the XCD order.  Every tiled
+ * geometry maps its kernel index onto the amplitude index through a bit
+ * permutation the plan builder chooses per operator (fewer source tiles per
+ * output tile, more L2 reuse); results differ only in summation order. */
 enum {
     TCP_TILE_AUTO = 0,
     TCP_TILE_512x8 = 1,  /* 2^12-amplitude tiles, two workgroups per CU */
     TCP_TILE_1024x8 = 2, /* 2^13-amplitude tiles */
     TCP_TILE_512x16 = 3, /* 2^13-amplitude tiles */
-    TCP_TILE_512x4 = 4   /* 2^11-amplitude tiles, four workgroups per CU */
+    TCP_TILE_512x4 = 4,  /* 2^11-amplitude tiles, four workgroups per CU */
+    TCP_TILE_1024x4 = 5  /* 2^12-amplitude tiles, 16 waves per workgroup, two per CU */
 };
 #define TCP_TILE_XCD_ORDER 0x100u
 #define TCP_TILE_LINEAR_ORDER 0x200u /* plain tile order, also for TCP_TILE_AUTO */
+/* Source tiles in flight per workgroup for the 512x8, 512x4 and 1024x4
+ * geometries: (depth - 1) << TCP_TILE_PREFETCH_SHIFT, depth 1..3 (default 1). */
+#define TCP_TILE_PREFETCH_SHIFT 10
+#define TCP_TILE_PREFETCH_MASK 0xC00u
+#define TCP_TILE_NT_LOADS 0x1000u /* non-temporal source-tile loads (same geometries) */
 int tcp_mvp_plan_create_tiled(const tcp_operator *op, int storage, uint64_t max_bytes, uint32_t tiling,
                               tcp_mvp_plan **out);
 /* Geometry the given dtype's applies use (after the small-space fallback;
@@ -434,6 +443,9 @@ typedef struct tcp_davidson_options {
     uint64_t seed;            /* deterministic start vector */
     tcp_log_multiplier_fn log_multiplier_fn; /* NULL: plain Jacobi preconditioner */
     void *log_multiplier_ctx;
+    uint64_t log_multiplier_len; /* floats the callback's buffer holds; must equal the plan's
+                                    dimension when log_multiplier_fn is set (else
+                                    TCP_ERR_INVALID_STRUCTURE_LENGTH before any launch) */
     double *trace;            /* host [trace_capacity x 4]: seconds, theta, ||r||, MVP bits (32/64) */
     uint32_t trace_capacity;
     void *stream;             /* hipStream_t (NULL = default stream) */

@@ -6,7 +6,10 @@ No reference counterpart: the reference applies one plan per call
 ``nrhs`` vectors back to back starting at ``offsets[p]``.  ``autotune`` times
 every tile geometry (threads per workgroup x amplitudes per thread) on the
 caller's buffers and keeps the fastest; ``capture`` records one apply into a
-hipGraph that ``replay`` launches.
+hipGraph that ``replay`` launches.  A captured graph is bound to the input
+and output tensors it was captured with: the plan keeps references to them
+(so their memory is not handed to other tensors while the graph can still
+run) until the next ``capture`` or ``release_graph_buffers``.
 """
 
 from __future__ import annotations
@@ -41,6 +44,7 @@ class BatchedMVP:
         self.offsets: List[int] = [int(L.tcp_batch_plan_offset(out, p)) for p in range(len(self.operators))]
         self.total_amplitudes = int(L.tcp_batch_plan_total_amplitudes(out))
         self.work = float(L.tcp_batch_plan_work(out))  # term * amp * rhs per apply
+        self._graph_buffers: Any = None
 
     def __del__(self) -> None:
         if getattr(self, "_ptr", None) and getattr(_lib, "_lib", None) is not None:
@@ -95,11 +99,23 @@ class BatchedMVP:
         return {"times_ms": times.tolist(), "chosen": int(chosen.value), "configs": self.configs()}
 
     def capture(self, inp: Any, out: Any, stream: Any = None) -> None:
+        """Record one apply of `inp` -> `out` into a hipGraph; `replay` reruns it on these tensors."""
         code = self._check(inp, out)
         handle = int(stream.cuda_stream) if stream is not None else 0
         _lib.check(_lib.lib().tcp_batch_plan_capture(self._ptr, inp.data_ptr(), out.data_ptr(), code,
                                                      handle or None))
+        self._graph_buffers = (inp, out)
 
     def replay(self, stream: Any = None) -> None:
+        """Launch the captured graph on `stream` (reads and writes the captured tensors)."""
+        if self._graph_buffers is None:
+            raise ValueError("no captured graph")
         handle = int(stream.cuda_stream) if stream is not None else 0
         _lib.check(_lib.lib().tcp_batch_plan_replay(self._ptr, handle or None))
+        if stream is not None:
+            for tensor in self._graph_buffers:  # the allocator waits for this stream before reusing them
+                tensor.record_stream(stream)
+
+    def release_graph_buffers(self) -> None:
+        """Drop the references to the captured tensors; replay is refused until the next capture."""
+        self._graph_buffers = None

@@ -660,6 +660,10 @@ int davidson(const Plan *plan, const tcp_davidson_options *opt, void *evec, tcp_
         return fail(TCP_ERR_INVALID_ARGUMENT, "max_subspace must be at most " + std::to_string(kMaxBasis));
     *res = tcp_davidson_result{};
     if (tcp::plan_dimension(plan) < 2) return fail(TCP_ERR_INVALID_ARGUMENT, "dimension must be at least 2");
+    if (opt->log_multiplier_fn && opt->log_multiplier_len != tcp::plan_dimension(plan))
+        return fail(TCP_ERR_INVALID_STRUCTURE_LENGTH,
+                    "expected structure length " + std::to_string(tcp::plan_dimension(plan)) + ", got " +
+                        std::to_string(opt->log_multiplier_len) + " (learned preconditioner multipliers)");
     tcp::DeviceGuard dg;
     int st = dg.enter(tcp::plan_device(plan));
     if (st) return st;

@@ -36,6 +36,7 @@ class DavidsonOptions(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("log_multiplier_fn", ctypes.c_void_p),
         ("log_multiplier_ctx", ctypes.c_void_p),
+        ("log_multiplier_len", ctypes.c_uint64),
         ("trace", ctypes.POINTER(ctypes.c_double)),
         ("trace_capacity", ctypes.c_uint32),
         ("stream", ctypes.c_void_p),
@@ -121,7 +122,11 @@ def lowest_eigenpair(plan: Any, *, tol: float = 1e-8, mvp_dtype: str = "complex1
     opt.trace = trace.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     opt.trace_capacity = trace_capacity
     if preconditioner is not None:
+        if int(preconditioner.dimension) != int(inner.dimension):
+            raise ValueError(f"expected structure length {inner.dimension}, got {preconditioner.dimension} "
+                             "(the preconditioner was built for another plan)")
         opt.log_multiplier_fn, opt.log_multiplier_ctx = preconditioner.callback()
+        opt.log_multiplier_len = int(preconditioner.dimension)
     vector = None
     vec_ptr = None
     if return_vector:

@@ -191,6 +191,22 @@ def gaussian_state(dimension: int, seed: int, dtype=np.complex128) -> np.ndarray
     return v
 
 
+def batched_random_sums(seed: int = BASE_SEED + 2, count: int = 512, qubits=(10, 22),
+                        terms=(64, 4096)) -> List[Terms]:
+    """`count` independent random Pauli sums for one batched launch: qubit counts
+    uniform in qubits[0]..qubits[1] and raw term counts uniform in
+    terms[0]..terms[1] (inclusive), both drawn from ``default_rng(seed)`` in
+    that order per sum; sum i is then ``random_pauli_sum(n, t, seed + 9998 + i)``
+    (the default seed gives the per-sum seeds BASE_SEED + 10000 + i)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(qubits[0], qubits[1] + 1))
+        t = int(rng.integers(terms[0], terms[1] + 1))
+        out.append(random_pauli_sum(n, t, seed=seed + 9998 + i))
+    return out
+
+
 def hubbard_ladder_terms(rows: int, cols: int, hopping: float, interaction: float):
     """Raw ladder terms of the spinful Fermi-Hubbard model, up modes 0..S-1 and
     down modes S..2S-1, open bonds, U n_up n_dn on every site and -t on every

@@ -232,3 +232,31 @@ def test_bench_refuses_more_than_one_gpu():
                           text=True, cwd=root, timeout=120)
     assert proc.returncode == 2
     assert "one GPU" in json.loads(proc.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_davidson_rejects_a_preconditioner_of_another_dimension(monkeypatch):
+    # The learned preconditioner's callback returns a bare float buffer; a hook
+    # built for another plan must be refused before the solver launches
+    # anything (and the ABI carries the buffer length for the same check).
+    from tencirpauli_amd import davidson
+
+    class Inner:
+        dimension = 924
+
+    calls = []
+    monkeypatch.setattr(davidson, "_plan_handle",
+                        lambda plan: (None, lambda *a: calls.append(a) or 0, None, Inner()))
+
+    class Hook:
+        dimension = 495
+
+        def callback(self):  # pragma: no cover - must not be reached
+            raise AssertionError("callback requested for a mismatched plan")
+
+    with pytest.raises(ValueError, match="expected structure length 924, got 495"):
+        davidson.lowest_eigenpair(object(), preconditioner=Hook())
+    assert not calls
+    names = [f[0] for f in davidson.DavidsonOptions._fields_]
+    assert names.index("log_multiplier_len") == names.index("log_multiplier_ctx") + 1
+    header = open(os.path.join(ROOT, "include", "tencirpauli_amd.h")).read()
+    assert re.search(r"void \*log_multiplier_ctx;\s*uint64_t log_multiplier_len;", header)

@@ -156,3 +156,24 @@ def test_davidson_small_subspace_restarts_and_limits():
     assert res32.converged and abs(res32.eigenvalue - exact) <= 1e-9
     with pytest.raises(ValueError, match="max_subspace"):
         davidson.lowest_eigenpair(restricted, max_subspace=33)
+
+
+def test_abi_rejects_multiplier_buffer_of_another_length():
+    # tcp_davidson_* checks log_multiplier_len against the plan's dimension
+    # before any launch; the callback is never invoked.
+    import ctypes
+
+    from tencirpauli_amd import _lib
+
+    restricted = _heisenberg(2, 4).restrict_charge(tcp.U1Sector(8, 4))
+    plan = restricted.mvp_plan()
+    called = []
+    fn = davidson.LOG_MULTIPLIER_FN(lambda ctx, theta, stream: called.append(theta) or None)
+    opt = davidson.DavidsonOptions()
+    _lib.lib().tcp_davidson_default_options(ctypes.byref(opt))
+    opt.log_multiplier_fn = ctypes.cast(fn, ctypes.c_void_p).value
+    opt.log_multiplier_len = plan.dimension - 1
+    res = davidson.DavidsonResultStruct()
+    status = _lib.lib().tcp_davidson_sector(plan._ptr, ctypes.byref(opt), None, ctypes.byref(res))
+    assert status == _lib.TCP_ERR_INVALID_STRUCTURE_LENGTH
+    assert not called
