@@ -5,12 +5,24 @@ Jxy ~ U[0.5, 1.5]) with a random Z field h ~ U[-1, 1] and 1,885 random
 weight-1..4 terms (N(0, 0.1)), 2,048 raw terms, seed 20260801; the state is
 2^28 complex128 Gaussian amplitudes normalised to 1, generated on the GPU.
 One step is one y = H v through the library's C ABI with v and y resident in
-HBM.  ``value`` is the grouped-gather algorithmic bytes of one MVP divided by
-its time (GB/s); term*amp/s is reported beside it.
+HBM.
 
-Printed as one JSON line with ``roofline`` (HBM bound, 8 TB/s peak) and
-``cpu_baseline`` (the C oracle's TermDirect row gather -- the reference's
-default lazy algorithm -- on a bounded row sample, all host threads).
+``value`` is the HBM bandwidth one MVP draws: the kernel's HBM bytes per
+launch, read by rocprofv3 counters (FETCH_SIZE and WRITE_SIZE, each scaled by
+its reading for a plain device copy of known size) and recorded in
+profiles/mvp_tile_28q_traffic.json together with the plan signature and a
+hash of the kernel sources, divided by the kernel time measured here.  The
+record is used only when its signature matches the plan and sources that ran;
+otherwise ``value`` falls back to the kernel's own tile-load bytes and says
+so.  term*amp/s is reported beside it, and the grouped-gather byte model (one
+full read of v per x-group) under ``model``.
+
+``cpu_baseline``: the C oracle's TermDirect row gather -- the reference's
+default lazy algorithm -- on a bounded row sample, on every host core the
+process may use, with a one-thread control.
+
+``python bench.py --record-traffic DIR ...`` writes the traffic record from
+rocprofv3 counter CSVs (see ``record_traffic``).
 """
 
 from __future__ import annotations
@@ -46,10 +58,34 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run one plain device copy (y <- v, 2^n amplitudes each way) before the MVPs, "
                         "the known-size reference for HBM counter scaling")
-    p.add_argument("--extras", default="fp32,sector,eager,wide,hubbard_half,davidson,eigsh",
-                   help="comma list of extra sections: fp32, sector, eager, wide, hubbard_half, davidson, "
-                        "eigsh, batched ('' for none)")
+    p.add_argument("--extras", default="fp32,lattice1000,sector,eager,wide,hubbard_half,davidson,eigsh,batched",
+                   help="comma list of extra sections: fp32, lattice1000, sector, eager, wide, hubbard_half, "
+                        "davidson, eigsh, batched ('' for none)")
+    p.add_argument("--json-out", default=None, help="also write the JSON line to this file")
+    p.add_argument("--record-traffic", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "DTYPE"),
+                   help="write the traffic record for DTYPE from two rocprofv3 --pmc runs of this bench "
+                        "(FETCH_SIZE, then WRITE_SIZE, each with --calibrate-copy) and exit")
     return p.parse_args()
+
+
+def host_cores() -> dict:
+    """Host CPUs this process may run on: the affinity mask, capped by a cgroup
+    CPU quota when one is set (the rest of the machine is not ours)."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q, period = fh.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(q) // int(period))
+    except (OSError, ValueError):
+        quota = None
+    threads = affinity if quota is None else min(affinity, quota)
+    return {"nproc": nproc, "affinity": affinity, "cgroup_cpu_quota": quota, "threads": threads}
 
 
 def build_operator(args):
@@ -65,34 +101,163 @@ def build_operator(args):
     return tcp.PauliOperator.from_code_arrays(codes, coef), name, int(codes.shape[0])
 
 
-def cpu_baseline(op, v_host: np.ndarray, y_dev_host_rows, args, bytes_alg: int, dim: int):
-    """Time the oracle's TermDirect gather on a contiguous row sample."""
+def cpu_baseline(op, v_host: np.ndarray, y_dev_host_rows, args, dim: int):
+    """Time the oracle's TermDirect gather (core/hamiltonian.rs:229-247) on a
+    contiguous row sample, on every usable host core and on one thread."""
     import oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    cores = host_cores()
+    threads = cores["threads"]
     codes, coef = op.code_arrays()
     masks = oracle.matrix_terms(op.nqubits, codes, coef)
-    probe = np.arange(4096, dtype=np.uint64)
-    t0 = time.perf_counter()
-    oracle.termdirect_rows(op.nqubits, codes, coef, v_host, probe, nthreads=threads, masks=masks)
-    rate = 4096 / max(time.perf_counter() - t0, 1e-6)
-    nrows = int(min(dim, max(4096, rate * args.cpu_seconds)))
-    rows = np.arange(nrows, dtype=np.uint64)
-    t0 = time.perf_counter()
-    y_rows = oracle.termdirect_rows(op.nqubits, codes, coef, v_host, rows, nthreads=threads, masks=masks)
-    elapsed = time.perf_counter() - t0
+
+    def timed(nrows: int, nthreads: int):
+        rows = np.arange(nrows, dtype=np.uint64)
+        t0 = time.perf_counter()
+        out = oracle.termdirect_rows(op.nqubits, codes, coef, v_host, rows, nthreads=nthreads, masks=masks)
+        return out, time.perf_counter() - t0
+
+    _, probe_s = timed(4096, threads)
+    nrows = int(min(dim, max(4096, 4096 / max(probe_s, 1e-6) * args.cpu_seconds)))
+    y_rows, elapsed = timed(nrows, threads)
     full_seconds = elapsed * dim / nrows
+    rows1 = int(min(dim, max(1024, nrows // max(threads, 1) // 4)))
+    _, elapsed1 = timed(rows1, 1)
+    full_seconds1 = elapsed1 * dim / rows1
     gpu_rows = y_dev_host_rows(nrows)
     err = float(np.linalg.norm(gpu_rows - y_rows) / max(np.linalg.norm(y_rows), 1e-300))
+    term_amp = op.term_count * dim
     return {
-        "value": round(bytes_alg / full_seconds / 1e9, 3),
-        "unit": "GB/s",
+        "value": round(term_amp / full_seconds, 1),
+        "unit": "term*amp/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"TermDirect rows 0..{nrows - 1} of {dim} ({elapsed:.1f} s, extrapolated to one full MVP: "
-                  f"{full_seconds:.1f} s)",
-        "term_amp_per_s": round(op.term_count * nrows / elapsed, 1),
+        "sample": f"TermDirect rows 0..{nrows - 1} of {dim} on {threads} threads ({elapsed:.1f} s; one full MVP "
+                  f"extrapolates to {full_seconds:.1f} s)",
+        "seconds_per_mvp": round(full_seconds, 2),
+        "host": cores,
+        "one_thread_control": {"value": round(term_amp / full_seconds1, 1), "unit": "term*amp/s",
+                               "seconds_per_mvp": round(full_seconds1, 1),
+                               "sample": f"rows 0..{rows1 - 1} on 1 thread ({elapsed1:.1f} s)"},
     }, {"rows": nrows, "rel_err": err}
+
+
+KERNEL_SOURCES = ("tencirpauli_amd/csrc/mvp_plan.hip", "tencirpauli_amd/csrc/tile_tables.hpp",
+                  "tencirpauli_amd/csrc/tcp_internal.hpp")
+
+
+def kernel_sha() -> str:
+    """Hash of the sources that define the full-space MVP kernel and its tables."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def plan_signature(plan, tag: str, name: str) -> dict:
+    """What a counter record must match to describe this run: the workload,
+    the plan's term/group/class counts and tiling for `tag`, and the kernel
+    sources."""
+    c64 = tag == "complex64"
+    return {"workload": name, "nqubits": plan.nqubits, "terms": plan.term_count, "x_groups": plan.group_count,
+            "classes": plan.class_count_c64 if c64 else plan.class_count,
+            "tiling": plan.tiling_c64 if c64 else plan.tiling, "kernel_sha": kernel_sha()}
+
+
+def load_traffic(path: str, tag: str, signature: dict):
+    """The recorded HBM traffic of `tag` if its signature equals `signature`, else None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        rec = json.load(fh).get(tag)
+    if not rec or rec.get("signature") != signature:
+        return None
+    return rec
+
+
+def _counter_rows(directory: str):
+    """Rows of every rocprofv3 counter-collection CSV under `directory`."""
+    import csv
+    import glob
+
+    rows = []
+    for fname in sorted(glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True)):
+        with open(fname, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _per_dispatch(rows, counter: str, kernel_substring: str, largest: bool = False):
+    """Mean of `counter` per dispatch (summed over its dimensions) over the
+    dispatches whose kernel name contains `kernel_substring`, or with
+    `largest` the largest such dispatch (the calibration copy among the
+    runtime's small staging copies)."""
+    totals = {}
+    for r in rows:
+        if r.get("Counter_Name") == counter and kernel_substring in r.get("Kernel_Name", ""):
+            key = r.get("Dispatch_Id") or r.get("Correlation_Id")
+            totals[key] = totals.get(key, 0.0) + float(r["Counter_Value"])
+    if not totals:
+        raise SystemExit(f"no {counter} rows for kernels matching {kernel_substring!r}")
+    if largest:
+        return max(totals.values()), 1
+    return sum(totals.values()) / len(totals), len(totals)
+
+
+def record_traffic(fetch_dir: str, write_dir: str, tag: str, out_path: str) -> dict:
+    """Traffic record of the MVP kernel for `tag` from two rocprofv3 --pmc runs
+    of this bench with --calibrate-copy and --json-out DIR/bench.json:
+    FETCH_SIZE in `fetch_dir`, WRITE_SIZE in `write_dir` (KiB per dispatch).
+    Each counter is scaled by its reading for the plain device copy y <- v
+    (__amd_rocclr_copyBuffer, 2^n amplitudes read and written), so the record
+    is in bytes.  The signature is taken from the profiled run's JSON line."""
+    with open(os.path.join(fetch_dir, "bench.json")) as fh:
+        run = json.loads(fh.read().strip().splitlines()[-1])
+    cfg = run["config"]
+    if cfg["amplitudes"] != tag:
+        raise SystemExit(f"the profiled run used {cfg['amplitudes']}, not {tag}")
+    s = 16 if tag == "complex128" else 8
+    copy_bytes = (1 << cfg["nqubits"]) * s
+    fetch_rows, write_rows = _counter_rows(fetch_dir), _counter_rows(write_dir)
+    fetch_kib, nf = _per_dispatch(fetch_rows, "FETCH_SIZE", "mvp_tile_kernel")
+    write_kib, nw = _per_dispatch(write_rows, "WRITE_SIZE", "mvp_tile_kernel")
+    copy_fetch, _ = _per_dispatch(fetch_rows, "FETCH_SIZE", "copyBuffer", largest=True)
+    copy_write, _ = _per_dispatch(write_rows, "WRITE_SIZE", "copyBuffer", largest=True)
+    read_scale = copy_bytes / (copy_fetch * 1024.0)
+    write_scale = copy_bytes / (copy_write * 1024.0)
+    read_bytes = fetch_kib * 1024.0 * read_scale
+    write_bytes = write_kib * 1024.0 * write_scale
+    rec = {
+        "signature": {"workload": cfg["workload"], "nqubits": cfg["nqubits"], "terms": cfg["terms"],
+                      "x_groups": cfg["x_groups"], "classes": cfg["tile_classes"], "tiling": cfg["tiling"],
+                      "kernel_sha": cfg["kernel_sha"]},
+        "hbm_bytes_per_launch": int(round(read_bytes + write_bytes)),
+        "read_bytes": int(round(read_bytes)),
+        "write_bytes": int(round(write_bytes)),
+        "fetch_kib_raw": fetch_kib, "write_kib_raw": write_kib, "dispatches": [nf, nw],
+        "copy_bytes": copy_bytes, "copy_fetch_kib": copy_fetch, "copy_write_kib": copy_write,
+        "read_scale": read_scale, "write_scale": write_scale,
+        "kernel_ms_in_profiled_run": run["ms_per_step"],
+    }
+    data = {}
+    if os.path.exists(out_path):
+        with open(out_path) as fh:
+            data = json.load(fh)
+    old = data.get(tag, {})
+    if old.get("signature") == rec["signature"] and "limiter" in old:
+        rec["limiter"] = old["limiter"]
+    data.setdefault("kernel", "mvp_tile_kernel")
+    data.setdefault("method", "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE (KiB per dispatch), each scaled by "
+                              "the same counter's reading for a plain device copy of known size; written by "
+                              "python bench.py --record-traffic")
+    data[tag] = rec
+    with open(out_path, "w") as fh:
+        json.dump(data, fh, indent=1)
+        fh.write("\n")
+    return rec
 
 
 def tile_model(plan, tag: str, dim: int, ms: float) -> dict:
@@ -136,6 +301,31 @@ def extra_fp32(plan, v, dim) -> dict:
     return {"ms_per_mvp": round(ms, 4), "bytes_alg_GBps": round(alg / (ms * 1e-3) / 1e9, 2),
             "term_amp_per_s": round(plan.term_count * dim / (ms * 1e-3), 1), "rel_err_vs_fp64": err,
             "tile_model": tile_model(plan, "complex64", dim, ms)}
+
+
+def extra_lattice1000() -> dict:
+    """The 28-qubit workload as written: the 4 x 7 lattice with 1,000 random
+    weight-1..4 terms (1,163 raw terms), complex128 and complex64 storage."""
+    import torch
+
+    import tencirpauli_amd as tcp
+    from tencirpauli_amd import lattice
+
+    codes, coef = lattice.heisenberg_4x7_with_random_terms(SEED, 1000)
+    plan = tcp.PauliOperator.from_code_arrays(codes, coef).native_mvp_plan()
+    dim = 1 << plan.nqubits
+    gen = torch.Generator(device="cuda").manual_seed(SEED + 1)
+    v = torch.randn(dim, dtype=torch.complex128, device="cuda", generator=gen)
+    v /= torch.linalg.vector_norm(v)
+    rec = {"raw_terms": int(codes.shape[0]), "terms": plan.term_count, "x_groups": plan.group_count}
+    for tag, dt in (("complex128", torch.complex128), ("complex64", torch.complex64)):
+        vv = v.to(dt)
+        yy = torch.empty_like(vv)
+        ms = time_applies(lambda: plan.apply_into(vv, yy), 5)
+        rec[tag] = {"ms_per_mvp": round(ms, 3), "term_amp_per_s": round(plan.term_count * dim / (ms * 1e-3), 1),
+                    "tile_model": tile_model(plan, tag, dim, ms)}
+        del vv, yy
+    return rec
 
 
 def sample_cpu(run, total: int, seconds: float) -> dict:
@@ -292,7 +482,7 @@ def extra_eager(cpu_seconds: float = 0.0) -> dict:
 
         indptr, indices, values = eager.mvp_plan(storage="eager").csr_arrays()
         vh = v.cpu().numpy()
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_cores()["threads"]
         smp = sample_cpu(lambda rng: oracle.csr_apply(indptr, indices, values, vh, rows=rng, nthreads=threads),
                          dim, cpu_seconds)
         rec["cpu_baseline"] = {"value_ms_per_mvp": round(1e3 * smp["full_apply_s"], 2), "cores": threads,
@@ -388,8 +578,14 @@ def extra_davidson() -> dict:
     for target in targets.values():  # warm-up (allocations, code load)
         davidson.lowest_eigenpair(target, tol=1e-3, max_iterations=3)
         davidson.lowest_eigenpair(target, tol=1e-3, max_iterations=3, mvp_dtype="complex64", preconditioner=hook)
-    e_ref = davidson.lowest_eigenpair(targets["lazy"], tol=1e-9).eigenvalue  # FP64 Jacobi to ||r|| <= 1e-9
-    out = {"energy_fp64": e_ref}
+    # The FP64 energy every run is judged against: SciPy eigsh (tol 1e-12) over the oracle's U(1) scatter,
+    # committed with its residual (tests/golden/make_ground_energy.py); the GPU's own FP64 Jacobi energy beside it.
+    with open(os.path.join(ROOT, "tests", "golden", "ground_energy_xxz5x5.json")) as fh:
+        fixture = json.load(fh)
+    e_ref = float(fixture["energy"])
+    e_gpu = davidson.lowest_eigenpair(targets["lazy"], tol=1e-9).eigenvalue
+    out = {"energy_fp64": e_ref, "energy_source": "tests/golden/ground_energy_xxz5x5.json (" + fixture["method"] + ")",
+           "gpu_jacobi_fp64_energy": e_gpu, "gpu_jacobi_abs_dE_vs_fixture": abs(e_gpu - e_ref)}
     # The FP32 phase stops at ||r|| <= 8e-5 and one FP64 MVP gives the FP64 Rayleigh quotient, accepted at
     # ||r_64|| <= 2e-4: late in the Jacobi run dE ~ ||r||^2 / 15 on this lattice, so that lands near 3e-10
     # (abs_dE below reports the achieved value against the FP64 energy).
@@ -489,13 +685,7 @@ def extra_batched() -> dict:
     from tencirpauli_amd import lattice
     from tencirpauli_amd.batched import BatchedMVP
 
-    rng = np.random.default_rng(SEED + 2)
-    ops = []
-    for i in range(512):
-        n = int(rng.integers(10, 23))
-        t = int(rng.integers(64, 4097))
-        codes, coef = lattice.random_pauli_sum(n, t, seed=SEED + 10000 + i)
-        ops.append(tcp.PauliOperator.from_code_arrays(codes, coef))
+    ops = [tcp.PauliOperator.from_code_arrays(codes, coef) for codes, coef in lattice.batched_random_sums()]
     t0 = time.perf_counter()
     batch = BatchedMVP(ops, nrhs=8)
     build_s = time.perf_counter() - t0
@@ -521,6 +711,10 @@ def extra_batched() -> dict:
 
 def main() -> int:
     args = parse_args()
+    if args.record_traffic:
+        fetch_dir, write_dir, tag = args.record_traffic
+        print(json.dumps(record_traffic(fetch_dir, write_dir, tag, args.traffic_json)), flush=True)
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "the MVP path is measured on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding of the state vector is not implemented)"}))
@@ -561,12 +755,20 @@ def main() -> int:
 
     tag = "complex64" if cdtype == torch.complex64 else "complex128"
     bytes_alg = plan.bytes_alg(tag)
-    achieved = bytes_alg / (ms_per_step * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(args.traffic_json):
-        with open(args.traffic_json) as fh:
-            recorded = json.load(fh)
-        traffic = recorded.get(tag, {}).get("hbm_bytes_per_launch")
+    tile = tile_model(plan, tag, dim, ms_per_step)
+    signature = plan_signature(plan, tag, name)
+    traffic_rec = load_traffic(args.traffic_json, tag, signature)
+    if traffic_rec is not None:
+        traffic = int(traffic_rec["hbm_bytes_per_launch"])
+        hbm_bytes = traffic
+        basis = ("HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (copy-calibrated), "
+                 f"{os.path.relpath(args.traffic_json, ROOT)}, signature matched")
+    else:
+        traffic = None
+        hbm_bytes = tile["bytes"]
+        basis = ("no counter record matches this plan and kernel source: the kernel's tile-load bytes "
+                 "2^n (classes s_in + s_out) stand in")
+    achieved = hbm_bytes / (ms_per_step * 1e-3) / 1e9
     record = {
         "metric": METRIC,
         "value": round(achieved, 2),
@@ -586,9 +788,10 @@ def main() -> int:
             "raw_terms": raw_terms,
             "terms": plan.term_count,
             "x_groups": plan.group_count,
-            "tile_classes": plan.class_count,
+            "tile_classes": signature["classes"],
+            "tiling": signature["tiling"],
             "amplitudes": tag,
-            "bytes_alg": bytes_alg,
+            "kernel_sha": signature["kernel_sha"],
         },
         "term_amp_per_s": round(plan.term_count * dim / (ms_per_step * 1e-3), 1),
         "plan_build_s": round(plan_seconds, 4),
@@ -599,11 +802,17 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "basis": basis,
+            "limiter": None if traffic_rec is None else traffic_rec.get("limiter"),
         },
-        # The grouped-gather model above charges one full read of v per x-group;
-        # the tile kernel reads one source tile per (output tile, class), so its
-        # own minimum HBM traffic is 2^n (classes * s_in + s_out).
-        "tile_model": tile_model(plan, tag, dim, ms_per_step),
+        # The kernel's own tile loads (one source tile per output tile and
+        # class, through L2), and the grouped-gather model that charges one
+        # full read of v per x-group -- neither is HBM traffic.
+        "tile_model": tile,
+        "model": {"grouped_gather_bytes": bytes_alg,
+                  "grouped_gather_GBps": round(bytes_alg / (ms_per_step * 1e-3) / 1e9, 1),
+                  "note": "2^n (G s_in + s_out) + T (16 W + s): one read of v per x-group; the tile kernel "
+                          "reads one source tile per class instead, so this is a work model, not traffic"},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
@@ -613,12 +822,18 @@ def main() -> int:
         def gpu_rows(count: int) -> np.ndarray:
             return y_full[:count].cpu().numpy()
 
-        record["cpu_baseline"], record["parity"] = cpu_baseline(op, v_host, gpu_rows, args, bytes_alg, dim)
+        record["cpu_baseline"], record["parity"] = cpu_baseline(op, v_host, gpu_rows, args, dim)
+        record["cpu_baseline"]["gpu_speedup"] = round(record["term_amp_per_s"] / record["cpu_baseline"]["value"], 1)
         del v_host
     extras = [e for e in args.extras.split(",") if e]
     if "fp32" in extras and tag == "complex128":
         record["fp32_storage"] = extra_fp32(plan, v, dim)
     del y
+    if "lattice1000" in extras and args.nqubits == 28:
+        del v, plan
+        torch.cuda.empty_cache()
+        record["lattice_4x7_1000_random_terms"] = extra_lattice1000()
+        v = plan = None
     if any(e in extras for e in ("sector", "eager", "wide", "hubbard_half", "davidson", "eigsh", "batched")):
         del v, plan
         torch.cuda.empty_cache()
@@ -638,7 +853,12 @@ def main() -> int:
         record["eigsh_4x5_xxz"] = extra_eigsh()
     if "batched" in extras:
         record["batched_512x8"] = extra_batched()
-    print(json.dumps(record), flush=True)
+    line = json.dumps(record)
+    if args.json_out:
+        os.makedirs(os.path.dirname(os.path.abspath(args.json_out)), exist_ok=True)
+        with open(args.json_out, "w") as fh:
+            fh.write(line + "\n")
+    print(line, flush=True)
     return 0
 
 

@@ -42,8 +42,12 @@ TCP_TILE_512x8 = 1
 TCP_TILE_1024x8 = 2
 TCP_TILE_512x16 = 3
 TCP_TILE_512x4 = 4
+TCP_TILE_1024x4 = 5
 TCP_TILE_XCD_ORDER = 0x100
 TCP_TILE_LINEAR_ORDER = 0x200
+TCP_TILE_PREFETCH_SHIFT = 10
+TCP_TILE_PREFETCH_MASK = 0xC00
+TCP_TILE_NT_LOADS = 0x1000
 
 # Every exported entry point: (name, restype, argtypes).
 _c = ctypes

@@ -52,6 +52,7 @@ __global__ __launch_bounds__(TPB) void batch_kernel(const VT *__restrict__ v, VT
                                                     TileArgs shared, uint64_t total_items,
                                                     unsigned int *__restrict__ counter) {
     __shared__ dvec2 lds[TPB * A];
+    __shared__ tcp::u32x4 lblk[tcp::block_words(TPB) + 2];
     __shared__ unsigned int item_s;
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(counter, 1u);
@@ -71,17 +72,16 @@ __global__ __launch_bounds__(TPB) void batch_kernel(const VT *__restrict__ v, VT
         const uint64_t tile_id = local - rhs * P.tiles;
         TileArgs a = shared;
         a.cls += P.cls_base;
-        a.zoff += static_cast<uint64_t>(P.cls_base) * tcp::zoff_stride(A);
         a.H = P.H;
         a.L = P.L;
         const uint64_t off = P.in_off + rhs * P.dim;
-        tcp::tile_apply<A, VT, TPB, true>(v + off, y + off, a, tile_id, P.active, lds);
+        tcp::tile_apply<A, VT, TPB, true>(v + off, y + off, a, tile_id, P.active, lds, lblk);
     }
 }
 
 struct Tables {
     bool built = false;
-    tcp::DeviceBuffer cls, zoff, grp64, grp32, term64, term32, probs;
+    tcp::DeviceBuffer cls, blk64, blk32, grp64, grp32, term64, term32, probs;
     uint64_t total_items = 0;
 };
 
@@ -163,7 +163,8 @@ int build_tables(tcp_batch_plan *plan, uint32_t cfg) {
     tab.pad_empty();
     int st;
     if ((st = tb.cls.upload(tab.cls.data(), sizeof(tcp::TileClass) * tab.cls.size())) ||
-        (st = tb.zoff.upload(tab.zoff.data(), sizeof(uint32_t) * tab.zoff.size())) ||
+        (st = tb.blk64.upload(tab.blk64.data(), sizeof(tcp::u32x4) * tab.blk64.size())) ||
+        (st = tb.blk32.upload(tab.blk32.data(), sizeof(tcp::u32x4) * tab.blk32.size())) ||
         (st = tb.grp64.upload(tab.grp64.data(), sizeof(tcp::TileGroup) * tab.grp64.size())) ||
         (st = tb.grp32.upload(tab.grp32.data(), sizeof(tcp::TileGroup) * tab.grp32.size())) ||
         (st = tb.term64.upload(tab.term64.data(), sizeof(tcp::TileTerm) * tab.term64.size())) ||
@@ -189,12 +190,15 @@ void launch_cfg(const tcp_batch_plan *plan, uint32_t cfg, const void *in, void *
     const Tables &tb = plan->tables[cfg];
     TileArgs a;
     a.cls = static_cast<const tcp::TileClass *>(tb.cls.ptr);
-    a.zoff = static_cast<const uint32_t *>(tb.zoff.ptr);
+    a.blk = static_cast<const tcp::u32x4 *>(dtype == TCP_C64 ? tb.blk32.ptr : tb.blk64.ptr);
     a.xorder = nullptr;
     a.grp = static_cast<const tcp::TileGroup *>(dtype == TCP_C64 ? tb.grp32.ptr : tb.grp64.ptr);
     a.term = static_cast<const tcp::TileTerm *>(dtype == TCP_C64 ? tb.term32.ptr : tb.term64.ptr);
     a.H = 0;
     a.L = 0;
+    a.n = 0;
+    a.permuted = 0;
+    for (int c = 0; c < 64; ++c) a.pos[c] = static_cast<uint8_t>(c);
     const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(plan->grid[cfg], tb.total_items));
     hipLaunchKernelGGL((batch_kernel<A, VT, TPB>), dim3(grid), dim3(TPB), 0, s, static_cast<const VT *>(in),
                        static_cast<VT *>(out), static_cast<const BatchProblem *>(tb.probs.ptr),

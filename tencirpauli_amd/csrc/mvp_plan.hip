@@ -39,12 +39,21 @@ using MvpArgs = tcp::TileArgs;
 struct Geometry {
     uint32_t threads, amps;
 };
-constexpr Geometry kGeometries[] = {{0, 0}, {512, 8}, {1024, 8}, {512, 16}, {512, 4}};
+constexpr Geometry kGeometries[] = {{0, 0}, {512, 8}, {1024, 8}, {512, 16}, {512, 4}, {1024, 4}};
 constexpr uint32_t kGeometryCount = sizeof(kGeometries) / sizeof(kGeometries[0]);
-constexpr uint32_t kSmallThreads = 512;  // below 2^12 amplitudes: 512 threads, 1-4 amplitudes each
+constexpr uint32_t kSmallThreads = 512;
+// Host-pointer applies stage through two device buffers; up to this size they
+// stay cached on the plan between calls (and count in its retained bytes),
+// larger ones are freed after each call.
+constexpr size_t kStagingKeepBytes = size_t(64) << 20;  // below 2^12 amplitudes: 512 threads, 1-4 amplitudes each
 constexpr uint32_t kSmallBits = 9;
 
 uint32_t log2u(uint32_t x) { return 31u - static_cast<uint32_t>(__builtin_clz(x)); }
+
+// Source tiles in flight per workgroup encoded in a tiling word (field 0 = one).
+constexpr uint32_t tile_prefetch_depth(uint32_t tiling) {
+    return ((tiling & TCP_TILE_PREFETCH_MASK) >> TCP_TILE_PREFETCH_SHIFT) + 1u;
+}
 
 // One workgroup per output tile and right-hand side (blockIdx.y).  With
 // `xcd_order` the tile of workgroup b is (b mod 8) * (tiles / 8) + b / 8:
@@ -52,20 +61,27 @@ uint32_t log2u(uint32_t x) { return 31u - static_cast<uint32_t>(__builtin_clz(x)
 // contiguous eighth of the tiles and its concurrent workgroups read source
 // tiles that lie close together (shared in that XCD's L2).
 // PARTIAL: the whole space is smaller than one tile; l >= dim idle.
-// Waves per SIMD the register budget is sized for: 2^11-amplitude and
-// smaller tiles run four 512-thread workgroups per CU (8 waves, <= 64 VGPRs);
-// 2^12-amplitude tiles (64 KiB of LDS) two per CU or one 1024-thread
-// workgroup (4 waves, <= 128 VGPRs); 512 x 16 one per CU (2 waves).
-template <int A, uint32_t TPB>
+// Waves per SIMD the register budget is sized for: as many workgroups per CU
+// as their LDS (the FP64 tile, 16 B per amplitude, and the class block, 8 B
+// per thread) fits in the CU's 160 KiB, spread over the 4 SIMDs, at most 8
+// (512 x 8: 4 waves, <= 128 VGPRs; 512 x 4 and 1024 x 4: 8 waves, <= 64
+// VGPRs, 6 for complex128 storage, whose four FP64 accumulators and
+// prefetched amplitudes do not fit in 64; 512 x 16: 2 waves).
+template <int A, uint32_t TPB, typename VT>
 constexpr int waves_per_simd() {
-    return A * TPB <= 2048 ? 8 : (A * TPB <= 4096 || TPB == 1024 ? 4 : 2);
+    constexpr int wgs = (160 * 1024) / ((2 * A + 1) * TPB * 8 + 32);
+    constexpr int w = wgs * static_cast<int>(TPB / 64) / 4;
+    constexpr int cap = (sizeof(VT) == 16 && A == 4) ? 6 : 8;
+    return w < 1 ? 1 : (w > cap ? cap : w);
 }
 
-template <int A, typename VT, uint32_t TPB, bool PARTIAL>
-__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(waves_per_simd<A, TPB>())))
+// PF: source tiles in flight (register ring depth); NT: non-temporal tile loads.
+template <int A, typename VT, uint32_t TPB, bool PARTIAL, int PF = 1, bool NT = false>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(waves_per_simd<A, TPB, VT>())))
 void mvp_tile_kernel(const VT *__restrict__ v, VT *__restrict__ y, MvpArgs p,
                                                        uint64_t dim, uint32_t xcd_order) {
     __shared__ dvec2 lds[TPB * A];
+    __shared__ tcp::u32x4 lblk[tcp::block_words(TPB) + 2];
     const uint64_t rhs = static_cast<uint64_t>(blockIdx.y) * dim;
     uint64_t tile = blockIdx.x;
     if (xcd_order) {
@@ -73,7 +89,7 @@ void mvp_tile_kernel(const VT *__restrict__ v, VT *__restrict__ y, MvpArgs p,
         tile = (tile & 7u) * per + (tile >> 3);
     }
     const uint32_t active = PARTIAL ? static_cast<uint32_t>(dim) : TPB * A;
-    tcp::tile_apply<A, VT, TPB, PARTIAL>(v + rhs, y + rhs, p, tile, active, lds);
+    tcp::tile_apply<A, VT, TPB, PARTIAL, PF, NT, !PARTIAL>(v + rhs, y + rhs, p, tile, active, lds, lblk);
 }
 
 // Fill kernel used instead of hipMemsetAsync (safe under graph capture).
@@ -92,8 +108,9 @@ __global__ void mvp_count_kernel(MvpArgs p, uint32_t tb, uint32_t passes, uint64
     unsigned long long local = 0;
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
          i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        const uint64_t tile_id = i >> p.L;
-        const uint32_t l = static_cast<uint32_t>(i & ((1ull << p.L) - 1));
+        const uint64_t k = tcp::kernel_index(p, i);
+        const uint64_t tile_id = k >> p.L;
+        const uint32_t l = static_cast<uint32_t>(k & ((1ull << p.L) - 1));
         const uint32_t t = l & ((1u << tb) - 1);
         const uint32_t slot = l >> tb;
         // in x-mask order the passes of one x-mask are consecutive (`same`
@@ -116,8 +133,9 @@ __global__ void mvp_diagonal_kernel(MvpArgs p, uint32_t tb, uint32_t diag_passes
                                     double *__restrict__ diag) {
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
          i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        const uint64_t tile_id = i >> p.L;
-        const uint32_t l = static_cast<uint32_t>(i & ((1ull << p.L) - 1));
+        const uint64_t k = tcp::kernel_index(p, i);
+        const uint64_t tile_id = k >> p.L;
+        const uint32_t l = static_cast<uint32_t>(k & ((1ull << p.L) - 1));
         const uint32_t t = l & ((1u << tb) - 1);
         double d = 0.0;
         for (uint32_t i = 0; i < diag_passes; ++i) d += tcp::group_sum(p, p.xorder[i], t, l >> tb, tile_id).x;
@@ -133,13 +151,19 @@ struct PlanTiles {
     uint32_t tb = 0, ab = 0, L = 0;
     uint64_t H = 0, passes = 0;
     uint32_t diag_passes = 0;  // passes with x = 0
-    tcp::DeviceBuffer cls, zoff, xorder, grp64, grp32, term64, term32;
+    uint32_t n = 0;
+    bool permuted = false;     // kernel coordinates differ from the index bits
+    uint8_t pos[64] = {};
+    tcp::DeviceBuffer cls, blk64, blk32, xorder, grp64, grp32, term64, term32;
     uint64_t bytes = 0;
 
     MvpArgs args(int dtype) const {
         MvpArgs a;
+        a.n = n;
+        a.permuted = permuted ? 1u : 0u;
+        for (int c = 0; c < 64; ++c) a.pos[c] = pos[c];
         a.cls = static_cast<const tcp::TileClass *>(cls.ptr);
-        a.zoff = static_cast<const uint32_t *>(zoff.ptr);
+        a.blk = static_cast<const tcp::u32x4 *>(dtype == TCP_C64 ? blk32.ptr : blk64.ptr);
         a.xorder = static_cast<const uint32_t *>(xorder.ptr);
         a.grp = static_cast<const tcp::TileGroup *>(dtype == TCP_C64 ? grp32.ptr : grp64.ptr);
         a.term = static_cast<const tcp::TileTerm *>(dtype == TCP_C64 ? term32.ptr : term64.ptr);
@@ -173,22 +197,27 @@ namespace {
 using tcp::fail;
 
 // Geometry index for `tiling` on an n-qubit space; 0 means the small-space
-// geometry (512 threads, 1-4 amplitudes each).  Automatic: 512 x 8 for
-// complex128 and 512 x 4 for complex64 storage (measured fastest on the
-// 28-qubit lattice workload: FP64 tiles are bounded by HBM traffic, which the
-// larger tile lowers; FP32 ones by LDS and issue latency, which the higher
-// occupancy of the smaller tile hides).
+// geometry (512 threads, 1-4 amplitudes each).  Automatic, measured on the
+// 28-qubit lattice workload: spaces of 2^24 amplitudes or more use 1024 x 8
+// for complex128 (its FP64 source tiles stream through L2 at a fixed rate, so
+// the larger tile's fewer classes win) and 512 x 8 for complex64 storage
+// (bound by the pass loop, which the eight amplitudes per thread amortise);
+// smaller spaces use 512 x 8 and 512 x 4, which give more workgroups.
 uint32_t resolve_geometry(uint32_t n, uint32_t tiling, int dtype) {
     uint32_t g = tiling & 0xffu;
-    if (g == TCP_TILE_AUTO) g = dtype == TCP_C64 ? TCP_TILE_512x4 : TCP_TILE_512x8;
+    if (g == TCP_TILE_AUTO) {
+        if (n >= 24) g = dtype == TCP_C64 ? TCP_TILE_512x8 : TCP_TILE_1024x8;
+        else g = dtype == TCP_C64 ? TCP_TILE_512x4 : TCP_TILE_512x8;
+    }
     const uint32_t L = log2u(kGeometries[g].threads) + log2u(kGeometries[g].amps);
     if (n < L) g = n >= 12 ? TCP_TILE_512x8 : (n >= 11 ? TCP_TILE_512x4 : 0);
     return g;
 }
 
-int build_tiles(const tcp::Operator &op, uint32_t geo, bool xcd, PlanTiles &pt, tcp::TileTables &tab) {
+int build_tiles(const tcp::Operator &op, uint32_t geo, bool xcd, uint32_t pipeline, PlanTiles &pt,
+                tcp::TileTables &tab) {
     const uint32_t n = op.nqubits;
-    pt.tiling = geo | (xcd ? TCP_TILE_XCD_ORDER : 0u);
+    pt.tiling = geo | (xcd ? TCP_TILE_XCD_ORDER : 0u) | pipeline;
     if (geo == 0) {
         pt.tb = std::min(n, kSmallBits);
         pt.ab = n > kSmallBits ? n - kSmallBits : 0;
@@ -197,7 +226,23 @@ int build_tiles(const tcp::Operator &op, uint32_t geo, bool xcd, PlanTiles &pt, 
         pt.ab = log2u(kGeometries[geo].amps);
     }
     pt.L = pt.tb + pt.ab;
-    tab.append(op, pt.tb, pt.ab);
+    pt.n = n;
+    for (uint32_t c = 0; c < 64; ++c) pt.pos[c] = static_cast<uint8_t>(c);
+    if (geo != 0) {
+        // tile bits that vary across the workgroups resident on one XCD (32 CUs)
+        const uint32_t threads = kGeometries[geo].threads, amps = kGeometries[geo].amps;
+        const uint32_t per_cu = std::max<uint32_t>(
+            1, std::min<uint32_t>((160u * 1024u) / ((2 * amps + 1) * threads * 8u + 32u), 2048u / threads));
+        const uint32_t window = log2u(32u * per_cu);
+        uint8_t pos[64];
+        if (tcp::choose_bit_map(op, pt.tb, pt.ab, window, pos)) {
+            for (uint32_t c = 0; c < n; ++c) {
+                pt.pos[c] = pos[c];
+                pt.permuted = pt.permuted || pos[c] != c;
+            }
+        }
+    }
+    tab.append(op, pt.tb, pt.ab, pt.permuted ? pt.pos : nullptr, geo == 0 ? kSmallThreads : kGeometries[geo].threads);
     pt.H = tab.H;
     pt.passes = tab.passes;
     pt.diag_passes = static_cast<uint32_t>(tab.diag_passes);
@@ -209,7 +254,8 @@ int build_tiles(const tcp::Operator &op, uint32_t geo, bool xcd, PlanTiles &pt, 
 int upload_tiles(PlanTiles &pt, const tcp::TileTables &tab) {
     int st;
     if ((st = pt.cls.upload(tab.cls.data(), tab.cls.size() * sizeof(tcp::TileClass)))) return st;
-    if ((st = pt.zoff.upload(tab.zoff.data(), tab.zoff.size() * sizeof(uint32_t)))) return st;
+    if ((st = pt.blk64.upload(tab.blk64.data(), tab.blk64.size() * sizeof(tcp::u32x4)))) return st;
+    if ((st = pt.blk32.upload(tab.blk32.data(), tab.blk32.size() * sizeof(tcp::u32x4)))) return st;
     if ((st = pt.xorder.upload(tab.xorder.data(), tab.xorder.size() * sizeof(uint32_t)))) return st;
     if ((st = pt.grp64.upload(tab.grp64.data(), tab.grp64.size() * sizeof(tcp::TileGroup)))) return st;
     if ((st = pt.grp32.upload(tab.grp32.data(), tab.grp32.size() * sizeof(tcp::TileGroup)))) return st;
@@ -222,7 +268,9 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, uint32_
     *out = nullptr;
     if (storage != TCP_STORAGE_LAZY && storage != TCP_STORAGE_EAGER)
         return fail(TCP_ERR_INVALID_ARGUMENT, "storage must be either 'eager' or 'lazy'");
-    if ((tiling & 0xffu) >= kGeometryCount || (tiling & ~(0xffu | TCP_TILE_XCD_ORDER | TCP_TILE_LINEAR_ORDER)) ||
+    if ((tiling & 0xffu) >= kGeometryCount ||
+        (tiling & ~(0xffu | TCP_TILE_XCD_ORDER | TCP_TILE_LINEAR_ORDER | TCP_TILE_PREFETCH_MASK | TCP_TILE_NT_LOADS)) ||
+        tile_prefetch_depth(tiling) > 3 ||
         ((tiling & TCP_TILE_XCD_ORDER) && (tiling & TCP_TILE_LINEAR_ORDER)))
         return fail(TCP_ERR_INVALID_ARGUMENT, "unknown tile geometry");
     const uint32_t n = op.nqubits;
@@ -240,10 +288,13 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, uint32_
                      ((tiling & 0xffu) == TCP_TILE_AUTO && !(tiling & TCP_TILE_LINEAR_ORDER));
     const uint32_t g128 = resolve_geometry(n, tiling, TCP_C128), g64 = resolve_geometry(n, tiling, TCP_C64);
     tcp::TileTables tab128, tab64;
-    build_tiles(op, g128, xcd, plan->t128, tab128);
+    // prefetch depth and load policy apply to the geometries that have them
+    const uint32_t pipeline = tiling & (TCP_TILE_PREFETCH_MASK | TCP_TILE_NT_LOADS);
+    auto pipelined = [](uint32_t g) { return g == TCP_TILE_512x8 || g == TCP_TILE_512x4 || g == TCP_TILE_1024x4; };
+    build_tiles(op, g128, xcd, pipelined(g128) ? pipeline : 0u, plan->t128, tab128);
     plan->G = tab128.G;
     plan->own64 = g64 != g128;
-    if (plan->own64) build_tiles(op, g64, xcd, plan->t64, tab64);
+    if (plan->own64) build_tiles(op, g64, xcd, pipelined(g64) ? pipeline : 0u, plan->t64, tab64);
 
     plan->retained_bytes = plan->t128.bytes + (plan->own64 ? plan->t64.bytes : 0) + sizeof(tcp_mvp_plan);
     if (tcp::exceeds_budget(plan->retained_bytes, max_bytes))
@@ -273,14 +324,28 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, uint32_
     return tcp::ok();
 }
 
-template <int A, typename VT, uint32_t TPB, bool PARTIAL = false>
+template <int A, typename VT, uint32_t TPB, bool PARTIAL = false, int PF = 1, bool NT = false>
 hipError_t launch_tile(const tcp_mvp_plan *plan, const PlanTiles &pt, const void *in, void *out, uint32_t nrhs,
                        int dtype, hipStream_t stream) {
     const uint64_t tiles = PARTIAL ? 1 : plan->dim() >> pt.L;
-    hipLaunchKernelGGL((mvp_tile_kernel<A, VT, TPB, PARTIAL>), dim3(static_cast<uint32_t>(tiles), nrhs),
+    hipLaunchKernelGGL((mvp_tile_kernel<A, VT, TPB, PARTIAL, PF, NT>), dim3(static_cast<uint32_t>(tiles), nrhs),
                        dim3(TPB), 0, stream, static_cast<const VT *>(in), static_cast<VT *>(out), pt.args(dtype),
                        plan->dim(), plan->xcd_order(pt) ? 1u : 0u);
     return hipGetLastError();
+}
+
+// Geometries with a selectable prefetch depth (1-3 tiles in flight) and load policy.
+template <int A, typename VT, uint32_t TPB>
+hipError_t launch_pipelined(const tcp_mvp_plan *plan, const PlanTiles &pt, const void *in, void *out,
+                            uint32_t nrhs, int dtype, hipStream_t stream) {
+    const uint32_t depth = tile_prefetch_depth(pt.tiling);
+    const bool nt = (pt.tiling & TCP_TILE_NT_LOADS) != 0;
+    if (depth == 3) return nt ? launch_tile<A, VT, TPB, false, 3, true>(plan, pt, in, out, nrhs, dtype, stream)
+                              : launch_tile<A, VT, TPB, false, 3, false>(plan, pt, in, out, nrhs, dtype, stream);
+    if (depth == 2) return nt ? launch_tile<A, VT, TPB, false, 2, true>(plan, pt, in, out, nrhs, dtype, stream)
+                              : launch_tile<A, VT, TPB, false, 2, false>(plan, pt, in, out, nrhs, dtype, stream);
+    return nt ? launch_tile<A, VT, TPB, false, 1, true>(plan, pt, in, out, nrhs, dtype, stream)
+              : launch_tile<A, VT, TPB, false, 1, false>(plan, pt, in, out, nrhs, dtype, stream);
 }
 
 template <typename VT>
@@ -294,10 +359,11 @@ hipError_t launch_typed(const tcp_mvp_plan *plan, const void *in, void *out, uin
         return hipGetLastError();
     }
     switch (pt.tiling & 0xffu) {
-        case TCP_TILE_512x8: return launch_tile<8, VT, 512>(plan, pt, in, out, nrhs, dtype, stream);
+        case TCP_TILE_512x8: return launch_pipelined<8, VT, 512>(plan, pt, in, out, nrhs, dtype, stream);
         case TCP_TILE_1024x8: return launch_tile<8, VT, 1024>(plan, pt, in, out, nrhs, dtype, stream);
         case TCP_TILE_512x16: return launch_tile<16, VT, 512>(plan, pt, in, out, nrhs, dtype, stream);
-        case TCP_TILE_512x4: return launch_tile<4, VT, 512>(plan, pt, in, out, nrhs, dtype, stream);
+        case TCP_TILE_512x4: return launch_pipelined<4, VT, 512>(plan, pt, in, out, nrhs, dtype, stream);
+        case TCP_TILE_1024x4: return launch_pipelined<4, VT, 1024>(plan, pt, in, out, nrhs, dtype, stream);
         default: break;
     }
     if (plan->n < kSmallBits) return launch_tile<1, VT, kSmallThreads, true>(plan, pt, in, out, nrhs, dtype, stream);
@@ -440,8 +506,14 @@ int tcp_mvp_plan_apply_host(const tcp_mvp_plan *plan, const double *input, doubl
     if (mp->stage_in.bytes < bytes && (st = mp->stage_in.allocate(bytes))) return st;
     if (mp->stage_out.bytes < bytes && (st = mp->stage_out.allocate(bytes))) return st;
     TCP_HIP_CHECK(hipMemcpy(mp->stage_in.ptr, input, bytes, hipMemcpyHostToDevice));
-    if ((st = apply_device(plan, mp->stage_in.ptr, mp->stage_out.ptr, 1, TCP_C128, nullptr))) return st;
-    TCP_HIP_CHECK(hipMemcpy(output, mp->stage_out.ptr, bytes, hipMemcpyDeviceToHost));
+    st = apply_device(plan, mp->stage_in.ptr, mp->stage_out.ptr, 1, TCP_C128, nullptr);
+    hipError_t e = st ? hipSuccess : hipMemcpy(output, mp->stage_out.ptr, bytes, hipMemcpyDeviceToHost);
+    if (bytes > kStagingKeepBytes) {  // large staging is released, never held for the plan's lifetime
+        mp->stage_in.release();
+        mp->stage_out.release();
+    }
+    if (st) return st;
+    if (e != hipSuccess) return tcp::hip_fail(e, "hipMemcpy device to host");
     return tcp::ok();
 }
 

@@ -17,6 +17,28 @@
 // one per-thread sum S.  Inside a class the passes are ordered by zs and the
 // kernel runs one loop per slot pattern, so the slot signs are compile-time
 // constants of straight-line code.
+//
+// Class blocks.  The terms of a class's passes are stored as 32-byte records
+// in one block (a header of pass ranges per slot pattern, then the records),
+// at most TPB 16-byte words.  Each thread loads one word of the next class's
+// block together with its part of the next source tile, and the block is
+// staged in LDS beside the tile, so the pass loop reads its terms from LDS
+// (the same address in every lane) and issues no scalar memory loads: on
+// gfx9 a scalar load and the LDS reads share one wait counter, and a
+// descriptor fetched per pass through the scalar cache serialised every
+// pass behind a scalar-cache round trip.
+//
+// Kernel coordinates.  All of the above is stated on the kernel's own index
+// k, whose bits are, from the lowest, the thread bits, the slot bits and the
+// tile bits.  Coordinate c of k is bit pos[c] of the amplitude's index in
+// memory (a permutation of the index bits; identity unless the plan builder
+// chose another).  Masks in the tables are in kernel coordinates; a class
+// also carries its source tile's offset in memory.  The builder keeps the
+// lowest four coordinates on the lowest four index bits, so that every group
+// of 16 lanes loads 16 consecutive amplitudes (whole 128-byte lines), and
+// picks the other tile-local bits and the six tile bits that vary fastest
+// across the workgroups of one XCD so that few distinct source tiles are
+// loaded per output tile and few distinct tiles are live in one XCD's L2.
 #pragma once
 
 #include <algorithm>
@@ -53,13 +75,33 @@ __device__ __forceinline__ double flip_sign(double x, uint32_t parity) {
     return __longlong_as_double(__double_as_longlong(x) ^ (static_cast<long long>(parity) << 63));
 }
 
-// One source tile per class; its passes are [gbegin, gend), and those with
-// slot pattern z are [zoff[c*S + z], zoff[c*S + z + 1]) with the record
-// stride S = zoff_stride(A) (one 64- or 128-byte scalar load per class).
+// One source tile per class, and the class block holding its passes.
 struct alignas(16) TileClass {
-    uint64_t hi;  // x >> L
-    uint32_t gbegin, gend;
+    uint64_t off;          // source tile of this class relative to the output tile, as an index offset in memory
+    uint32_t boff, bwords; // its block: words [boff, boff + bwords) of the block array
 };
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// A term of a pass in a class block (two 16-byte words).
+struct alignas(16) TermRecord {
+    uint32_t lo_last;  // bits 0..15: the pass's x & (2^L - 1); bit 31: the pass's last term
+    uint32_t zlo;      // z & (TPB - 1)
+    uint64_t zhi;      // z >> L
+    double wr, wi;     // w' (rounded to FP32 in the complex64 blocks)
+};
+
+// Record slots (32 bytes) taken by a block's header: A + 1 u16 record indices
+// (slot-pattern z's passes are records [zr[z], zr[z+1])).
+__host__ __device__ constexpr uint32_t header_slots(uint32_t A) { return ((A + 1) * 2 + 31) / 32; }
+// 16-byte words of one class block in a TPB-thread workgroup (threads
+// t < block_words(TPB) each load and stage one), and the most term records
+// it holds.  The LDS copy has two more words, so that reading one record past
+// a block's last stays inside it.
+__host__ __device__ constexpr uint32_t block_words(uint32_t TPB) { return TPB / 2; }
+__host__ __device__ constexpr uint32_t block_records(uint32_t TPB, uint32_t A) {
+    return block_words(TPB) / 2 - header_slots(A);
+}
 
 // 64-byte pass descriptor (one scalar load).  The first term is stored
 // inline, the others are [tbegin, tend) of the term array.
@@ -112,16 +154,32 @@ __device__ __forceinline__ TileTerm load_term(cptr<TileTerm> p) {
 // (several operators may share the arrays).
 struct TileArgs {
     const TileClass *cls;   // [H]
-    const uint32_t *zoff;   // [H * zoff_stride(A)] pass ranges per slot pattern
-    const TileGroup *grp;   // passes
-    const TileTerm *term;   // extra terms
+    const u32x4 *blk;       // class blocks
+    const TileGroup *grp;   // passes (plan builders)
+    const TileTerm *term;   // extra terms (plan builders)
     const uint32_t *xorder; // passes in x-mask order (plan builders only)
     uint32_t H;
     uint32_t L;
+    uint32_t n;             // index bits (kernel coordinates 0..n-1)
+    uint32_t permuted;      // 0: pos is the identity
+    uint8_t pos[64];        // kernel coordinate c -> index bit pos[c]
 };
 
-// Words per class in the pass-range table: A + 1 entries padded to 16 or 32.
-__host__ __device__ constexpr uint32_t zoff_stride(uint32_t A) { return A + 1 <= 16 ? 16 : 32; }
+// Kernel coordinates of the in-memory index `phys` (plan builders).
+__device__ __forceinline__ uint64_t kernel_index(const TileArgs &p, uint64_t phys) {
+    if (!p.permuted) return phys;
+    uint64_t k = 0;
+    for (uint32_t c = 0; c < p.n; ++c) k |= ((phys >> p.pos[c]) & 1ull) << c;
+    return k;
+}
+
+// In-memory index bits of kernel coordinates [first, first + count) holding `bits`.
+__host__ __device__ __forceinline__ uint64_t deposit(const uint8_t *pos, uint32_t first, uint32_t count,
+                                                     uint64_t bits) {
+    uint64_t out = 0;
+    for (uint32_t j = 0; j < count; ++j) out |= ((bits >> j) & 1ull) << pos[first + j];
+    return out;
+}
 
 template <int A>
 struct SlotSign {  // (-1)^popc(zs & a) as a compile-time bit
@@ -136,117 +194,182 @@ __device__ __forceinline__ uint32_t src_offset(uint32_t tbase, uint32_t ls, int 
     return tbase + ((static_cast<uint32_t>(a) ^ ls) * TPB * static_cast<uint32_t>(sizeof(dvec2)));
 }
 
-// acc[a] += (-1)^popc(ZS & a) * S * src[a] for every pass of slot pattern ZS
-// in [g0, g1) (straight-line code: the slot signs are compile-time).  The A
-// LDS reads of a pass are issued together, then the per-thread sum S of the
-// pass's terms, then the FMAs.
+struct Rec {
+    uint32_t lo_last, zlo;
+    uint64_t zhi;
+    double wr, wi;
+};
+
+// Record r of a block staged in LDS (every lane reads the same address).
+__device__ __forceinline__ Rec read_rec(const u32x4 *blk, uint32_t r) {
+    const u32x4 a = blk[2 * r], b = blk[2 * r + 1];
+    Rec R;
+    R.lo_last = a[0];
+    R.zlo = a[1];
+    R.zhi = a[2] | (static_cast<uint64_t>(a[3]) << 32);
+    R.wr = __longlong_as_double(static_cast<long long>(b[0] | (static_cast<uint64_t>(b[1]) << 32)));
+    R.wi = __longlong_as_double(static_cast<long long>(b[2] | (static_cast<uint64_t>(b[3]) << 32)));
+    return R;
+}
+
+// S += w' (-1)^popc(z & l) for the term of record R at thread t of tile `tile_id`.
+__device__ __forceinline__ void add_term(const Rec &R, uint32_t t, uint64_t tile_id, double &sr, double &si) {
+    const uint32_t par = __popc(R.zlo & t) + __popcll(R.zhi & tile_id);
+    sr += flip_sign(R.wr, par & 1u);
+    si += flip_sign(R.wi, par & 1u);
+}
+
+// acc[a] += (-1)^popc(ZS & a) * S * src[a] for every pass of slot pattern ZS,
+// records [r0, r1) of the staged block (straight-line code: the slot signs are
+// compile-time).  The LDS reads of the first four slots of a pass are issued
+// together, then its terms' per-thread sum S is formed, then the FMAs run
+// four slots at a time.
 template <int ZS, int A, uint32_t TPB>
-__device__ __forceinline__ void passes_uniform(cptr<TileGroup> grp, cptr<TileTerm> term, uint32_t g0, uint32_t g1,
-                                               uint32_t t, uint32_t t16, uint64_t tile_id, const char *tile,
-                                               double *accr, double *acci) {
+__device__ __forceinline__ void passes_uniform(const u32x4 *blk, uint32_t r0, uint32_t r1, uint32_t t, uint32_t t16,
+                                               uint64_t tile_id, const char *tile, double *accr, double *acci) {
     constexpr uint32_t TB = __builtin_ctz(TPB);
-    for (uint32_t g = g0; g < g1; ++g) {
-        const TileGroup G = load_group(grp + g);
-        const uint32_t tbase = t16 ^ ((G.lo & (TPB - 1)) * static_cast<uint32_t>(sizeof(dvec2)));
-        const uint32_t ls = G.lo >> TB;
-        dvec2 s[A];
+    constexpr int C = A >= 8 ? 4 : (A >= 4 ? 2 : A);  // slots read from LDS at a time (bounds live registers)
+    uint32_t r = r0;
+    while (r < r1) {
+        Rec R = read_rec(blk, r);
+        const uint32_t head = __builtin_amdgcn_readfirstlane(R.lo_last);
+        const uint32_t lo = head & 0xFFFFu;
+        const uint32_t tbase = t16 ^ ((lo & (TPB - 1)) * static_cast<uint32_t>(sizeof(dvec2)));
+        const uint32_t ls = lo >> TB;
+        dvec2 s[C];
 #pragma unroll
-        for (int a = 0; a < A; ++a) s[a] = *reinterpret_cast<const dvec2 *>(tile + src_offset<TPB>(tbase, ls, a));
-        uint32_t par = __popc(G.zlo & t) + __popcll(G.zhi & tile_id);
-        double sr = flip_sign(G.wr, par & 1u), si = flip_sign(G.wi, par & 1u);
-        for (uint32_t k = G.tbegin; k < G.tend; ++k) {
-            const TileTerm T = load_term(term + k);
-            par = __popc(T.zlo & t) + __popcll(T.zhi & tile_id);
-            sr += flip_sign(T.wr, par & 1u);
-            si += flip_sign(T.wi, par & 1u);
+        for (int a = 0; a < C; ++a) s[a] = *reinterpret_cast<const dvec2 *>(tile + src_offset<TPB>(tbase, ls, a));
+        double sr = 0.0, si = 0.0;
+        add_term(R, t, tile_id, sr, si);
+        ++r;
+        for (uint32_t last = head >> 31; !last; ++r) {
+            R = read_rec(blk, r);
+            last = __builtin_amdgcn_readfirstlane(R.lo_last) >> 31;
+            add_term(R, t, tile_id, sr, si);
         }
 #pragma unroll
-        for (int a = 0; a < A; ++a) {
-            if (SlotSign<A>::neg(ZS, a)) {
-                accr[a] = fma(-sr, s[a].x, accr[a]);
-                accr[a] = fma(si, s[a].y, accr[a]);
-                acci[a] = fma(-sr, s[a].y, acci[a]);
-                acci[a] = fma(-si, s[a].x, acci[a]);
-            } else {
-                accr[a] = fma(sr, s[a].x, accr[a]);
-                accr[a] = fma(-si, s[a].y, accr[a]);
-                acci[a] = fma(sr, s[a].y, acci[a]);
-                acci[a] = fma(si, s[a].x, acci[a]);
+        for (int a0 = 0; a0 < A; a0 += C) {
+            if (a0 > 0) {
+#pragma unroll
+                for (int c = 0; c < C; ++c)
+                    s[c] = *reinterpret_cast<const dvec2 *>(tile + src_offset<TPB>(tbase, ls, a0 + c));
+            }
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int a = a0 + c;
+                if (SlotSign<A>::neg(ZS, a)) {
+                    accr[a] = fma(-sr, s[c].x, accr[a]);
+                    accr[a] = fma(si, s[c].y, accr[a]);
+                    acci[a] = fma(-sr, s[c].y, acci[a]);
+                    acci[a] = fma(-si, s[c].x, acci[a]);
+                } else {
+                    accr[a] = fma(sr, s[c].x, accr[a]);
+                    accr[a] = fma(-si, s[c].y, accr[a]);
+                    acci[a] = fma(sr, s[c].y, acci[a]);
+                    acci[a] = fma(si, s[c].x, acci[a]);
+                }
             }
         }
     }
 }
 
-// All passes of one class, slot pattern by slot pattern (zr: the class's
-// pass-range record).
+// All passes of one class, slot pattern by slot pattern (zr: the block
+// header, record index where slot pattern z's passes begin, zr[A] the end).
 template <int A, uint32_t TPB, int ZS = 0>
-__device__ __forceinline__ void passes_all(cptr<TileGroup> grp, cptr<TileTerm> term, cptr<uint32_t> zr, uint32_t t,
-                                           uint32_t t16, uint64_t tile_id, const char *tile, double *accr,
-                                           double *acci) {
-    passes_uniform<ZS, A, TPB>(grp, term, zr[ZS], zr[ZS + 1], t, t16, tile_id, tile, accr, acci);
-    if constexpr (ZS + 1 < A) passes_all<A, TPB, ZS + 1>(grp, term, zr, t, t16, tile_id, tile, accr, acci);
+__device__ __forceinline__ void passes_all(const u32x4 *blk, const uint32_t *zr, uint32_t t, uint32_t t16,
+                                           uint64_t tile_id, const char *tile, double *accr, double *acci) {
+    passes_uniform<ZS, A, TPB>(blk, zr[ZS], zr[ZS + 1], t, t16, tile_id, tile, accr, acci);
+    if constexpr (ZS + 1 < A) passes_all<A, TPB, ZS + 1>(blk, zr, t, t16, tile_id, tile, accr, acci);
+}
+
+// Loads of one source tile into registers; NT marks them non-temporal.
+template <bool NT, typename VT>
+__device__ __forceinline__ VT load_amp(const VT *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
 }
 
 // y[tile] = sum over classes/passes of the gathered source tiles.  `lds` is
 // the workgroup's buffer of TPB * A FP64 amplitudes (FP32 storage is widened
 // once per class).  `active` amplitudes of the tile exist (2^L except when
 // the whole space is smaller than one tile; then only l < active are read
-// and written).  The next class's source tile is loaded into registers while
-// the current class is applied.  Every thread of the workgroup must call
-// this (it synchronises).
-template <int A, typename VT, uint32_t TPB, bool PARTIAL>
+// and written).  Source tiles of the next PF classes are in flight in
+// registers (a ring of PF buffers) while the current class is applied.
+// Every thread of the workgroup must call this (it synchronises).
+template <int A, typename VT, uint32_t TPB, bool PARTIAL, int PF = 1, bool NT = false, bool PERM = false>
 __device__ __forceinline__ void tile_apply(const VT *__restrict__ vin, VT *__restrict__ yout, const TileArgs &p,
-                                           uint64_t tile_id, uint32_t active, dvec2 *lds) {
+                                           uint64_t tile_id, uint32_t active, dvec2 *lds, u32x4 *lblk) {
     cptr<TileClass> cls = (cptr<TileClass>)p.cls;
-    cptr<TileGroup> grp = (cptr<TileGroup>)p.grp;
-    cptr<TileTerm> term = (cptr<TileTerm>)p.term;
-    cptr<uint32_t> zoff = (cptr<uint32_t>)p.zoff;
+    constexpr uint32_t TB = __builtin_ctz(TPB), AB = __builtin_ctz(static_cast<uint32_t>(A));
 
     const uint32_t t = threadIdx.x;
-    const uint64_t base = tile_id << p.L;
-    const VT *__restrict__ src_base = vin + t;
+    // in-memory index = base | toff | slot_off(a) (disjoint bits)
+    uint64_t base, toff;
+    if constexpr (PERM) {
+        base = deposit(p.pos, p.L, p.n - p.L, tile_id);
+        toff = deposit(p.pos, 0, TB, t);
+    } else {
+        base = tile_id << p.L;
+        toff = t;
+    }
+    auto slot_off = [&](int a) -> uint64_t {
+        if constexpr (PERM) return deposit(p.pos, TB, AB, static_cast<uint64_t>(a));
+        else return static_cast<uint64_t>(a) * TPB;
+    };
+    const VT *__restrict__ src_base = vin + toff;
     auto live = [&](int a) { return !PARTIAL || t + a * TPB < active; };
     if (p.H == 0) {  // empty operator: H v = 0
 #pragma unroll
         for (int a = 0; a < A; ++a)
-            if (live(a)) yout[base + t + a * TPB] = Amp<VT>::make(0.0, 0.0);
+            if (live(a)) yout[base + toff + slot_off(a)] = Amp<VT>::make(0.0, 0.0);
         return;
     }
 
     const uint32_t t16 = t * static_cast<uint32_t>(sizeof(dvec2));
     const char *tile = reinterpret_cast<const char *>(lds);
     double accr[A], acci[A];
-    VT pre[A];
+    VT pre[PF][A];
+    u32x4 bpre[PF];  // this thread's word of the in-flight classes' blocks
 #pragma unroll
     for (int a = 0; a < A; ++a) {
         accr[a] = 0.0;
         acci[a] = 0.0;
-        pre[a] = Amp<VT>::make(0.0, 0.0);
     }
-    {
-        const uint64_t src = base ^ (static_cast<uint64_t>(cls[0].hi) << p.L);
+    auto fetch = [&](VT *buf, u32x4 &bw, uint32_t h) {
+        const uint64_t src = base ^ cls[h].off;
+        const uint32_t boff = cls[h].boff, bwords = cls[h].bwords;
 #pragma unroll
         for (int a = 0; a < A; ++a)
-            if (live(a)) pre[a] = src_base[src + a * TPB];
-    }
-    constexpr uint32_t ZSTRIDE = zoff_stride(A);
-    for (uint32_t h = 0; h < p.H; ++h) {
-        __syncthreads();  // every wave is done reading the previous class's tile
+            buf[a] = live(a) ? load_amp<NT>(src_base + (src | slot_off(a))) : Amp<VT>::make(0.0, 0.0);
+        bw = t < bwords ? p.blk[boff + t] : u32x4{0u, 0u, 0u, 0u};  // bwords <= block_words(TPB)
+    };
 #pragma unroll
-        for (int a = 0; a < A; ++a) lds[t + a * TPB] = Amp<VT>::widen(pre[a]);
-        __syncthreads();
-        if (h + 1 < p.H) {  // prefetch the next class's source tile
-            const uint64_t src = base ^ (static_cast<uint64_t>(cls[h + 1].hi) << p.L);
+    for (int k = 0; k < PF; ++k)
+        if (static_cast<uint32_t>(k) < p.H) fetch(pre[k], bpre[k], k);
+    for (uint32_t h0 = 0; h0 < p.H; h0 += PF) {
 #pragma unroll
-            for (int a = 0; a < A; ++a)
-                if (live(a)) pre[a] = src_base[src + a * TPB];
+        for (int k = 0; k < PF; ++k) {
+            const uint32_t h = h0 + k;
+            if (h >= p.H) break;
+            __syncthreads();  // every wave is done reading the previous class's tile and block
+#pragma unroll
+            for (int a = 0; a < A; ++a) lds[t + a * TPB] = Amp<VT>::widen(pre[k][a]);
+            if (t < block_words(TPB)) lblk[t] = bpre[k];
+            __syncthreads();
+            if (h + PF < p.H) fetch(pre[k], bpre[k], h + PF);  // the class PF ahead into the freed buffers
+            uint32_t zr[A + 1];  // block header: u16 entries, read once into scalar registers
+#pragma unroll
+            for (int z = 0; z <= A; ++z) {
+                const uint32_t word = reinterpret_cast<const uint32_t *>(lblk)[z >> 1];
+                zr[z] = __builtin_amdgcn_readfirstlane((word >> ((z & 1) * 16)) & 0xFFFFu);
+            }
+            passes_all<A, TPB>(lblk, zr, t, t16, tile_id, tile, accr, acci);
         }
-        passes_all<A, TPB>(grp, term, zoff + h * ZSTRIDE, t, t16, tile_id, tile, accr, acci);
     }
-    VT *__restrict__ out = yout + base + t;
+    VT *__restrict__ out = yout + base + toff;
 #pragma unroll
     for (int a = 0; a < A; ++a)
-        if (live(a)) out[a * TPB] = Amp<VT>::make(accr[a], acci[a]);
+        if (live(a)) out[slot_off(a)] = Amp<VT>::make(accr[a], acci[a]);
 }
 
 // Diagonal sum of pass g at tile-local position (t, slot) of tile `tile_id`,
@@ -265,10 +388,87 @@ __device__ inline dvec2 group_sum(const TileArgs &p, uint32_t g, uint32_t t, uin
     return dvec2{dr, di};
 }
 
+// Kernel coordinates for the tile kernel of geometry (tb, ab) on `op`, with
+// `window` tile bits varying fastest across the workgroups that run together
+// on one XCD.  Coordinates 0..3 stay on index bits 0..3; the other tile-local
+// bits are picked greedily to leave the fewest distinct x-mask parts outside
+// the tile (source tiles per output tile), then the window bits to leave the
+// fewest outside tile and window (distinct source tiles an XCD streams per
+// window), then, among the tile-local bits, the slot bits that split the
+// fewest x-groups into passes.  Ties go to the lowest bit.  Returns false
+// (identity) for spaces too small to tile, and for operators with more than
+// 16,384 distinct x-masks (where the search would cost more than it saves).
+inline bool choose_bit_map(const Operator &op, uint32_t tb, uint32_t ab, uint32_t window, uint8_t *pos) {
+    const uint32_t n = op.nqubits, L = tb + ab;
+    if (n > 64 || tb < 4 || n < L + window) return false;
+    const std::vector<IndexTerm> terms = index_terms(op);
+    std::vector<uint64_t> xs(terms.size());
+    for (size_t i = 0; i < terms.size(); ++i) xs[i] = terms[i].x;
+    std::sort(xs.begin(), xs.end());
+    xs.erase(std::unique(xs.begin(), xs.end()), xs.end());
+    if (xs.size() > 16384) return false;
+    std::vector<uint64_t> scratch;
+    auto distinct_outside = [&](uint64_t inside) {
+        scratch.assign(xs.begin(), xs.end());
+        for (uint64_t &v : scratch) v &= ~inside;
+        std::sort(scratch.begin(), scratch.end());
+        return static_cast<size_t>(std::unique(scratch.begin(), scratch.end()) - scratch.begin());
+    };
+    auto grow = [&](uint64_t &set, uint32_t count, uint64_t fixed) {
+        for (uint32_t step = 0; step < count; ++step) {
+            size_t best = SIZE_MAX;
+            uint32_t best_b = 0;
+            for (uint32_t b = 0; b < n; ++b) {
+                if (((set | fixed) >> b) & 1ull) continue;
+                const size_t c = distinct_outside(fixed | set | (1ull << b));
+                if (c < best) {
+                    best = c;
+                    best_b = b;
+                }
+            }
+            set |= 1ull << best_b;
+        }
+    };
+    uint64_t local = 0xFull;
+    grow(local, L - 4, 0);
+    uint64_t win = 0;
+    grow(win, window, local);
+    // slot bits: fewest distinct (x, z restricted to the slot bits) pairs
+    std::vector<std::pair<uint64_t, uint64_t>> pairs(terms.size());
+    uint64_t slot = 0;
+    for (uint32_t step = 0; step < ab; ++step) {
+        size_t best = SIZE_MAX;
+        uint32_t best_b = 0;
+        for (uint32_t b = 4; b < n; ++b) {
+            if (!((local >> b) & 1ull) || ((slot >> b) & 1ull)) continue;
+            const uint64_t m = slot | (1ull << b);
+            for (size_t i = 0; i < terms.size(); ++i) pairs[i] = {terms[i].x, terms[i].z & m};
+            std::sort(pairs.begin(), pairs.end());
+            const size_t c = static_cast<size_t>(std::unique(pairs.begin(), pairs.end()) - pairs.begin());
+            if (c < best) {
+                best = c;
+                best_b = b;
+            }
+        }
+        slot |= 1ull << best_b;
+    }
+    uint32_t c = 0;
+    auto place = [&](uint64_t set) {
+        for (uint32_t b = 0; b < n; ++b)
+            if ((set >> b) & 1ull) pos[c++] = static_cast<uint8_t>(b);
+    };
+    place(local & ~slot);  // thread coordinates (bits 0..3 first)
+    place(slot);
+    place(win);
+    const uint64_t all = n == 64 ? ~0ull : (1ull << n) - 1;
+    place(all & ~(local | win));
+    return true;
+}
+
 // Host-side tables of one or more operators for tile geometry (tb, ab).
 struct TileTables {
     std::vector<TileClass> cls;
-    std::vector<uint32_t> zoff;            // (A+1) per class
+    std::vector<u32x4> blk64, blk32;       // class blocks (blk32: coefficients rounded to FP32)
     std::vector<uint32_t> xorder;          // passes in x-mask order
     std::vector<TileGroup> grp64, grp32;   // grp32: coefficients rounded to FP32
     std::vector<TileTerm> term64, term32;
@@ -278,12 +478,30 @@ struct TileTables {
     size_t cls_base = 0;        // its first entry in cls
     size_t grp_base = 0;        // its first pass
 
-    // Appends the tables of `op`.
-    void append(const Operator &op, uint32_t tb, uint32_t ab) {
+    // Appends the tables of `op`, with kernel coordinate c at index bit
+    // pos[c] (pos = nullptr: the identity), for a kernel of `wg_threads`
+    // threads per workgroup (0: 2^tb), which sets the class block size.
+    void append(const Operator &op, uint32_t tb, uint32_t ab, const uint8_t *pos = nullptr,
+                uint32_t wg_threads = 0) {
         const uint32_t L = tb + ab, A = 1u << ab;
         const uint64_t lmask = (1ull << L) - 1, tmask = (1ull << tb) - 1;
-        const std::vector<IndexTerm> terms = index_terms(op);
+        std::vector<IndexTerm> terms = index_terms(op);
         const uint64_t T = terms.size();
+        const uint32_t n = op.nqubits;
+        if (pos) {
+            auto to_kernel = [&](uint64_t phys) {
+                uint64_t k = 0;
+                for (uint32_t c = 0; c < n; ++c) k |= ((phys >> pos[c]) & 1ull) << c;
+                return k;
+            };
+            for (IndexTerm &term : terms) {  // popc(z & x) is the same in both coordinates
+                term.x = to_kernel(term.x);
+                term.z = to_kernel(term.z);
+            }
+        }
+        auto class_offset = [&](uint64_t hi) {
+            return pos ? deposit(pos, L, n > L ? n - L : 0, hi) : hi << L;
+        };
         struct Key {
             uint64_t hi, lo;
             uint32_t zs;
@@ -316,11 +534,16 @@ struct TileTables {
             uint32_t zs, index;
         };
         std::vector<PassKey> pkeys;
+        struct ClassPasses {
+            uint64_t off;
+            size_t gbegin, gend;
+        };
+        std::vector<ClassPasses> classes;
         uint64_t k = 0;
         while (k < T) {
-            if (cls.size() == cls_base || cls.back().hi != keys[k].hi) {
-                if (cls.size() > cls_base) cls.back().gend = static_cast<uint32_t>(grp64.size());
-                cls.push_back(TileClass{keys[k].hi, static_cast<uint32_t>(grp64.size()), 0});
+            if (classes.empty() || keys[k].hi != keys[k - 1].hi) {
+                if (!classes.empty()) classes.back().gend = grp64.size();
+                classes.push_back(ClassPasses{class_offset(keys[k].hi), grp64.size(), 0});
             }
             uint64_t e = k;
             while (e < T && keys[e].hi == keys[k].hi && keys[e].zs == keys[k].zs && keys[e].lo == keys[k].lo) ++e;
@@ -343,18 +566,11 @@ struct TileTables {
             }
             k = e;
         }
-        if (cls.size() > cls_base) cls.back().gend = static_cast<uint32_t>(grp64.size());
+        if (!classes.empty()) classes.back().gend = grp64.size();
+        for (const ClassPasses &c : classes)
+            append_blocks(c.off, c.gbegin, c.gend, ab, wg_threads ? wg_threads : 1u << tb);
         H = cls.size() - cls_base;
         passes = grp64.size() - grp_base;
-        // pass ranges per slot pattern
-        const uint32_t stride = zoff_stride(A);
-        for (size_t c = cls_base; c < cls.size(); ++c) {
-            uint32_t g = cls[c].gbegin;
-            for (uint32_t z = 0; z < stride; ++z) {
-                while (z <= A && g < cls[c].gend && grp64[g].zs < z) ++g;
-                zoff.push_back(z <= A ? g : cls[c].gend);
-            }
-        }
         // x-mask order for the plan builders
         std::sort(pkeys.begin(), pkeys.end(), [](const PassKey &a, const PassKey &b) {
             if (a.hi != b.hi) return a.hi < b.hi;
@@ -372,10 +588,77 @@ struct TileTables {
         }
     }
 
+    // The blocks of one class: passes [gb, ge) (ordered by slot pattern) as
+    // term records, packed into blocks of at most block_records(wg_threads, 2^ab)
+    // records; a class that does not fit in one block becomes several
+    // classes with the same source tile, and a pass with more terms than a
+    // block holds is split into passes of consecutive terms (the same sum,
+    // accumulated in two steps).
+    void append_blocks(uint64_t off, size_t gb, size_t ge, uint32_t ab, uint32_t wg_threads) {
+        const uint32_t A = 1u << ab, HS = header_slots(A);
+        // 2 * (HS + records) words per block <= block_words(wg_threads); the kernels run
+        // 64 or more threads per workgroup, so a block holds 14 or more records
+        const uint32_t RMAX = block_records(std::max<uint32_t>(wg_threads, 64), A);
+        struct Chunk {
+            uint32_t zs;
+            std::vector<TermRecord> r64, r32;
+        };
+        std::vector<Chunk> chunks;
+        for (size_t g = gb; g < ge; ++g) {
+            auto record = [](const TileGroup &P, const TileTerm &T) {
+                return TermRecord{P.lo, T.zlo, T.zhi, T.wr, T.wi};
+            };
+            std::vector<TermRecord> all64, all32;
+            const TileGroup &P64 = grp64[g], &P32 = grp32[g];
+            all64.push_back(record(P64, TileTerm{P64.zlo, P64.zs, P64.zhi, P64.wr, P64.wi}));
+            all32.push_back(record(P32, TileTerm{P32.zlo, P32.zs, P32.zhi, P32.wr, P32.wi}));
+            for (uint32_t q = P64.tbegin; q < P64.tend; ++q) {
+                all64.push_back(record(P64, term64[q]));
+                all32.push_back(record(P32, term32[q]));
+            }
+            for (size_t i = 0; i < all64.size(); i += RMAX) {
+                Chunk c{P64.zs, {}, {}};
+                const size_t e = std::min(all64.size(), i + RMAX);
+                c.r64.assign(all64.begin() + i, all64.begin() + e);
+                c.r32.assign(all32.begin() + i, all32.begin() + e);
+                c.r64.back().lo_last |= 1u << 31;
+                c.r32.back().lo_last |= 1u << 31;
+                chunks.push_back(std::move(c));
+            }
+        }
+        auto push_words = [](std::vector<u32x4> &blk, const void *data, size_t words) {
+            const u32x4 *w = static_cast<const u32x4 *>(data);
+            blk.insert(blk.end(), w, w + words);
+        };
+        size_t i = 0;
+        while (i < chunks.size()) {
+            size_t j = i, nrec = 0;
+            while (j < chunks.size() && nrec + chunks[j].r64.size() <= RMAX) nrec += chunks[j++].r64.size();
+            std::vector<uint16_t> header(HS * 16, 0);
+            uint32_t r = HS;
+            size_t q = i;
+            for (uint32_t z = 0; z <= A; ++z) {
+                while (q < j && chunks[q].zs < z) r += static_cast<uint32_t>(chunks[q++].r64.size());
+                header[z] = static_cast<uint16_t>(z == A ? HS + nrec : r);
+            }
+            cls.push_back(TileClass{off, static_cast<uint32_t>(blk64.size()), static_cast<uint32_t>(2 * (HS + nrec))});
+            push_words(blk64, header.data(), 2 * HS);
+            push_words(blk32, header.data(), 2 * HS);
+            for (size_t c = i; c < j; ++c) {
+                push_words(blk64, chunks[c].r64.data(), 2 * chunks[c].r64.size());
+                push_words(blk32, chunks[c].r32.data(), 2 * chunks[c].r32.size());
+            }
+            i = j;
+        }
+    }
+
     // Keeps every device array non-empty (a zero-size upload has no pointer).
     void pad_empty() {
         if (cls.empty()) cls.push_back(TileClass{0, 0, 0});
-        if (zoff.empty()) zoff.push_back(0);
+        if (blk64.empty()) {
+            blk64.push_back(u32x4{0u, 0u, 0u, 0u});
+            blk32.push_back(u32x4{0u, 0u, 0u, 0u});
+        }
         if (xorder.empty()) xorder.push_back(0);
         if (grp64.empty()) {
             grp64.push_back(TileGroup{});
@@ -387,7 +670,8 @@ struct TileTables {
         }
     }
     uint64_t device_bytes() const {
-        return cls.size() * sizeof(TileClass) + (zoff.size() + xorder.size()) * sizeof(uint32_t) +
+        return cls.size() * sizeof(TileClass) + (blk64.size() + blk32.size()) * sizeof(u32x4) +
+               xorder.size() * sizeof(uint32_t) +
                (grp64.size() + grp32.size()) * sizeof(TileGroup) +
                (term64.size() + term32.size()) * sizeof(TileTerm);
     }

@@ -28,7 +28,7 @@ from ._plan import (
 
 # tile geometries of the full-space kernel (threads x amplitudes per thread)
 TILINGS = {"512x8": _lib.TCP_TILE_512x8, "1024x8": _lib.TCP_TILE_1024x8, "512x16": _lib.TCP_TILE_512x16,
-           "512x4": _lib.TCP_TILE_512x4}
+           "512x4": _lib.TCP_TILE_512x4, "1024x4": _lib.TCP_TILE_1024x4}
 
 _LETTERS = {"I": 0, "X": 1, "Y": 2, "Z": 3}
 PauliInput = Union[str, Sequence[int]]
@@ -392,14 +392,17 @@ class PauliOperator:
 
     def native_mvp_plan(self, *, storage: str = "lazy",
                         max_bytes: Optional[int] = DEFAULT_MAX_BYTES,
-                        tiling: Optional[str] = None, xcd_order: Optional[bool] = None) -> "NativeMVPPlan":
+                        tiling: Optional[str] = None, xcd_order: Optional[bool] = None,
+                        prefetch: Optional[int] = None, nontemporal: bool = False) -> "NativeMVPPlan":
         """Compile a reusable matrix-free plan on the current GPU.
 
         ``tiling`` picks the kernel's tile geometry ("512x8", "512x4",
-        "1024x8", "512x16"; None = automatic: 512x8 for complex128 and 512x4
-        for complex64 storage, with the XCD order) and ``xcd_order`` walks the
-        tiles XCD by XCD; neither changes the result beyond floating-point
-        summation order.
+        "1024x4", "1024x8", "512x16"; None = automatic: 512x8 for complex128
+        and 512x4 for complex64 storage, with the XCD order) and ``xcd_order``
+        walks the tiles XCD by XCD; ``prefetch`` (1-3) is the number of source
+        tiles each workgroup keeps in flight and ``nontemporal`` marks those
+        loads non-temporal (512x8, 512x4 and 1024x4 only).  None of these
+        changes the result beyond floating-point summation order.
         """
         validate_max_bytes(max_bytes)
         if storage not in ("lazy", "eager"):
@@ -414,6 +417,12 @@ class PauliOperator:
             code |= _lib.TCP_TILE_XCD_ORDER
         elif xcd_order is False:
             code |= _lib.TCP_TILE_LINEAR_ORDER
+        if prefetch is not None:
+            if prefetch not in (1, 2, 3):
+                raise ValueError("prefetch must be 1, 2 or 3")
+            code |= (prefetch - 1) << _lib.TCP_TILE_PREFETCH_SHIFT
+        if nontemporal:
+            code |= _lib.TCP_TILE_NT_LOADS
         _lib.check(_lib.lib().tcp_mvp_plan_create_tiled(self._handle.ptr, st, effective_max_bytes(max_bytes),
                                                          code, ctypes.byref(out)))
         return NativeMVPPlan(out, self, storage)

@@ -121,7 +121,8 @@ def test_many_classes_and_large_groups():
     coef = np.concatenate([coef, rng.normal(size=300) + 0j])
     op = tcp.PauliOperator.from_code_arrays(codes, coef)
     plan = op.native_mvp_plan()
-    assert plan.class_count == 64 and plan.group_count > 500  # every 6-bit high part occurs
+    # every 6-bit high part occurs; the 300-term diagonal group spans several class blocks
+    assert plan.class_count >= 64 and plan.group_count > 500
     state = _random_state(18, 8)
     assert rel_err(plan.apply(state), _oracle_apply(op, state)) <= FP64_TOL
 
