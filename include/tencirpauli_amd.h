@@ -244,6 +244,20 @@ void tcp_mvp_plan_destroy(tcp_mvp_plan *plan);
 int tcp_mvp(const tcp_operator *op, const double *input, double *output, uint64_t input_len,
             uint64_t output_len);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): y = op v on device buffers of dtype TCP_C128 or TCP_C64
+ * with no plan and no device allocation.  The operator's tile tables (the
+ * ones tcp_mvp_plan_create builds) are staged in a static 8 MiB workspace
+ * that each device holds until the process ends; operators whose tables need
+ * more (roughly 250,000 terms) are TCP_ERR_UNSUPPORTED.  The call returns
+ * once the tables are copied (it synchronises `stream` once); the MVP then
+ * runs on `stream`.  Every call on one device shares the workspace: calls
+ * that overlap on one device race, so issue them on one stream or serialise
+ * them.  Same results as tcp_mvp_plan_apply_into on a plan of `op`. */
+int tcp_mvp_apply_static(const tcp_operator *op, const void *input, void *output, uint64_t input_len,
+                         uint64_t output_len, int dtype, void *stream);
+
 /* ---- sectors ------------------------------------------------------------ */
 
 /* U1Sector (core/sector.rs:79-238): dimension C(n, k); rank/unrank of an

@@ -105,6 +105,7 @@ SIGNATURES = {
     "tcp_mvp_plan_apply_host": (_c.c_int, [_vp, _dp, _dp, _c.c_uint64, _c.c_uint64]),
     "tcp_mvp_plan_destroy": (None, [_vp]),
     "tcp_mvp": (_c.c_int, [_vp, _dp, _dp, _c.c_uint64, _c.c_uint64]),
+    "tcp_mvp_apply_static": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _c.c_uint64, _c.c_int, _vp]),
     "tcp_u1_dimension": (_c.c_int, [_c.c_uint32, _c.c_uint32, _u64p]),
     "tcp_u1_rank": (_c.c_int, [_c.c_uint32, _c.c_uint32, _c.c_uint64, _u64p]),
     "tcp_u1_unrank": (_c.c_int, [_c.c_uint32, _c.c_uint32, _c.c_uint64, _u64p]),

@@ -324,54 +324,70 @@ int build_plan(const tcp::Operator &op, int storage, uint64_t max_bytes, uint32_
     return tcp::ok();
 }
 
+// Everything one apply launch needs: the space, the geometry word and the
+// device tables (a plan's own, or tables staged in the static workspace).
+struct LaunchSpec {
+    uint64_t dim;
+    uint32_t n, tiling, L, ab;
+    uint64_t H;
+    bool xcd;
+    MvpArgs args;
+};
+
+LaunchSpec launch_spec(const tcp_mvp_plan *plan, int dtype) {
+    const PlanTiles &pt = plan->tiles(dtype);
+    return LaunchSpec{plan->dim(), plan->n, pt.tiling, pt.L, pt.ab, pt.H, plan->xcd_order(pt), pt.args(dtype)};
+}
+
 template <int A, typename VT, uint32_t TPB, bool PARTIAL = false, int PF = 1, bool NT = false>
-hipError_t launch_tile(const tcp_mvp_plan *plan, const PlanTiles &pt, const void *in, void *out, uint32_t nrhs,
-                       int dtype, hipStream_t stream) {
-    const uint64_t tiles = PARTIAL ? 1 : plan->dim() >> pt.L;
+hipError_t launch_tile(const LaunchSpec &ls, const void *in, void *out, uint32_t nrhs, hipStream_t stream) {
+    const uint64_t tiles = PARTIAL ? 1 : ls.dim >> ls.L;
     hipLaunchKernelGGL((mvp_tile_kernel<A, VT, TPB, PARTIAL, PF, NT>), dim3(static_cast<uint32_t>(tiles), nrhs),
-                       dim3(TPB), 0, stream, static_cast<const VT *>(in), static_cast<VT *>(out), pt.args(dtype),
-                       plan->dim(), plan->xcd_order(pt) ? 1u : 0u);
+                       dim3(TPB), 0, stream, static_cast<const VT *>(in), static_cast<VT *>(out), ls.args, ls.dim,
+                       ls.xcd ? 1u : 0u);
     return hipGetLastError();
 }
 
 // Geometries with a selectable prefetch depth (1-3 tiles in flight) and load policy.
 template <int A, typename VT, uint32_t TPB>
-hipError_t launch_pipelined(const tcp_mvp_plan *plan, const PlanTiles &pt, const void *in, void *out,
-                            uint32_t nrhs, int dtype, hipStream_t stream) {
-    const uint32_t depth = tile_prefetch_depth(pt.tiling);
-    const bool nt = (pt.tiling & TCP_TILE_NT_LOADS) != 0;
-    if (depth == 3) return nt ? launch_tile<A, VT, TPB, false, 3, true>(plan, pt, in, out, nrhs, dtype, stream)
-                              : launch_tile<A, VT, TPB, false, 3, false>(plan, pt, in, out, nrhs, dtype, stream);
-    if (depth == 2) return nt ? launch_tile<A, VT, TPB, false, 2, true>(plan, pt, in, out, nrhs, dtype, stream)
-                              : launch_tile<A, VT, TPB, false, 2, false>(plan, pt, in, out, nrhs, dtype, stream);
-    return nt ? launch_tile<A, VT, TPB, false, 1, true>(plan, pt, in, out, nrhs, dtype, stream)
-              : launch_tile<A, VT, TPB, false, 1, false>(plan, pt, in, out, nrhs, dtype, stream);
+hipError_t launch_pipelined(const LaunchSpec &ls, const void *in, void *out, uint32_t nrhs, hipStream_t stream) {
+    const uint32_t depth = tile_prefetch_depth(ls.tiling);
+    const bool nt = (ls.tiling & TCP_TILE_NT_LOADS) != 0;
+    if (depth == 3) return nt ? launch_tile<A, VT, TPB, false, 3, true>(ls, in, out, nrhs, stream)
+                              : launch_tile<A, VT, TPB, false, 3, false>(ls, in, out, nrhs, stream);
+    if (depth == 2) return nt ? launch_tile<A, VT, TPB, false, 2, true>(ls, in, out, nrhs, stream)
+                              : launch_tile<A, VT, TPB, false, 2, false>(ls, in, out, nrhs, stream);
+    return nt ? launch_tile<A, VT, TPB, false, 1, true>(ls, in, out, nrhs, stream)
+              : launch_tile<A, VT, TPB, false, 1, false>(ls, in, out, nrhs, stream);
 }
 
 template <typename VT>
-hipError_t launch_typed(const tcp_mvp_plan *plan, const void *in, void *out, uint32_t nrhs, int dtype,
-                        hipStream_t stream) {
-    const PlanTiles &pt = plan->tiles(dtype);
-    if (pt.H == 0) {  // empty operator: H v = 0
-        const uint64_t count = plan->dim() * nrhs;
+hipError_t launch_typed(const LaunchSpec &ls, const void *in, void *out, uint32_t nrhs, hipStream_t stream) {
+    if (ls.H == 0) {  // empty operator: H v = 0
+        const uint64_t count = ls.dim * nrhs;
         const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((count + 255) / 256, 65536));
         hipLaunchKernelGGL(zero_kernel<VT>, dim3(blocks), dim3(256), 0, stream, static_cast<VT *>(out), count);
         return hipGetLastError();
     }
-    switch (pt.tiling & 0xffu) {
-        case TCP_TILE_512x8: return launch_pipelined<8, VT, 512>(plan, pt, in, out, nrhs, dtype, stream);
-        case TCP_TILE_1024x8: return launch_tile<8, VT, 1024>(plan, pt, in, out, nrhs, dtype, stream);
-        case TCP_TILE_512x16: return launch_tile<16, VT, 512>(plan, pt, in, out, nrhs, dtype, stream);
-        case TCP_TILE_512x4: return launch_pipelined<4, VT, 512>(plan, pt, in, out, nrhs, dtype, stream);
-        case TCP_TILE_1024x4: return launch_pipelined<4, VT, 1024>(plan, pt, in, out, nrhs, dtype, stream);
+    switch (ls.tiling & 0xffu) {
+        case TCP_TILE_512x8: return launch_pipelined<8, VT, 512>(ls, in, out, nrhs, stream);
+        case TCP_TILE_1024x8: return launch_tile<8, VT, 1024>(ls, in, out, nrhs, stream);
+        case TCP_TILE_512x16: return launch_tile<16, VT, 512>(ls, in, out, nrhs, stream);
+        case TCP_TILE_512x4: return launch_pipelined<4, VT, 512>(ls, in, out, nrhs, stream);
+        case TCP_TILE_1024x4: return launch_pipelined<4, VT, 1024>(ls, in, out, nrhs, stream);
         default: break;
     }
-    if (plan->n < kSmallBits) return launch_tile<1, VT, kSmallThreads, true>(plan, pt, in, out, nrhs, dtype, stream);
-    switch (pt.ab) {
-        case 0: return launch_tile<1, VT, kSmallThreads>(plan, pt, in, out, nrhs, dtype, stream);
-        case 1: return launch_tile<2, VT, kSmallThreads>(plan, pt, in, out, nrhs, dtype, stream);
-        default: return launch_tile<4, VT, kSmallThreads>(plan, pt, in, out, nrhs, dtype, stream);
+    if (ls.n < kSmallBits) return launch_tile<1, VT, kSmallThreads, true>(ls, in, out, nrhs, stream);
+    switch (ls.ab) {
+        case 0: return launch_tile<1, VT, kSmallThreads>(ls, in, out, nrhs, stream);
+        case 1: return launch_tile<2, VT, kSmallThreads>(ls, in, out, nrhs, stream);
+        default: return launch_tile<4, VT, kSmallThreads>(ls, in, out, nrhs, stream);
     }
+}
+
+hipError_t launch_any(const LaunchSpec &ls, const void *in, void *out, uint32_t nrhs, int dtype, hipStream_t stream) {
+    return dtype == TCP_C128 ? launch_typed<dvec2>(ls, in, out, nrhs, stream)
+                             : launch_typed<fvec2>(ls, in, out, nrhs, stream);
 }
 
 int apply_device(const tcp_mvp_plan *plan, const void *in, void *out, uint64_t nrhs, int dtype, void *stream) {
@@ -389,11 +405,39 @@ int apply_device(const tcp_mvp_plan *plan, const void *in, void *out, uint64_t n
     tcp::DeviceGuard dg;
     int st = dg.enter(plan->device);
     if (st) return st;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    hipError_t e = dtype == TCP_C128 ? launch_typed<dvec2>(plan, in, out, static_cast<uint32_t>(nrhs), dtype, s)
-                                     : launch_typed<fvec2>(plan, in, out, static_cast<uint32_t>(nrhs), dtype, s);
+    hipError_t e = launch_any(launch_spec(plan, dtype), in, out, static_cast<uint32_t>(nrhs), dtype,
+                              static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return tcp::hip_fail(e, "mvp kernel launch");
     return tcp::ok();
+}
+
+}  // namespace
+
+// Workspace of tcp_mvp_apply_static: the class table and class blocks of the
+// operator being applied are staged here instead of in allocated memory.
+// Each device holds its own copy until the process ends; host code reaches
+// it only through the address hipGetSymbolAddress returns on that device.
+constexpr size_t kStaticTableBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticTableBytes];
+
+namespace {
+
+std::mutex g_scratch_mutex;
+std::vector<void *> g_scratch_ptr;  // per device ordinal
+
+// Address of the current device's workspace (looked up once per device).
+int scratch_address(void **out) {
+    int device = 0;
+    TCP_HIP_CHECK(hipGetDevice(&device));
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr.size() <= static_cast<size_t>(device)) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        TCP_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return TCP_OK;
 }
 
 }  // namespace
@@ -487,6 +531,54 @@ int tcp_mvp_plan_apply_batch(const tcp_mvp_plan *plan, const void *input, void *
                              uint64_t nrhs, int dtype, void *stream) {
     if (!plan) return fail(TCP_ERR_INVALID_ARGUMENT, "null plan");
     return apply_device(plan, input, output, nrhs, dtype, stream);
+}
+
+int tcp_mvp_apply_static(const tcp_operator *op, const void *input, void *output, uint64_t input_len,
+                         uint64_t output_len, int dtype, void *stream) {
+    if (!op) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
+    if (dtype != TCP_C128 && dtype != TCP_C64)
+        return fail(TCP_ERR_INVALID_ARGUMENT, "dtype must be TCP_C128 or TCP_C64");
+    const tcp::Operator &o = tcp::operator_of(op);
+    const uint32_t n = o.nqubits;
+    if (n >= 64) return fail(TCP_ERR_OVERFLOW, "integer overflow while computing matrix dimension");
+    const uint64_t dim = 1ull << n;
+    int st;
+    if ((st = check_len(dim, input_len)) || (st = check_len(dim, output_len))) return st;
+    if (!input || !output) return fail(TCP_ERR_INVALID_ARGUMENT, "null state buffer");
+    const size_t vec_bytes = static_cast<size_t>(dim) * (dtype == TCP_C128 ? 16 : 8);
+    if (tcp::ranges_overlap(input, vec_bytes, output, vec_bytes))
+        return fail(TCP_ERR_INVALID_ARGUMENT, "apply_into input and output must not overlap");
+    // the plan builder's tables, on the host
+    PlanTiles pt;
+    tcp::TileTables tab;
+    build_tiles(o, resolve_geometry(n, TCP_TILE_AUTO, dtype), true, 0u, pt, tab);
+    if ((dim >> pt.L) > 0x7fffffffull) return fail(TCP_ERR_UNSUPPORTED, "too many tiles for one launch");
+    const std::vector<tcp::u32x4> &blk = dtype == TCP_C64 ? tab.blk32 : tab.blk64;
+    const size_t cls_bytes = tab.cls.size() * sizeof(tcp::TileClass);
+    const size_t blk_off = (cls_bytes + 255) & ~size_t(255);
+    const size_t blk_bytes = blk.size() * sizeof(tcp::u32x4);
+    if (blk_off + blk_bytes > kStaticTableBytes)
+        return fail(TCP_ERR_UNSUPPORTED, "tcp_mvp_apply_static: the operator's tables need " +
+                                             std::to_string(blk_off + blk_bytes) + " bytes, more than the " +
+                                             std::to_string(kStaticTableBytes) + "-byte static workspace");
+    void *ws = nullptr;
+    if ((st = scratch_address(&ws))) return st;
+    char *base = static_cast<char *>(ws);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    TCP_HIP_CHECK(hipMemcpyAsync(base, tab.cls.data(), cls_bytes, hipMemcpyHostToDevice, s));
+    TCP_HIP_CHECK(hipMemcpyAsync(base + blk_off, blk.data(), blk_bytes, hipMemcpyHostToDevice, s));
+    TCP_HIP_CHECK(hipStreamSynchronize(s));  // the host tables are released on return
+    MvpArgs a = pt.args(dtype);
+    a.cls = reinterpret_cast<const tcp::TileClass *>(base);
+    a.blk = reinterpret_cast<const tcp::u32x4 *>(base + blk_off);
+    a.grp = nullptr;  // pass and term tables serve the plan builders only
+    a.term = nullptr;
+    a.xorder = nullptr;
+    const LaunchSpec ls{dim, n, pt.tiling, pt.L, pt.ab, pt.H, (pt.tiling & TCP_TILE_XCD_ORDER) && (dim >> pt.L) >= 8,
+                        a};
+    hipError_t e = launch_any(ls, input, output, 1, dtype, s);
+    if (e != hipSuccess) return tcp::hip_fail(e, "mvp kernel launch");
+    return tcp::ok();
 }
 
 int tcp_mvp_plan_apply_host(const tcp_mvp_plan *plan, const double *input, double *output,
