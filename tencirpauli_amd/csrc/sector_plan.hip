@@ -682,9 +682,14 @@ constexpr uint32_t kDiagIndex = 255;
 constexpr uint32_t kColBits = 24;
 constexpr uint32_t kColMask = (1u << kColBits) - 1;
 
+// Bit patterns of a value.  Vector elements are passed by value: clang's
+// __builtin_bit_cast applied directly to an ext_vector element such as v.y
+// reads element 0 instead (which made v and conj(v) hash and compare equal).
+__device__ __forceinline__ unsigned long long bits_of(double d) { return __builtin_bit_cast(unsigned long long, d); }
+__device__ __forceinline__ uint32_t bits_of(float f) { return __builtin_bit_cast(uint32_t, f); }
+
 __device__ __forceinline__ unsigned long long value_hash(dvec2 v) {
-    unsigned long long z = __builtin_bit_cast(unsigned long long, v.x) ^
-                           (__builtin_bit_cast(unsigned long long, v.y) * 0x9e3779b97f4a7c15ull);
+    unsigned long long z = bits_of(v.x) ^ (bits_of(v.y) * 0x9e3779b97f4a7c15ull);
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
     return (z ^ (z >> 31)) | 1ull;  // never 0
@@ -778,13 +783,11 @@ __global__ __launch_bounds__(256) void csr_pack_kernel(const uint64_t *__restric
             uint32_t s = slot_of(h), probe = 0;
             while (slots[s] != h && ++probe < kDictSlots) s = (s + 1) & (kDictSlots - 1);
             const uint32_t idx = probe < kDictSlots ? slot_index[s] : kDiagIndex;
-            const bool same = idx < kDictMax &&
-                              __builtin_bit_cast(unsigned long long, table64[idx].x) ==
-                                  __builtin_bit_cast(unsigned long long, v.x) &&
-                              __builtin_bit_cast(unsigned long long, table64[idx].y) ==
-                                  __builtin_bit_cast(unsigned long long, v.y) &&
-                              __builtin_bit_cast(uint32_t, table32[idx].x) == __builtin_bit_cast(uint32_t, vals32[k].x) &&
-                              __builtin_bit_cast(uint32_t, table32[idx].y) == __builtin_bit_cast(uint32_t, vals32[k].y);
+            const dvec2 t64 = idx < kDictMax ? table64[idx] : dvec2{0.0, 0.0};
+            const fvec2 t32 = idx < kDictMax ? table32[idx] : fvec2{0.f, 0.f};
+            const fvec2 v32 = vals32[k];
+            const bool same = idx < kDictMax && bits_of(t64.x) == bits_of(v.x) && bits_of(t64.y) == bits_of(v.y) &&
+                              bits_of(t32.x) == bits_of(v32.x) && bits_of(t32.y) == bits_of(v32.y);
             if (!same) {
                 atomicExch(&state[1], 1u);
                 return;

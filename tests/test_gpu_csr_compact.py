@@ -107,3 +107,34 @@ def test_degenerate_matrices():
     one = hop.restrict_charge(tcp.U1Sector(2, 2)).mvp_plan(storage="eager")
     assert one.dimension == 1
     np.testing.assert_allclose(one.apply(np.array([2.0 + 1.0j])), [0.25 * (2.0 + 1.0j)], rtol=1e-15)
+
+
+def test_values_that_differ_only_in_the_sign_of_an_imaginary_part():
+    # A Hermitian complex operator holds v and conj(v) as separate table
+    # values: X0 X1 + 2 X0 Y1 maps |00> -> |11> with 1 + 2i and |01> -> |10>
+    # with 1 - 2i.  Both must keep their own table entry (a hash or equality
+    # test that reads only the real part merges them).
+    n = 4
+    op = tcp.PauliOperator.from_terms(n, [("XXII", 1.0), ("XYII", 2.0), ("IIZI", 0.3)])
+    plan = op.csr_mvp_plan()
+    assert plan.storage_format == "compact"
+    _check(plan, dense.dense_operator(n, *op.code_arrays()), 3)
+    # a complex-hopping U(1) lattice sector: every hop and its reverse are conjugates
+    codes, coef = lattice.heisenberg_periodic(3, 3, seed=5)
+    rows, vals = [], []
+    for (a, b) in lattice.periodic_lattice_bonds(3, 3):
+        for pa, pb, c in ((1, 1, 0.8), (2, 2, 0.8), (1, 2, 0.6), (2, 1, -0.6)):
+            w = np.zeros(9, dtype=np.uint8)
+            w[a], w[b] = pa, pb
+            rows.append(w)
+            vals.append(c)
+    rows = np.concatenate([np.asarray(rows), codes])
+    vals = np.concatenate([np.asarray(vals, dtype=np.complex128), coef])
+    op = tcp.PauliOperator.from_code_arrays(rows, vals)
+    sector = tcp.U1Sector(9, 4)
+    plan = op.restrict_charge(sector).mvp_plan(storage="eager")
+    assert plan.storage_format == "compact"
+    basis = np.asarray(dense.blocks_basis([(9, 4)]), dtype=np.int64)
+    full = dense.dense_operator(9, rows, vals)[np.ix_(basis, basis)]
+    assert np.any(np.abs(full.imag) > 0.1)
+    _check(plan, full, 12)
