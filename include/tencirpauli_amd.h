@@ -317,6 +317,38 @@ int tcp_sector_plan_create_blocks(const tcp_operator *op, uint32_t nblocks, cons
 int tcp_sector_plan_create_fermion(const tcp_fermion *f, uint32_t nblocks, const uint32_t *sizes,
                                    const uint32_t *pops, uint64_t max_bytes, tcp_sector_plan **out);
 
+/* General additive-charge sectors on binary axes (qubits or fermion modes):
+ * ChargeSectorPlan's suffix-count rank authority (core/charge_sector.rs:8-16,
+ * rank/unrank :1426-1580, build_charge_sector_plan :1620) for sectors that are
+ * not contiguous fixed-population blocks -- e.g. total number and 2 S_z on
+ * interleaved spin orbitals.  Axis a is index bit n-1-a; constraint j reads
+ * levels[(j * naxes + a) * 2 + bit], an integer, and targets[j] (charge
+ * offsets already subtracted).  Order: lexicographic over axes, axis 0 most
+ * significant (ascending integer order).  At most 63 axes and 4 constraints;
+ * the rank tables hold (naxes + 1) x prod_j (sum_a |level_1 - level_0| + 1)
+ * counts (TCP_ERR_UNSUPPORTED above 2^24); dimensions below 2^32.  Host only. */
+typedef struct tcp_charge_sector tcp_charge_sector;
+int tcp_charge_sector_create(uint32_t naxes, uint32_t nconstraints, const int64_t *levels, const int64_t *targets,
+                             uint64_t max_bytes, tcp_charge_sector **out);
+uint64_t tcp_charge_sector_dimension(const tcp_charge_sector *sector);
+/* rank of an occupation (TCP_ERR_INVALID_SECTOR when a constraint fails) */
+int tcp_charge_sector_rank(const tcp_charge_sector *sector, uint64_t bitstring, uint64_t *rank);
+int tcp_charge_sector_unrank(const tcp_charge_sector *sector, uint64_t index, uint64_t *bitstring);
+/* states of ranks first .. first + count - 1 (ChargeSector.basis_states, py/charge.py:770-791) */
+int tcp_charge_sector_basis_words(const tcp_charge_sector *sector, uint64_t first, uint64_t count, uint64_t *out);
+void tcp_charge_sector_destroy(tcp_charge_sector *sector);
+/* Restricted plan of a Pauli operator on a general sector: the generic charge
+ * loop apply_charge_mvp_from_prepared_plan_into (core/charge/generic.rs:854-1051)
+ * as a destination gather.  Every charge must be exactly conserved
+ * (TCP_ERR_SECTOR_LEAKAGE otherwise); an aggregated transition that still
+ * leaves the sector makes applies fail with TCP_ERR_INVALID_SECTOR, as the
+ * reference's loop does. */
+int tcp_sector_plan_create_charge(const tcp_operator *op, const tcp_charge_sector *sector, uint64_t max_bytes,
+                                  tcp_sector_plan **out);
+/* The same for a fermion operator's ladder words (as tcp_sector_plan_create_fermion). */
+int tcp_sector_plan_create_fermion_charge(const tcp_fermion *f, const tcp_charge_sector *sector,
+                                          uint64_t max_bytes, tcp_sector_plan **out);
+
 uint64_t tcp_sector_plan_dimension(const tcp_sector_plan *plan);
 uint32_t tcp_sector_plan_nqubits(const tcp_sector_plan *plan);
 uint64_t tcp_sector_plan_term_count(const tcp_sector_plan *plan);
@@ -379,6 +411,13 @@ int tcp_csr_plan_create_full(const tcp_operator *op, uint64_t max_bytes, tcp_csr
  * leaves the sector fails the build with TCP_ERR_INVALID_SECTOR). */
 int tcp_csr_plan_create_fermion(const tcp_fermion *f, uint32_t nblocks, const uint32_t *sizes,
                                 const uint32_t *pops, uint64_t max_bytes, tcp_csr_plan **out);
+/* Eager CSR on a general charge sector (compile_charge_csr_from_prepared_plan,
+ * core/charge/eager.rs:133-204): no conservation precondition, leakage fails
+ * the build with TCP_ERR_INVALID_SECTOR. */
+int tcp_csr_plan_create_charge(const tcp_operator *op, const tcp_charge_sector *sector, uint64_t max_bytes,
+                               tcp_csr_plan **out);
+int tcp_csr_plan_create_fermion_charge(const tcp_fermion *f, const tcp_charge_sector *sector, uint64_t max_bytes,
+                                       tcp_csr_plan **out);
 uint64_t tcp_csr_plan_dimension(const tcp_csr_plan *plan);
 uint64_t tcp_csr_plan_transition_count(const tcp_csr_plan *plan); /* stored entries */
 uint64_t tcp_csr_plan_estimated_bytes(const tcp_csr_plan *plan);

@@ -10,7 +10,8 @@ Two independent oracles live here:
   reference algorithms -- canonicalisation (core/operator.rs:117-155),
   TermDirect apply (core/hamiltonian.rs:204-270), U(1) rank/unrank and the lazy
   restricted scatter (core/sector.rs:79-342, 579-642, 1051-1127), contiguous
-  fixed-population blocks with the generic charge loop
+  fixed-population blocks and general sectors (the suffix-count plan,
+  core/charge_sector.rs:1426-1578, 1620-1722) with the generic charge loop
   (core/charge_sector.rs:973-1038, core/charge/generic.rs:949-1045), the eager
   CSR apply (core/csr.rs:194-238) and ``analyze_charge``
   (core/operator.rs:396-445).
@@ -107,6 +108,12 @@ def _declare(L: ctypes.CDLL) -> None:
         c.c_uint32, _u32p, _u32p, c.c_uint64, _u8p, _dp, _dp, _dp, _u64p]
     L.orc_blocks_generic_apply_range.argtypes = [
         c.c_uint32, _u32p, _u32p, c.c_uint64, _u8p, _dp, _dp, _dp, c.c_uint64, c.c_uint64, _u64p]
+    i64p = c.POINTER(c.c_int64)
+    L.orc_charge_dimension.argtypes = [c.c_uint32, c.c_uint32, i64p, i64p, _u64p]
+    L.orc_charge_rank_many.argtypes = [c.c_uint32, c.c_uint32, i64p, i64p, _u64p, c.c_uint64, _u64p]
+    L.orc_charge_unrank_many.argtypes = [c.c_uint32, c.c_uint32, i64p, i64p, _u64p, c.c_uint64, _u64p]
+    L.orc_charge_generic_apply_range.argtypes = [
+        c.c_uint32, c.c_uint32, i64p, i64p, c.c_uint64, _u8p, _dp, _dp, _dp, c.c_uint64, c.c_uint64, _u64p]
     L.orc_csr_apply.argtypes = [c.c_uint64, _u64p, _u64p, _dp, _dp, _dp, c.c_uint64, c.c_uint64, c.c_int]
     L.orc_analyze_charge.argtypes = [c.c_uint32, c.c_uint64, _u8p, _dp, _dp, _u64p]
 
@@ -266,6 +273,67 @@ def blocks_generic_apply(blocks: Sequence[Tuple[int, int]], codes: np.ndarray, c
     leak = ctypes.c_uint64(0)
     lo, hi = sources if sources is not None else (0, 2**64 - 1)
     st = lib().orc_blocks_generic_apply_range(len(s), _p(s, _u32p), _p(p, _u32p), T, _p(codes, _u8p),
+                                              _p(cf.view(np.float64), _dp), _p(v.view(np.float64), _dp),
+                                              _p(y.view(np.float64), _dp), lo, hi, ctypes.byref(leak))
+    _check(st, f"source_index={leak.value}")
+    return y
+
+
+def _charge_arrays(levels, targets) -> Tuple[np.ndarray, np.ndarray]:  # type: ignore[no-untyped-def]
+    """levels [m][n][2] and targets [m] as contiguous int64 arrays."""
+    lv = np.ascontiguousarray(np.asarray(levels, dtype=np.int64))
+    if lv.ndim != 3 or lv.shape[2] != 2:
+        raise ValueError("levels must have shape [constraints, axes, 2]")
+    tg = np.ascontiguousarray(np.asarray(targets, dtype=np.int64).reshape(-1))
+    if tg.shape[0] != lv.shape[0]:
+        raise ValueError("one target per constraint")
+    return lv, tg
+
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def charge_dimension(levels, targets) -> int:  # type: ignore[no-untyped-def]
+    """Dimension of the general charge sector (build_charge_sector_plan)."""
+    lv, tg = _charge_arrays(levels, targets)
+    d = ctypes.c_uint64(0)
+    _check(lib().orc_charge_dimension(lv.shape[1], lv.shape[0], _p(lv, _i64p), _p(tg, _i64p), ctypes.byref(d)))
+    return int(d.value)
+
+
+def charge_rank_many(levels, targets, bitstrings: np.ndarray) -> np.ndarray:  # type: ignore[no-untyped-def]
+    """ChargeSectorPlan::rank of every bitstring (UINT64_MAX outside the sector)."""
+    lv, tg = _charge_arrays(levels, targets)
+    b = np.ascontiguousarray(bitstrings, dtype=np.uint64)
+    r = np.zeros(len(b), dtype=np.uint64)
+    _check(lib().orc_charge_rank_many(lv.shape[1], lv.shape[0], _p(lv, _i64p), _p(tg, _i64p), _p(b, _u64p), len(b),
+                                      _p(r, _u64p)))
+    return r
+
+
+def charge_unrank_many(levels, targets, indices: np.ndarray) -> np.ndarray:  # type: ignore[no-untyped-def]
+    """ChargeSectorPlan::unrank of every index."""
+    lv, tg = _charge_arrays(levels, targets)
+    i = np.ascontiguousarray(indices, dtype=np.uint64)
+    b = np.zeros(len(i), dtype=np.uint64)
+    _check(lib().orc_charge_unrank_many(lv.shape[1], lv.shape[0], _p(lv, _i64p), _p(tg, _i64p), _p(i, _u64p),
+                                        len(i), _p(b, _u64p)))
+    return b
+
+
+def charge_generic_apply(levels, targets, codes: np.ndarray, coef: np.ndarray,  # type: ignore[no-untyped-def]
+                         state: np.ndarray, sources: Optional[Tuple[int, int]] = None) -> np.ndarray:
+    """The generic charge loop of a Pauli operator on a general charge sector
+    (``sources=(begin, end)``: those source ranks only)."""
+    lv, tg = _charge_arrays(levels, targets)
+    n = lv.shape[1]
+    codes, cf = _terms(n, codes, coef)
+    T = codes.shape[0]
+    v = np.ascontiguousarray(state, dtype=np.complex128)
+    y = np.empty_like(v)
+    leak = ctypes.c_uint64(0)
+    lo, hi = sources if sources is not None else (0, 2**64 - 1)
+    st = lib().orc_charge_generic_apply_range(n, lv.shape[0], _p(lv, _i64p), _p(tg, _i64p), T, _p(codes, _u8p),
                                               _p(cf.view(np.float64), _dp), _p(v.view(np.float64), _dp),
                                               _p(y.view(np.float64), _dp), lo, hi, ctypes.byref(leak))
     _check(st, f"source_index={leak.value}")

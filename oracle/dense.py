@@ -70,6 +70,28 @@ def blocks_basis(blocks: Sequence[Tuple[int, int]]) -> List[int]:
     return out
 
 
+def charge_basis(levels, targets) -> List[int]:  # type: ignore[no-untyped-def]
+    """Brute-force basis of a general charge sector (qubit 0 = MSB).
+
+    ``levels[j][a] = (value with bit a clear, value with bit a set)`` of
+    constraint j; a state is selected when every charge, summed over the
+    axes, equals its target.  Ascending integer order, as
+    tests/test_charge.py:63-83 enumerates ``itertools.product`` over axes.
+    Vectorised over all 2^n integers (n <= ~24).
+    """
+    lv = np.asarray(levels, dtype=np.int64)
+    m, n = lv.shape[0], lv.shape[1]
+    states = np.arange(1 << n, dtype=np.int64)
+    keep = np.ones(states.shape[0], dtype=bool)
+    for j in range(m):
+        total = np.zeros(states.shape[0], dtype=np.int64)
+        for a in range(n):
+            bit = (states >> (n - 1 - a)) & 1
+            total += np.where(bit == 1, lv[j, a, 1], lv[j, a, 0])
+        keep &= total == int(targets[j])
+    return [int(b) for b in states[keep]]
+
+
 def comb_rank(bits: int, k: int) -> int:
     """Combinatorial number system: sum_j C(p_j, j) over ascending set bits."""
     rank = 0

@@ -697,22 +697,28 @@ int orc_blocks_unrank_many(uint32_t nblocks, const uint32_t *sizes, const uint32
 /* same way).  Exact-zero sums are skipped; a nonzero value whose            */
 /* destination is outside the sector is InvalidSector (leakage).            */
 /* ------------------------------------------------------------------------ */
+/* The sector is given by its rank/unrank pair (the plan's rank authority). */
+typedef struct {
+    const void *ctx;
+    uint32_t n;
+    uint64_t dim;
+    int (*rank)(const void *ctx, uint64_t bitstring, uint64_t *rank);
+    int (*unrank)(const void *ctx, uint64_t index, uint64_t *bitstring);
+} sector_ops;
+
 /* Sources [col_begin, col_end) only (a bounded sample for timing). */
-int orc_blocks_generic_apply_range(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops,
-                                   uint64_t T, const uint8_t *codes, const double *coef,
-                                   const double *state, double *output, uint64_t col_begin, uint64_t col_end,
-                                   uint64_t *leak_source) {
-    blocks_sector b;
-    int st = blocks_new(nblocks, sizes, pops, &b);
-    if (st) return st;
-    if (col_end > b.total_dim) col_end = b.total_dim;
-    uint32_t n = b.n;
+static int generic_apply_range(const sector_ops *sec, uint64_t T, const uint8_t *codes, const double *coef,
+                               const double *state, double *output, uint64_t col_begin, uint64_t col_end,
+                               uint64_t *leak_source) {
+    int st = ORC_OK;
+    if (col_end > sec->dim) col_end = sec->dim;
+    uint32_t n = sec->n;
     uint64_t *keys = (uint64_t *)malloc((T + 1) * sizeof(uint64_t));
     cplx *vals = (cplx *)malloc((T + 1) * sizeof(cplx));
-    for (uint64_t i = 0; i < 2 * b.total_dim; ++i) output[i] = 0.0;
+    for (uint64_t i = 0; i < 2 * sec->dim; ++i) output[i] = 0.0;
     for (uint64_t col = col_begin; col < col_end && !st; ++col) {
         uint64_t src;
-        st = blocks_unrank(&b, col, &src);
+        st = sec->unrank(sec->ctx, col, &src);
         if (st) break;
         cplx sv = {state[2 * col], state[2 * col + 1]};
         uint64_t nk = 0;
@@ -751,7 +757,7 @@ int orc_blocks_generic_apply_range(uint32_t nblocks, const uint32_t *sizes, cons
         for (uint64_t j = 0; j < nk; ++j) {
             if (c_is_zero(vals[j])) continue;
             uint64_t row;
-            int rs = blocks_rank(&b, keys[j], &row);
+            int rs = sec->rank(sec->ctx, keys[j], &row);
             if (rs) {
                 if (leak_source) *leak_source = col;
                 st = ORC_INVALID_SECTOR;
@@ -767,11 +773,223 @@ int orc_blocks_generic_apply_range(uint32_t nblocks, const uint32_t *sizes, cons
     return st;
 }
 
+static int blocks_rank_cb(const void *ctx, uint64_t bitstring, uint64_t *rank) {
+    return blocks_rank((const blocks_sector *)ctx, bitstring, rank);
+}
+static int blocks_unrank_cb(const void *ctx, uint64_t index, uint64_t *bitstring) {
+    return blocks_unrank((const blocks_sector *)ctx, index, bitstring);
+}
+
+int orc_blocks_generic_apply_range(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops,
+                                   uint64_t T, const uint8_t *codes, const double *coef,
+                                   const double *state, double *output, uint64_t col_begin, uint64_t col_end,
+                                   uint64_t *leak_source) {
+    blocks_sector b;
+    int st = blocks_new(nblocks, sizes, pops, &b);
+    if (st) return st;
+    sector_ops sec = {&b, b.n, b.total_dim, blocks_rank_cb, blocks_unrank_cb};
+    return generic_apply_range(&sec, T, codes, coef, state, output, col_begin, col_end, leak_source);
+}
+
 int orc_blocks_generic_apply(uint32_t nblocks, const uint32_t *sizes, const uint32_t *pops,
                              uint64_t T, const uint8_t *codes, const double *coef,
                              const double *state, double *output, uint64_t *leak_source) {
     return orc_blocks_generic_apply_range(nblocks, sizes, pops, T, codes, coef, state, output, 0, UINT64_MAX,
                                           leak_source);
+}
+
+/* ------------------------------------------------------------------------ */
+/* General charge sector: ChargeSectorPlan (core/charge_sector.rs:8-16) as   */
+/* built by build_charge_sector_plan (:1620-1722) for binary axes.  Axis a   */
+/* (bit n-1-a) has the two candidate contributions levels[j][a][0|1] per     */
+/* constraint j.  suffix[p] maps a remainder vector to the number of         */
+/* assignments of axes p..n-1 whose contributions sum to it: suffix[n] =     */
+/* {0: 1}; suffix[p][c + r] += suffix[p+1][r] for every candidate c of axis  */
+/* p.  The reference keys hash maps; here each table is an array sorted by   */
+/* remainder and searched by bisection (the lookup order is not observable). */
+/* rank (:1442-1503): for each axis, the counts of the smaller candidate     */
+/* (remaining - c0) are added when the axis is set; the final remainder must */
+/* be zero (InvalidSector).  unrank (:1529-1578): candidate 0 is taken when  */
+/* the index is below its count, else the index drops by that count.        */
+/* ------------------------------------------------------------------------ */
+#define ORC_MAX_CHARGES 4
+typedef struct {
+    int64_t key[ORC_MAX_CHARGES];
+    uint64_t count;
+} charge_entry;
+
+typedef struct {
+    uint32_t n, m;
+    int64_t *levels; /* [m][n][2] */
+    int64_t target[ORC_MAX_CHARGES];
+    charge_entry **suffix; /* [n + 1] tables */
+    uint64_t *len;         /* [n + 1] */
+    uint64_t dim;
+} charge_sector;
+
+static int cmp_charge_entry(const void *pa, const void *pb) {
+    const charge_entry *a = (const charge_entry *)pa, *b = (const charge_entry *)pb;
+    for (int j = 0; j < ORC_MAX_CHARGES; ++j)
+        if (a->key[j] != b->key[j]) return a->key[j] < b->key[j] ? -1 : 1;
+    return 0;
+}
+
+static uint64_t charge_lookup(const charge_sector *s, uint32_t p, const int64_t *key) {
+    charge_entry probe;
+    memset(&probe, 0, sizeof probe);
+    for (uint32_t j = 0; j < s->m; ++j) probe.key[j] = key[j];
+    const charge_entry *hit =
+        (const charge_entry *)bsearch(&probe, s->suffix[p], s->len[p], sizeof(charge_entry), cmp_charge_entry);
+    return hit ? hit->count : 0;
+}
+
+static void charge_free(charge_sector *s) {
+    if (s->suffix)
+        for (uint32_t p = 0; p <= s->n; ++p) free(s->suffix[p]);
+    free(s->suffix);
+    free(s->len);
+    free(s->levels);
+}
+
+static int charge_new(uint32_t n, uint32_t m, const int64_t *levels, const int64_t *targets, charge_sector *s) {
+    memset(s, 0, sizeof *s);
+    if (m == 0 || m > ORC_MAX_CHARGES || n > 63) return ORC_INVALID_SECTOR;
+    s->n = n;
+    s->m = m;
+    s->levels = (int64_t *)malloc(sizeof(int64_t) * 2 * n * m + 1);
+    memcpy(s->levels, levels, sizeof(int64_t) * 2 * n * m);
+    for (uint32_t j = 0; j < m; ++j) s->target[j] = targets[j];
+    s->suffix = (charge_entry **)calloc(n + 1, sizeof(charge_entry *));
+    s->len = (uint64_t *)calloc(n + 1, sizeof(uint64_t));
+    s->suffix[n] = (charge_entry *)calloc(1, sizeof(charge_entry));
+    s->len[n] = 1;
+    s->suffix[n][0].count = 1;
+    for (int p = (int)n - 1; p >= 0; --p) {
+        const uint64_t prev = s->len[p + 1];
+        charge_entry *t = (charge_entry *)malloc(sizeof(charge_entry) * 2 * prev + 1);
+        uint64_t k = 0;
+        for (uint32_t v = 0; v < 2; ++v)
+            for (uint64_t i = 0; i < prev; ++i) {
+                t[k] = s->suffix[p + 1][i];
+                for (uint32_t j = 0; j < m; ++j) t[k].key[j] += levels[((uint64_t)j * n + p) * 2 + v];
+                ++k;
+            }
+        qsort(t, k, sizeof(charge_entry), cmp_charge_entry);
+        uint64_t w = 0;
+        for (uint64_t i = 0; i < k; ++i) {
+            if (w && cmp_charge_entry(&t[w - 1], &t[i]) == 0) {
+                if (t[w - 1].count > UINT64_MAX - t[i].count) {
+                    free(t);
+                    charge_free(s);
+                    return ORC_OVERFLOW;
+                }
+                t[w - 1].count += t[i].count;
+            } else {
+                t[w++] = t[i];
+            }
+        }
+        s->suffix[p] = t;
+        s->len[p] = w;
+    }
+    s->dim = charge_lookup(s, 0, s->target);
+    return ORC_OK;
+}
+
+static int charge_rank(const charge_sector *s, uint64_t bitstring, uint64_t *rank) {
+    const uint32_t n = s->n, m = s->m;
+    if (n < 64 && (bitstring >> n)) return ORC_INVALID_INDEX;
+    int64_t rem[ORC_MAX_CHARGES], cand[ORC_MAX_CHARGES];
+    for (uint32_t j = 0; j < m; ++j) rem[j] = s->target[j];
+    uint64_t r = 0;
+    for (uint32_t a = 0; a < n; ++a) {
+        const uint32_t value = (uint32_t)((bitstring >> (n - 1 - a)) & 1);
+        if (value) {
+            for (uint32_t j = 0; j < m; ++j) cand[j] = rem[j] - s->levels[((uint64_t)j * n + a) * 2 + 0];
+            r += charge_lookup(s, a + 1, cand);
+        }
+        for (uint32_t j = 0; j < m; ++j) rem[j] -= s->levels[((uint64_t)j * n + a) * 2 + value];
+    }
+    for (uint32_t j = 0; j < m; ++j)
+        if (rem[j] != 0) return ORC_INVALID_SECTOR;
+    *rank = r;
+    return ORC_OK;
+}
+
+static int charge_unrank(const charge_sector *s, uint64_t index, uint64_t *bitstring) {
+    const uint32_t n = s->n, m = s->m;
+    if (index >= s->dim) return ORC_INVALID_INDEX;
+    int64_t rem[ORC_MAX_CHARGES], cand[ORC_MAX_CHARGES];
+    for (uint32_t j = 0; j < m; ++j) rem[j] = s->target[j];
+    uint64_t word = 0;
+    for (uint32_t a = 0; a < n; ++a) {
+        int selected = -1;
+        for (uint32_t v = 0; v < 2 && selected < 0; ++v) {
+            for (uint32_t j = 0; j < m; ++j) cand[j] = rem[j] - s->levels[((uint64_t)j * n + a) * 2 + v];
+            const uint64_t c = charge_lookup(s, a + 1, cand);
+            if (index < c) selected = (int)v;
+            else index -= c;
+        }
+        if (selected < 0) return ORC_INVALID_SECTOR;
+        if (selected) word |= 1ull << (n - 1 - a);
+        for (uint32_t j = 0; j < m; ++j) rem[j] = cand[j];
+    }
+    *bitstring = word;
+    return ORC_OK;
+}
+
+static int charge_rank_cb(const void *ctx, uint64_t bitstring, uint64_t *rank) {
+    return charge_rank((const charge_sector *)ctx, bitstring, rank);
+}
+static int charge_unrank_cb(const void *ctx, uint64_t index, uint64_t *bitstring) {
+    return charge_unrank((const charge_sector *)ctx, index, bitstring);
+}
+
+int orc_charge_dimension(uint32_t n, uint32_t m, const int64_t *levels, const int64_t *targets, uint64_t *dim) {
+    charge_sector s;
+    int st = charge_new(n, m, levels, targets, &s);
+    if (st) return st;
+    *dim = s.dim;
+    charge_free(&s);
+    return ORC_OK;
+}
+
+/* ranks[i] = UINT64_MAX where bitstrings[i] is outside the sector */
+int orc_charge_rank_many(uint32_t n, uint32_t m, const int64_t *levels, const int64_t *targets,
+                         const uint64_t *bitstrings, uint64_t count, uint64_t *ranks) {
+    charge_sector s;
+    int st = charge_new(n, m, levels, targets, &s);
+    if (st) return st;
+    for (uint64_t i = 0; i < count && !st; ++i) {
+        st = charge_rank(&s, bitstrings[i], &ranks[i]);
+        if (st == ORC_INVALID_SECTOR) {
+            ranks[i] = UINT64_MAX;
+            st = ORC_OK;
+        }
+    }
+    charge_free(&s);
+    return st;
+}
+
+int orc_charge_unrank_many(uint32_t n, uint32_t m, const int64_t *levels, const int64_t *targets,
+                           const uint64_t *indices, uint64_t count, uint64_t *bitstrings) {
+    charge_sector s;
+    int st = charge_new(n, m, levels, targets, &s);
+    if (st) return st;
+    for (uint64_t i = 0; i < count && !st; ++i) st = charge_unrank(&s, indices[i], &bitstrings[i]);
+    charge_free(&s);
+    return st;
+}
+
+int orc_charge_generic_apply_range(uint32_t n, uint32_t m, const int64_t *levels, const int64_t *targets,
+                                   uint64_t T, const uint8_t *codes, const double *coef, const double *state,
+                                   double *output, uint64_t col_begin, uint64_t col_end, uint64_t *leak_source) {
+    charge_sector s;
+    int st = charge_new(n, m, levels, targets, &s);
+    if (st) return st;
+    sector_ops sec = {&s, n, s.dim, charge_rank_cb, charge_unrank_cb};
+    st = generic_apply_range(&sec, T, codes, coef, state, output, col_begin, col_end, leak_source);
+    charge_free(&s);
+    return st;
 }
 
 /* ------------------------------------------------------------------------ */

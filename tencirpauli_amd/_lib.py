@@ -137,6 +137,17 @@ SIGNATURES = {
                                                 _c.c_uint64, _c.POINTER(_vp)]),
     "tcp_sector_plan_create_fermion": (_c.c_int, [_vp, _c.c_uint32, _c.POINTER(_c.c_uint32),
                                                    _c.POINTER(_c.c_uint32), _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_charge_sector_create": (_c.c_int, [_c.c_uint32, _c.c_uint32, _c.POINTER(_c.c_int64),
+                                            _c.POINTER(_c.c_int64), _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_charge_sector_dimension": (_c.c_uint64, [_vp]),
+    "tcp_charge_sector_rank": (_c.c_int, [_vp, _c.c_uint64, _u64p]),
+    "tcp_charge_sector_unrank": (_c.c_int, [_vp, _c.c_uint64, _u64p]),
+    "tcp_charge_sector_basis_words": (_c.c_int, [_vp, _c.c_uint64, _c.c_uint64, _u64p]),
+    "tcp_charge_sector_destroy": (None, [_vp]),
+    "tcp_sector_plan_create_charge": (_c.c_int, [_vp, _vp, _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_sector_plan_create_fermion_charge": (_c.c_int, [_vp, _vp, _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_csr_plan_create_charge": (_c.c_int, [_vp, _vp, _c.c_uint64, _c.POINTER(_vp)]),
+    "tcp_csr_plan_create_fermion_charge": (_c.c_int, [_vp, _vp, _c.c_uint64, _c.POINTER(_vp)]),
     "tcp_csr_plan_dimension": (_c.c_uint64, [_vp]),
     "tcp_csr_plan_transition_count": (_c.c_uint64, [_vp]),
     "tcp_csr_plan_estimated_bytes": (_c.c_uint64, [_vp]),

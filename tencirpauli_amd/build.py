@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtencirpauli_amd.so")
-SOURCES = ["operator.cpp", "fermion.cpp", "mvp_plan.hip", "sector_plan.hip", "wide_sector.hip", "davidson.hip", "mlp.hip",
+SOURCES = ["operator.cpp", "fermion.cpp", "mvp_plan.hip", "sector_plan.hip", "wide_sector.hip", "charge_sector.hip", "davidson.hip", "mlp.hip",
            "batched.hip", "full_matrix.hip", "lanczos.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -5,8 +5,12 @@ Charges act on binary axes: qubits, or fermion modes of an ``OperatorSpace``
 number of every block of a partition of the axes into at most four
 contiguous blocks -- e.g. total number plus 2 S_z fixing N_up and N_down --
 is the factorized layout (core/charge_sector.rs:150-217, 1786-1868) and runs
-on the GPU block kernel.  Rank order is lexicographic over axes with axis 0
-most significant (``operator_space_axis0_msb_mixed_radix``).
+on the GPU block kernel.  Every other sector of integer charges on at most
+63 axes (interleaved spin orbitals, weights other than 0/1, free spectator
+axes, unreachable targets) uses the suffix-count rank tables of
+ChargeSectorPlan (core/charge_sector.rs:1211-1580) and the generic gather
+kernel.  Rank order is lexicographic over axes with axis 0 most significant
+(``operator_space_axis0_msb_mixed_radix``).
 """
 
 from __future__ import annotations
@@ -20,7 +24,8 @@ from typing import Any, Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _lib
-from ._plan import DEFAULT_MAX_BYTES, NativePlanFacade, effective_max_bytes, validate_max_bytes
+from ._plan import (DEFAULT_MAX_BYTES, NativePlanFacade, check_allocation, effective_max_bytes,
+                    validate_max_bytes)
 
 
 @dataclass(frozen=True, eq=True)
@@ -117,29 +122,39 @@ class ChargeSector:
         self.nqubits = n
         self.space = spaces.pop() if spaces else None
         self.blocks = _factorized_blocks(constraints)
+        self._general: Optional[_GeneralSector] = None
+        if self.blocks is None:
+            self._general = _GeneralSector(constraints, max_bytes)
+        self.execution_family = "factorized" if self.blocks is not None else "generic"
 
     @property
     def dimension(self) -> int:
-        if self.blocks is None:
-            raise NotImplementedError("only contiguous fixed-population qubit blocks are implemented")
+        if self._general is not None:
+            return self._general.dimension
         return math.prod(math.comb(s, k) for s, k in self.blocks)
 
     def _arrays(self) -> Tuple[np.ndarray, np.ndarray]:
         if self.blocks is None:
-            raise NotImplementedError("only contiguous fixed-population qubit blocks are implemented")
+            raise ValueError("the sector is not a product of contiguous fixed-population blocks")
         sizes = np.ascontiguousarray([b[0] for b in self.blocks], dtype=np.uint32)
         pops = np.ascontiguousarray([b[1] for b in self.blocks], dtype=np.uint32)
         return sizes, pops
 
     def rank(self, occupations: Sequence[int]) -> int:
+        """Rank of a selected occupation (ValueError when a constraint fails)."""
         bits = tuple(int(b) for b in occupations)
-        if len(bits) != self.nqubits or any(b not in (0, 1) for b in bits):
-            raise ValueError("occupations must contain exactly nqubits binary values")
+        if len(bits) != self.nqubits:
+            raise ValueError("occupations must match OperatorSpace.axes length")
+        if any(b not in (0, 1) for b in bits):
+            raise ValueError("occupation is outside the finite sector layout")
         value = 0
         for b in bits:
             value = (value << 1) | b
-        sizes, pops = self._arrays()
         out = ctypes.c_uint64()
+        if self._general is not None:
+            _lib.check(_lib.lib().tcp_charge_sector_rank(self._general.ptr, value, ctypes.byref(out)))
+            return int(out.value)
+        sizes, pops = self._arrays()
         u32 = ctypes.POINTER(ctypes.c_uint32)
         _lib.check(_lib.lib().tcp_blocks_rank(len(sizes), sizes.ctypes.data_as(u32), pops.ctypes.data_as(u32),
                                               value, ctypes.byref(out)))
@@ -147,20 +162,64 @@ class ChargeSector:
 
     def unrank(self, index: int) -> Tuple[int, ...]:
         if not 0 <= int(index) < self.dimension:
-            raise IndexError("restricted basis index is out of range")
-        sizes, pops = self._arrays()
+            raise IndexError("sector index is out of range")
         out = ctypes.c_uint64()
-        u32 = ctypes.POINTER(ctypes.c_uint32)
-        _lib.check(_lib.lib().tcp_blocks_unrank(len(sizes), sizes.ctypes.data_as(u32),
-                                                pops.ctypes.data_as(u32), int(index), ctypes.byref(out)))
+        if self._general is not None:
+            _lib.check(_lib.lib().tcp_charge_sector_unrank(self._general.ptr, int(index), ctypes.byref(out)))
+        else:
+            sizes, pops = self._arrays()
+            u32 = ctypes.POINTER(ctypes.c_uint32)
+            _lib.check(_lib.lib().tcp_blocks_unrank(len(sizes), sizes.ctypes.data_as(u32),
+                                                    pops.ctypes.data_as(u32), int(index), ctypes.byref(out)))
         value = int(out.value)
         return tuple((value >> (self.nqubits - 1 - q)) & 1 for q in range(self.nqubits))
 
-    def basis_states(self) -> np.ndarray:
-        rows = np.asarray([self.unrank(i) for i in range(self.dimension)], dtype=np.uint8)
-        rows = rows.reshape(self.dimension, self.nqubits)
+    def basis_states(self, *, max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> np.ndarray:
+        """Every selected occupation as read-only uint64 rows in rank order (py/charge.py:770-791)."""
+        validate_max_bytes(max_bytes)
+        dim, n = self.dimension, self.nqubits
+        check_allocation(dim * max(n, 1) * 8, max_bytes, "charge-sector basis states")
+        if self._general is not None:
+            words = np.empty(dim, dtype=np.uint64)
+            _lib.check(_lib.lib().tcp_charge_sector_basis_words(
+                self._general.ptr, 0, dim, words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+            shifts = np.arange(n - 1, -1, -1, dtype=np.uint64)
+            rows = (words[:, None] >> shifts[None, :]) & np.uint64(1)
+        else:
+            rows = np.asarray([self.unrank(i) for i in range(dim)], dtype=np.uint64)
+        rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(dim, n)
         rows.flags.writeable = False
         return rows
+
+
+class _GeneralSector:
+    """Owner of a native general-sector rank authority (``tcp_charge_sector``)."""
+
+    def __init__(self, constraints: Sequence[Tuple[AdditiveCharge, int]], max_bytes: Optional[int]) -> None:
+        self.ptr = None
+        n, m = constraints[0][0].nqubits, len(constraints)
+        levels = np.zeros((m, n, 2), dtype=np.int64)
+        targets = np.zeros(m, dtype=np.int64)
+        for j, (charge, target) in enumerate(constraints):
+            pairs = charge.qubit_levels
+            values = [v for pair in pairs for v in pair] + [charge.offset]
+            if any(float(v) != int(v) for v in values):
+                raise ValueError("charges of sectors that are not contiguous fixed-population blocks need "
+                                 "integer levels")
+            levels[j] = np.asarray(pairs, dtype=np.float64).reshape(n, 2).astype(np.int64)
+            targets[j] = int(target) - int(charge.offset)
+        ptr = ctypes.c_void_p()
+        i64 = ctypes.POINTER(ctypes.c_int64)
+        _lib.check(_lib.lib().tcp_charge_sector_create(n, m, levels.ctypes.data_as(i64),
+                                                       targets.ctypes.data_as(i64), effective_max_bytes(max_bytes),
+                                                       ctypes.byref(ptr)))
+        self.ptr = ptr
+        self.dimension = int(_lib.lib().tcp_charge_sector_dimension(ptr))
+
+    def __del__(self) -> None:
+        if getattr(self, "ptr", None) and getattr(_lib, "_lib", None) is not None:
+            _lib.lib().tcp_charge_sector_destroy(self.ptr)
+            self.ptr = None
 
 
 def _factorized_blocks(constraints: Sequence[Tuple[AdditiveCharge, int]]) -> Optional[Tuple[Tuple[int, int], ...]]:
@@ -172,7 +231,8 @@ def _factorized_blocks(constraints: Sequence[Tuple[AdditiveCharge, int]]) -> Opt
     sum_B w_{c,B} n_B = target_c - const_c.  The sector is the factorized
     layout (core/charge_sector.rs:150-217, 1786-1868) when exactly one
     population vector within the block sizes solves them; a block that no
-    charge weighs, or several solutions, make a union of layouts (None).
+    charge weighs, more than four blocks, no solution (an empty sector) or
+    several solutions leave it to the general tables (None).
     """
     n = constraints[0][0].nqubits
     weights = [[l1 - l0 for l0, l1 in c.qubit_levels] for c, _ in constraints]
@@ -189,9 +249,7 @@ def _factorized_blocks(constraints: Sequence[Tuple[AdditiveCharge, int]]) -> Opt
     if len(runs) > 4:
         return None
     solutions = _block_populations([r[2] for r in runs], [r[1] for r in runs], rhs)
-    if not solutions:
-        raise ValueError("charge target is unreachable")
-    if len(solutions) > 1:
+    if len(solutions) != 1:
         return None
     return tuple((run[1], pop) for run, pop in zip(runs, solutions[0]))
 
@@ -223,10 +281,11 @@ def _block_populations(columns: Sequence[Tuple[float, ...]], sizes: Sequence[int
 
 
 class ChargeRestrictedOperator(NativePlanFacade):
-    """Pauli operator restricted to a factorized charge sector (py/charge.py:1075-1412).
+    """Pauli operator restricted to a charge sector (py/charge.py:1075-1412).
 
     ``storage="eager"`` materialises the restricted operator as device CSR
     (core/charge/eager.rs); ``"lazy"`` keeps the matrix-free plan.
+    Factorized sectors run the block kernel, the others the generic one.
     """
 
     def __init__(self, operator: Any, sector: ChargeSector, plan: Any, storage: str, csr_plan: Any = None) -> None:
@@ -238,8 +297,8 @@ class ChargeRestrictedOperator(NativePlanFacade):
         self._plan = plan
         self._csr = csr_plan
         self._lock = threading.Lock()
-        self.native_variant = "factorized"
-        self.execution_family = "factorized"
+        self.native_variant = sector.execution_family
+        self.execution_family = sector.execution_family
 
     @property
     def plan(self) -> Any:
@@ -258,10 +317,7 @@ class ChargeRestrictedOperator(NativePlanFacade):
     def _ensure_eager(self, max_bytes: Optional[int]) -> Any:
         with self._lock:
             if self._csr is None:
-                from .eager import csr_plan_blocks
-
-                sizes, pops = self.sector._arrays()
-                self._csr = csr_plan_blocks(self.operator, sizes, pops, max_bytes)
+                self._csr = _eager_pauli_plan(self.operator, self.sector, max_bytes)
             return self._csr
 
     def mvp_plan(self, *, storage: str = "lazy", max_bytes: Optional[int] = DEFAULT_MAX_BYTES) -> Any:
@@ -314,15 +370,21 @@ def _fermion_plan(operator: Any, sector: ChargeSector, max_bytes: Optional[int],
     from .symmetry import SectorMvpPlan
 
     _lib.require_device()
-    sizes, pops = sector._arrays()
     out = ctypes.c_void_p()
-    u32 = ctypes.POINTER(ctypes.c_uint32)
-    create = _lib.lib().tcp_csr_plan_create_fermion if eager else _lib.lib().tcp_sector_plan_create_fermion
-    _lib.check(create(operator._handle.ptr, len(sizes), sizes.ctypes.data_as(u32), pops.ctypes.data_as(u32),
-                      effective_max_bytes(max_bytes), ctypes.byref(out)))
+    L = _lib.lib()
+    if sector._general is not None:
+        create = L.tcp_csr_plan_create_fermion_charge if eager else L.tcp_sector_plan_create_fermion_charge
+        _lib.check(create(operator._handle.ptr, sector._general.ptr, effective_max_bytes(max_bytes),
+                          ctypes.byref(out)))
+    else:
+        sizes, pops = sector._arrays()
+        u32 = ctypes.POINTER(ctypes.c_uint32)
+        create = L.tcp_csr_plan_create_fermion if eager else L.tcp_sector_plan_create_fermion
+        _lib.check(create(operator._handle.ptr, len(sizes), sizes.ctypes.data_as(u32), pops.ctypes.data_as(u32),
+                          effective_max_bytes(max_bytes), ctypes.byref(out)))
     if eager:
-        return CsrMvpPlan(out, "factorized", "fermion", operator.term_count)
-    return SectorMvpPlan(out, "factorized", "fermion")
+        return CsrMvpPlan(out, sector.execution_family, "fermion", operator.term_count)
+    return SectorMvpPlan(out, sector.execution_family, "fermion")
 
 
 def restrict_fermion(operator: Any, sector: ChargeSector, *, storage: str = "lazy",
@@ -330,9 +392,6 @@ def restrict_fermion(operator: Any, sector: ChargeSector, *, storage: str = "laz
     if storage not in ("lazy", "eager"):
         raise ValueError("storage must be either 'eager' or 'lazy'")
     validate_max_bytes(max_bytes)
-    if sector.blocks is None:
-        raise NotImplementedError("only sectors that fix the particle number of contiguous mode blocks are "
-                                  "implemented on the GPU")
     plan = _fermion_plan(operator, sector, max_bytes, eager=storage == "eager")
     restricted = FermionRestrictedOperator(operator, sector, None if storage == "eager" else plan, storage,
                                            plan if storage == "eager" else None)
@@ -344,13 +403,30 @@ def _lazy_blocks_plan(operator: Any, sector: ChargeSector, max_bytes: Optional[i
     from .symmetry import SectorMvpPlan
 
     _lib.require_device()
-    sizes, pops = sector._arrays()
     out = ctypes.c_void_p()
+    if sector._general is not None:
+        _lib.check(_lib.lib().tcp_sector_plan_create_charge(operator._handle.ptr, sector._general.ptr,
+                                                             effective_max_bytes(max_bytes), ctypes.byref(out)))
+        return SectorMvpPlan(out, "generic", "generic")
+    sizes, pops = sector._arrays()
     u32 = ctypes.POINTER(ctypes.c_uint32)
     _lib.check(_lib.lib().tcp_sector_plan_create_blocks(operator._handle.ptr, len(sizes), sizes.ctypes.data_as(u32),
                                                          pops.ctypes.data_as(u32), effective_max_bytes(max_bytes),
                                                          ctypes.byref(out)))
     return SectorMvpPlan(out, "factorized", "factorized")
+
+
+def _eager_pauli_plan(operator: Any, sector: ChargeSector, max_bytes: Optional[int]) -> Any:
+    from .eager import CsrMvpPlan, csr_plan_blocks
+
+    if sector._general is None:
+        sizes, pops = sector._arrays()
+        return csr_plan_blocks(operator, sizes, pops, max_bytes)
+    _lib.require_device()
+    out = ctypes.c_void_p()
+    _lib.check(_lib.lib().tcp_csr_plan_create_charge(operator._handle.ptr, sector._general.ptr,
+                                                      effective_max_bytes(max_bytes), ctypes.byref(out)))
+    return CsrMvpPlan(out, "generic", "generic", operator.term_count)
 
 
 def restrict_charge(operator: Any, sector: ChargeSector, *, storage: str = "lazy",
@@ -360,12 +436,6 @@ def restrict_charge(operator: Any, sector: ChargeSector, *, storage: str = "lazy
     validate_max_bytes(max_bytes)
     if operator.nqubits != sector.nqubits:
         raise ValueError("Pauli operator and charge sector layouts are incompatible")
-    if sector.blocks is None:
-        raise NotImplementedError("only contiguous fixed-population qubit blocks are implemented")
     if storage == "eager":
-        from .eager import csr_plan_blocks
-
-        sizes, pops = sector._arrays()
-        return ChargeRestrictedOperator(operator, sector, None, storage,
-                                        csr_plan_blocks(operator, sizes, pops, max_bytes))
+        return ChargeRestrictedOperator(operator, sector, None, storage, _eager_pauli_plan(operator, sector, max_bytes))
     return ChargeRestrictedOperator(operator, sector, _lazy_blocks_plan(operator, sector, max_bytes), storage)

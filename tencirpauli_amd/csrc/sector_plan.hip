@@ -35,6 +35,7 @@
 
 #include "full_matrix.hpp"
 #include "tcp_internal.hpp"
+#include "charge_sector.hpp"
 #include "wide_sector.hpp"
 
 namespace {
@@ -851,6 +852,11 @@ __global__ __launch_bounds__(256) void csr_compact_spmv_kernel(const uint64_t *_
 
 }  // namespace
 
+// Host rank authority of a general charge sector.
+struct tcp_charge_sector {
+    tcp::ChargeGeometry geo;
+};
+
 struct tcp_sector_plan {
     uint32_t n = 0;
     uint32_t nblocks = 0;
@@ -867,7 +873,8 @@ struct tcp_sector_plan {
     SectorArgs host_args{};
     bool cond = false;  // terms are fermion ladder words with occupancy conditions
     tcp::DeviceBuffer gx, goff, tz, tw64, tw32, lut, choose, tc1, tc0, gblk, cbase;
-    std::unique_ptr<tcp::WideSector> wide;  // U(1) sectors of 64 or more qubits
+    std::unique_ptr<tcp::WideSector> wide;        // U(1) sectors of 64 or more qubits
+    std::unique_ptr<tcp::ChargeSector> general;   // charge sectors that are not contiguous blocks
     uint64_t retained_bytes = 0;
     std::mutex staging_mutex;
     tcp::DeviceBuffer stage_in, stage_out;
@@ -1168,6 +1175,41 @@ int build_sector_plan_impl(const tcp::Operator *op, const tcp_fermion *fermion, 
     return tcp::ok();
 }
 
+// A sector plan on a general charge sector (see charge_sector.hpp) over the
+// terms of a Pauli operator or of a fermion operator.
+int build_charge_plan(const tcp::Operator *op, const tcp_fermion *fermion, const tcp::ChargeGeometry &geo,
+                      uint64_t max_bytes, bool require_conserved, tcp_sector_plan **out) {
+    if (!out) return fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    const uint32_t width = op ? op->nqubits : fermion->n_modes;
+    if (width != geo.n)
+        return fail(TCP_ERR_INCOMPATIBLE_QUBIT_COUNTS, "incompatible qubit counts: " + std::to_string(width) +
+                                                            " and " + std::to_string(geo.n));
+    auto plan = new tcp_sector_plan();
+    std::unique_ptr<tcp_sector_plan> guard(plan);
+    plan->general.reset(new tcp::ChargeSector());
+    std::vector<tcp::CondTerm> fterms;
+    if (fermion) fterms = tcp::fermion_index_terms(*fermion);
+    int st = tcp::charge_build(op, fermion ? &fterms : nullptr, geo, max_bytes, require_conserved, *plan->general);
+    if (st) return st;
+    const tcp::ChargeSector &c = *plan->general;
+    plan->n = geo.n;
+    plan->nblocks = 0;
+    plan->dim = geo.dim;
+    plan->T = c.T;
+    plan->G = c.G;
+    plan->diag_group = c.diag_group;
+    plan->energy_scale = c.energy_scale;
+    plan->leaks = c.leaks;
+    plan->leak_source = c.leak_source;
+    plan->u1 = false;
+    plan->cond = fermion != nullptr;
+    plan->retained_bytes = sizeof(tcp_sector_plan) + c.retained_bytes;
+    TCP_HIP_CHECK(hipGetDevice(&plan->device));
+    *out = guard.release();
+    return tcp::ok();
+}
+
 int build_sector_plan(const tcp::Operator &op, const BlockSpec &spec, bool u1, uint64_t max_bytes,
                       tcp_sector_plan **out, bool require_conserved = true) {
     return build_sector_plan_impl(&op, nullptr, spec, u1, max_bytes, out, require_conserved);
@@ -1238,6 +1280,8 @@ double plan_energy_scale(const tcp_sector_plan *plan) { return plan->energy_scal
 int plan_features(const tcp_sector_plan *plan, double scale, uint16_t *features, double *diag, hipStream_t stream) {
     if (plan->wide)
         return fail(TCP_ERR_UNSUPPORTED, "learned-preconditioner features support sectors of at most 63 qubits");
+    if (plan->general)
+        return fail(TCP_ERR_UNSUPPORTED, "learned-preconditioner features support U(1) and block sectors only");
     if (plan->dim == 0) return TCP_OK;
     const uint32_t blocks = static_cast<uint32_t>((plan->dim + 255) / 256);
     if (plan->cond)
@@ -1251,6 +1295,7 @@ int plan_features(const tcp_sector_plan *plan, double scale, uint16_t *features,
 }
 int plan_diagonal(const tcp_sector_plan *plan, double *diag, hipStream_t stream) {
     if (plan->wide) return wide_diagonal(*plan->wide, diag, stream);
+    if (plan->general) return charge_diagonal(*plan->general, diag, stream);
     if (plan->dim == 0) return TCP_OK;
     const uint64_t threads = (plan->dim + kRanksPerThread - 1) / kRanksPerThread;
     const uint32_t blocks = static_cast<uint32_t>((threads + 255) / 256);
@@ -1305,6 +1350,7 @@ hipError_t launch_gather(const tcp_sector_plan *plan, const void *in, void *out,
 int apply_device(const tcp_sector_plan *plan, const void *in, void *out, int dtype, void *stream) {
     if (dtype != TCP_C128 && dtype != TCP_C64)
         return fail(TCP_ERR_INVALID_ARGUMENT, "dtype must be TCP_C128 or TCP_C64");
+    if (plan->dim == 0) return TCP_OK;  // an empty sector: nothing to read or write
     if (!in || !out) return fail(TCP_ERR_INVALID_ARGUMENT, "null state buffer");
     if (plan->leaks) return leak_error(plan);
     const size_t bytes = static_cast<size_t>(plan->dim) * (dtype == TCP_C128 ? 16 : 8);
@@ -1315,6 +1361,7 @@ int apply_device(const tcp_sector_plan *plan, const void *in, void *out, int dty
     if (st) return st;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (plan->wide) return (st = tcp::wide_apply(*plan->wide, in, out, dtype, s)) ? st : tcp::ok();
+    if (plan->general) return (st = tcp::charge_apply(*plan->general, in, out, dtype, s)) ? st : tcp::ok();
     hipError_t e;
     if (plan->cond)
         e = dtype == TCP_C128 ? launch_gather<dvec2, true>(plan, in, out, dtype, s)
@@ -1516,6 +1563,59 @@ int tcp_sector_plan_create_fermion(const tcp_fermion *f, uint32_t nblocks, const
     return st ? st : tcp::ok();
 }
 
+int tcp_charge_sector_create(uint32_t naxes, uint32_t nconstraints, const int64_t *levels, const int64_t *targets,
+                             uint64_t max_bytes, tcp_charge_sector **out) {
+    if (!out) return fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    auto sector = new tcp_charge_sector();
+    std::unique_ptr<tcp_charge_sector> guard(sector);
+    int st = tcp::charge_geometry(naxes, nconstraints, levels, targets, max_bytes, sector->geo);
+    if (st) return st;
+    *out = guard.release();
+    return tcp::ok();
+}
+
+uint64_t tcp_charge_sector_dimension(const tcp_charge_sector *sector) { return sector ? sector->geo.dim : 0; }
+
+int tcp_charge_sector_rank(const tcp_charge_sector *sector, uint64_t bitstring, uint64_t *rank) {
+    if (!sector || !rank) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    const int st = tcp::charge_rank(sector->geo, bitstring, *rank);
+    return st ? st : tcp::ok();
+}
+
+int tcp_charge_sector_unrank(const tcp_charge_sector *sector, uint64_t index, uint64_t *bitstring) {
+    if (!sector || !bitstring) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    const int st = tcp::charge_unrank(sector->geo, index, *bitstring);
+    return st ? st : tcp::ok();
+}
+
+int tcp_charge_sector_basis_words(const tcp_charge_sector *sector, uint64_t first, uint64_t count, uint64_t *out) {
+    if (!sector || (!out && count)) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    if (first > sector->geo.dim || count > sector->geo.dim - first)
+        return fail(TCP_ERR_INVALID_INDEX, "invalid index: sector index is out of range");
+    for (uint64_t i = 0; i < count; ++i) {
+        const int st = tcp::charge_unrank(sector->geo, first + i, out[i]);
+        if (st) return st;
+    }
+    return tcp::ok();
+}
+
+void tcp_charge_sector_destroy(tcp_charge_sector *sector) { delete sector; }
+
+int tcp_sector_plan_create_charge(const tcp_operator *op, const tcp_charge_sector *sector, uint64_t max_bytes,
+                                  tcp_sector_plan **out) {
+    if (!op || !sector) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator or sector");
+    const int st = build_charge_plan(&tcp::operator_of(op), nullptr, sector->geo, max_bytes, true, out);
+    return st ? st : tcp::ok();
+}
+
+int tcp_sector_plan_create_fermion_charge(const tcp_fermion *f, const tcp_charge_sector *sector,
+                                          uint64_t max_bytes, tcp_sector_plan **out) {
+    if (!f || !sector) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator or sector");
+    const int st = build_charge_plan(nullptr, f, sector->geo, max_bytes, true, out);
+    return st ? st : tcp::ok();
+}
+
 uint64_t tcp_sector_plan_dimension(const tcp_sector_plan *plan) { return plan ? plan->dim : 0; }
 uint32_t tcp_sector_plan_nqubits(const tcp_sector_plan *plan) { return plan ? plan->n : 0; }
 uint64_t tcp_sector_plan_term_count(const tcp_sector_plan *plan) { return plan ? plan->T : 0; }
@@ -1539,6 +1639,7 @@ int tcp_sector_plan_apply_host(const tcp_sector_plan *plan, const double *input,
     int st;
     if ((st = check_len(plan->dim, input_len))) return st;
     if ((st = check_len(plan->dim, output_len))) return st;
+    if (plan->dim == 0) return tcp::ok();
     if (!input || !output) return fail(TCP_ERR_INVALID_ARGUMENT, "null state buffer");
     const size_t bytes = static_cast<size_t>(plan->dim) * 16;
     if (tcp::ranges_overlap(input, bytes, output, bytes))
@@ -1565,6 +1666,7 @@ int tcp_sector_plan_rank_device(const tcp_sector_plan *plan, const uint64_t *bit
     tcp::DeviceGuard dg;
     int st = dg.enter(plan->device);
     if (st) return st;
+    if (plan->general) return (st = tcp::charge_rank_device(*plan->general, bitstrings, ranks, count, static_cast<hipStream_t>(stream))) ? st : tcp::ok();
     const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((count + 255) / 256, 8192));
     hipLaunchKernelGGL(sector_rank_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
                        plan->args(TCP_C128), bitstrings, ranks, count, plan->n);
@@ -1582,6 +1684,7 @@ int tcp_sector_plan_unrank_device(const tcp_sector_plan *plan, const uint64_t *r
     tcp::DeviceGuard dg;
     int st = dg.enter(plan->device);
     if (st) return st;
+    if (plan->general) return (st = tcp::charge_unrank_device(*plan->general, ranks, bitstrings, count, static_cast<hipStream_t>(stream))) ? st : tcp::ok();
     const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((count + 255) / 256, 8192));
     hipLaunchKernelGGL(sector_unrank_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
                        plan->args(TCP_C128), ranks, bitstrings, count);
@@ -1601,6 +1704,10 @@ int tcp_sector_plan_walk_device(const tcp_sector_plan *plan, uint64_t first, uin
     tcp::DeviceGuard dg;
     int st = dg.enter(plan->device);
     if (st) return st;
+    if (plan->general)
+        return (st = tcp::charge_walk_device(*plan->general, first, bitstrings, count, static_cast<hipStream_t>(stream)))
+                   ? st
+                   : tcp::ok();
     const uint64_t threads = (count + run - 1) / run;
     const uint32_t blocks = static_cast<uint32_t>((threads + 255) / 256);
     hipLaunchKernelGGL(sector_walk_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
@@ -1780,10 +1887,12 @@ int build_csr(const tcp_sector_plan *sp, uint64_t max_bytes, tcp_csr_plan **out)
     };
     auto count = [&](uint32_t *counts) -> int {
         if (sp->wide) return tcp::wide_csr_counts(*sp->wide, counts, nullptr);
+        if (sp->general) return tcp::charge_csr_counts(*sp->general, counts, nullptr);
         return sp->cond ? count_with(std::true_type(), counts) : count_with(std::false_type(), counts);
     };
     auto fill = [&](const uint64_t *indptr, uint32_t *cols, void *v64, void *v32) -> int {
         if (sp->wide) return tcp::wide_csr_fill(*sp->wide, indptr, cols, v64, v32, nullptr);
+        if (sp->general) return tcp::charge_csr_fill(*sp->general, indptr, cols, v64, v32, nullptr);
         return sp->cond ? fill_with(std::true_type(), indptr, cols, v64, v32)
                         : fill_with(std::false_type(), indptr, cols, v64, v32);
     };
@@ -1793,6 +1902,7 @@ int build_csr(const tcp_sector_plan *sp, uint64_t max_bytes, tcp_csr_plan **out)
 int csr_apply_device(const tcp_csr_plan *plan, const void *in, void *out, int dtype, void *stream) {
     if (dtype != TCP_C128 && dtype != TCP_C64)
         return fail(TCP_ERR_INVALID_ARGUMENT, "dtype must be TCP_C128 or TCP_C64");
+    if (plan->dim == 0) return TCP_OK;
     if (!in || !out) return fail(TCP_ERR_INVALID_ARGUMENT, "null state buffer");
     const size_t bytes = static_cast<size_t>(plan->dim) * (dtype == TCP_C128 ? 16 : 8);
     if (tcp::ranges_overlap(in, bytes, out, bytes))
@@ -1931,6 +2041,40 @@ int tcp_csr_plan_create_fermion(const tcp_fermion *f, uint32_t nblocks, const ui
     return st ? st : tcp::ok();
 }
 
+}  // extern "C"
+
+namespace {
+int csr_from_charge(const tcp::Operator *op, const tcp_fermion *f, const tcp_charge_sector *sector,
+                    uint64_t max_bytes, tcp_csr_plan **out) {
+    if (!out) return fail(TCP_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (!sector) return fail(TCP_ERR_INVALID_ARGUMENT, "null sector");
+    tcp_sector_plan *sp = nullptr;
+    int st = build_charge_plan(op, f, sector->geo, max_bytes, false, &sp);
+    if (st) return st;
+    std::unique_ptr<tcp_sector_plan> guard(sp);
+    tcp::DeviceGuard dg;
+    if ((st = dg.enter(sp->device))) return st;
+    return build_csr(sp, max_bytes, out);
+}
+}  // namespace
+
+extern "C" {
+
+int tcp_csr_plan_create_charge(const tcp_operator *op, const tcp_charge_sector *sector, uint64_t max_bytes,
+                               tcp_csr_plan **out) {
+    if (!op) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
+    const int st = csr_from_charge(&tcp::operator_of(op), nullptr, sector, max_bytes, out);
+    return st ? st : tcp::ok();
+}
+
+int tcp_csr_plan_create_fermion_charge(const tcp_fermion *f, const tcp_charge_sector *sector, uint64_t max_bytes,
+                                       tcp_csr_plan **out) {
+    if (!f) return fail(TCP_ERR_INVALID_ARGUMENT, "null operator");
+    const int st = csr_from_charge(nullptr, f, sector, max_bytes, out);
+    return st ? st : tcp::ok();
+}
+
 uint64_t tcp_csr_plan_dimension(const tcp_csr_plan *plan) { return plan ? plan->dim : 0; }
 uint64_t tcp_csr_plan_transition_count(const tcp_csr_plan *plan) { return plan ? plan->nnz : 0; }
 uint64_t tcp_csr_plan_estimated_bytes(const tcp_csr_plan *plan) { return plan ? plan->retained_bytes : 0; }
@@ -1951,6 +2095,7 @@ int tcp_csr_plan_apply_host(const tcp_csr_plan *plan, const double *input, doubl
     int st;
     if ((st = check_len(plan->dim, input_len))) return st;
     if ((st = check_len(plan->dim, output_len))) return st;
+    if (plan->dim == 0) return tcp::ok();
     if (!input || !output) return fail(TCP_ERR_INVALID_ARGUMENT, "null state buffer");
     const size_t bytes = static_cast<size_t>(plan->dim) * 16;
     if (tcp::ranges_overlap(input, bytes, output, bytes))

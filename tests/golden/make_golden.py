@@ -65,6 +65,45 @@ def kats() -> dict:
             "population": 1,
             "order": [[0, 0, 1], [0, 1, 0], [1, 0, 0]],
         },
+        "charge_spin_and_number": {
+            "source": "tests/test_charge.py:263-270",
+            "levels": [[[0, 1], [0, 1]], [[0, 1], [0, -1]]],
+            "targets": [1, 1],
+            "dimension": 1,
+            "order": [[1, 0]],
+        },
+        "charge_free_spectators": {
+            "source": "tests/test_charge.py:855-861",
+            "levels": [[[0, 1], [0, 0], [0, 0], [0, 0]]],
+            "targets": [0],
+            "dimension": 8,
+        },
+        "charge_wide_one_particle": {
+            "source": "tests/test_charge.py:103-115",
+            "modes": [62, 63],
+            "number_dimension": "modes",
+            "marked_dimension": 1,
+            "marked_first": "mode 0 occupied, every other mode empty",
+        },
+        "charge_qubit_hopping": {
+            "source": "tests/test_charge.py:310-345",
+            "nqubits": 2,
+            "terms": [["XX", [1.0, 0.0]], ["YY", [1.0, 0.0]]],
+            "levels": [[[0, 1], [0, 1]]],
+            "targets": [1],
+            "dense": [[[0.0, 0.0], [2.0, 0.0]], [[2.0, 0.0], [0.0, 0.0]]],
+            "state": [[0.5, -0.2], [-0.3, 0.7]],
+            "transition_count": 2,
+        },
+        "charge_aggregated_cancellation": {
+            "source": "tests/test_charge.py:369-385",
+            "nqubits": 2,
+            "terms": [["XX", [1.0, 0.0]], ["YY", [1.0, 0.0]]],
+            "levels": [[[0, 1], [0, 1]]],
+            "targets": [0],
+            "dense": [[[0.0, 0.0]]],
+            "transition_count": 0,
+        },
         "charge_two_modes_zero_charge": {
             "source": "tests/test_execution_invariants.py:76-93",
             "order": [[0, 0], [0, 1], [1, 0], [1, 1]],

@@ -74,3 +74,35 @@ def conserving_u1_terms(rng: np.random.Generator, n: int, count: int):
             rows.append(w)
             coefs.append(complex(rng.normal(), 0.0))
     return np.asarray(rows, dtype=np.uint8).reshape(-1, n), np.asarray(coefs, dtype=np.complex128)
+
+
+def interleave_spins(codes: np.ndarray) -> np.ndarray:
+    """Relabel the qubits of Pauli words from the block layout (spin up on
+    qubits 0..L-1, spin down on L..2L-1) to interleaved spin orbitals
+    (site i up on 2i, down on 2i + 1).  A qubit relabeling is exact for Pauli
+    operators (no Jordan-Wigner string is re-derived)."""
+    codes = np.asarray(codes, dtype=np.uint8)
+    n = codes.shape[1]
+    sites = n // 2
+    out = np.zeros_like(codes)
+    for i in range(sites):
+        out[:, 2 * i] = codes[:, i]
+        out[:, 2 * i + 1] = codes[:, sites + i]
+    return out
+
+
+def spin_levels(n: int, interleaved: bool = True) -> np.ndarray:
+    """levels [2][n][2] of the total number N and of 2 S_z (+1 up, -1 down)."""
+    levels = np.zeros((2, n, 2), dtype=np.int64)
+    levels[0, :, 1] = 1
+    for a in range(n):
+        up = (a % 2 == 0) if interleaved else (a < n // 2)
+        levels[1, a, 1] = 1 if up else -1
+    return levels
+
+
+def words_of(bits: np.ndarray, n: int) -> np.ndarray:
+    """uint64 words (qubit 0 = MSB) of occupation rows."""
+    rows = np.asarray(bits, dtype=np.uint64).reshape(-1, n)
+    shifts = np.arange(n - 1, -1, -1, dtype=np.uint64)
+    return (rows << shifts[None, :]).sum(axis=1, dtype=np.uint64)

@@ -197,6 +197,69 @@ def test_block_sector_rank_matches_oracle():
     assert [three.unrank(i) for i in range(3)] == [(0, 0, 1), (0, 1, 0), (1, 0, 0)]
 
 
+def _fermion_charges(n, levels):
+    space = tcp.OperatorSpace(fermions=n)
+    return [tcp.AdditiveCharge(space, fermions={a: int(levels[j, a, 1]) for a in range(n)})
+            for j in range(levels.shape[0])]
+
+
+def test_general_charge_sector_rank_matches_oracle():
+    # interleaved spin orbitals (N, 2 S_z) and weighted qubit charges: the
+    # sectors that are not contiguous blocks use the native suffix-count tables
+    from tests.helpers import spin_levels
+
+    for n, targets in ((10, (4, 0)), (12, (5, -1)), (32, (8, 0))):
+        levels = spin_levels(n)
+        sector = tcp.ChargeSector(list(zip(_fermion_charges(n, levels), targets)))
+        assert sector.blocks is None and sector.execution_family == "generic"
+        assert sector.dimension == oracle.charge_dimension(levels, targets)
+        idx = np.unique(np.linspace(0, sector.dimension - 1, num=min(sector.dimension, 300)).astype(np.uint64))
+        words = oracle.charge_unrank_many(levels, targets, idx)
+        for i, w in zip(idx, words):
+            occ = tuple((int(w) >> (n - 1 - q)) & 1 for q in range(n))
+            assert sector.unrank(int(i)) == occ
+            assert sector.rank(occ) == int(i)
+        if sector.dimension <= 5000:
+            basis = sector.basis_states()
+            assert basis.dtype == np.uint64 and basis.flags.writeable is False
+            np.testing.assert_array_equal(basis, np.asarray(
+                [[(int(w) >> (n - 1 - q)) & 1 for q in range(n)] for w in dense.charge_basis(levels, targets)]))
+    weighted = tcp.AdditiveCharge(7, {0: (0, 2), 1: (1, -1), 2: (0, 3), 4: (2, 1), 5: (0, 1), 6: (-1, 1)})
+    sector = weighted.sector(2)
+    lv = np.asarray([weighted.qubit_levels], dtype=np.int64)
+    assert [int("".join(map(str, r)), 2) for r in sector.basis_states()] == dense.charge_basis(lv, [2])
+    with pytest.raises(ValueError, match="constraint"):
+        sector.rank((1, 1, 1, 1, 1, 1, 1))
+    with pytest.raises(IndexError):
+        sector.unrank(sector.dimension)
+
+
+def test_general_charge_sector_edges():
+    space = tcp.OperatorSpace(fermions=4)
+    number = tcp.AdditiveCharge(space, fermions={m: 1 for m in range(4)})
+    marked = tcp.AdditiveCharge(space, fermions={0: 1})
+    # an unreachable target is an empty sector, as build_charge_sector_plan gives
+    empty = tcp.ChargeSector(((number, 1), (marked, 2)))
+    assert empty.dimension == 0 and empty.basis_states().shape == (0, 4)
+    # unweighted axes are free spectators (tests/test_charge.py:855-861)
+    assert marked.sector(0, max_bytes=1 << 20).dimension == 8
+    with pytest.raises(ValueError, match="integer"):
+        tcp.ChargeSector(((tcp.AdditiveCharge(3, {0: (0, 0.5), 2: (0, 1)}), 1),))
+    # 64 axes exceed the general tables; 2^24 table entries bound the levels
+    wide = tcp.OperatorSpace(fermions=64)
+    interleaved = tcp.AdditiveCharge(wide, fermions={m: (1 if m % 2 else -1) for m in range(64)})
+    with pytest.raises(tcp.UnsupportedOnDeviceError, match="63"):
+        tcp.ChargeSector(((interleaved, 0),))
+    heavy = tcp.AdditiveCharge(20, {a: (0, 1 << (a % 20)) for a in range(20)})
+    with pytest.raises(tcp.UnsupportedOnDeviceError, match="2\\^24"):
+        heavy.sector(5)
+    # two population solutions: a union of block layouts; budgets below the
+    # enforced floor are advisory (tests/test_charge.py:882-897)
+    union = tcp.ChargeSector(((number, 2), (marked, 1), (tcp.AdditiveCharge(space, fermions={1: 1, 3: 1}), 1)),
+                             max_bytes=16)
+    assert union.blocks is None and [int("".join(map(str, union.unrank(i))), 2) for i in range(2)] == [0b1001, 0b1100]
+
+
 def test_apply_into_buffer_protocol():
     good = np.zeros(4, dtype=np.complex128)
     with pytest.raises(TypeError, match="NumPy arrays"):

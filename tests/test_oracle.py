@@ -16,7 +16,8 @@ import pytest
 
 import oracle
 from oracle import dense
-from tests.helpers import conserving_u1_terms, cplx, kats, rel_err, seeded, word_codes
+from tests.helpers import (conserving_u1_terms, cplx, interleave_spins, kats, rel_err, seeded, spin_levels,
+                           word_codes)
 
 
 def _terms(spec):
@@ -234,6 +235,106 @@ def test_blocks_generic_apply_reports_leakage():
     codes = np.asarray([[1, 0, 0, 0]], dtype=np.uint8)
     with pytest.raises(oracle.OracleError) as err:
         oracle.blocks_generic_apply([(2, 1), (2, 1)], codes, [1.0], np.ones(4, dtype=np.complex128))
+    assert err.value.status == 6
+
+
+# ---------------------------------------------------------------- general charge sectors
+GENERAL_CASES = [
+    # (levels [m][n][2], targets): interleaved N and 2 S_z, weights other than
+    # 0/1 with nonzero level_0, free spectator axes, three constraints, and an
+    # unreachable target
+    (spin_levels(8), [4, 0]),
+    (spin_levels(10), [5, 1]),
+    (spin_levels(10), [3, -3]),
+    (np.asarray([[[0, 2], [1, -1], [0, 3], [0, 0], [2, 1], [0, 1], [-1, 1]]]), [2]),
+    (np.asarray([[[0, 1]] * 3 + [[0, 0]] * 4 + [[0, 1]] * 2]), [2]),
+    (np.concatenate([spin_levels(9), np.asarray([[[0, 1 if a % 3 == 0 else 0] for a in range(9)]])]), [4, 0, 1]),
+    (spin_levels(6), [7, 1]),
+]
+
+
+def test_charge_known_answers():
+    k = kats()
+    for name in ("charge_spin_and_number", "charge_free_spectators"):
+        case = k[name]
+        assert oracle.charge_dimension(case["levels"], case["targets"]) == case["dimension"]
+    case = k["charge_spin_and_number"]
+    assert [int(b) for b in oracle.charge_unrank_many(case["levels"], case["targets"], np.arange(1))] == [0b10]
+    for n in k["charge_wide_one_particle"]["modes"]:
+        number = np.zeros((1, n, 2), dtype=np.int64)
+        number[0, :, 1] = 1
+        assert oracle.charge_dimension(number, [1]) == n
+        last = oracle.charge_unrank_many(number, [1], np.asarray([n - 1]))
+        assert int(oracle.charge_rank_many(number, [1], last)[0]) == n - 1
+        marked = np.concatenate([number, np.zeros((1, n, 2), dtype=np.int64)])
+        marked[1, 0, 1] = 1
+        assert oracle.charge_dimension(marked, [1, 1]) == 1
+        assert int(oracle.charge_unrank_many(marked, [1, 1], np.arange(1))[0]) == 1 << (n - 1)
+    # the reference's restricted qubit hopping: dense matrix and its product
+    for name in ("charge_qubit_hopping", "charge_aggregated_cancellation"):
+        case = k[name]
+        codes, coef = _terms(case["terms"])
+        dim = oracle.charge_dimension(case["levels"], case["targets"])
+        expected = np.asarray([[cplx(v) for v in row] for row in case["dense"]], dtype=np.complex128)
+        columns = [oracle.charge_generic_apply(case["levels"], case["targets"], codes, coef, np.eye(dim)[j] + 0j)
+                   for j in range(dim)]
+        np.testing.assert_array_equal(np.stack(columns, axis=1), expected)
+        assert int(np.count_nonzero(expected)) == case["transition_count"]
+
+
+@pytest.mark.parametrize("case", range(len(GENERAL_CASES)))
+def test_charge_rank_against_brute_force(case):
+    levels, targets = GENERAL_CASES[case]
+    n = levels.shape[1]
+    basis = np.asarray(dense.charge_basis(levels, targets), dtype=np.uint64)
+    assert oracle.charge_dimension(levels, targets) == len(basis)
+    np.testing.assert_array_equal(oracle.charge_rank_many(levels, targets, basis), np.arange(len(basis)))
+    np.testing.assert_array_equal(oracle.charge_unrank_many(levels, targets, np.arange(len(basis))), basis)
+    outside = np.setdiff1d(np.arange(1 << n, dtype=np.uint64), basis)
+    assert np.all(oracle.charge_rank_many(levels, targets, outside) == np.iinfo(np.uint64).max)
+
+
+def test_charge_sector_agrees_with_the_block_restatement_on_factorized_sectors():
+    # two independent restatements of one order: contiguous blocks (closed-form
+    # binomial ranks) and the suffix-count tables
+    for blocks in ([(3, 1), (3, 2)], [(4, 2), (4, 2)], [(2, 1), (3, 0), (3, 3)]):
+        n = sum(s for s, _ in blocks)
+        levels = np.zeros((len(blocks), n, 2), dtype=np.int64)
+        start = 0
+        for j, (size, _) in enumerate(blocks):
+            levels[j, start:start + size, 1] = 1
+            start += size
+        targets = [p for _, p in blocks]
+        dim = oracle.blocks_dimension(blocks)
+        assert oracle.charge_dimension(levels, targets) == dim
+        np.testing.assert_array_equal(oracle.charge_unrank_many(levels, targets, np.arange(dim)),
+                                      oracle.blocks_unrank_many(blocks, np.arange(dim)))
+
+
+def test_charge_generic_apply_matches_dense_projection_interleaved_hubbard():
+    from tencirpauli_amd import lattice
+
+    codes, coef = lattice.fermi_hubbard_jw(1, 3, 1.0, 4.0, seed=3)
+    cc, cf = oracle.canonicalize(6, interleave_spins(codes), coef)
+    rng = np.random.default_rng(5)
+    for number, spin in ((2, 0), (3, 1), (4, 0), (3, -3)):
+        levels, targets = spin_levels(6), [number, spin]
+        basis = dense.charge_basis(levels, targets)
+        state = rng.normal(size=len(basis)) + 1j * rng.normal(size=len(basis))
+        expected = dense.restricted_dense(6, cc, cf, basis) @ state
+        got = oracle.charge_generic_apply(levels, targets, cc, cf, state)
+        np.testing.assert_allclose(got, expected, atol=1e-12)
+    # the non-interleaved image equals the block-sector loop exactly
+    bc, bf = oracle.canonicalize(6, codes, coef)
+    state = rng.normal(size=9) + 0j
+    np.testing.assert_array_equal(oracle.charge_generic_apply(spin_levels(6, interleaved=False), [2, 0], bc, bf, state),
+                                  oracle.blocks_generic_apply([(3, 1), (3, 1)], bc, bf, state))
+
+
+def test_charge_generic_apply_reports_leakage():
+    codes = np.asarray([[1, 0, 0, 0]], dtype=np.uint8)
+    with pytest.raises(oracle.OracleError) as err:
+        oracle.charge_generic_apply(spin_levels(4), [2, 0], codes, [1.0], np.ones(4, dtype=np.complex128))
     assert err.value.status == 6
 
 
