@@ -1741,20 +1741,32 @@ namespace {
 
 constexpr uint32_t kCsrLanes = 8;  // lanes per row in the SpMV (rows average ~27 entries on lattice models)
 
+// The compact form is optional: when its buffers cannot be allocated the plan
+// keeps plain CSR, as when it does not apply.
+void drop_compact(tcp_csr_plan *plan) {
+    plan->packed.release();
+    plan->diag64.release();
+    plan->diag32.release();
+    plan->table64.release();
+    plan->table32.release();
+    (void)hipGetLastError();  // the failed allocation is not this call's error
+}
+
 // Builds the compact form of a filled plan when it applies: fewer than 2^24
-// rows, at most kDictMax distinct off-diagonal values, and room in the budget
-// while both forms exist.  Otherwise the plan keeps plain CSR (not an error).
-// On success the FP32 values are dropped (the compact form carries them).
+// rows, at most kDictMax distinct off-diagonal values, room in the budget
+// while both forms exist, and device memory for it.  Otherwise the plan keeps
+// plain CSR (not an error).  On success the FP32 values are dropped (the
+// compact form carries them).
 int compact_csr(tcp_csr_plan *plan, uint64_t max_bytes) {
     const uint64_t dim = plan->dim, nnz = plan->nnz;
     if (dim == 0 || nnz == 0 || dim > kColMask + 1ull) return TCP_OK;
     const uint64_t extra = 4 * nnz + 24 * dim + 64 * kDictSlots;
     if (tcp::exceeds_budget(plan->retained_bytes + extra, max_bytes)) return TCP_OK;
     tcp::DeviceBuffer slots, slot_vals, state, slot_index;
-    int st;
-    if ((st = slots.allocate(8 * kDictSlots)) || (st = slot_vals.allocate(16 * kDictSlots)) ||
-        (st = state.allocate(8)))
-        return st;
+    if (slots.allocate(8 * kDictSlots) || slot_vals.allocate(16 * kDictSlots) || state.allocate(8)) {
+        drop_compact(plan);
+        return TCP_OK;
+    }
     TCP_HIP_CHECK(hipMemset(slots.ptr, 0, 8 * kDictSlots));
     TCP_HIP_CHECK(hipMemset(state.ptr, 0, 8));
     const auto *ip = static_cast<const uint64_t *>(plan->indptr.ptr);
@@ -1786,10 +1798,12 @@ int compact_csr(tcp_csr_plan *plan, uint64_t max_bytes) {
     const size_t tb = std::max<size_t>(ntable, 1);
     t64.resize(tb, dvec2{0.0, 0.0});
     t32.resize(tb, fvec2{0.f, 0.f});
-    if ((st = slot_index.upload(index.data(), 4 * kDictSlots)) || (st = plan->table64.upload(t64.data(), 16 * tb)) ||
-        (st = plan->table32.upload(t32.data(), 8 * tb)) || (st = plan->packed.allocate(4 * nnz)) ||
-        (st = plan->diag64.allocate(16 * dim)) || (st = plan->diag32.allocate(8 * dim)))
-        return st;
+    if (slot_index.upload(index.data(), 4 * kDictSlots) || plan->table64.upload(t64.data(), 16 * tb) ||
+        plan->table32.upload(t32.data(), 8 * tb) || plan->packed.allocate(4 * nnz) ||
+        plan->diag64.allocate(16 * dim) || plan->diag32.allocate(8 * dim)) {
+        drop_compact(plan);
+        return TCP_OK;
+    }
     TCP_HIP_CHECK(hipMemset(plan->diag64.ptr, 0, 16 * dim));
     TCP_HIP_CHECK(hipMemset(plan->diag32.ptr, 0, 8 * dim));
     hipLaunchKernelGGL(csr_pack_kernel, dim3(blocks), dim3(256), 0, nullptr, ip, cp,
@@ -1801,11 +1815,7 @@ int compact_csr(tcp_csr_plan *plan, uint64_t max_bytes) {
     TCP_HIP_CHECK(hipGetLastError());
     TCP_HIP_CHECK(hipMemcpy(hstate, state.ptr, 8, hipMemcpyDeviceToHost));
     if (hstate[1]) {  // a hash collision: keep plain CSR
-        plan->packed.release();
-        plan->diag64.release();
-        plan->diag32.release();
-        plan->table64.release();
-        plan->table32.release();
+        drop_compact(plan);
         return TCP_OK;
     }
     plan->compact = true;
