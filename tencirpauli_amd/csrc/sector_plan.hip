@@ -75,7 +75,6 @@ struct SectorArgs {
     const void *tw;          // [T] (re, im) of w = c * i^(#Y mod 4)
     const uint64_t *tc1;     // [T] fermion terms: modes that must be occupied in the source (null: Pauli)
     const uint64_t *tc0;     // [T] fermion terms: modes that must be empty in the source
-    const uint32_t *gblk;    // [G] bit j set when group g's x-mask touches block j
     const uint32_t *lut;     // rank tables
     const uint32_t *cbase;   // [kMaxBlocks][8] table base of (block, chunk), offset by its lowest row
     const uint64_t *choose;  // [64 * 64] C(p, j), p, j < 64
@@ -104,7 +103,8 @@ __host__ __device__ __forceinline__ bool in_sector(const SectorArgs &p, uint64_t
     return ok;
 }
 
-// Rank of block j's bits of s among the block's fixed-population patterns.
+// Rank of block j's bits of s among the block's fixed-population patterns,
+// times the block's mixed-radix stride (the rank-table entries carry it).
 __host__ __device__ __forceinline__ uint32_t block_rank(const SectorArgs &p, const uint32_t *lut, uint32_t j,
                                                          uint64_t s) {
     const uint64_t w = (s >> p.shift[j]) & p.bmask[j];
@@ -120,7 +120,7 @@ __host__ __device__ __forceinline__ uint32_t block_rank(const SectorArgs &p, con
 
 __host__ __device__ __forceinline__ uint32_t sector_rank(const SectorArgs &p, const uint32_t *lut, uint64_t s) {
     uint32_t rank = 0;
-    for (uint32_t j = 0; j < p.nblocks; ++j) rank += block_rank(p, lut, j, s) * p.stride[j];
+    for (uint32_t j = 0; j < p.nblocks; ++j) rank += block_rank(p, lut, j, s);  // entries carry the stride
     return rank;
 }
 
@@ -190,16 +190,91 @@ struct Amp<fvec2> {
 // 4x4 Hubbard one (whose groups mostly leave the sector); 8 adds 19% there.
 constexpr uint32_t kGatherChunk = 4;
 
-// NB = number of blocks, fixed at compile time so the per-group block loops
-// keep their block fields in registers without spilling kernel arguments.
-// RPT = consecutive destination ranks per thread (one unrank, then successor
-// steps): 1 for one-block (U(1)) sectors, 2 for block sectors -- on the 5x5
-// XXZ sector one rank per thread is 8% faster than two, on the 4x4 Hubbard
-// sectors two are 4-5% faster than one.
+// The gather kernel's own tables (built once per plan by gather_tables):
+//  * byte chunks flattened over the blocks -- chunk t covers index bits
+//    cshift[t] .. cshift[t] + 7 of one block -- so a destination keeps each
+//    chunk's rank-table entry in a register and a group re-ranks, in each
+//    block, only the chunks from the lowest to the highest one its x-mask
+//    flips: the chunks below are unchanged, and above them the ones-below
+//    count is unchanged because the flip keeps the block's population;
+//  * one 64-byte record per group (a few scalar loads): Pauli terms whose z
+//    covers all of x or none of it are summed per z & ~x, their signs on an
+//    in-sector source being fixed up to popc(z & ~x & s) -- the XX + YY of a
+//    bond becomes one constant, a Jordan-Wigner hop one term -- and one
+//    term sits inline; further terms and fermion words are in a term list;
+//  * groups that cannot keep every block's population (an odd number of
+//    flipped bits in a block, or more ones or zeros to move than the block
+//    holds) are left out: they never reach the sector;
+//  * the diagonal stored per state (when the memory budget allows), which
+//    takes the x = 0 group out of the loop.
+constexpr uint32_t kMaxFlatChunks = 12;  // sum of ceil(size / 8) over <= 4 blocks of <= 63 qubits
+constexpr uint32_t kInlineTerm = 1u << 31;
+
+struct GatherGroup {
+    uint64_t x;
+    uint32_t chunks;  // bits 0..11: flat chunks to re-rank; 12..23: those that start a run; 24..27: blocks x touches
+    uint32_t halves;  // block q: bits 8q .. 8q+7 half the number of bits x flips in block q
+    double cre, cim;  // the folded z = 0 terms
+    uint64_t z1;      // the inline term (when nterms has kInlineTerm)
+    double w1re, w1im;
+    uint32_t toff;    // further terms: term list [toff, toff + (nterms & ~kInlineTerm))
+    uint32_t nterms;
+};
+static_assert(sizeof(GatherGroup) == 64, "one scalar load per group");
+
+struct GatherArgs {
+    uint32_t G;       // groups the kernel visits
+    uint32_t nflat;   // flat chunks
+    uint32_t cfirst;  // bit t: chunk t is its block's lowest
+    // chunk t is byte (index >> cshift[t]) & 0xff: a block's last chunk may
+    // carry low bits of the next block up, which its rank-table entries
+    // ignore (they count positions inside the block only), and no chunk of
+    // the same block follows it to need its ones count
+    uint32_t cshift[kMaxFlatChunks];
+    uint32_t cbase[kMaxFlatChunks];    // rank-table row base of chunk t (entries carry the block's stride)
+    uint64_t blockmask[kMaxBlocks];    // index bits of block q
+    const GatherGroup *groups;         // [G]
+    const uint64_t *tz;
+    const void *tw;
+    const uint64_t *tc1, *tc0;  // fermion words: occupancy conditions (null: Pauli)
+    const double *diag;         // [dim] stored diagonal, or null
+};
+
+// A group record read field by field (one scalar load after merging).
+__device__ __forceinline__ GatherGroup load_group(cptr<GatherGroup> groups, uint32_t g) {
+    GatherGroup r;
+    r.x = groups[g].x;
+    r.chunks = groups[g].chunks;
+    r.halves = groups[g].halves;
+    r.cre = groups[g].cre;
+    r.cim = groups[g].cim;
+    r.z1 = groups[g].z1;
+    r.w1re = groups[g].w1re;
+    r.w1im = groups[g].w1im;
+    r.toff = groups[g].toff;
+    r.nterms = groups[g].nterms;
+    return r;
+}
+
+template <bool COND>
+__device__ __forceinline__ bool gather_term_acts(const GatherArgs &q, uint32_t k, uint64_t s) {
+    if (!COND) return true;
+    const uint64_t need = ((cptr<uint64_t>)q.tc1)[k];
+    return (s & need) == need && (s & ((cptr<uint64_t>)q.tc0)[k]) == 0;
+}
+
+// NB = number of blocks, fixed at compile time so the per-group loops keep
+// their block and chunk fields in registers.  RPT = consecutive destination
+// ranks per thread (one unrank, then successor steps): 1 for one-block (U(1))
+// sectors, 2 for block sectors.
 template <typename VT, bool LDS, bool COND, uint32_t NB, uint32_t RPT>
 __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT *__restrict__ v,
-                                                                       VT *__restrict__ y,
-                                                                       SectorArgs p) {
+                                                                       VT *__restrict__ y, SectorArgs p,
+                                                                       GatherArgs q) {
+    constexpr uint32_t F = 7 + NB < kMaxFlatChunks ? 7 + NB : kMaxFlatChunks;  // >= sum of ceil(size / 8)
+    // groups in flight together: with several blocks, four 64-byte records
+    // at once would not fit the scalar registers
+    constexpr uint32_t GC = NB == 1 ? kGatherChunk : 2;
     extern __shared__ uint32_t lds_lut[];
     const uint32_t *lut = p.lut;
     if (LDS) {
@@ -207,11 +282,9 @@ __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT 
         __syncthreads();
         lut = lds_lut;
     }
-    cptr<uint64_t> gx = (cptr<uint64_t>)p.gx;
-    cptr<uint32_t> goff = (cptr<uint32_t>)p.goff;
-    cptr<uint64_t> tz = (cptr<uint64_t>)p.tz;
-    cptr<dvec2> tw = (cptr<dvec2>)p.tw;
-    cptr<uint32_t> gblk = (cptr<uint32_t>)p.gblk;
+    cptr<GatherGroup> groups = (cptr<GatherGroup>)q.groups;
+    cptr<uint64_t> tz = (cptr<uint64_t>)q.tz;
+    cptr<dvec2> tw = (cptr<dvec2>)q.tw;
 
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * RPT;
     for (uint64_t r0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * RPT;
@@ -220,37 +293,61 @@ __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT 
         for (uint32_t j = 0; j < RPT; ++j) {
             const uint64_t r = r0 + j;
             if (r >= p.dim) break;
-            // a group leaves the blocks its x-mask does not touch as they are in
-            // b: only touched blocks are tested and re-ranked
-            uint32_t lr[kMaxBlocks];
+            // rank-table entry and ones-below count of every chunk of b
+            uint32_t contrib[F], below_at[F];
+            uint32_t below = 0;
 #pragma unroll
-            for (uint32_t q = 0; q < kMaxBlocks; ++q) lr[q] = q < NB ? block_rank(p, lut, q, b) : 0u;
+            for (uint32_t t = 0; t < F; ++t) {
+                contrib[t] = 0;
+                below_at[t] = 0;
+                if (t < q.nflat) {
+                    if ((q.cfirst >> t) & 1u) below = 0;
+                    const uint32_t byte = static_cast<uint32_t>(b >> q.cshift[t]) & 0xffu;
+                    contrib[t] = lut[q.cbase[t] + below * 256u + byte];
+                    below_at[t] = below;
+                    below += __popc(byte);
+                }
+            }
             double ar = 0.0, ai = 0.0;
-            // kGatherChunk groups at a time: their source ranks and weights
+            if (q.diag) {
+                const double d = q.diag[r];
+                const VT sv = v[r];
+                ar = d * Amp<VT>::re(sv);
+                ai = d * Amp<VT>::im(sv);
+            }
+            // GC groups at a time: their source ranks and weights
             // first, then all their gathers in flight together (a group that
             // leaves the sector or sums to an exact zero -- skipped as
             // core/sector.rs:1099 does -- gathers v[r] with weight zero)
-            for (uint32_t g0 = 0; g0 < p.G; g0 += kGatherChunk) {
-                uint64_t rs[kGatherChunk];
-                double wr[kGatherChunk], wi[kGatherChunk];
+            for (uint32_t g0 = 0; g0 < q.G; g0 += GC) {
+                uint64_t rs[GC];
+                double wr[GC], wi[GC];
                 uint32_t used = 0;  // bit u: group g0 + u contributes
 #pragma unroll
-                for (uint32_t u = 0; u < kGatherChunk; ++u) {
+                for (uint32_t u = 0; u < GC; ++u) {
                     const uint32_t g = g0 + u;
-                    const bool live = g < p.G;
-                    const uint64_t x = live ? gx[g] : 0;
-                    const uint32_t touched = live ? gblk[g] : 0u;
-                    const uint64_t s = b ^ x;
-                    double dr = 0.0, di = 0.0;
+                    const bool live = g < q.G;
+                    const GatherGroup gr = load_group(groups, live ? g : 0);
+                    const uint64_t s = b ^ gr.x;
+                    // s keeps every block's population when, inside each
+                    // block, half of the bits x flips are ones in b
                     bool ok = live;
 #pragma unroll
-                    for (uint32_t q = 0; q < NB; ++q)
-                        if (NB == 1 || ((touched >> q) & 1u))
-                            ok &= static_cast<uint32_t>(__popcll((s >> p.shift[q]) & p.bmask[q])) == p.pop[q];
-                    if (ok) {
-                        const uint32_t k1 = goff[g + 1];
-                        for (uint32_t k = goff[g]; k < k1; ++k) {
-                            if (!term_acts<COND>(p, k, s)) continue;
+                    for (uint32_t k = 0; k < NB; ++k) {
+                        if (!((gr.chunks >> (24 + k)) & 1u)) continue;
+                        const uint64_t xk = NB == 1 ? gr.x : gr.x & q.blockmask[k];
+                        ok &= static_cast<uint32_t>(__popcll(b & xk)) == ((gr.halves >> (8 * k)) & 0xffu);
+                    }
+                    double dr = gr.cre, di = gr.cim;
+                    if (ok && gr.nterms != 0) {  // else: a nonzero constant
+                        if (gr.nterms & kInlineTerm) {
+                            const uint32_t par = __popcll(gr.z1 & s) & 1u;
+                            dr += flip_sign(gr.w1re, par);
+                            di += flip_sign(gr.w1im, par);
+                        }
+                        const uint32_t k1 = gr.toff + (gr.nterms & ~kInlineTerm);
+                        for (uint32_t k = gr.toff; k < k1; ++k) {
+                            if (!gather_term_acts<COND>(q, k, s)) continue;
                             const uint32_t par = __popcll(tz[k] & s) & 1u;
                             const dvec2 w = tw[k];
                             dr += flip_sign(w.x, par);
@@ -259,26 +356,33 @@ __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT 
                         ok = dr != 0.0 || di != 0.0;
                     }
                     uint64_t src = r;
-                    if (ok && x != 0) {
-                        uint32_t rank = 0;
+                    if (ok && gr.x != 0) {
+                        // the chunks of each block from the lowest to the highest
+                        // one x flips; a run starts from b's ones-below count
+                        uint32_t delta = 0, nb = 0;
 #pragma unroll
-                        for (uint32_t q = 0; q < NB; ++q)
-                            rank += ((NB == 1 || ((touched >> q) & 1u)) ? block_rank(p, lut, q, s) : lr[q]) *
-                                    p.stride[q];
-                        src = rank;
+                        for (uint32_t t = 0; t < F; ++t) {
+                            if ((gr.chunks >> t) & 1u) {
+                                if ((gr.chunks >> (12 + t)) & 1u) nb = below_at[t];
+                                const uint32_t byte = static_cast<uint32_t>(s >> q.cshift[t]) & 0xffu;
+                                delta += lut[q.cbase[t] + nb * 256u + byte] - contrib[t];
+                                nb += __popc(byte);
+                            }
+                        }
+                        src = static_cast<uint32_t>(static_cast<uint32_t>(r) + delta);
                         ok = src < p.dim;  // unreachable for in-sector s; keeps the gather in bounds
                         if (!ok) src = r;
                     }
                     rs[u] = src;
-                    wr[u] = dr;
-                    wi[u] = di;
+                    wr[u] = ok ? dr : 0.0;
+                    wi[u] = ok ? di : 0.0;
                     used |= static_cast<uint32_t>(ok) << u;
                 }
-                VT sv[kGatherChunk];
+                VT sv[GC];
 #pragma unroll
-                for (uint32_t u = 0; u < kGatherChunk; ++u) sv[u] = v[rs[u]];
+                for (uint32_t u = 0; u < GC; ++u) sv[u] = v[rs[u]];
 #pragma unroll
-                for (uint32_t u = 0; u < kGatherChunk; ++u) {
+                for (uint32_t u = 0; u < GC; ++u) {
                     // unused slots add an exact zero whatever v[r] holds
                     const bool on = (used >> u) & 1u;
                     const double sr = on ? Amp<VT>::re(sv[u]) : 0.0, si = on ? Amp<VT>::im(sv[u]) : 0.0;
@@ -872,7 +976,10 @@ struct tcp_sector_plan {
     uint32_t leak_weight = 0;  // its destination weight (U(1) message)
     SectorArgs host_args{};
     bool cond = false;  // terms are fermion ladder words with occupancy conditions
-    tcp::DeviceBuffer gx, goff, tz, tw64, tw32, lut, choose, tc1, tc0, gblk, cbase;
+    tcp::DeviceBuffer gx, goff, tz, tw64, tw32, lut, choose, tc1, tc0, cbase;
+    // the gather kernel's tables (see GatherArgs)
+    GatherArgs host_gather{};
+    tcp::DeviceBuffer grec64, grec32, gtz, gtw64, gtw32, gtc1, gtc0, diag64;
     std::unique_ptr<tcp::WideSector> wide;        // U(1) sectors of 64 or more qubits
     std::unique_ptr<tcp::ChargeSector> general;   // charge sectors that are not contiguous blocks
     uint64_t retained_bytes = 0;
@@ -887,11 +994,20 @@ struct tcp_sector_plan {
         a.tw = dtype == TCP_C64 ? tw32.ptr : tw64.ptr;
         a.tc1 = cond ? static_cast<const uint64_t *>(tc1.ptr) : nullptr;
         a.tc0 = cond ? static_cast<const uint64_t *>(tc0.ptr) : nullptr;
-        a.gblk = static_cast<const uint32_t *>(gblk.ptr);
         a.lut = static_cast<const uint32_t *>(lut.ptr);
         a.choose = static_cast<const uint64_t *>(choose.ptr);
         a.cbase = static_cast<const uint32_t *>(cbase.ptr);
         return a;
+    }
+    GatherArgs gather_args(int dtype) const {
+        GatherArgs q = host_gather;
+        q.groups = static_cast<const GatherGroup *>(dtype == TCP_C64 ? grec32.ptr : grec64.ptr);
+        q.tz = static_cast<const uint64_t *>(gtz.ptr);
+        q.tw = dtype == TCP_C64 ? gtw32.ptr : gtw64.ptr;
+        q.tc1 = cond ? static_cast<const uint64_t *>(gtc1.ptr) : nullptr;
+        q.tc0 = cond ? static_cast<const uint64_t *>(gtc0.ptr) : nullptr;
+        q.diag = static_cast<const double *>(diag64.ptr);
+        return q;
     }
 };
 
@@ -990,7 +1106,7 @@ int sector_geometry(const BlockSpec &spec, SectorArgs &a, uint64_t &dim, std::ve
                         ++seen;
                         if (pos < spec.sizes[j] && index <= k) value += choose_table[pos * 64 + index];
                     }
-                    lut.push_back(static_cast<uint32_t>(value));
+                    lut.push_back(static_cast<uint32_t>(value * a.stride[j]));  // < dim < 2^32 in sector
                 }
         }
     }
@@ -999,6 +1115,155 @@ int sector_geometry(const BlockSpec &spec, SectorArgs &a, uint64_t &dim, std::ve
 }
 
 int scan_leakage(tcp_sector_plan *plan, const std::vector<uint64_t> &gx);
+int store_diagonal(tcp_sector_plan *plan);
+
+// The gather kernel's tables of a block-sector plan (see GatherArgs): flat
+// byte chunks, one record per group it visits (FP64 and FP32 weights), the
+// further terms and, when the budget allows dim * 8 more bytes, the stored
+// diagonal in place of the x = 0 group.
+int gather_tables(tcp_sector_plan *plan, const std::vector<tcp::CondTerm> &terms,
+                  const std::vector<std::vector<uint64_t>> &members, const std::vector<uint64_t> &gx,
+                  const std::vector<uint32_t> &cbase, uint64_t max_bytes) {
+    const SectorArgs &a = plan->host_args;
+    GatherArgs q{};
+    uint32_t first_chunk[kMaxBlocks] = {};
+    for (uint32_t j = 0; j < a.nblocks; ++j) {
+        first_chunk[j] = q.nflat;
+        q.blockmask[j] = a.bmask[j] << a.shift[j];
+        for (uint32_t c = 0; c < a.nchunk[j]; ++c) {
+            const uint32_t t = q.nflat++;
+            if (t >= kMaxFlatChunks) return fail(TCP_ERR_UNSUPPORTED, "too many rank-table chunks");
+            q.cshift[t] = a.shift[j] + 8 * c;
+            q.cbase[t] = cbase[j * kMaxChunksPerBlock + c];
+            if (c == 0) q.cfirst |= 1u << t;
+        }
+    }
+    const bool cond = plan->cond;
+    const bool store = plan->diag_group != UINT32_MAX && plan->dim > 0 &&
+                       !tcp::exceeds_budget(plan->retained_bytes + plan->dim * 8, max_bytes);
+    std::vector<GatherGroup> rec64, rec32;
+    std::vector<uint64_t> gtz, gtc1, gtc0;
+    std::vector<dvec2> gtw64, gtw32;
+    for (size_t g = 0; g < gx.size() && g < members.size(); ++g) {
+        if (store && g == plan->diag_group) continue;
+        GatherGroup r{};
+        r.x = gx[g];
+        bool reachable = true;
+        for (uint32_t j = 0; j < kMaxBlocks; ++j) {
+            const uint64_t xj = j < a.nblocks ? (gx[g] >> a.shift[j]) & a.bmask[j] : 0;
+            if (!xj) {
+                    continue;
+            }
+            const uint32_t flips = static_cast<uint32_t>(__builtin_popcountll(xj));
+            if ((flips & 1u) || flips / 2 > a.pop[j] || flips / 2 > a.size[j] - a.pop[j]) reachable = false;
+            const uint32_t lo = first_chunk[j] + static_cast<uint32_t>(__builtin_ctzll(xj)) / 8;
+            const uint32_t hi = first_chunk[j] + (63u - static_cast<uint32_t>(__builtin_clzll(xj))) / 8;
+            for (uint32_t t = lo; t <= hi; ++t) r.chunks |= 1u << t;
+            r.chunks |= (1u << (12 + lo)) | (1u << (24 + j));
+            r.halves |= (flips / 2) << (8 * j);
+        }
+        if (!reachable) continue;  // never maps a sector state into the sector
+        GatherGroup r32 = r;
+        r.toff = r32.toff = static_cast<uint32_t>(gtz.size());
+        // On an in-sector source s = b ^ x, x & s = x & ~b holds exactly
+        // H = popc(x) / 2 ones, so a Pauli term whose z covers all of x or
+        // none of it has the sign (-1)^(H [z & x = x]) (-1)^popc(z' & s),
+        // z' = z & ~x: such terms are summed per z' (z' = 0: the group's
+        // constant, the XX and YY of a bond), each sum kept as one term.
+        // Other terms and fermion words (whose conditions read s) stay as
+        // they are.
+        const uint32_t half_x = static_cast<uint32_t>(__builtin_popcountll(gx[g])) / 2;
+        std::vector<uint64_t> keys;
+        std::vector<dvec2> sum64, sum32;
+        std::vector<const tcp::CondTerm *> rest;
+        for (uint64_t k : members[g]) {
+            const tcp::CondTerm &t = terms[k];
+            const uint64_t zx = t.z & gx[g];
+            if (cond || (zx != 0 && zx != gx[g])) {
+                rest.push_back(&t);
+                continue;
+            }
+            const double sign = (zx != 0 && (half_x & 1u)) ? -1.0 : 1.0;
+            const uint64_t key = t.z & ~gx[g];
+            const size_t i = std::find(keys.begin(), keys.end(), key) - keys.begin();
+            if (i == keys.size()) {
+                keys.push_back(key);
+                sum64.push_back(dvec2{0.0, 0.0});
+                sum32.push_back(dvec2{0.0, 0.0});
+            }
+            sum64[i] += sign * dvec2{t.w.re, t.w.im};
+            sum32[i] += sign * dvec2{static_cast<double>(static_cast<float>(t.w.re)),
+                                     static_cast<double>(static_cast<float>(t.w.im))};
+        }
+        auto add_term = [&](uint64_t z, uint64_t c1, uint64_t c0, dvec2 w64, dvec2 w32) {
+            if (!(r.nterms & kInlineTerm) && !cond) {
+                r.z1 = r32.z1 = z;
+                r.w1re = w64.x;
+                r.w1im = w64.y;
+                r32.w1re = w32.x;
+                r32.w1im = w32.y;
+                r.nterms |= kInlineTerm;
+                r32.nterms |= kInlineTerm;
+                return;
+            }
+            gtz.push_back(z);
+            gtc1.push_back(c1);
+            gtc0.push_back(c0);
+            gtw64.push_back(w64);
+            gtw32.push_back(w32);
+            ++r.nterms;
+            ++r32.nterms;
+        };
+        for (size_t i = 0; i < keys.size(); ++i) {
+            if (keys[i] == 0) {
+                r.cre = sum64[i].x;
+                r.cim = sum64[i].y;
+                r32.cre = sum32[i].x;
+                r32.cim = sum32[i].y;
+            } else if (sum64[i].x != 0.0 || sum64[i].y != 0.0 || sum32[i].x != 0.0 || sum32[i].y != 0.0) {
+                add_term(keys[i], 0, 0, sum64[i], sum32[i]);
+            }
+        }
+        for (const tcp::CondTerm *t : rest)
+            add_term(t->z, t->c1, t->c0, dvec2{t->w.re, t->w.im},
+                     dvec2{static_cast<double>(static_cast<float>(t->w.re)),
+                           static_cast<double>(static_cast<float>(t->w.im))});
+        const bool constant = r.nterms == 0;
+        if (constant && r.cre == 0.0 && r.cim == 0.0 && r32.cre == 0.0 && r32.cim == 0.0)
+            continue;  // sums to an exact zero on every in-sector source
+        rec64.push_back(r);
+        rec32.push_back(r32);
+    }
+    q.G = static_cast<uint32_t>(rec64.size());
+    if (rec64.empty()) {
+        rec64.push_back(GatherGroup{});
+        rec32.push_back(GatherGroup{});
+    }
+    if (gtz.empty()) {
+        gtz.push_back(0);
+        gtc1.push_back(0);
+        gtc0.push_back(0);
+        gtw64.push_back(dvec2{0, 0});
+        gtw32.push_back(dvec2{0, 0});
+    }
+    plan->host_gather = q;
+    int st;
+    if ((st = plan->grec64.upload(rec64.data(), rec64.size() * sizeof(GatherGroup))) ||
+        (st = plan->grec32.upload(rec32.data(), rec32.size() * sizeof(GatherGroup))) ||
+        (st = plan->gtz.upload(gtz.data(), gtz.size() * 8)) ||
+        (st = plan->gtw64.upload(gtw64.data(), gtw64.size() * 16)) ||
+        (st = plan->gtw32.upload(gtw32.data(), gtw32.size() * 16)))
+        return st;
+    if (cond && ((st = plan->gtc1.upload(gtc1.data(), gtc1.size() * 8)) ||
+                 (st = plan->gtc0.upload(gtc0.data(), gtc0.size() * 8))))
+        return st;
+    plan->retained_bytes += rec64.size() * 2 * sizeof(GatherGroup) + gtz.size() * (cond ? 56 : 40);
+    if (store) {
+        if ((st = store_diagonal(plan))) return st;
+        plan->retained_bytes += plan->dim * 8;
+    }
+    return TCP_OK;
+}
 
 // require_conserved: lazy restriction's precondition (exact conservation of
 // every block's particle number).  The eager builders skip it, as the
@@ -1159,11 +1424,6 @@ int build_sector_plan_impl(const tcp::Operator *op, const tcp_fermion *fermion, 
     if ((st = plan->lut.upload(lut.data(), lut.size() * 4))) return st;
     if ((st = plan->choose.upload(choose_table.data(), choose_table.size() * 8))) return st;
     if ((st = plan->cbase.upload(cbase.data(), cbase.size() * 4))) return st;
-    std::vector<uint32_t> gblk(gx.size(), 0);
-    for (size_t g = 0; g < gx.size(); ++g)
-        for (uint32_t j = 0; j < spec.nblocks; ++j)
-            if ((gx[g] >> a.shift[j]) & a.bmask[j]) gblk[g] |= 1u << j;
-    if ((st = plan->gblk.upload(gblk.data(), gblk.size() * 4))) return st;
     if (fermion) {
         if ((st = plan->tc1.upload(tc1.data(), tc1.size() * 8))) return st;
         if ((st = plan->tc0.upload(tc0.data(), tc0.size() * 8))) return st;
@@ -1171,6 +1431,8 @@ int build_sector_plan_impl(const tcp::Operator *op, const tcp_fermion *fermion, 
     }
     plan->u1 = u1;
     if ((st = scan_leakage(plan, gx))) return st;
+    // plans built for the eager CSR (no conservation precondition) never run the gather
+    if (require_conserved && (st = gather_tables(plan, terms, members, gx, cbase, max_bytes))) return st;
     *out = guard.release();
     return tcp::ok();
 }
@@ -1312,6 +1574,15 @@ int plan_diagonal(const tcp_sector_plan *plan, double *diag, hipStream_t stream)
 
 namespace {
 
+// The diagonal H_rr of every state, kept for the gather kernel.
+int store_diagonal(tcp_sector_plan *plan) {
+    int st = plan->diag64.allocate(plan->dim * 8);
+    if (st) return st;
+    if ((st = tcp::plan_diagonal(plan, static_cast<double *>(plan->diag64.ptr), nullptr))) return st;
+    TCP_HIP_CHECK(hipDeviceSynchronize());
+    return TCP_OK;
+}
+
 int check_len(uint64_t expected, uint64_t actual) {
     if (expected != actual)
         return fail(TCP_ERR_INVALID_STRUCTURE_LENGTH, "expected structure length " +
@@ -1324,6 +1595,7 @@ template <typename VT, bool COND>
 hipError_t launch_gather(const tcp_sector_plan *plan, const void *in, void *out, int dtype,
                          hipStream_t stream) {
     SectorArgs a = plan->args(dtype);
+    const GatherArgs q = plan->gather_args(dtype);
     auto launch = [&](auto nb) {
         constexpr uint32_t NB = decltype(nb)::value;
         constexpr uint32_t RPT = NB == 1 ? 1 : 2;
@@ -1332,11 +1604,11 @@ hipError_t launch_gather(const tcp_sector_plan *plan, const void *in, void *out,
         if (plan->lut_in_lds)
             hipLaunchKernelGGL((sector_gather_kernel<VT, true, COND, NB, RPT>), dim3(static_cast<uint32_t>(blocks)),
                                dim3(kGatherThreads), a.lut_words * 4, stream, static_cast<const VT *>(in),
-                               static_cast<VT *>(out), a);
+                               static_cast<VT *>(out), a, q);
         else
             hipLaunchKernelGGL((sector_gather_kernel<VT, false, COND, NB, RPT>), dim3(static_cast<uint32_t>(blocks)),
                                dim3(kGatherThreads), 0, stream, static_cast<const VT *>(in),
-                               static_cast<VT *>(out), a);
+                               static_cast<VT *>(out), a, q);
     };
     switch (a.nblocks) {
         case 1: launch(std::integral_constant<uint32_t, 1>()); break;
