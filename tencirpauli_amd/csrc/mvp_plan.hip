@@ -50,16 +50,22 @@ constexpr uint32_t kSmallBits = 9;
 
 uint32_t log2u(uint32_t x) { return 31u - static_cast<uint32_t>(__builtin_clz(x)); }
 
+constexpr uint32_t kNoXcdOrder = ~0u;
+
 // Source tiles in flight per workgroup encoded in a tiling word (field 0 = one).
 constexpr uint32_t tile_prefetch_depth(uint32_t tiling) {
     return ((tiling & TCP_TILE_PREFETCH_MASK) >> TCP_TILE_PREFETCH_SHIFT) + 1u;
 }
 
-// One workgroup per output tile and right-hand side (blockIdx.y).  With
-// `xcd_order` the tile of workgroup b is (b mod 8) * (tiles / 8) + b / 8:
-// workgroups are dealt to the eight XCDs round-robin, so each XCD walks one
-// contiguous eighth of the tiles and its concurrent workgroups read source
-// tiles that lie close together (shared in that XCD's L2).
+// One workgroup per output tile and right-hand side (blockIdx.y).
+// Workgroups are dealt to the eight XCDs round-robin (XCD = b mod 8).  With
+// an XCD order (xcd_shift != kNoXcdOrder) tile bits xcd_shift .. +2 name the
+// XCD and j = b / 8 fills the others: the workgroups one XCD runs together
+// differ in the low xcd_shift tile bits (the bit map's window, so their
+// source tiles coincide class after class, shared in that XCD's L2), and at
+// one moment the eight XCDs work on tiles that differ in the bit map's XCD
+// bits, so they share source tiles through the last-level cache.  With
+// xcd_shift = log2(tiles) - 3 each XCD walks one contiguous eighth.
 // PARTIAL: the whole space is smaller than one tile; l >= dim idle.
 // Waves per SIMD the register budget is sized for: as many workgroups per CU
 // as their LDS (the FP64 tile, 16 B per amplitude, and the class block, 8 B
@@ -79,14 +85,14 @@ constexpr int waves_per_simd() {
 template <int A, typename VT, uint32_t TPB, bool PARTIAL, int PF = 1, bool NT = false>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(waves_per_simd<A, TPB, VT>())))
 void mvp_tile_kernel(const VT *__restrict__ v, VT *__restrict__ y, MvpArgs p,
-                                                       uint64_t dim, uint32_t xcd_order) {
+                                                       uint64_t dim, uint32_t xcd_shift) {
     __shared__ dvec2 lds[TPB * A];
     __shared__ tcp::u32x4 lblk[tcp::block_words(TPB) + 2];
     const uint64_t rhs = static_cast<uint64_t>(blockIdx.y) * dim;
     uint64_t tile = blockIdx.x;
-    if (xcd_order) {
-        const uint64_t per = gridDim.x >> 3;
-        tile = (tile & 7u) * per + (tile >> 3);
+    if (xcd_shift != kNoXcdOrder) {
+        const uint64_t j = tile >> 3;
+        tile = (j & ((1ull << xcd_shift) - 1)) | ((tile & 7u) << xcd_shift) | ((j >> xcd_shift) << (xcd_shift + 3));
     }
     const uint32_t active = PARTIAL ? static_cast<uint32_t>(dim) : TPB * A;
     tcp::tile_apply<A, VT, TPB, PARTIAL, PF, NT, !PARTIAL>(v + rhs, y + rhs, p, tile, active, lds, lblk);
@@ -146,6 +152,9 @@ __global__ void mvp_diagonal_kernel(MvpArgs p, uint32_t tb, uint32_t diag_passes
 }  // namespace
 
 // Device tables of one tile geometry.
+struct PlanTiles;
+uint32_t tile_xcd_shift(const PlanTiles &pt, uint64_t dim);
+
 struct PlanTiles {
     uint32_t tiling = 0;  // geometry index (0: small space) | TCP_TILE_XCD_ORDER
     uint32_t tb = 0, ab = 0, L = 0;
@@ -154,6 +163,7 @@ struct PlanTiles {
     uint32_t n = 0;
     bool permuted = false;     // kernel coordinates differ from the index bits
     uint8_t pos[64] = {};
+    uint32_t xcd_bits_at = kNoXcdOrder;  // tile bit of the bit map's XCD bits (kNoXcdOrder: none chosen)
     tcp::DeviceBuffer cls, blk64, blk32, xorder, grp64, grp32, term64, term32;
     uint64_t bytes = 0;
 
@@ -173,6 +183,15 @@ struct PlanTiles {
     }
 };
 
+// XCD-order shift of a launch (see mvp_tile_kernel): the bit map's XCD bits
+// when it chose them and the space has tiles above them, else the top three
+// tile bits; kNoXcdOrder without an XCD order or with fewer than 8 tiles.
+uint32_t tile_xcd_shift(const PlanTiles &pt, uint64_t dim) {
+    if (!(pt.tiling & TCP_TILE_XCD_ORDER) || (dim >> pt.L) < 8) return kNoXcdOrder;
+    const uint32_t top = 63u - static_cast<uint32_t>(__builtin_clzll(dim >> pt.L)) - 3u;
+    return pt.xcd_bits_at <= top ? pt.xcd_bits_at : top;
+}
+
 struct tcp_mvp_plan {
     uint32_t n = 0;
     uint64_t T = 0, G = 0;
@@ -189,7 +208,7 @@ struct tcp_mvp_plan {
     uint64_t dim() const { return 1ull << n; }
     const PlanTiles &tiles(int dtype) const { return dtype == TCP_C64 && own64 ? t64 : t128; }
     MvpArgs args(int dtype) const { return tiles(dtype).args(dtype); }
-    bool xcd_order(const PlanTiles &pt) const { return (pt.tiling & TCP_TILE_XCD_ORDER) && (dim() >> pt.L) >= 8; }
+    uint32_t xcd_shift(const PlanTiles &pt) const { return tile_xcd_shift(pt, dim()); }
 };
 
 namespace {
@@ -233,13 +252,17 @@ int build_tiles(const tcp::Operator &op, uint32_t geo, bool xcd, uint32_t pipeli
         const uint32_t threads = kGeometries[geo].threads, amps = kGeometries[geo].amps;
         const uint32_t per_cu = std::max<uint32_t>(
             1, std::min<uint32_t>((160u * 1024u) / ((2 * amps + 1) * threads * 8u + 32u), 2048u / threads));
-        const uint32_t window = log2u(32u * per_cu);
+        // window: half the workgroups one XCD runs at once (measured 2-3% faster
+        // on the 28-qubit lattice than all of them, whose source tiles fill the
+        // 4 MiB L2 exactly); three XCD bits above it
+        const uint32_t window = log2u(32u * per_cu) - 1;
         uint8_t pos[64];
-        if (tcp::choose_bit_map(op, pt.tb, pt.ab, window, pos)) {
+        if (tcp::choose_bit_map(op, pt.tb, pt.ab, window, 3, pos)) {
             for (uint32_t c = 0; c < n; ++c) {
                 pt.pos[c] = pos[c];
                 pt.permuted = pt.permuted || pos[c] != c;
             }
+            pt.xcd_bits_at = window;
         }
     }
     tab.append(op, pt.tb, pt.ab, pt.permuted ? pt.pos : nullptr, geo == 0 ? kSmallThreads : kGeometries[geo].threads);
@@ -330,13 +353,13 @@ struct LaunchSpec {
     uint64_t dim;
     uint32_t n, tiling, L, ab;
     uint64_t H;
-    bool xcd;
+    uint32_t xcd_shift;  // kNoXcdOrder: launch order
     MvpArgs args;
 };
 
 LaunchSpec launch_spec(const tcp_mvp_plan *plan, int dtype) {
     const PlanTiles &pt = plan->tiles(dtype);
-    return LaunchSpec{plan->dim(), plan->n, pt.tiling, pt.L, pt.ab, pt.H, plan->xcd_order(pt), pt.args(dtype)};
+    return LaunchSpec{plan->dim(), plan->n, pt.tiling, pt.L, pt.ab, pt.H, plan->xcd_shift(pt), pt.args(dtype)};
 }
 
 template <int A, typename VT, uint32_t TPB, bool PARTIAL = false, int PF = 1, bool NT = false>
@@ -344,7 +367,7 @@ hipError_t launch_tile(const LaunchSpec &ls, const void *in, void *out, uint32_t
     const uint64_t tiles = PARTIAL ? 1 : ls.dim >> ls.L;
     hipLaunchKernelGGL((mvp_tile_kernel<A, VT, TPB, PARTIAL, PF, NT>), dim3(static_cast<uint32_t>(tiles), nrhs),
                        dim3(TPB), 0, stream, static_cast<const VT *>(in), static_cast<VT *>(out), ls.args, ls.dim,
-                       ls.xcd ? 1u : 0u);
+                       ls.xcd_shift);
     return hipGetLastError();
 }
 
@@ -574,8 +597,7 @@ int tcp_mvp_apply_static(const tcp_operator *op, const void *input, void *output
     a.grp = nullptr;  // pass and term tables serve the plan builders only
     a.term = nullptr;
     a.xorder = nullptr;
-    const LaunchSpec ls{dim, n, pt.tiling, pt.L, pt.ab, pt.H, (pt.tiling & TCP_TILE_XCD_ORDER) && (dim >> pt.L) >= 8,
-                        a};
+    const LaunchSpec ls{dim, n, pt.tiling, pt.L, pt.ab, pt.H, tile_xcd_shift(pt, dim), a};
     hipError_t e = launch_any(ls, input, output, 1, dtype, s);
     if (e != hipSuccess) return tcp::hip_fail(e, "mvp kernel launch");
     return tcp::ok();

@@ -390,17 +390,21 @@ __device__ inline dvec2 group_sum(const TileArgs &p, uint32_t g, uint32_t t, uin
 
 // Kernel coordinates for the tile kernel of geometry (tb, ab) on `op`, with
 // `window` tile bits varying fastest across the workgroups that run together
-// on one XCD.  Coordinates 0..3 stay on index bits 0..3; the other tile-local
-// bits are picked greedily to leave the fewest distinct x-mask parts outside
-// the tile (source tiles per output tile), then the window bits to leave the
-// fewest outside tile and window (distinct source tiles an XCD streams per
-// window), then, among the tile-local bits, the slot bits that split the
-// fewest x-groups into passes.  Ties go to the lowest bit.  Returns false
+// on one XCD and `xcd` tile bits (placed right above the window) that tell
+// the XCDs apart.  Coordinates 0..3 stay on index bits 0..3; the other
+// tile-local bits are picked greedily to leave the fewest distinct x-mask
+// parts outside the tile (source tiles per output tile), then the window bits
+// to leave the fewest outside tile and window (distinct source tiles an XCD
+// streams while its workgroups run one class each), then the XCD bits the
+// same way (source tiles the eight XCDs share through the last-level cache),
+// then, among the tile-local bits, the slot bits that split the fewest
+// x-groups into passes.  Ties go to the lowest bit.  Returns false
 // (identity) for spaces too small to tile, and for operators with more than
 // 16,384 distinct x-masks (where the search would cost more than it saves).
-inline bool choose_bit_map(const Operator &op, uint32_t tb, uint32_t ab, uint32_t window, uint8_t *pos) {
+inline bool choose_bit_map(const Operator &op, uint32_t tb, uint32_t ab, uint32_t window, uint32_t xcd,
+                           uint8_t *pos) {
     const uint32_t n = op.nqubits, L = tb + ab;
-    if (n > 64 || tb < 4 || n < L + window) return false;
+    if (n > 64 || tb < 4 || n < L + window + xcd) return false;
     const std::vector<IndexTerm> terms = index_terms(op);
     std::vector<uint64_t> xs(terms.size());
     for (size_t i = 0; i < terms.size(); ++i) xs[i] = terms[i].x;
@@ -433,6 +437,8 @@ inline bool choose_bit_map(const Operator &op, uint32_t tb, uint32_t ab, uint32_
     grow(local, L - 4, 0);
     uint64_t win = 0;
     grow(win, window, local);
+    uint64_t xb = 0;
+    grow(xb, xcd, local | win);
     // slot bits: fewest distinct (x, z restricted to the slot bits) pairs
     std::vector<std::pair<uint64_t, uint64_t>> pairs(terms.size());
     uint64_t slot = 0;
@@ -460,8 +466,9 @@ inline bool choose_bit_map(const Operator &op, uint32_t tb, uint32_t ab, uint32_
     place(local & ~slot);  // thread coordinates (bits 0..3 first)
     place(slot);
     place(win);
+    place(xb);
     const uint64_t all = n == 64 ? ~0ull : (1ull << n) - 1;
-    place(all & ~(local | win));
+    place(all & ~(local | win | xb));
     return true;
 }
 
