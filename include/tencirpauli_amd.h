@@ -459,11 +459,24 @@ uint32_t tcp_batch_config_count(void);
 int tcp_batch_config(uint32_t index, uint32_t *threads, uint32_t *amplitudes);
 uint32_t tcp_batch_plan_selected(const tcp_batch_plan *plan);
 int tcp_batch_plan_select(tcp_batch_plan *plan, uint32_t config);
-/* Times every geometry on the given buffers (after one warm-up each), keeps
- * the fastest; times_ms (optional, tcp_batch_config_count() entries) gets
- * the per-apply times, -1 for geometries that do not fit. */
+/* Classes per work item (split-K over an operator's classes; 0 = all):
+ * an operator with more classes is applied as several items per output
+ * tile whose partial tiles are added with atomic adds to a cleared output,
+ * so the summation order, and the last bits of the result, vary between
+ * runs.  The plan starts at index 0 (no split). */
+uint32_t tcp_batch_split_count(void);
+int tcp_batch_split(uint32_t index, uint32_t *classes_per_item);
+uint32_t tcp_batch_plan_selected_split(const tcp_batch_plan *plan);
+int tcp_batch_plan_select_split(tcp_batch_plan *plan, uint32_t index);
+/* Times every geometry on the given buffers (after one warm-up each) with
+ * whole operators per item, keeps the fastest, then times every split on it
+ * and keeps the fastest; times_ms (optional, tcp_batch_config_count()
+ * entries) gets the geometry times, -1 for geometries that do not fit;
+ * tcp_batch_plan_split_times the split times (tcp_batch_split_count()
+ * entries, -1 for splits not timed because they split no operator). */
 int tcp_batch_plan_autotune(tcp_batch_plan *plan, const void *input, void *output, int dtype,
                             uint32_t repeats, float *times_ms, uint32_t *chosen);
+int tcp_batch_plan_split_times(const tcp_batch_plan *plan, float *times_ms);
 /* output <- H_p input for every operator and right-hand side, on `stream`. */
 int tcp_batch_plan_apply(const tcp_batch_plan *plan, const void *input, void *output, int dtype,
                          void *stream);

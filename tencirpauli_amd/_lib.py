@@ -166,6 +166,11 @@ SIGNATURES = {
     "tcp_batch_plan_selected": (_c.c_uint32, [_vp]),
     "tcp_batch_plan_select": (_c.c_int, [_vp, _c.c_uint32]),
     "tcp_batch_plan_autotune": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _c.c_uint32, _vp, _u32p]),
+    "tcp_batch_split_count": (_c.c_uint32, []),
+    "tcp_batch_split": (_c.c_int, [_c.c_uint32, _u32p]),
+    "tcp_batch_plan_selected_split": (_c.c_uint32, [_vp]),
+    "tcp_batch_plan_select_split": (_c.c_int, [_vp, _c.c_uint32]),
+    "tcp_batch_plan_split_times": (_c.c_int, [_vp, _vp]),
     "tcp_batch_plan_apply": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _vp]),
     "tcp_batch_plan_capture": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _vp]),
     "tcp_batch_plan_replay": (_c.c_int, [_vp, _vp]),

@@ -5,7 +5,10 @@ No reference counterpart: the reference applies one plan per call
 (core/hamiltonian.rs:204-270).  The batch buffer holds, for operator p, its
 ``nrhs`` vectors back to back starting at ``offsets[p]``.  ``autotune`` times
 every tile geometry (threads per workgroup x amplitudes per thread) on the
-caller's buffers and keeps the fastest; ``capture`` records one apply into a
+caller's buffers and keeps the fastest, then the classes per work item on
+it (``splits``: an operator with more classes is applied as several items
+per output tile added with atomic adds, whose summation order varies from
+run to run within rounding); ``capture`` records one apply into a
 hipGraph that ``replay`` launches.  A captured graph is bound to the input
 and output tensors it was captured with: the plan keeps references to them
 (so their memory is not handed to other tensors while the graph can still
@@ -60,9 +63,26 @@ class BatchedMVP:
             out.append({"threads": t.value, "amplitudes_per_thread": a.value})
         return out
 
+    @staticmethod
+    def splits() -> List[int]:
+        """Classes per work item the autotuner tries (0: whole operators)."""
+        out = []
+        for i in range(int(_lib.lib().tcp_batch_split_count())):
+            c = ctypes.c_uint32()
+            _lib.check(_lib.lib().tcp_batch_split(i, ctypes.byref(c)))
+            out.append(int(c.value))
+        return out
+
     @property
     def selected(self) -> int:
         return int(_lib.lib().tcp_batch_plan_selected(self._ptr))
+
+    @property
+    def selected_split(self) -> int:
+        return int(_lib.lib().tcp_batch_plan_selected_split(self._ptr))
+
+    def select_split(self, index: int) -> None:
+        _lib.check(_lib.lib().tcp_batch_plan_select_split(self._ptr, index))
 
     def select(self, config: int) -> None:
         _lib.check(_lib.lib().tcp_batch_plan_select(self._ptr, config))
@@ -96,7 +116,11 @@ class BatchedMVP:
         chosen = ctypes.c_uint32()
         _lib.check(_lib.lib().tcp_batch_plan_autotune(self._ptr, inp.data_ptr(), out.data_ptr(), code, repeats,
                                                       times.ctypes.data, ctypes.byref(chosen)))
-        return {"times_ms": times.tolist(), "chosen": int(chosen.value), "configs": self.configs()}
+        split_times = np.zeros(int(_lib.lib().tcp_batch_split_count()), dtype=np.float32)
+        _lib.check(_lib.lib().tcp_batch_plan_split_times(self._ptr, split_times.ctypes.data))
+        return {"times_ms": times.tolist(), "chosen": int(chosen.value), "configs": self.configs(),
+                "split_times_ms": split_times.tolist(), "chosen_split": self.selected_split,
+                "classes_per_item": self.splits()}
 
     def capture(self, inp: Any, out: Any, stream: Any = None) -> None:
         """Record one apply of `inp` -> `out` into a hipGraph; `replay` reruns it on these tensors."""

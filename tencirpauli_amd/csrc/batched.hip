@@ -11,7 +11,12 @@
 // cleared by a one-thread kernel, so the whole apply can be captured into a
 // hipGraph and replayed.  The autotuner times each tile geometry (threads per
 // workgroup x amplitudes per thread) on the caller's buffers and keeps the
-// fastest.
+// fastest, then the number of classes per work item on that geometry: an
+// operator with more classes than that is split into several items per
+// output tile (split-K over its classes), whose partial tiles are added to
+// the output with atomic adds -- more items to balance few heavy operators
+// over the device, at the price of a cleared output, atomic traffic and a
+// summation order that varies from run to run (within rounding).
 #include <algorithm>
 #include <memory>
 #include <numeric>
@@ -35,6 +40,8 @@ struct BatchProblem {
     uint32_t L;           // tile bits
     uint32_t cls_base;    // first entry in the class table
     uint32_t active;      // amplitudes per tile (2^L, or 2^n when the space is smaller than a tile)
+    uint32_t chunk;       // classes per work item (H: one item per tile)
+    uint32_t chunks;      // work items per output tile, ceil(H / chunk)
     uint32_t pad;
 };
 
@@ -43,8 +50,20 @@ struct Config {
 };
 constexpr Config kConfigs[] = {{256, 2}, {256, 4}, {512, 2}, {128, 8}, {512, 4}};
 constexpr uint32_t kConfigCount = sizeof(kConfigs) / sizeof(kConfigs[0]);
+// classes per work item the autotuner tries (0: all of an operator's classes)
+constexpr uint32_t kSplits[] = {0, 256, 64, 16};
+constexpr uint32_t kSplitCount = sizeof(kSplits) / sizeof(kSplits[0]);
 
 __global__ void reset_counter_kernel(unsigned int *counter) { *counter = 0u; }
+
+// Clears the output before split items add into it (a kernel, not a memset,
+// so the apply can be captured).
+template <typename VT>
+__global__ void clear_kernel(VT *__restrict__ y, uint64_t count) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < count;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        y[i] = VT{0, 0};
+}
 
 template <int A, typename VT, uint32_t TPB>
 __global__ __launch_bounds__(TPB) void batch_kernel(const VT *__restrict__ v, VT *__restrict__ y,
@@ -68,21 +87,26 @@ __global__ __launch_bounds__(TPB) void batch_kernel(const VT *__restrict__ v, VT
         }
         const BatchProblem P = probs[lo];
         const uint64_t local = item - P.item_begin;
-        const uint64_t rhs = local / P.tiles;
-        const uint64_t tile_id = local - rhs * P.tiles;
+        const uint64_t per_rhs = static_cast<uint64_t>(P.tiles) * P.chunks;
+        const uint64_t rhs = local / per_rhs;
+        const uint64_t rest = local - rhs * per_rhs;
+        const uint32_t chunk = static_cast<uint32_t>(rest / P.tiles);  // chunk-major: fewer concurrent adds per tile
+        const uint64_t tile_id = rest - static_cast<uint64_t>(chunk) * P.tiles;
         TileArgs a = shared;
-        a.cls += P.cls_base;
-        a.H = P.H;
+        a.cls += P.cls_base + chunk * P.chunk;
+        a.H = P.chunks > 1 ? min(P.chunk, P.H - chunk * P.chunk) : P.H;
         a.L = P.L;
         const uint64_t off = P.in_off + rhs * P.dim;
-        tcp::tile_apply<A, VT, TPB, true>(v + off, y + off, a, tile_id, P.active, lds, lblk);
+        tcp::tile_apply<A, VT, TPB, true>(v + off, y + off, a, tile_id, P.active, lds, lblk, P.chunks > 1);
     }
 }
 
 struct Tables {
     bool built = false;
-    tcp::DeviceBuffer cls, blk64, blk32, grp64, grp32, term64, term32, probs;
-    uint64_t total_items = 0;
+    tcp::DeviceBuffer cls, blk64, blk32, grp64, grp32, term64, term32;
+    tcp::DeviceBuffer probs[kSplitCount];  // work items for each classes-per-item setting
+    uint64_t total_items[kSplitCount] = {};
+    bool any_split[kSplitCount] = {};
 };
 
 }  // namespace
@@ -95,6 +119,8 @@ struct tcp_batch_plan {
     double work = 0;                // sum of terms x 2^n x nrhs
     int device = 0;
     uint32_t config = 0;
+    uint32_t split = 0;  // index into kSplits
+    float split_times[kSplitCount] = {-1.0f, -1.0f, -1.0f, -1.0f};
     Tables tables[kConfigCount];
     tcp::DeviceBuffer counter;
     uint32_t grid[kConfigCount] = {};
@@ -151,25 +177,34 @@ int build_tables(tcp_batch_plan *plan, uint32_t cfg) {
     std::vector<size_t> order(P);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-    std::vector<BatchProblem> sorted(P);
-    uint64_t item = 0;
-    for (size_t i = 0; i < P; ++i) {
-        sorted[i] = probs[order[i]];
-        sorted[i].item_begin = item;
-        item += static_cast<uint64_t>(sorted[i].tiles) * plan->nrhs;
-    }
-    if (item >= (1ull << 32)) return fail(TCP_ERR_UNSUPPORTED, "more than 2^32 work items");
-    tb.total_items = item;
-    tab.pad_empty();
     int st;
+    for (uint32_t k = 0; k < kSplitCount; ++k) {
+        std::vector<BatchProblem> sorted(P);
+        uint64_t item = 0;
+        bool split = false;
+        for (size_t i = 0; i < P; ++i) {
+            BatchProblem &bp = sorted[i];
+            bp = probs[order[i]];
+            bp.chunk = kSplits[k] && bp.H > kSplits[k] ? kSplits[k] : std::max<uint32_t>(bp.H, 1u);
+            bp.chunks = bp.H > bp.chunk ? (bp.H + bp.chunk - 1) / bp.chunk : 1u;
+            split = split || bp.chunks > 1;
+            bp.item_begin = item;
+            item += static_cast<uint64_t>(bp.tiles) * bp.chunks * plan->nrhs;
+        }
+        if (item >= (1ull << 32)) return fail(TCP_ERR_UNSUPPORTED, "more than 2^32 work items");
+        tb.total_items[k] = item;
+        tb.any_split[k] = split;
+        if ((st = tb.probs[k].upload(sorted.data(), sizeof(BatchProblem) * std::max<size_t>(sorted.size(), 1))))
+            return st;
+    }
+    tab.pad_empty();
     if ((st = tb.cls.upload(tab.cls.data(), sizeof(tcp::TileClass) * tab.cls.size())) ||
         (st = tb.blk64.upload(tab.blk64.data(), sizeof(tcp::u32x4) * tab.blk64.size())) ||
         (st = tb.blk32.upload(tab.blk32.data(), sizeof(tcp::u32x4) * tab.blk32.size())) ||
         (st = tb.grp64.upload(tab.grp64.data(), sizeof(tcp::TileGroup) * tab.grp64.size())) ||
         (st = tb.grp32.upload(tab.grp32.data(), sizeof(tcp::TileGroup) * tab.grp32.size())) ||
         (st = tb.term64.upload(tab.term64.data(), sizeof(tcp::TileTerm) * tab.term64.size())) ||
-        (st = tb.term32.upload(tab.term32.data(), sizeof(tcp::TileTerm) * tab.term32.size())) ||
-        (st = tb.probs.upload(sorted.data(), sizeof(BatchProblem) * sorted.size())))
+        (st = tb.term32.upload(tab.term32.data(), sizeof(tcp::TileTerm) * tab.term32.size())))
         return st;
     // persistent grid: every resident workgroup of the device
     hipDeviceProp_t prop;
@@ -186,7 +221,8 @@ int build_tables(tcp_batch_plan *plan, uint32_t cfg) {
 }
 
 template <int A, typename VT, uint32_t TPB>
-void launch_cfg(const tcp_batch_plan *plan, uint32_t cfg, const void *in, void *out, int dtype, hipStream_t s) {
+void launch_cfg(const tcp_batch_plan *plan, uint32_t cfg, uint32_t split, const void *in, void *out, int dtype,
+                hipStream_t s) {
     const Tables &tb = plan->tables[cfg];
     TileArgs a;
     a.cls = static_cast<const tcp::TileClass *>(tb.cls.ptr);
@@ -199,29 +235,35 @@ void launch_cfg(const tcp_batch_plan *plan, uint32_t cfg, const void *in, void *
     a.n = 0;
     a.permuted = 0;
     for (int c = 0; c < 64; ++c) a.pos[c] = static_cast<uint8_t>(c);
-    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(plan->grid[cfg], tb.total_items));
+    if (tb.any_split[split]) {
+        const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((plan->total + 255) / 256, 256u * 64u));
+        hipLaunchKernelGGL(clear_kernel<VT>, dim3(blocks), dim3(256), 0, s, static_cast<VT *>(out), plan->total);
+    }
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(plan->grid[cfg], tb.total_items[split]));
     hipLaunchKernelGGL((batch_kernel<A, VT, TPB>), dim3(grid), dim3(TPB), 0, s, static_cast<const VT *>(in),
-                       static_cast<VT *>(out), static_cast<const BatchProblem *>(tb.probs.ptr),
-                       static_cast<uint32_t>(plan->ops.size()), a, tb.total_items,
+                       static_cast<VT *>(out), static_cast<const BatchProblem *>(tb.probs[split].ptr),
+                       static_cast<uint32_t>(plan->ops.size()), a, tb.total_items[split],
                        static_cast<unsigned int *>(plan->counter.ptr));
 }
 
 template <typename VT>
-void launch_typed(const tcp_batch_plan *plan, uint32_t cfg, const void *in, void *out, int dtype, hipStream_t s) {
+void launch_typed(const tcp_batch_plan *plan, uint32_t cfg, uint32_t split, const void *in, void *out, int dtype,
+                  hipStream_t s) {
     switch (cfg) {
-        case 0: launch_cfg<2, VT, 256>(plan, cfg, in, out, dtype, s); break;
-        case 1: launch_cfg<4, VT, 256>(plan, cfg, in, out, dtype, s); break;
-        case 2: launch_cfg<2, VT, 512>(plan, cfg, in, out, dtype, s); break;
-        case 3: launch_cfg<8, VT, 128>(plan, cfg, in, out, dtype, s); break;
-        default: launch_cfg<4, VT, 512>(plan, cfg, in, out, dtype, s); break;
+        case 0: launch_cfg<2, VT, 256>(plan, cfg, split, in, out, dtype, s); break;
+        case 1: launch_cfg<4, VT, 256>(plan, cfg, split, in, out, dtype, s); break;
+        case 2: launch_cfg<2, VT, 512>(plan, cfg, split, in, out, dtype, s); break;
+        case 3: launch_cfg<8, VT, 128>(plan, cfg, split, in, out, dtype, s); break;
+        default: launch_cfg<4, VT, 512>(plan, cfg, split, in, out, dtype, s); break;
     }
 }
 
-int enqueue(const tcp_batch_plan *plan, uint32_t cfg, const void *in, void *out, int dtype, hipStream_t s) {
-    if (plan->tables[cfg].total_items == 0) return TCP_OK;
+int enqueue(const tcp_batch_plan *plan, uint32_t cfg, uint32_t split, const void *in, void *out, int dtype,
+            hipStream_t s) {
+    if (plan->tables[cfg].total_items[split] == 0) return TCP_OK;
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, static_cast<unsigned int *>(plan->counter.ptr));
-    if (dtype == TCP_C128) launch_typed<dvec2>(plan, cfg, in, out, dtype, s);
-    else launch_typed<fvec2>(plan, cfg, in, out, dtype, s);
+    if (dtype == TCP_C128) launch_typed<dvec2>(plan, cfg, split, in, out, dtype, s);
+    else launch_typed<fvec2>(plan, cfg, split, in, out, dtype, s);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TCP_OK : tcp::hip_fail(e, "batched kernel launch");
 }
@@ -292,6 +334,28 @@ int tcp_batch_plan_select(tcp_batch_plan *plan, uint32_t config) {
     return tcp::ok();
 }
 
+uint32_t tcp_batch_split_count(void) { return kSplitCount; }
+
+int tcp_batch_split(uint32_t index, uint32_t *classes_per_item) {
+    if (index >= kSplitCount) return fail(TCP_ERR_INVALID_ARGUMENT, "split index out of range");
+    if (classes_per_item) *classes_per_item = kSplits[index];
+    return tcp::ok();
+}
+
+uint32_t tcp_batch_plan_selected_split(const tcp_batch_plan *plan) { return plan ? plan->split : 0; }
+
+int tcp_batch_plan_select_split(tcp_batch_plan *plan, uint32_t index) {
+    if (!plan || index >= kSplitCount) return fail(TCP_ERR_INVALID_ARGUMENT, "bad split");
+    plan->split = index;
+    return tcp::ok();
+}
+
+int tcp_batch_plan_split_times(const tcp_batch_plan *plan, float *times_ms) {
+    if (!plan || !times_ms) return fail(TCP_ERR_INVALID_ARGUMENT, "null argument");
+    for (uint32_t k = 0; k < kSplitCount; ++k) times_ms[k] = plan->split_times[k];
+    return tcp::ok();
+}
+
 int tcp_batch_plan_apply(const tcp_batch_plan *plan, const void *input, void *output, int dtype, void *stream) {
     if (!plan) return fail(TCP_ERR_INVALID_ARGUMENT, "null plan");
     int st = check_buffers(plan, input, output, dtype);
@@ -299,7 +363,7 @@ int tcp_batch_plan_apply(const tcp_batch_plan *plan, const void *input, void *ou
     if (plan->ops.empty()) return tcp::ok();
     tcp::DeviceGuard dg;
     if ((st = dg.enter(plan->device))) return st;
-    st = enqueue(plan, plan->config, input, output, dtype, static_cast<hipStream_t>(stream));
+    st = enqueue(plan, plan->config, plan->split, input, output, dtype, static_cast<hipStream_t>(stream));
     return st ? st : tcp::ok();
 }
 
@@ -314,30 +378,50 @@ int tcp_batch_plan_autotune(tcp_batch_plan *plan, const void *input, void *outpu
     hipEvent_t e0, e1;
     TCP_HIP_CHECK(hipEventCreate(&e0));
     TCP_HIP_CHECK(hipEventCreate(&e1));
-    float best = 3.4e38f;
-    uint32_t best_cfg = plan->config;
-    for (uint32_t c = 0; c < kConfigCount; ++c) {
-        if (times_ms) times_ms[c] = -1.0f;
-        if ((st = build_tables(plan, c))) break;
-        if ((st = enqueue(plan, c, input, output, dtype, nullptr))) break;  // warm-up
+    // per-apply time of (geometry c, split k), after one warm-up
+    auto time_one = [&](uint32_t c, uint32_t k, float &ms) -> int {
+        int s2;
+        if ((s2 = build_tables(plan, c))) return s2;
+        if ((s2 = enqueue(plan, c, k, input, output, dtype, nullptr))) return s2;
         TCP_HIP_CHECK(hipEventRecord(e0, nullptr));
-        for (uint32_t r = 0; r < repeats && !st; ++r) st = enqueue(plan, c, input, output, dtype, nullptr);
-        if (st) break;
+        for (uint32_t r = 0; r < repeats; ++r)
+            if ((s2 = enqueue(plan, c, k, input, output, dtype, nullptr))) return s2;
         TCP_HIP_CHECK(hipEventRecord(e1, nullptr));
         TCP_HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0.0f;
         TCP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
         ms /= static_cast<float>(repeats);
+        return TCP_OK;
+    };
+    // the geometry with whole operators per tile, then the classes per item on it
+    float best = 3.4e38f;
+    uint32_t best_cfg = plan->config, best_split = 0;
+    for (uint32_t c = 0; c < kConfigCount && !st; ++c) {
+        if (times_ms) times_ms[c] = -1.0f;
+        float ms = 0.0f;
+        if ((st = time_one(c, 0, ms))) break;
         if (times_ms) times_ms[c] = ms;
         if (ms < best) {
             best = ms;
             best_cfg = c;
         }
     }
+    for (uint32_t k = 0; k < kSplitCount; ++k) plan->split_times[k] = -1.0f;
+    if (!st) plan->split_times[0] = best;
+    for (uint32_t k = 1; k < kSplitCount && !st; ++k) {
+        if (!plan->tables[best_cfg].any_split[k]) continue;  // splits nothing: same items as whole operators
+        float ms = 0.0f;
+        if ((st = time_one(best_cfg, k, ms))) break;
+        plan->split_times[k] = ms;
+        if (ms < best) {
+            best = ms;
+            best_split = k;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st) return st;
     plan->config = best_cfg;
+    plan->split = best_split;
     if (chosen) *chosen = best_cfg;
     return tcp::ok();
 }
@@ -364,7 +448,7 @@ int tcp_batch_plan_capture(tcp_batch_plan *plan, const void *input, void *output
     }
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        st = plan->ops.empty() ? TCP_OK : enqueue(plan, plan->config, input, output, dtype, s);
+        st = plan->ops.empty() ? TCP_OK : enqueue(plan, plan->config, plan->split, input, output, dtype, s);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(s, &g);
         if (!st && e2 == hipSuccess) {

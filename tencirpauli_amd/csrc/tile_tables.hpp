@@ -60,6 +60,11 @@ template <>
 struct Amp<dvec2> {
     __device__ static dvec2 widen(const dvec2 &v) { return v; }
     __device__ static dvec2 make(double r, double i) { return dvec2{r, i}; }
+    __device__ static void atomic_add(dvec2 *y, double r, double i) {
+        double *d = reinterpret_cast<double *>(y);
+        unsafeAtomicAdd(d, r);
+        unsafeAtomicAdd(d + 1, i);
+    }
 };
 template <>
 struct Amp<fvec2> {
@@ -68,6 +73,11 @@ struct Amp<fvec2> {
     }
     __device__ static fvec2 make(double r, double i) {
         return fvec2{static_cast<float>(r), static_cast<float>(i)};
+    }
+    __device__ static void atomic_add(fvec2 *y, double r, double i) {
+        float *f = reinterpret_cast<float *>(y);
+        unsafeAtomicAdd(f, static_cast<float>(r));
+        unsafeAtomicAdd(f + 1, static_cast<float>(i));
     }
 };
 
@@ -295,10 +305,13 @@ __device__ __forceinline__ VT load_amp(const VT *p) {
 // the whole space is smaller than one tile; then only l < active are read
 // and written).  Source tiles of the next PF classes are in flight in
 // registers (a ring of PF buffers) while the current class is applied.
-// Every thread of the workgroup must call this (it synchronises).
+// Every thread of the workgroup must call this (it synchronises).  With
+// `accumulate` the tile is added to y with atomic adds (the classes of one
+// output tile split over several calls; y cleared beforehand).
 template <int A, typename VT, uint32_t TPB, bool PARTIAL, int PF = 1, bool NT = false, bool PERM = false>
 __device__ __forceinline__ void tile_apply(const VT *__restrict__ vin, VT *__restrict__ yout, const TileArgs &p,
-                                           uint64_t tile_id, uint32_t active, dvec2 *lds, u32x4 *lblk) {
+                                           uint64_t tile_id, uint32_t active, dvec2 *lds, u32x4 *lblk,
+                                           bool accumulate = false) {
     cptr<TileClass> cls = (cptr<TileClass>)p.cls;
     constexpr uint32_t TB = __builtin_ctz(TPB), AB = __builtin_ctz(static_cast<uint32_t>(A));
 
@@ -367,6 +380,12 @@ __device__ __forceinline__ void tile_apply(const VT *__restrict__ vin, VT *__res
         }
     }
     VT *__restrict__ out = yout + base + toff;
+    if (accumulate) {
+#pragma unroll
+        for (int a = 0; a < A; ++a)
+            if (live(a)) Amp<VT>::atomic_add(out + slot_off(a), accr[a], acci[a]);
+        return;
+    }
 #pragma unroll
     for (int a = 0; a < A; ++a)
         if (live(a)) out[slot_off(a)] = Amp<VT>::make(accr[a], acci[a]);

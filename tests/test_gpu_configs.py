@@ -141,7 +141,10 @@ def test_batched_512_operators_8_rhs_autotuned_graph():
         torch.cuda.synchronize()
         batch.replay(stream)
         stream.synchronize()
-        assert torch.equal(y, eager)
+        if report["chosen_split"] == 0:
+            assert torch.equal(y, eager)
+        else:  # split items add with atomics: the summation order varies
+            assert _device_rel_err(y, eager.to(torch.complex128)) <= FP32_TOL
     # every block against its own plan (FP64 apply of the same complex64 inputs)
     worst = 0.0
     for p, op in enumerate(ops):
