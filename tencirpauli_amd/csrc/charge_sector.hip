@@ -18,16 +18,20 @@ typedef double dvec2 __attribute__((ext_vector_type(2)));
 typedef float fvec2 __attribute__((ext_vector_type(2)));
 
 constexpr uint32_t kChargeThreads = 256;
-constexpr uint32_t kChargeLdsLimit = 48 * 1024;  // count tables up to this size are copied to LDS
+constexpr uint32_t kChargeLdsLimit = 64 * 1024;  // count tables up to this size are copied to LDS
 
 struct ChargeArgs {
     uint32_t n;
+    uint32_t m;
     uint32_t G;
     uint64_t dim;
     uint64_t box;
     int64_t enc_target, enc_zero;
     const uint32_t *count;   // [(n + 1) * box]
     const int64_t *enc;      // [n]
+    const int64_t *wt;       // [n * m] weight of axis a in constraint j
+    int64_t target[tcp::kChargeMaxConstraints];
+    int64_t lo[tcp::kChargeMaxConstraints], hi[tcp::kChargeMaxConstraints];
     const uint64_t *gx;      // [G]
     const uint32_t *goff;    // [G + 1]
     const uint64_t *tz;      // [T]
@@ -56,6 +60,34 @@ __host__ __device__ __forceinline__ uint64_t gen_rank(uint32_t n, uint64_t box, 
     return idx == enc_zero ? rank : UINT64_MAX;
 }
 
+// Rank of an arbitrary bitstring s: the walk of gen_rank, returning
+// UINT64_MAX as soon as a prefix remainder leaves the box (no completion).
+__host__ __device__ __forceinline__ uint64_t gen_rank_checked(uint32_t n, uint32_t m, uint64_t box, int64_t enc_target,
+                                                              int64_t enc_zero, const uint32_t *count,
+                                                              const int64_t *enc, const int64_t *wt,
+                                                              const int64_t *target, const int64_t *lo,
+                                                              const int64_t *hi, uint64_t s) {
+    int64_t R[tcp::kChargeMaxConstraints];
+    for (uint32_t j = 0; j < m; ++j) {
+        R[j] = target[j];
+        if (R[j] < lo[j] || R[j] > hi[j]) return UINT64_MAX;  // empty sector
+    }
+    int64_t idx = enc_target;
+    uint64_t rank = 0;
+    while (s) {
+        const uint32_t bit = 63u - static_cast<uint32_t>(__builtin_clzll(s));
+        const uint32_t a = n - 1 - bit;
+        rank += count[static_cast<uint64_t>(a + 1) * box + static_cast<uint64_t>(idx)];
+        idx -= enc[a];
+        for (uint32_t j = 0; j < m; ++j) {
+            R[j] -= wt[static_cast<uint64_t>(a) * m + j];
+            if (R[j] < lo[j] || R[j] > hi[j]) return UINT64_MAX;
+        }
+        s &= ~(1ull << bit);
+    }
+    return idx == enc_zero ? rank : UINT64_MAX;
+}
+
 // State of rank r < dim.
 __host__ __device__ __forceinline__ uint64_t gen_unrank(uint32_t n, uint64_t box, int64_t enc_target,
                                                         const uint32_t *count, const int64_t *enc, uint64_t r) {
@@ -70,6 +102,30 @@ __host__ __device__ __forceinline__ uint64_t gen_unrank(uint32_t n, uint64_t box
         }
     }
     return bits;
+}
+
+// Whether b ^ x keeps every charge of the in-sector state b: per constraint,
+// the weights of the bits x flips, + where b has the bit clear and - where
+// set, sum to zero.  (x is the same in every lane: a scalar loop over its
+// bits.)  Then every prefix of b ^ x stays in the box and gen_rank needs no
+// checks.
+__device__ __forceinline__ bool keeps_charges(const ChargeArgs &p, uint64_t b, uint64_t x) {
+    int64_t d[tcp::kChargeMaxConstraints] = {0, 0, 0, 0};
+    for (uint64_t r = x; r; r &= r - 1) {
+        const uint32_t bit = static_cast<uint32_t>(__builtin_ctzll(r));
+        const uint32_t a = p.n - 1 - bit;
+        const bool set = (b >> bit) & 1ull;
+#pragma unroll
+        for (uint32_t j = 0; j < tcp::kChargeMaxConstraints; ++j)
+            if (j < p.m) {
+                const int64_t w = p.wt[static_cast<uint64_t>(a) * p.m + j];
+                d[j] += set ? -w : w;
+            }
+    }
+    bool ok = true;
+#pragma unroll
+    for (uint32_t j = 0; j < tcp::kChargeMaxConstraints; ++j) ok &= j >= p.m || d[j] == 0;
+    return ok;
 }
 
 template <bool COND>
@@ -132,11 +188,12 @@ __global__ __launch_bounds__(kChargeThreads) void charge_gather_kernel(const VT 
         const uint64_t b = gen_unrank(p.n, p.box, p.enc_target, count, p.enc, r);
         double ar = 0.0, ai = 0.0;
         for (uint32_t g = 0; g < p.G; ++g) {
-            const uint64_t s = b ^ p.gx[g];
-            const uint64_t rs = s == b ? r : gen_rank(p.n, p.box, p.enc_target, p.enc_zero, count, p.enc, s);
-            if (rs == UINT64_MAX) continue;
+            const uint64_t x = p.gx[g];
+            if (x != 0 && !keeps_charges(p, b, x)) continue;
+            const uint64_t s = b ^ x;
             const dvec2 w = group_value<COND>(p, g, s);
             if (w.x == 0.0 && w.y == 0.0) continue;
+            const uint64_t rs = x == 0 ? r : gen_rank(p.n, p.box, p.enc_target, p.enc_zero, count, p.enc, s);
             const VT sv = v[rs];
             const double sr = Amp<VT>::re(sv), si = Amp<VT>::im(sv);
             ar = fma(w.x, sr, ar);
@@ -157,8 +214,7 @@ __global__ __launch_bounds__(kChargeThreads) void charge_leak_kernel(ChargeArgs 
          r += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
         const uint64_t b = gen_unrank(p.n, p.box, p.enc_target, p.count, p.enc, r);
         for (uint32_t g = 0; g < p.G; ++g) {
-            const uint64_t d = b ^ p.gx[g];
-            if (gen_rank(p.n, p.box, p.enc_target, p.enc_zero, p.count, p.enc, d) != UINT64_MAX) continue;
+            if (keeps_charges(p, b, p.gx[g])) continue;  // the destination b ^ x is in the sector
             const dvec2 w = group_value<COND>(p, g, b);
             if (w.x != 0.0 || w.y != 0.0) {
                 atomicMin(first, static_cast<unsigned long long>(r) * p.G + g);
@@ -183,7 +239,9 @@ __global__ void charge_rank_kernel(ChargeArgs p, const uint64_t *__restrict__ bi
          i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
         const uint64_t s = bits[i];
         const bool inside = p.n == 64 || (s >> p.n) == 0;
-        ranks[i] = inside ? gen_rank(p.n, p.box, p.enc_target, p.enc_zero, p.count, p.enc, s) : UINT64_MAX;
+        ranks[i] = inside ? gen_rank_checked(p.n, p.m, p.box, p.enc_target, p.enc_zero, p.count, p.enc, p.wt,
+                                             p.target, p.lo, p.hi, s)
+                          : UINT64_MAX;
     }
 }
 
@@ -205,9 +263,9 @@ __global__ __launch_bounds__(kChargeThreads) void charge_count_kernel(ChargeArgs
         const uint64_t b = gen_unrank(p.n, p.box, p.enc_target, p.count, p.enc, r);
         uint32_t c = 0;
         for (uint32_t g = 0; g < p.G; ++g) {
-            const uint64_t s = b ^ p.gx[g];
-            if (s != b && gen_rank(p.n, p.box, p.enc_target, p.enc_zero, p.count, p.enc, s) == UINT64_MAX) continue;
-            const dvec2 w = group_value<COND>(p, g, s);
+            const uint64_t x = p.gx[g];
+            if (x != 0 && !keeps_charges(p, b, x)) continue;
+            const dvec2 w = group_value<COND>(p, g, b ^ x);
             c += (w.x != 0.0 || w.y != 0.0);
         }
         counts[r] = c;
@@ -226,11 +284,12 @@ __global__ __launch_bounds__(kChargeThreads) void charge_fill_kernel(ChargeArgs 
         const uint64_t len = indptr[r + 1] - pos;
         uint32_t k = 0;
         for (uint32_t g = 0; g < p.G && k < len; ++g) {
-            const uint64_t s = b ^ p.gx[g];
-            const uint64_t rs = s == b ? r : gen_rank(p.n, p.box, p.enc_target, p.enc_zero, p.count, p.enc, s);
-            if (rs == UINT64_MAX) continue;
+            const uint64_t x = p.gx[g];
+            if (x != 0 && !keeps_charges(p, b, x)) continue;
+            const uint64_t s = b ^ x;
             const dvec2 w = group_value<COND>(p, g, s);
             if (w.x == 0.0 && w.y == 0.0) continue;
+            const uint64_t rs = x == 0 ? r : gen_rank(p.n, p.box, p.enc_target, p.enc_zero, p.count, p.enc, s);
             // insertion into the row's sorted prefix (rows hold at most G entries)
             uint32_t i = k;
             while (i > 0 && cols[pos + i - 1] > rs) {
@@ -259,6 +318,13 @@ ChargeArgs args_of(const tcp::ChargeSector &c, int dtype) {
     a.enc_zero = c.geo.enc_zero;
     a.count = static_cast<const uint32_t *>(c.count.ptr);
     a.enc = static_cast<const int64_t *>(c.enc.ptr);
+    a.m = c.geo.m;
+    a.wt = static_cast<const int64_t *>(c.wt.ptr);
+    for (uint32_t j = 0; j < tcp::kChargeMaxConstraints; ++j) {
+        a.target[j] = c.geo.target[j];
+        a.lo[j] = c.geo.lo[j];
+        a.hi[j] = c.geo.hi[j];
+    }
     a.gx = static_cast<const uint64_t *>(c.gx.ptr);
     a.goff = static_cast<const uint32_t *>(c.goff.ptr);
     a.tz = static_cast<const uint64_t *>(c.tz.ptr);
@@ -310,9 +376,11 @@ int charge_geometry(uint32_t n, uint32_t m, const int64_t *levels, const int64_t
             const int64_t w = g.weight[static_cast<size_t>(a) * m + j];
             (w > 0 ? pos : neg) += w;
         }
-        g.lo[j] = g.target[j] - pos;
+        g.lo[j] = std::max(g.target[j] - pos, neg);
+        g.hi[j] = std::min(g.target[j] - neg, pos);
+        if (g.hi[j] < g.lo[j]) g.hi[j] = g.lo[j];  // an empty sector (its target is out of reach)
         g.stride[j] = static_cast<uint64_t>(box);
-        box *= static_cast<unsigned __int128>(pos - neg + 1);
+        box *= static_cast<unsigned __int128>(g.hi[j] - g.lo[j] + 1);
         if (box * (n + 1) > kChargeMaxTableEntries)
             return fail(TCP_ERR_UNSUPPORTED, "charge sector rank tables would exceed 2^24 entries (charge levels "
                                              "too large for the GPU suffix-count tables)");
@@ -374,14 +442,18 @@ int charge_geometry(uint32_t n, uint32_t m, const int64_t *levels, const int64_t
         }
         std::swap(cur, next);
     }
-    g.dim = g.count[static_cast<uint64_t>(g.enc_target)];
+    bool reachable = true;
+    for (uint32_t j = 0; j < m; ++j) reachable &= g.target[j] >= g.lo[j] && g.target[j] <= g.hi[j];
+    g.dim = reachable ? g.count[static_cast<uint64_t>(g.enc_target)] : 0;
     return TCP_OK;
 }
 
 int charge_rank(const ChargeGeometry &g, uint64_t bits, uint64_t &rank) {
     if (g.n < 64 && (bits >> g.n))
         return fail(TCP_ERR_INVALID_INDEX, "occupation is outside the finite sector layout");
-    rank = gen_rank(g.n, g.box, g.enc_target, g.enc_zero, g.count.data(), g.enc.data(), bits);
+    rank = g.dim == 0 ? UINT64_MAX
+                      : gen_rank_checked(g.n, g.m, g.box, g.enc_target, g.enc_zero, g.count.data(), g.enc.data(),
+                                         g.weight.data(), g.target, g.lo, g.hi, bits);
     if (rank == UINT64_MAX)
         return fail(TCP_ERR_INVALID_SECTOR, "invalid sector: occupation does not satisfy every charge constraint");
     return TCP_OK;
@@ -476,14 +548,15 @@ int charge_build(const Operator *op, const std::vector<CondTerm> *fermion_terms,
         tw32.push_back(dvec2{0, 0});
     }
     out.retained_bytes = gx.size() * 8 + goff.size() * 4 + tz.size() * (out.cond ? 8 + 32 + 16 : 8 + 32) +
-                         geo.count.size() * 4 + geo.enc.size() * 8;
+                         geo.count.size() * 4 + geo.enc.size() * 8 + geo.weight.size() * 8;
     if (exceeds_budget(out.retained_bytes, max_bytes)) return memory_limit(out.retained_bytes, max_bytes);
     int st;
     if ((st = out.gx.upload(gx.data(), gx.size() * 8)) || (st = out.goff.upload(goff.data(), goff.size() * 4)) ||
         (st = out.tz.upload(tz.data(), tz.size() * 8)) || (st = out.tw64.upload(tw64.data(), tw64.size() * 16)) ||
         (st = out.tw32.upload(tw32.data(), tw32.size() * 16)) ||
         (st = out.count.upload(geo.count.data(), geo.count.size() * 4)) ||
-        (st = out.enc.upload(geo.enc.data(), std::max<size_t>(1, geo.enc.size()) * 8)))
+        (st = out.enc.upload(geo.enc.data(), std::max<size_t>(1, geo.enc.size()) * 8)) ||
+        (st = out.wt.upload(geo.weight.data(), std::max<size_t>(1, geo.weight.size()) * 8)))
         return st;
     if (out.cond && ((st = out.tc1.upload(tc1.data(), tc1.size() * 8)) ||
                      (st = out.tc0.upload(tc0.data(), tc0.size() * 8))))

@@ -14,10 +14,15 @@
 // R_a being the target minus the contributions of axes 0..a-1.  Levels are
 // shifted so every axis contributes 0 when its bit is clear (the constant
 // part moves into the target); a set bit of axis a then subtracts w_a.
-// Remainders are stored densely: every prefix remainder lies in the box
-// [target - sum of positive weights, target - sum of negative weights] per
-// constraint, encoded mixed radix, so the update R -= w_a is one
-// subtraction of the encoded weight and no remainder leaves the table.
+// Remainders are stored densely over the box, per constraint, of the values
+// a prefix remainder of an in-sector state can take: within [target - sum of
+// positive weights, target - sum of negative weights] (what the prefix can
+// have used) and within [sum of negative weights, sum of positive weights]
+// (what a suffix can still supply), encoded mixed radix, so the update
+// R -= w_a is one subtraction of the encoded weight.  A state whose prefix
+// leaves the box is outside the sector: ranks of arbitrary bitstrings check
+// each step; the kernels first test that a flip keeps every charge, after
+// which the walk stays in the box.
 #pragma once
 
 #include <cstdint>
@@ -36,7 +41,8 @@ struct ChargeGeometry {
     uint32_t n = 0, m = 0;
     std::vector<int64_t> weight;  // [n * m]: level_1 - level_0 of axis a, constraint j
     int64_t target[kChargeMaxConstraints] = {};  // after removing every axis's level_0
-    int64_t lo[kChargeMaxConstraints] = {};      // remainder box
+    int64_t lo[kChargeMaxConstraints] = {};      // remainder box [lo, hi]
+    int64_t hi[kChargeMaxConstraints] = {};
     uint64_t stride[kChargeMaxConstraints] = {};
     uint64_t box = 0;                            // encoded remainders per axis position
     std::vector<uint32_t> count;                 // [(n + 1) * box] completions
@@ -63,7 +69,7 @@ struct ChargeSector {
     bool cond = false;
     bool leaks = false;
     uint64_t leak_source = 0;
-    DeviceBuffer gx, goff, tz, tw64, tw32, tc1, tc0, count, enc;
+    DeviceBuffer gx, goff, tz, tw64, tw32, tc1, tc0, count, enc, wt;
     uint64_t retained_bytes = 0;
 };
 

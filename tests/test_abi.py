@@ -250,9 +250,13 @@ def test_general_charge_sector_edges():
     interleaved = tcp.AdditiveCharge(wide, fermions={m: (1 if m % 2 else -1) for m in range(64)})
     with pytest.raises(tcp.UnsupportedOnDeviceError, match="63"):
         tcp.ChargeSector(((interleaved, 0),))
-    heavy = tcp.AdditiveCharge(20, {a: (0, 1 << (a % 20)) for a in range(20)})
+    # the remainder box holds only what a prefix and a suffix can both reach:
+    # powers of two with a small target stay small, a middle target does not
+    heavy = tcp.AdditiveCharge(24, {a: (0, 1 << a) for a in range(24)})
+    five = heavy.sector(5)  # distinct powers of two: only the binary digits of 5
+    assert five.dimension == 1 and five.unrank(0) == tuple(1 if q in (0, 2) else 0 for q in range(24))
     with pytest.raises(tcp.UnsupportedOnDeviceError, match="2\\^24"):
-        heavy.sector(5)
+        heavy.sector(1 << 23)
     # two population solutions: a union of block layouts; budgets below the
     # enforced floor are advisory (tests/test_charge.py:882-897)
     union = tcp.ChargeSector(((number, 2), (marked, 1), (tcp.AdditiveCharge(space, fermions={1: 1, 3: 1}), 1)),
