@@ -360,8 +360,17 @@ def extra_sector(cpu_seconds: float = 0.0) -> dict:
     dn = tcp.AdditiveCharge.number(32, range(16, 32))
     codes5, coef5 = lattice.heisenberg_periodic(5, 5, seed=SEED)
     op5 = tcp.PauliOperator.from_code_arrays(codes5, coef5)
+    # the same Hubbard model on interleaved spin orbitals (site i up on qubit 2i, down on 2i + 1):
+    # N and 2 S_z are then a general charge sector (suffix-count rank tables)
+    inter = np.zeros_like(codes)
+    inter[:, 0::2], inter[:, 1::2] = codes[:, :16], codes[:, 16:]
+    op_i = tcp.PauliOperator.from_code_arrays(inter, coef)
+    n_i = tcp.AdditiveCharge(32, {a: (0, 1) for a in range(32)})
+    sz_i = tcp.AdditiveCharge(32, {a: (0, 1 if a % 2 == 0 else -1) for a in range(32)})
     cases = [("hubbard_4x4_jw32_nup4_ndn4", op.restrict_charge(tcp.ChargeSector(((up, 4), (dn, 4)))), op,
               "blocks"),
+             ("hubbard_4x4_interleaved_n8_sz0_general", op_i.restrict_charge(tcp.ChargeSector(((n_i, 8), (sz_i, 0)))),
+              op_i, "general"),
              ("xxz_5x5_periodic_sz_half", op5.restrict_charge(tcp.U1Sector(25, 12)), op5, "u1")]
     for name, restricted, operator, kind in cases:
         gen = torch.Generator(device="cuda").manual_seed(SEED + 7)
@@ -389,6 +398,12 @@ def extra_sector(cpu_seconds: float = 0.0) -> dict:
             if kind == "u1":
                 run = lambda rng: oracle.u1_lazy_apply(25, 12, cc, cf, vh, sources=rng)  # noqa: E731
                 algo = "U(1) lazy serial scatter (core/sector.rs:624-640)"
+            elif kind == "general":
+                lv = np.zeros((2, 32, 2), dtype=np.int64)
+                lv[0, :, 1] = 1
+                lv[1, :, 1] = [1 if a % 2 == 0 else -1 for a in range(32)]
+                run = lambda rng: oracle.charge_generic_apply(lv, [8, 0], cc, cf, vh, sources=rng)  # noqa: E731
+                algo = "generic charge loop over the suffix-count sector (core/charge/generic.rs:949-1045)"
             else:
                 run = lambda rng: oracle.blocks_generic_apply([(16, 4), (16, 4)], cc, cf, vh,  # noqa: E731
                                                               sources=rng)
