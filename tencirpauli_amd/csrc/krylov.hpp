@@ -9,7 +9,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdint>
+#include <limits>
 #include <type_traits>
 #include <vector>
 
@@ -117,116 +119,165 @@ inline uint32_t stream_blocks(uint64_t D) {
     return static_cast<uint32_t>(std::min<uint64_t>((D + kThreads - 1) / kThreads, kStreamBlocks));
 }
 
-// Cyclic Jacobi eigensolver for a real symmetric n x n matrix (row-major).
-// Eigenvalues ascending; eigenvector k is column k of `vecs`.
-void jacobi_eigh(int n, std::vector<double> a, std::vector<double> &vals, std::vector<double> &vecs) {
-    vecs.assign(static_cast<size_t>(n) * n, 0.0);
-    for (int i = 0; i < n; ++i) vecs[i * n + i] = 1.0;
-    for (int sweep = 0; sweep < 100; ++sweep) {
-        double off = 0.0, scale = 0.0;
-        for (int i = 0; i < n; ++i) {
-            scale += a[i * n + i] * a[i * n + i];
-            for (int j = i + 1; j < n; ++j) off += a[i * n + j] * a[i * n + j];
+inline double conj_of(double x) { return x; }
+inline std::complex<double> conj_of(const std::complex<double> &x) { return std::conj(x); }
+inline double abs2_of(double x) { return x * x; }
+inline double abs2_of(const std::complex<double> &x) { return std::norm(x); }
+inline double real_of(double x) { return x; }
+inline double real_of(const std::complex<double> &x) { return x.real(); }
+inline double phase_of(double x) { return x < 0.0 ? -1.0 : 1.0; }
+inline std::complex<double> phase_of(const std::complex<double> &x) {
+    const double r = std::abs(x);
+    return r > 0.0 ? x / r : std::complex<double>(1.0, 0.0);
+}
+
+// Eigen-decomposition of a Hermitian n x n matrix `a` (row-major; S = double
+// for real symmetric, std::complex<double> for complex Hermitian input):
+// Householder reduction to tridiagonal form, A = Q T Q^H, a diagonal phase
+// that makes T real (T = D T' D^H), then the implicit QL iteration with
+// Wilkinson shifts on T' (Bowdler, Martin, Reinsch and Wilkinson, Numer.
+// Math. 11 (1968) 293-306), its plane rotations applied to the columns of
+// Q D.  Eigenvalues ascending in `vals`; eigenvector k is column k of the
+// row-major n x n `vecs`.  O(n^3) with a small constant: for n = 24 it takes
+// a few percent of the time of a cyclic Jacobi sweep sequence.
+template <typename S>
+void tridiagonal_eigh(int n, std::vector<S> a, std::vector<double> &vals, std::vector<S> &vecs) {
+    auto A = [&](int i, int j) -> S & { return a[static_cast<size_t>(i) * n + j]; };
+    std::vector<S> q(static_cast<size_t>(n) * n, S(0.0));
+    auto Q = [&](int i, int j) -> S & { return q[static_cast<size_t>(i) * n + j]; };
+    for (int i = 0; i < n; ++i) Q(i, i) = S(1.0);
+    std::vector<S> v(n), p(n);
+    // Householder steps: H_k = I - beta v v^H zeroes column k below the subdiagonal
+    for (int k = 0; k + 2 < n; ++k) {
+        double sigma = 0.0;
+        for (int i = k + 1; i < n; ++i) sigma += abs2_of(A(i, k));
+        const double tail = sigma - abs2_of(A(k + 1, k));
+        if (!(tail > 0.0)) continue;  // already tridiagonal in this column
+        const double norm = std::sqrt(sigma);
+        const S alpha = -phase_of(A(k + 1, k)) * norm;
+        for (int i = k + 1; i < n; ++i) v[i] = A(i, k);
+        v[k + 1] -= alpha;
+        double h = 0.0;
+        for (int i = k + 1; i < n; ++i) h += abs2_of(v[i]);
+        const double beta = 2.0 / h;
+        // p = beta B v, K = beta (v^H p) / 2, q = p - K v, B <- B - v q^H - q v^H
+        double vp = 0.0;
+        for (int i = k + 1; i < n; ++i) {
+            S acc(0.0);
+            for (int j = k + 1; j < n; ++j) acc += A(i, j) * v[j];
+            p[i] = beta * acc;
+            vp += real_of(conj_of(v[i]) * p[i]);
         }
-        if (off <= 1e-32 * (scale + 1e-300)) break;
-        for (int p = 0; p < n; ++p)
-            for (int q = p + 1; q < n; ++q) {
-                const double apq = a[p * n + q];
-                if (std::fabs(apq) < 1e-300) continue;
-                const double tau = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
-                const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(tau * tau + 1.0));
-                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-                for (int k = 0; k < n; ++k) {  // columns p, q
-                    const double akp = a[k * n + p], akq = a[k * n + q];
-                    a[k * n + p] = c * akp - s * akq;
-                    a[k * n + q] = s * akp + c * akq;
+        const double K = 0.5 * beta * vp;
+        for (int i = k + 1; i < n; ++i) p[i] -= K * v[i];
+        for (int i = k + 1; i < n; ++i)
+            for (int j = k + 1; j < n; ++j) A(i, j) -= v[i] * conj_of(p[j]) + p[i] * conj_of(v[j]);
+        A(k + 1, k) = alpha;
+        A(k, k + 1) = conj_of(alpha);
+        for (int i = k + 2; i < n; ++i) A(i, k) = A(k, i) = S(0.0);
+        // Q <- Q H_k (columns k+1..n-1)
+        for (int r = 0; r < n; ++r) {
+            S acc(0.0);
+            for (int j = k + 1; j < n; ++j) acc += Q(r, j) * v[j];
+            acc *= beta;
+            for (int j = k + 1; j < n; ++j) Q(r, j) -= acc * conj_of(v[j]);
+        }
+    }
+    // real tridiagonal T' = D^H T D; the columns of Q pick up the phases
+    std::vector<double> d(n), e(n, 0.0);
+    S ph(1.0);
+    for (int i = 0; i < n; ++i) {
+        d[i] = real_of(A(i, i));
+        if (i > 0)
+            for (int r = 0; r < n; ++r) Q(r, i) *= ph;
+        if (i + 1 < n) {
+            const S sub = A(i + 1, i);
+            e[i] = std::sqrt(abs2_of(sub));
+            ph *= phase_of(sub);
+        }
+    }
+    // implicit QL with Wilkinson shifts; e[i] couples d[i] and d[i + 1]
+    const double eps = std::numeric_limits<double>::epsilon();
+    for (int l = 0; l < n; ++l) {
+        for (int iter = 0; iter < 64; ++iter) {
+            int m = l;
+            while (m + 1 < n && std::fabs(e[m]) > eps * (std::fabs(d[m]) + std::fabs(d[m + 1]))) ++m;
+            if (m == l) break;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+            double s = 1.0, c = 1.0, pp = 0.0;
+            bool split = false;
+            for (int i = m - 1; i >= l; --i) {
+                const double f = s * e[i], b = c * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {  // the rotation vanished: T split at i + 1
+                    d[i + 1] -= pp;
+                    e[m] = 0.0;
+                    split = true;
+                    break;
                 }
-                for (int k = 0; k < n; ++k) {  // rows p, q
-                    const double apk = a[p * n + k], aqk = a[q * n + k];
-                    a[p * n + k] = c * apk - s * aqk;
-                    a[q * n + k] = s * apk + c * aqk;
-                }
-                for (int k = 0; k < n; ++k) {
-                    const double vkp = vecs[k * n + p], vkq = vecs[k * n + q];
-                    vecs[k * n + p] = c * vkp - s * vkq;
-                    vecs[k * n + q] = s * vkp + c * vkq;
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - pp;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                pp = s * r;
+                d[i + 1] = g + pp;
+                g = c * r - b;
+                for (int row = 0; row < n; ++row) {
+                    const S zi = Q(row, i), zj = Q(row, i + 1);
+                    Q(row, i + 1) = s * zi + c * zj;
+                    Q(row, i) = c * zi - s * zj;
                 }
             }
+            if (split) continue;
+            d[l] -= pp;
+            e[l] = g;
+            e[m] = 0.0;
+        }
     }
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) order[i] = i;
-    std::sort(order.begin(), order.end(), [&](int x, int y) { return a[x * n + x] < a[y * n + y]; });
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return d[x] < d[y]; });
     vals.resize(n);
-    std::vector<double> sorted(static_cast<size_t>(n) * n);
+    vecs.assign(static_cast<size_t>(n) * n, S(0.0));
     for (int k = 0; k < n; ++k) {
-        vals[k] = a[order[k] * n + order[k]];
-        for (int i = 0; i < n; ++i) sorted[i * n + k] = vecs[i * n + order[k]];
+        vals[k] = d[order[k]];
+        for (int i = 0; i < n; ++i) vecs[static_cast<size_t>(i) * n + k] = Q(i, order[k]);
     }
-    vecs.swap(sorted);
 }
 
 // Lowest `count` eigenpairs of a Hermitian m x m matrix G (interleaved
-// complex, row-major).  Real matrices are solved directly; complex ones via
-// the real symmetric embedding [[Re, -Im], [Im, Re]] whose spectrum is G's,
-// each eigenvalue twice (one copy per pair is kept).
+// complex, row-major); a matrix whose imaginary parts are all zero is solved
+// in real arithmetic.
 void hermitian_lowest(int m, const std::vector<double> &g, int count, std::vector<double> &vals,
                       std::vector<double> &vecs /* m x count complex, column-major by pair */) {
     bool real = true;
     for (int i = 0; i < m * m; ++i)
         if (g[2 * i + 1] != 0.0) real = false;
-    std::vector<double> ev, evec;
+    std::vector<double> ev;
     vals.assign(count, 0.0);
     vecs.assign(static_cast<size_t>(2) * m * count, 0.0);
     if (real) {
-        std::vector<double> a(static_cast<size_t>(m) * m);
+        std::vector<double> a(static_cast<size_t>(m) * m), evec;
         for (int i = 0; i < m * m; ++i) a[i] = g[2 * i];
-        jacobi_eigh(m, a, ev, evec);
+        tridiagonal_eigh<double>(m, std::move(a), ev, evec);
         for (int k = 0; k < count; ++k) {
             vals[k] = ev[k];
             for (int i = 0; i < m; ++i) vecs[2 * (k * m + i)] = evec[i * m + k];
         }
         return;
     }
-    const int n = 2 * m;
-    std::vector<double> a(static_cast<size_t>(n) * n);
-    for (int i = 0; i < m; ++i)
-        for (int j = 0; j < m; ++j) {
-            const double re = g[2 * (i * m + j)], im = g[2 * (i * m + j) + 1];
-            a[i * n + j] = re;
-            a[(i + m) * n + (j + m)] = re;
-            a[i * n + (j + m)] = -im;
-            a[(i + m) * n + j] = im;
-        }
-    jacobi_eigh(n, a, ev, evec);
-    int found = 0;
-    std::vector<std::vector<double>> kept;
-    for (int k = 0; k < n && found < count; ++k) {
-        std::vector<double> x(2 * m);
-        double norm = 0.0;
+    std::vector<std::complex<double>> a(static_cast<size_t>(m) * m), evec;
+    for (int i = 0; i < m * m; ++i) a[i] = std::complex<double>(g[2 * i], g[2 * i + 1]);
+    tridiagonal_eigh<std::complex<double>>(m, std::move(a), ev, evec);
+    for (int k = 0; k < count; ++k) {
+        vals[k] = ev[k];
         for (int i = 0; i < m; ++i) {
-            x[2 * i] = evec[i * n + k];
-            x[2 * i + 1] = evec[(i + m) * n + k];
+            vecs[2 * (k * m + i)] = evec[i * m + k].real();
+            vecs[2 * (k * m + i) + 1] = evec[i * m + k].imag();
         }
-        // Gram-Schmidt against kept vectors of the same pairs (drop the duplicate copy)
-        for (const auto &y : kept) {
-            double pr = 0.0, pi = 0.0;
-            for (int i = 0; i < m; ++i) {
-                pr += y[2 * i] * x[2 * i] + y[2 * i + 1] * x[2 * i + 1];
-                pi += y[2 * i] * x[2 * i + 1] - y[2 * i + 1] * x[2 * i];
-            }
-            for (int i = 0; i < m; ++i) {
-                x[2 * i] -= pr * y[2 * i] - pi * y[2 * i + 1];
-                x[2 * i + 1] -= pr * y[2 * i + 1] + pi * y[2 * i];
-            }
-        }
-        for (double v : x) norm += v * v;
-        if (norm < 1e-12) continue;
-        norm = std::sqrt(norm);
-        for (double &v : x) v /= norm;
-        kept.push_back(x);
-        vals[found] = ev[k];
-        for (int i = 0; i < 2 * m; ++i) vecs[static_cast<size_t>(found) * 2 * m + i] = x[i];
-        ++found;
     }
 }
 
