@@ -1,6 +1,8 @@
 // Learned diagonal preconditioner for Davidson: a 64-128-128-1 tanh MLP with
-// bf16 weights and FP32 accumulation on MFMA (v_mfma_f32_16x16x16_bf16),
-// trained and evaluated on the GPU.  No reference counterpart.
+// bf16 weights and FP32 accumulation on MFMA (the forward pass on gfx950's
+// v_mfma_f32_16x16x32_bf16, the backward products on
+// v_mfma_f32_16x16x16_bf16), trained and evaluated on the GPU.  No reference
+// counterpart.
 //
 // The network maps per-state features to the log of a diagonal
 // preconditioner, t_i = r_i e^{m_i}, replacing Jacobi's 1 / (theta - H_ii).
@@ -10,7 +12,10 @@
 // MFMA 16x16x16 operand layout (per 64-lane wave): A[i][k] with i = lane % 16,
 // k = 4 (lane / 16) + j; B[k][n] with n = lane % 16, k = 4 (lane / 16) + j;
 // D[i][n] with n = lane % 16, i = 4 (lane / 16) + j; j = 0..3 the element
-// within the lane's vector.
+// within the lane's vector.  16x16x32: A and B hold eight elements per lane,
+// element j of lane group g = lane / 16 multiplying element j of the same
+// group in the other operand (the contraction index of slot (g, j) is the
+// caller's choice, the same in both operands); D as for 16x16x16.
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -27,6 +32,9 @@ constexpr uint32_t kP = kF * kH + kH + kH * kH + kH + kH + 1;
 constexpr uint32_t kOffW1 = 0, kOffB1 = kF * kH, kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH,
                    kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kH;
 constexpr uint32_t kThreads = 256;
+// The forward pass: eight waves share one LDS copy of the weights (48 KiB),
+// so three workgroups fill a CU with six waves per SIMD.
+constexpr uint32_t kFwdThreads = 512;
 // The Davidson callback re-evaluates the network when theta has moved by more
 // than kRefresh times the coupling scale since its last evaluation (default;
 // tcp_learned_precond_set_refresh changes it).  A preconditioner for a
@@ -75,9 +83,34 @@ __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f4 mfma32(s8 a, s8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0,
+                                                   0);
+}
+// Eight bf16 values: lo[0..3], hi[0..3].
+__device__ __forceinline__ s8 pack_bf16x8(f4 lo, f4 hi) {
+    return __builtin_bit_cast(
+        s8, (u4{cvt_pk_bf16(lo.x, lo.y), cvt_pk_bf16(lo.z, lo.w), cvt_pk_bf16(hi.x, hi.y), cvt_pk_bf16(hi.z, hi.w)}));
+}
+
+// Forward-pass fragments (16x16x32).  Layer 1, block n of 16 hidden units and
+// block ks of 32 features: slot (lane, j) holds W1[32 ks + 8 g + j][16 n + lane % 16]
+// (features 8g..8g+7 of a row are one 16-byte load of X).  Layer 2, block m of
+// 16 outputs and block q of 32 hidden inputs: slot (lane, j) holds
+// W2[k2(q, g, j)][16 m + lane % 16] with k2 = 32 q + 4 g + j for j < 4 and
+// 32 q + 16 + 4 g + (j - 4) for j >= 4 -- the units a lane holds in the layer-1
+// results of blocks 2q and 2q + 1, so those results are the layer-2 B operand
+// as they stand.
+__host__ __device__ constexpr uint32_t w2_input(uint32_t q, uint32_t g, uint32_t j) {
+    return 32 * q + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+}
+
 struct MlpDev {
-    const s4 *w1f;   // [8 n][4 ks][64 lane] fragments of W1 (bf16 x4)
-    const s4 *w2f;   // [8 n][8 ks][64 lane] fragments of W2
+    const s8 *w1f;   // [8 n][2 ks][64 lane] forward fragments of W1 (bf16 x8)
+    const s8 *w2f;   // [8 m][4 q][64 lane] forward fragments of W2
     const uint16_t *w2b;  // W2 bf16, row-major [in][out]
     const float *params;  // FP32 master parameters
 };
@@ -86,15 +119,16 @@ struct MlpDev {
 __global__ void swizzle_kernel(const float *__restrict__ params, uint16_t *__restrict__ w1f,
                                uint16_t *__restrict__ w2f, uint16_t *__restrict__ w2b) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    // W1: 8 * 4 * 64 * 4 = 8192 entries
-    if (tid < 8 * 4 * 64 * 4) {
-        const uint32_t j = tid & 3, lane = (tid >> 2) & 63, ks = (tid >> 8) & 3, n = tid >> 10;
-        const uint32_t k = ks * 16 + 4 * (lane >> 4) + j, col = n * 16 + (lane & 15);
+    // W1: 8 * 2 * 64 * 8 = 8192 entries
+    if (tid < 8 * 2 * 64 * 8) {
+        const uint32_t j = tid & 7, lane = (tid >> 3) & 63, ks = (tid >> 9) & 1, n = tid >> 10;
+        const uint32_t k = ks * 32 + 8 * (lane >> 4) + j, col = n * 16 + (lane & 15);
         w1f[tid] = f2bf(params[kOffW1 + k * kH + col]);
     }
-    if (tid < 8 * 8 * 64 * 4) {
-        const uint32_t j = tid & 3, lane = (tid >> 2) & 63, ks = (tid >> 8) & 7, n = tid >> 11;
-        const uint32_t k = ks * 16 + 4 * (lane >> 4) + j, col = n * 16 + (lane & 15);
+    // W2: 8 * 4 * 64 * 8 = 16384 entries
+    if (tid < 8 * 4 * 64 * 8) {
+        const uint32_t j = tid & 7, lane = (tid >> 3) & 63, q = (tid >> 9) & 3, m = tid >> 11;
+        const uint32_t k = w2_input(q, lane >> 4, j), col = m * 16 + (lane & 15);
         w2f[tid] = f2bf(params[kOffW2 + k * kH + col]);
         w2b[tid] = f2bf(params[kOffW2 + tid]);
     }
@@ -115,25 +149,26 @@ __global__ void fill_feature0_kernel(uint16_t *__restrict__ X, const double *__r
 // Both layers run transposed, H^T = W^T X^T: the weight fragments are the A
 // operand and 16 rows of X (or of H1) the B operand.  The MFMA result then
 // holds, in each lane, hidden units 4 (lane / 16) + j of one 16-block for row
-// lane % 16 -- exactly the B-operand layout of the next layer, so the first
-// hidden layer goes from accumulator to bf16 fragment in registers (no LDS
-// staging, no barrier).  The output layer is a dot product over the lanes that
-// share a row.
+// lane % 16; two such blocks are the next layer's B fragment (the W2
+// fragments follow that unit order), so the first hidden layer goes from
+// accumulator to bf16 fragment in registers (no LDS staging, no barrier).
+// Each wave takes R tiles of 16 rows per weight-fragment read from LDS.  The
+// output layer is a dot product over the lanes that share a row.
 // Without STORE (inference) the output needs only sum_u w3_u tanh z_u =
 // sum_u w3_u - 2 sum_u w3_u / (e^{2 z_u} + 1): each lane starts its partial
 // sum at its share of sum_u w3_u and adds one FMA per hidden unit.
-template <bool STORE>
-__global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const uint16_t *__restrict__ X,
+template <bool STORE, int R>
+__global__ __launch_bounds__(kFwdThreads) void mlp_forward_kernel(MlpDev p, const uint16_t *__restrict__ X,
                                                                uint64_t B, float *__restrict__ y,
                                                                float *__restrict__ h1, float *__restrict__ h2,
                                                                const double *__restrict__ diag, double theta,
                                                                double inv_scale) {
-    __shared__ s4 w1s[8 * 4 * 64];
-    __shared__ s4 w2s[8 * 8 * 64];
+    __shared__ s8 w1s[8 * 2 * 64];
+    __shared__ s8 w2s[8 * 4 * 64];
     __shared__ float bs1[kH], bs2[kH], ws3[kH];  // biases times kTwoLog2e; w3 (times -2 without STORE)
-    for (uint32_t i = threadIdx.x; i < 8 * 4 * 64; i += kThreads) w1s[i] = p.w1f[i];
-    for (uint32_t i = threadIdx.x; i < 8 * 8 * 64; i += kThreads) w2s[i] = p.w2f[i];
-    for (uint32_t i = threadIdx.x; i < kH; i += kThreads) {
+    for (uint32_t i = threadIdx.x; i < 8 * 2 * 64; i += kFwdThreads) w1s[i] = p.w1f[i];
+    for (uint32_t i = threadIdx.x; i < 8 * 4 * 64; i += kFwdThreads) w2s[i] = p.w2f[i];
+    for (uint32_t i = threadIdx.x; i < kH; i += kFwdThreads) {
         bs1[i] = p.params[kOffB1 + i] * kTwoLog2e;
         bs2[i] = p.params[kOffB2 + i] * kTwoLog2e;
         ws3[i] = STORE ? p.params[kOffW3 + i] : -2.0f * p.params[kOffW3 + i];
@@ -146,63 +181,109 @@ __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(MlpDev p, const u
     if (!STORE)
         for (int n = 0; n < 8; ++n)
             for (int j = 0; j < 4; ++j) base += p.params[kOffW3 + n * 16 + lg * 4 + j];
-    const uint64_t ntile = (B + 63) / 64;
+    constexpr uint64_t kRows = (kFwdThreads / 64) * 16 * R;  // rows per workgroup step: R tiles of 16 per wave
+    const uint64_t ntile = (B + kRows - 1) / kRows;
+    // The next step's rows are loaded while this step computes (with three
+    // waves per SIMD the HBM latency of X would otherwise stall each step).
+    s8 xn[R][2];
+    double dn[R];
+    auto load_rows = [&](uint64_t tile) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint64_t rr = tile * kRows + (wave * R + r) * 16 + li;
+            const bool ok = tile < ntile && rr < B;
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks)
+                xn[r][ks] = ok ? *reinterpret_cast<const s8 *>(X + rr * kF + ks * 32 + lg * 8)
+                               : s8{0, 0, 0, 0, 0, 0, 0, 0};
+            dn[r] = diag && ok && lg == 0 ? diag[rr] : 0.0;
+        }
+    };
+    load_rows(blockIdx.x);
     for (uint64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-        // keeps the weight fragments in LDS: without it the compiler hoists all
-        // 96 loop-invariant fragment loads into registers (occupancy 1)
+        // keeps the weight fragments in LDS: without it the compiler hoists the
+        // loop-invariant fragment loads into registers (occupancy 1)
         asm volatile("" ::: "memory");
-        const uint64_t row = tile * 64 + wave * 16 + li;
-        const bool valid = row < B;
-        s4 x[4];
+        uint64_t row[R];
+        bool valid[R];
+        s8 x[R][2];
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-            x[ks] = valid ? *reinterpret_cast<const s4 *>(X + row * kF + ks * 16 + lg * 4) : s4{0, 0, 0, 0};
-        if (diag && valid && lg == 0)
-            x[0][0] = static_cast<short>(f2bf(static_cast<float>((diag[row] - theta) * inv_scale)));
-        f4 acc[8];
-#pragma unroll
-        for (int n = 0; n < 8; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-#pragma unroll
-            for (int n = 0; n < 8; ++n) acc[n] = mfma(w1s[(n * 4 + ks) * 64 + lane], x[ks], acc[n]);
-        s4 hb[8];  // layer-1 activations of block n as the layer-2 B fragment
-#pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            f4 v;
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                v[j] = fmaf(-2.0f, tanh_denominator_rcp(fmaf(acc[n][j], kTwoLog2e, bs1[n * 16 + lg * 4 + j])), 1.0f);
-            hb[n] = pack_bf16(v);
-            if (STORE && valid) *reinterpret_cast<f4 *>(h1 + row * kH + n * 16 + lg * 4) = v;
+        for (int r = 0; r < R; ++r) {
+            row[r] = tile * kRows + (wave * R + r) * 16 + li;
+            valid[r] = row[r] < B;
+            x[r][0] = xn[r][0];
+            x[r][1] = xn[r][1];
+            if (diag && valid[r] && lg == 0)
+                x[r][0][0] = static_cast<short>(f2bf(static_cast<float>((dn[r] - theta) * inv_scale)));
         }
-        f4 acc2[8];
+        load_rows(tile + gridDim.x);
+        f4 acc[R][8];
 #pragma unroll
-        for (int n = 0; n < 8; ++n) acc2[n] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int ks = 0; ks < 8; ++ks)
+            for (int n = 0; n < 8; ++n) acc[r][n] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int n = 0; n < 8; ++n) acc2[n] = mfma(w2s[(n * 8 + ks) * 64 + lane], hb[ks], acc2[n]);
-        float part = base;
+        for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            f4 v;
+            for (int n = 0; n < 8; ++n) {
+                const s8 a = w1s[(n * 2 + ks) * 64 + lane];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t u = n * 16 + lg * 4 + j;
-                const float r = tanh_denominator_rcp(fmaf(acc2[n][j], kTwoLog2e, bs2[u]));
-                if (STORE) {
-                    v[j] = fmaf(-2.0f, r, 1.0f);
-                    part = fmaf(v[j], ws3[u], part);
-                } else {
-                    part = fmaf(ws3[u], r, part);
-                }
+                for (int r = 0; r < R; ++r) acc[r][n] = mfma32(a, x[r][ks], acc[r][n]);
             }
-            if (STORE && valid) *reinterpret_cast<f4 *>(h2 + row * kH + n * 16 + lg * 4) = v;
+        s8 hb[R][4];  // layer-1 activations of blocks 2q, 2q + 1 as the layer-2 B fragment q
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                f4 v[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int n = 2 * q + h;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        v[h][j] = fmaf(-2.0f,
+                                       tanh_denominator_rcp(fmaf(acc[r][n][j], kTwoLog2e, bs1[n * 16 + lg * 4 + j])),
+                                       1.0f);
+                    if (STORE && valid[r]) *reinterpret_cast<f4 *>(h1 + row[r] * kH + n * 16 + lg * 4) = v[h];
+                }
+                hb[r][q] = pack_bf16x8(v[0], v[1]);
+            }
+        f4 acc2[R][8];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int m = 0; m < 8; ++m) acc2[r][m] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const s8 a = w2s[(m * 4 + q) * 64 + lane];
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc2[r][m] = mfma32(a, hb[r][q], acc2[r][m]);
+            }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float part = base;
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                f4 v;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t u = m * 16 + lg * 4 + j;
+                    const float rc = tanh_denominator_rcp(fmaf(acc2[r][m][j], kTwoLog2e, bs2[u]));
+                    if (STORE) {
+                        v[j] = fmaf(-2.0f, rc, 1.0f);
+                        part = fmaf(v[j], ws3[u], part);
+                    } else {
+                        part = fmaf(ws3[u], rc, part);
+                    }
+                }
+                if (STORE && valid[r]) *reinterpret_cast<f4 *>(h2 + row[r] * kH + m * 16 + lg * 4) = v;
+            }
+            part += __shfl_xor(part, 16);
+            part += __shfl_xor(part, 32);
+            if (lg == 0 && valid[r]) y[row[r]] = part + b3;
         }
-        part += __shfl_xor(part, 16);
-        part += __shfl_xor(part, 32);
-        if (lg == 0 && valid) y[row] = part + b3;
     }
 }
 
@@ -513,7 +594,7 @@ struct tcp_mlp {
     // training scratch (grown on demand)
     tcp::DeviceBuffer h1, h2, y, da2, da1, partial, small;
     MlpDev dev() const {
-        return MlpDev{static_cast<const s4 *>(w1f.ptr), static_cast<const s4 *>(w2f.ptr),
+        return MlpDev{static_cast<const s8 *>(w1f.ptr), static_cast<const s8 *>(w2f.ptr),
                       static_cast<const uint16_t *>(w2b.ptr), static_cast<const float *>(params.ptr)};
     }
     int refresh(hipStream_t s) {
@@ -563,18 +644,36 @@ int ensure(tcp::DeviceBuffer &b, size_t bytes) {
     return b.allocate(bytes);
 }
 
+// Row tiles of 16 per wave and weight read (inference; training stores the
+// hidden layers and keeps one).
+constexpr int kInferTiles = 1, kTrainTiles = 1;
+
+// Exactly the workgroups the device holds at once (a persistent grid: one
+// more partial round of blocks would leave most of the device idle), fewer
+// for small batches.
+template <typename K>
+uint32_t forward_blocks(K kernel, uint64_t B, uint64_t rows_per_block) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kFwdThreads, 0) != hipSuccess || per_cu < 1)
+        cus = 256, per_cu = 2;
+    const uint64_t tiles = (B + rows_per_block - 1) / rows_per_block;
+    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(tiles, static_cast<uint64_t>(cus) * per_cu)));
+}
+
 int forward(const tcp_mlp *mlp, const uint16_t *X, uint64_t B, float *y, float *h1, float *h2, hipStream_t s,
             const double *diag = nullptr, double theta = 0.0, double inv_scale = 0.0) {
     if (B == 0) return TCP_OK;
-    const uint64_t tiles = (B + 63) / 64;
-    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(tiles, 256 * 4));
     if (h1 || h2) {
         if (!h1 || !h2) return fail(TCP_ERR_INVALID_ARGUMENT, "mlp forward stores both hidden layers or neither");
-        hipLaunchKernelGGL(mlp_forward_kernel<true>, dim3(blocks), dim3(kThreads), 0, s, mlp->dev(), X, B, y, h1, h2,
-                           diag, theta, inv_scale);
+        const uint32_t blocks = forward_blocks(mlp_forward_kernel<true, kTrainTiles>, B, 16 * (kFwdThreads / 64) * kTrainTiles);
+        hipLaunchKernelGGL((mlp_forward_kernel<true, kTrainTiles>), dim3(blocks), dim3(kFwdThreads), 0, s, mlp->dev(), X,
+                           B, y, h1, h2, diag, theta, inv_scale);
     } else {
-        hipLaunchKernelGGL(mlp_forward_kernel<false>, dim3(blocks), dim3(kThreads), 0, s, mlp->dev(), X, B, y, h1,
-                           h2, diag, theta, inv_scale);
+        const uint32_t blocks = forward_blocks(mlp_forward_kernel<false, kInferTiles>, B, 16 * (kFwdThreads / 64) * kInferTiles);
+        hipLaunchKernelGGL((mlp_forward_kernel<false, kInferTiles>), dim3(blocks), dim3(kFwdThreads), 0, s, mlp->dev(),
+                           X, B, y, h1, h2, diag, theta, inv_scale);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TCP_OK : tcp::hip_fail(e, "mlp forward");
