@@ -38,7 +38,11 @@ def test_mlp_forward_matches_reference(batch):
     Xd = _bf16_tensor(X)
     got = mlp.forward(Xd).cpu().numpy()
     expected = ref.forward(p, Xd.float().cpu().numpy())[0]
-    np.testing.assert_allclose(got, expected, atol=2e-3 * max(1.0, np.abs(expected).max()))
+    # The reference rounds every product input to bf16 as the kernel does; what
+    # remains is FP32 accumulation order, the kernel's exp2/rcp tanh (~1e-7) and
+    # rare bf16 rounding-boundary flips of a first-layer activation (measured
+    # max |error| 1.5e-4 of the output scale).
+    np.testing.assert_allclose(got, expected, atol=5e-4 * max(1.0, np.abs(expected).max()))
 
 
 def test_mlp_gradient_matches_reference():
@@ -57,7 +61,9 @@ def test_mlp_gradient_matches_reference():
     offsets = np.cumsum([0, 64 * 128, 128, 128 * 128, 128, 128, 1])
     for name, lo, hi in zip(("W1", "b1", "W2", "b2", "w3", "b3"), offsets[:-1], offsets[1:]):
         err = np.linalg.norm(got[lo:hi] - expected[lo:hi]) / max(np.linalg.norm(expected[lo:hi]), 1e-12)
-        assert err <= 1e-2, (name, err)
+        # the same rounding model as the forward check; measured worst block
+        # (W1) 1.8e-4 over three seeds
+        assert err <= 1e-3, (name, err)
 
 
 @pytest.mark.parametrize("mode", ["learned", "jacobi"])
