@@ -194,6 +194,7 @@ __global__ __launch_bounds__(kChargeThreads) void charge_gather_kernel(const VT 
             const dvec2 w = group_value<COND>(p, g, s);
             if (w.x == 0.0 && w.y == 0.0) continue;
             const uint64_t rs = x == 0 ? r : gen_rank(p.n, p.box, p.enc_target, p.enc_zero, count, p.enc, s);
+            if (rs >= p.dim) continue;  // unreachable for an in-sector s; keeps the gather in bounds
             const VT sv = v[rs];
             const double sr = Amp<VT>::re(sv), si = Amp<VT>::im(sv);
             ar = fma(w.x, sr, ar);
