@@ -469,7 +469,8 @@ def extra_eager(cpu_seconds: float = 0.0) -> dict:
     """5x5 periodic XXZ, S_z = 1/2 (dim 5,200,300): the eager restricted operator (device CSR)
     against the lazy sector gather.  Bytes per apply: indptr 8 (D+1) + y 16|8 D + one read of
     v 16|8 D (gathers are assumed to hit in cache), plus, for plain CSR, columns 4 nnz + values
-    16|8 nnz, or, for the compact form, one packed word 4 nnz + the diagonal 16|8 D."""
+    16|8 nnz, or, for the compact form, one packed word 4 nnz + the diagonal 16|8 D; the slab
+    form reads one word per (x-group slot, row), 4 G_slab D, the diagonal and no indptr."""
     import torch
 
     import tencirpauli_amd as tcp
@@ -511,7 +512,9 @@ def extra_eager(cpu_seconds: float = 0.0) -> dict:
         ms_lazy = time_applies(lambda: lazy.apply_into(vv, yy), 10)
         eager.apply_into(vv, yy)
         err = (torch.linalg.vector_norm(yy.to(torch.complex128) - y_ref) / torch.linalg.vector_norm(y_ref)).item()
-        if plan.storage_format == "compact":
+        if plan.storage_format == "slab":
+            alg = 4 * plan.slab_groups * dim + 3 * s * dim
+        elif plan.storage_format == "compact":
             alg = 8 * (dim + 1) + 4 * nnz + 3 * s * dim
         else:
             alg = 8 * (dim + 1) + (4 + s) * nnz + 2 * s * dim

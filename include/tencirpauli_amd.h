@@ -428,6 +428,21 @@ uint64_t tcp_csr_plan_estimated_bytes(const tcp_csr_plan *plan);
  * No reference counterpart (the reference's CsrGraph, core/csr.rs, stores a
  * value per entry); results are the same either way. */
 int tcp_csr_plan_is_compact(const tcp_csr_plan *plan);
+/* The form the SpMV reads: TCP_CSR_PLAIN, TCP_CSR_COMPACT, or TCP_CSR_SLAB --
+ * the compact entries regrouped by x-mask, one 32-bit word per (x-group,
+ * row) with 0xFFFFFFFF where the row has no transition in the group, read one
+ * row per lane so that a wave's gathers under one x-mask coalesce (chosen
+ * for restricted operators of 64 qubits or fewer when the x-groups times the
+ * rows are at most 8/3 of the off-diagonal entries and the budget allows);
+ * -1 for a null plan.  No reference counterpart; results agree with plain
+ * CSR to FP64 rounding (the summation order differs). */
+#define TCP_CSR_PLAIN 0
+#define TCP_CSR_COMPACT 1
+#define TCP_CSR_SLAB 2
+int tcp_csr_plan_storage_form(const tcp_csr_plan *plan);
+/* x-group slots of the slab form (its words: slots x dimension); 0 for the
+ * other forms.  No reference counterpart. */
+uint32_t tcp_csr_plan_slab_groups(const tcp_csr_plan *plan);
 /* y = A v: row-parallel CSR gather (CsrGraph::apply_into), FP64 accumulation;
  * TCP_C64 uses complex64 vectors and FP32-rounded values. */
 int tcp_csr_plan_apply_into(const tcp_csr_plan *plan, const void *input, void *output, uint64_t input_len,
@@ -622,7 +637,7 @@ int tcp_learned_precond_create(const tcp_mlp *mlp, const tcp_sector_plan *plan,
 const void *tcp_learned_precond_log_multiplier(void *ctx, double theta, void *stream);
 /* The callback re-evaluates the network only when theta has moved by more
  * than threshold * scale (scale: the operator's coupling scale) since its last
- * evaluation; 0 re-evaluates on every call.  Default 0.5. */
+ * evaluation; 0 re-evaluates on every call.  Default 2. */
 int tcp_learned_precond_set_refresh(tcp_learned_precond *precond, double threshold);
 /* Writes feature 0, (H_ii - theta) / scale, of every state (the callback does
  * this itself; training calls it before tcp_mlp_forward / tcp_mlp_gradient on

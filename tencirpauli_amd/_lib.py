@@ -152,6 +152,8 @@ SIGNATURES = {
     "tcp_csr_plan_transition_count": (_c.c_uint64, [_vp]),
     "tcp_csr_plan_estimated_bytes": (_c.c_uint64, [_vp]),
     "tcp_csr_plan_is_compact": (_c.c_int, [_vp]),
+    "tcp_csr_plan_storage_form": (_c.c_int, [_vp]),
+    "tcp_csr_plan_slab_groups": (_c.c_uint32, [_vp]),
     "tcp_csr_plan_apply_into": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _c.c_uint64, _c.c_int, _vp]),
     "tcp_csr_plan_apply_host": (_c.c_int, [_vp, _dp, _dp, _c.c_uint64, _c.c_uint64]),
     "tcp_csr_plan_export": (_c.c_int, [_vp, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64), _dp]),

@@ -32,18 +32,18 @@ constexpr uint32_t kP = kF * kH + kH + kH * kH + kH + kH + 1;
 constexpr uint32_t kOffW1 = 0, kOffB1 = kF * kH, kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH,
                    kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kH;
 constexpr uint32_t kThreads = 256;
-// The forward pass: eight waves share one LDS copy of the weights (48 KiB),
-// so three workgroups fill a CU with six waves per SIMD.
+// The forward pass: eight waves share one LDS copy of the weights (48 KiB);
+// at its register count two workgroups fit a CU, four waves per SIMD.
 constexpr uint32_t kFwdThreads = 512;
 // The Davidson callback re-evaluates the network when theta has moved by more
 // than kRefresh times the coupling scale since its last evaluation (default;
 // tcp_learned_precond_set_refresh changes it).  A preconditioner for a
-// slightly different theta is as good a search direction: on the 4x4 and 4x5
-// training-family tori the iteration counts are the same for refresh
-// thresholds from 0 to 0.5 energy units (26 and 42), while the evaluations
-// fall from 25 and 41 to 7 and 13; on the 5x5 torus (D = 5,200,300) they are
-// the same from 0.25 to 1.0.
-constexpr double kRefresh = 0.5;
+// somewhat different theta is as good a search direction: on the seven
+// sectors the network is trained on (4x4, 3x6 and 4x5 tori) the learned
+// Davidson iteration counts at subspaces 12/4 and 24/4 are the same for
+// every threshold from 0 (evaluate at every iteration) to 2 (the largest
+// tried), while the evaluations fall several-fold.
+constexpr double kRefresh = 2.0;
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));

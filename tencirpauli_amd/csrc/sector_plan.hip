@@ -954,6 +954,104 @@ __global__ __launch_bounds__(256) void csr_compact_spmv_kernel(const uint64_t *_
     }
 }
 
+// ---------------------------------------------------------------- slab form
+// The compact entries regrouped by x-group: slot q's words for consecutive
+// rows are consecutive in memory, and one thread per row visits the slots in
+// turn, so a wave reads 64 packed words of one group in one access and its
+// gathers, the sources of consecutive rows under one x-mask, fall in few
+// cache lines (the CSR's lanes gather unrelated columns).  Rows without a
+// transition in a group hold kSlabEmpty.
+constexpr uint32_t kSlabEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kSlabUnroll = 8;  // slots whose loads are in flight together
+
+__global__ void fill_words_kernel(uint32_t *__restrict__ p, uint64_t count, uint32_t value) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < count;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        p[i] = value;
+}
+
+// Index of x in the ascending xs[0..nx), or nx.
+__device__ __forceinline__ uint32_t find_mask(const uint64_t *__restrict__ xs, uint32_t nx, uint64_t x) {
+    uint32_t lo = 0, hi = nx;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (xs[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < nx && xs[lo] == x ? lo : nx;
+}
+
+// Pass 1 (slot == nullptr): marks used[g] for the x-mask g of every
+// off-diagonal entry (x = state(row) ^ state(column)), counts those entries
+// into *offdiag and raises *bad when an x-mask is not in xs.  Pass 2: writes
+// each entry into its row of slot slot[g].
+__global__ __launch_bounds__(256) void slab_scatter_kernel(const uint64_t *__restrict__ indptr,
+                                                           const uint32_t *__restrict__ packed,
+                                                           const uint64_t *__restrict__ states, uint64_t dim,
+                                                           const uint64_t *__restrict__ xs, uint32_t nx,
+                                                           uint32_t *__restrict__ used,
+                                                           const uint32_t *__restrict__ slot,
+                                                           uint32_t *__restrict__ slab,
+                                                           unsigned long long *offdiag, uint32_t *bad) {
+    unsigned long long local = 0;
+    for (uint64_t r = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; r < dim;
+         r += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        for (uint64_t k = indptr[r]; k < indptr[r + 1]; ++k) {
+            const uint32_t w = packed[k];
+            if ((w >> kColBits) == kDiagIndex) continue;
+            const uint32_t c = w & kColMask;
+            const uint32_t g = c < dim ? find_mask(xs, nx, states[r] ^ states[c]) : nx;
+            if (g == nx) {
+                atomicExch(bad, 1u);
+                continue;
+            }
+            if (slot) slab[static_cast<uint64_t>(slot[g]) * dim + r] = w;
+            else used[g] = 1u;
+            ++local;
+        }
+    }
+    if (!slot && local) atomicAdd(offdiag, local);
+}
+
+// y = H v over the slab form: one row per thread, kSlabUnroll slots' words
+// and gathers in flight together, FP64 accumulation; the value table is in LDS.
+template <typename VT>
+__global__ __launch_bounds__(256) void slab_spmv_kernel(const uint32_t *__restrict__ slab, uint32_t nslab,
+                                                        const VT *__restrict__ table, uint32_t ntable,
+                                                        const VT *__restrict__ diag, const VT *__restrict__ v,
+                                                        VT *__restrict__ y, uint64_t dim) {
+    __shared__ VT tab[kDictMax];
+    for (uint32_t i = threadIdx.x; i < kDictMax; i += blockDim.x) tab[i] = i < ntable ? table[i] : VT{0, 0};
+    __syncthreads();
+    for (uint64_t r = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; r < dim;
+         r += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const VT d = diag[r], vr = v[r];
+        double ar = static_cast<double>(d.x) * vr.x - static_cast<double>(d.y) * vr.y;
+        double ai = static_cast<double>(d.x) * vr.y + static_cast<double>(d.y) * vr.x;
+        for (uint32_t q0 = 0; q0 < nslab; q0 += kSlabUnroll) {
+            uint32_t w[kSlabUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kSlabUnroll; ++u)
+                w[u] = q0 + u < nslab ? slab[static_cast<uint64_t>(q0 + u) * dim + r] : kSlabEmpty;
+            VT x[kSlabUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kSlabUnroll; ++u) x[u] = v[w[u] != kSlabEmpty ? (w[u] & kColMask) : r];
+#pragma unroll
+            for (uint32_t u = 0; u < kSlabUnroll; ++u) {
+                const bool on = w[u] != kSlabEmpty;  // an empty slot adds an exact zero
+                const VT c = tab[on ? (w[u] >> kColBits) : 0u];
+                const double cr = on ? static_cast<double>(c.x) : 0.0, ci = on ? static_cast<double>(c.y) : 0.0;
+                const double sr = static_cast<double>(x[u].x), si = static_cast<double>(x[u].y);
+                ar = fma(cr, sr, ar);
+                ar = fma(-ci, si, ar);
+                ai = fma(cr, si, ai);
+                ai = fma(ci, sr, ai);
+            }
+        }
+        y[r] = Amp<VT>::make(ar, ai);
+    }
+}
+
 }  // namespace
 
 // Host rank authority of a general charge sector.
@@ -2004,6 +2102,13 @@ struct tcp_csr_plan {
     bool compact = false;
     uint32_t ntable = 0;
     tcp::DeviceBuffer packed, table64, table32, diag64, diag32;
+    // slab form (used by the SpMV when `slab`; built on the compact form,
+    // whose packed entries it replaces): word [q * dim + row] holds row's
+    // entry of off-diagonal x-group slot q, packed as in the compact form, or
+    // kSlabEmpty when the row has no transition in that group
+    bool slab = false;
+    uint32_t nslab = 0;
+    tcp::DeviceBuffer slab_words;
     uint64_t retained_bytes = 0;
     std::mutex staging_mutex;
     tcp::DeviceBuffer stage_in, stage_out;
@@ -2097,12 +2202,92 @@ int compact_csr(tcp_csr_plan *plan, uint64_t max_bytes) {
     return TCP_OK;
 }
 
+// Builds the slab form of a compact plan when it pays: the slots (distinct
+// off-diagonal x-masks that hold entries) times the rows may exceed the
+// off-diagonal entries by at most kSlabMaxPad (the empty words are read too),
+// and the budget and device memory must allow it.  `states(bits)` writes the
+// bitstring of every row; `xs` lists the operator's x-masks.  Otherwise, or
+// when an entry's x-mask is not among them, the plan stays compact (not an
+// error).  On success the packed entries are dropped (the slab carries them).
+constexpr double kSlabMaxPad = 8.0 / 3.0;  // slab words per off-diagonal entry (fill >= 3/8)
+
+template <class States>
+int slab_csr(tcp_csr_plan *plan, States states, std::vector<uint64_t> xs, uint64_t max_bytes) {
+    if (!plan->compact || plan->dim == 0) return TCP_OK;
+    std::sort(xs.begin(), xs.end());
+    xs.erase(std::unique(xs.begin(), xs.end()), xs.end());
+    xs.erase(std::remove(xs.begin(), xs.end(), 0ull), xs.end());
+    if (xs.empty() || xs.size() > (1u << 20)) return TCP_OK;
+    const uint64_t dim = plan->dim;
+    const uint32_t nx = static_cast<uint32_t>(xs.size());
+    tcp::DeviceBuffer bits, dxs, used, slot, flags;
+    auto quit = [&]() {  // the slab is optional: a failed allocation is not this call's error
+        (void)hipGetLastError();
+        return TCP_OK;
+    };
+    if (tcp::exceeds_budget(plan->retained_bytes + 8 * dim + 12 * static_cast<uint64_t>(nx), max_bytes)) return TCP_OK;
+    if (bits.allocate(8 * dim) || dxs.upload(xs.data(), 8 * xs.size()) || used.allocate(4 * nx) ||
+        slot.allocate(4 * nx) || flags.allocate(16))
+        return quit();
+    int st;
+    if ((st = states(static_cast<uint64_t *>(bits.ptr)))) return st;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((dim + 255) / 256, 256 * 16));
+    hipLaunchKernelGGL(fill_words_kernel, dim3(std::max<uint32_t>(1, std::min<uint32_t>((nx + 255) / 256, 1024))),
+                       dim3(256), 0, nullptr, static_cast<uint32_t *>(used.ptr), uint64_t(nx), 0u);
+    hipLaunchKernelGGL(fill_words_kernel, dim3(1), dim3(64), 0, nullptr, static_cast<uint32_t *>(flags.ptr),
+                       uint64_t(4), 0u);
+    auto *offdiag = static_cast<unsigned long long *>(flags.ptr);
+    auto *bad = static_cast<uint32_t *>(flags.ptr) + 2;
+    const auto *ip = static_cast<const uint64_t *>(plan->indptr.ptr);
+    const auto *pk = static_cast<const uint32_t *>(plan->packed.ptr);
+    hipLaunchKernelGGL(slab_scatter_kernel, dim3(blocks), dim3(256), 0, nullptr, ip, pk,
+                       static_cast<const uint64_t *>(bits.ptr), dim, static_cast<const uint64_t *>(dxs.ptr), nx,
+                       static_cast<uint32_t *>(used.ptr), nullptr, nullptr, offdiag, bad);
+    TCP_HIP_CHECK(hipGetLastError());
+    uint32_t hflags[4];
+    std::vector<uint32_t> hused(nx);
+    TCP_HIP_CHECK(hipMemcpy(hflags, flags.ptr, 16, hipMemcpyDeviceToHost));
+    TCP_HIP_CHECK(hipMemcpy(hused.data(), used.ptr, 4 * nx, hipMemcpyDeviceToHost));
+    if (hflags[2]) return TCP_OK;  // an entry outside the listed x-masks: stay compact
+    const uint64_t noff = static_cast<uint64_t>(hflags[0]) | (static_cast<uint64_t>(hflags[1]) << 32);
+    std::vector<uint32_t> hslot(nx, 0);
+    uint32_t nslab = 0;
+    for (uint32_t g = 0; g < nx; ++g)
+        if (hused[g]) hslot[g] = nslab++;
+    if (nslab == 0 || static_cast<double>(nslab) * static_cast<double>(dim) > kSlabMaxPad * static_cast<double>(noff))
+        return TCP_OK;
+    const uint64_t words = static_cast<uint64_t>(nslab) * dim;
+    const uint64_t retained = plan->retained_bytes - 4 * plan->nnz + 4 * words;
+    if (tcp::exceeds_budget(plan->retained_bytes + 4 * words, max_bytes)) return TCP_OK;
+    if (slot.upload(hslot.data(), 4 * nx) || plan->slab_words.allocate(4 * words)) {
+        plan->slab_words.release();
+        return quit();
+    }
+    hipLaunchKernelGGL(fill_words_kernel, dim3(static_cast<uint32_t>(std::min<uint64_t>((words + 255) / 256, 256 * 64))),
+                       dim3(256), 0, nullptr, static_cast<uint32_t *>(plan->slab_words.ptr), words, kSlabEmpty);
+    hipLaunchKernelGGL(slab_scatter_kernel, dim3(blocks), dim3(256), 0, nullptr, ip, pk,
+                       static_cast<const uint64_t *>(bits.ptr), dim, static_cast<const uint64_t *>(dxs.ptr), nx,
+                       static_cast<uint32_t *>(used.ptr), static_cast<const uint32_t *>(slot.ptr),
+                       static_cast<uint32_t *>(plan->slab_words.ptr), offdiag, bad);
+    TCP_HIP_CHECK(hipGetLastError());
+    TCP_HIP_CHECK(hipDeviceSynchronize());
+    plan->slab = true;
+    plan->nslab = nslab;
+    plan->packed.release();
+    plan->retained_bytes = retained;
+    return TCP_OK;
+}
+
 // Two-pass CSR assembly: `count` writes the entries of every row, the host
 // scans them into indptr, `fill` writes each row's columns (ascending) and
 // values in FP64 and FP32.
-template <class Count, class Fill>
+struct NoSlab {
+    int operator()(tcp_csr_plan *) const { return TCP_OK; }
+};
+
+template <class Count, class Fill, class Slab = NoSlab>
 int assemble_csr(uint32_t n, uint64_t dim, int device, uint64_t max_bytes, Count count, Fill fill,
-                 tcp_csr_plan **out) {
+                 tcp_csr_plan **out, Slab slab = Slab()) {
     auto plan = new tcp_csr_plan();
     std::unique_ptr<tcp_csr_plan> guard(plan);
     plan->n = n;
@@ -2134,6 +2319,7 @@ int assemble_csr(uint32_t n, uint64_t dim, int device, uint64_t max_bytes, Count
         return st;
     TCP_HIP_CHECK(hipDeviceSynchronize());
     if ((st = compact_csr(plan, max_bytes))) return st;
+    if ((st = slab(plan))) return st;
     *out = guard.release();
     return TCP_OK;
 }
@@ -2178,7 +2364,17 @@ int build_csr(const tcp_sector_plan *sp, uint64_t max_bytes, tcp_csr_plan **out)
         return sp->cond ? fill_with(std::true_type(), indptr, cols, v64, v32)
                         : fill_with(std::false_type(), indptr, cols, v64, v32);
     };
-    return assemble_csr(sp->n, dim, sp->device, max_bytes, count, fill, out);
+    // the slab form's rows and x-masks: the plan's walk and its x-groups
+    auto slab = [&](tcp_csr_plan *plan) -> int {
+        if (sp->wide) return TCP_OK;
+        const tcp::DeviceBuffer &gxb = sp->general ? sp->general->gx : sp->gx;
+        const uint64_t ng = sp->general ? sp->general->G : sp->G;
+        std::vector<uint64_t> xs(ng);
+        if (ng) TCP_HIP_CHECK(hipMemcpy(xs.data(), gxb.ptr, 8 * ng, hipMemcpyDeviceToHost));
+        auto states = [&](uint64_t *bits) -> int { return tcp_sector_plan_walk_device(sp, 0, bits, dim, 64, nullptr); };
+        return slab_csr(plan, states, std::move(xs), max_bytes);
+    };
+    return assemble_csr(sp->n, dim, sp->device, max_bytes, count, fill, out, slab);
 }
 
 int csr_apply_device(const tcp_csr_plan *plan, const void *in, void *out, int dtype, void *stream) {
@@ -2198,7 +2394,20 @@ int csr_apply_device(const tcp_csr_plan *plan, const void *in, void *out, int dt
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint64_t *ip = static_cast<const uint64_t *>(plan->indptr.ptr);
     const uint32_t *cp = static_cast<const uint32_t *>(plan->cols.ptr);
-    if (plan->compact) {
+    if (plan->slab) {
+        const uint32_t sblocks = static_cast<uint32_t>((plan->dim + 255) / 256);
+        const auto *sw = static_cast<const uint32_t *>(plan->slab_words.ptr);
+        if (dtype == TCP_C128)
+            hipLaunchKernelGGL(slab_spmv_kernel<dvec2>, dim3(sblocks), dim3(256), 0, s, sw, plan->nslab,
+                               static_cast<const dvec2 *>(plan->table64.ptr), plan->ntable,
+                               static_cast<const dvec2 *>(plan->diag64.ptr), static_cast<const dvec2 *>(in),
+                               static_cast<dvec2 *>(out), plan->dim);
+        else
+            hipLaunchKernelGGL(slab_spmv_kernel<fvec2>, dim3(sblocks), dim3(256), 0, s, sw, plan->nslab,
+                               static_cast<const fvec2 *>(plan->table32.ptr), plan->ntable,
+                               static_cast<const fvec2 *>(plan->diag32.ptr), static_cast<const fvec2 *>(in),
+                               static_cast<fvec2 *>(out), plan->dim);
+    } else if (plan->compact) {
         const auto *pk = static_cast<const uint32_t *>(plan->packed.ptr);
         if (dtype == TCP_C128)
             hipLaunchKernelGGL((csr_compact_spmv_kernel<dvec2, kCsrLanes>), dim3(blocks), dim3(256), 0, s, ip, pk,
@@ -2361,6 +2570,10 @@ uint64_t tcp_csr_plan_dimension(const tcp_csr_plan *plan) { return plan ? plan->
 uint64_t tcp_csr_plan_transition_count(const tcp_csr_plan *plan) { return plan ? plan->nnz : 0; }
 uint64_t tcp_csr_plan_estimated_bytes(const tcp_csr_plan *plan) { return plan ? plan->retained_bytes : 0; }
 int tcp_csr_plan_is_compact(const tcp_csr_plan *plan) { return plan && plan->compact ? 1 : 0; }
+int tcp_csr_plan_storage_form(const tcp_csr_plan *plan) {
+    return !plan ? -1 : plan->slab ? TCP_CSR_SLAB : plan->compact ? TCP_CSR_COMPACT : TCP_CSR_PLAIN;
+}
+uint32_t tcp_csr_plan_slab_groups(const tcp_csr_plan *plan) { return plan && plan->slab ? plan->nslab : 0; }
 
 int tcp_csr_plan_apply_into(const tcp_csr_plan *plan, const void *input, void *output, uint64_t input_len,
                             uint64_t output_len, int dtype, void *stream) {

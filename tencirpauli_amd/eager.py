@@ -88,9 +88,12 @@ class CsrMvpPlan(NativePlanFacade):
         self.transition_count = int(L.tcp_csr_plan_transition_count(ptr))
         self.device_bytes = int(L.tcp_csr_plan_estimated_bytes(ptr))
         # "compact": 24-bit column + 8-bit index into a table of the distinct
-        # off-diagonal values per entry, diagonal in a dense array; "plain": a
-        # value per entry.  The SpMV reads whichever form the plan holds.
-        self.storage_format = "compact" if L.tcp_csr_plan_is_compact(ptr) else "plain"
+        # off-diagonal values per entry, diagonal in a dense array; "slab": the
+        # compact entries regrouped by x-mask, one word per (x-group, row);
+        # "plain": a value per entry.  The SpMV reads whichever form the plan
+        # holds.
+        self.storage_format = {0: "plain", 1: "compact", 2: "slab"}[L.tcp_csr_plan_storage_form(ptr)]
+        self.slab_groups = int(L.tcp_csr_plan_slab_groups(ptr))  # x-group slots of the slab form (else 0)
         self.native_variant = native_variant
         self.execution_family = execution_family
         self.term_count = term_count

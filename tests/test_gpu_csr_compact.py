@@ -1,13 +1,17 @@
-"""GPU checks of the compact CSR form against the Kronecker-product oracle.
+"""GPU checks of the compact and slab CSR forms against the Kronecker-product
+oracle.
 
 An eager plan whose matrix has fewer than 2^24 rows and at most 255 distinct
 off-diagonal values stores each entry as a 24-bit column plus an 8-bit index
 into a value table, with the diagonal in a dense array; otherwise it keeps a
-value per entry.  Both forms must give the oracle's product (FP64 within
-1e-12 relative, complex64 within FP32 rounding) and export the same CSR.
-The cases pin the table boundary (255 distinct values compact, 256 plain),
-rows without a diagonal entry, complex table values, and the choice made for
-the U(1) lattice sectors.
+value per entry.  A restricted plan whose x-groups (distinct off-diagonal
+x-masks holding entries) times rows are at most 8/3 of its off-diagonal
+entries regroups the compact entries by x-mask (the slab form).  Every form
+must give the oracle's product (FP64 within 1e-12 relative, complex64 within
+FP32 rounding) and export the same CSR.  The cases pin the table boundary
+(255 distinct values compact, 256 plain), rows without a diagonal entry,
+complex table values, the slab choice computed independently from the
+exported CSR and the sector basis, and sectors on either side of it.
 """
 
 from __future__ import annotations
@@ -18,7 +22,7 @@ import pytest
 import tencirpauli_amd as tcp
 from oracle import dense
 from tencirpauli_amd import lattice
-from tests.helpers import rel_err
+from tests.helpers import interleave_spins, rel_err, spin_levels
 
 pytestmark = pytest.mark.gpu
 
@@ -66,12 +70,26 @@ def test_table_boundary_and_rows_without_diagonal(distinct, form):
     np.testing.assert_allclose(csr.to_scipy().toarray(), full, atol=1e-13)
 
 
-def test_u1_lattice_sector_is_compact_and_matches_lazy_and_oracle():
+def _expected_form(plan, basis):
+    """The form the selection rule gives, from the exported CSR and the
+    sector basis (integers in rank order)."""
+    indptr, cols, vals = plan.csr_arrays()
+    rows = np.repeat(np.arange(plan.dimension), np.diff(indptr).astype(np.int64))
+    x = np.asarray(basis, dtype=np.int64)[rows] ^ np.asarray(basis, dtype=np.int64)[cols.astype(np.int64)]
+    off = x != 0
+    if len(np.unique(vals[off])) > 255 or plan.dimension > (1 << 24) or len(cols) == 0:
+        return "plain"
+    slots = len(np.unique(x[off]))
+    return "slab" if 0 < slots and 3 * slots * plan.dimension <= 8 * int(off.sum()) else "compact"
+
+
+def test_u1_lattice_sector_is_slab_and_matches_lazy_and_oracle():
     codes, coef = lattice.heisenberg_periodic(3, 4, seed=5)
     op = tcp.PauliOperator.from_code_arrays(codes, coef)
     sector = tcp.U1Sector(12, 6)
     plan = op.restrict_charge(sector).mvp_plan(storage="eager")
-    assert plan.storage_format == "compact"
+    basis = dense.blocks_basis([(12, 6)])
+    assert plan.storage_format == _expected_form(plan, basis) == "slab"
     basis = np.asarray(dense.blocks_basis([(12, 6)]), dtype=np.int64)
     full = dense.dense_operator(12, codes, coef)[np.ix_(basis, basis)]
     _check(plan, full, 11)
@@ -133,8 +151,56 @@ def test_values_that_differ_only_in_the_sign_of_an_imaginary_part():
     op = tcp.PauliOperator.from_code_arrays(rows, vals)
     sector = tcp.U1Sector(9, 4)
     plan = op.restrict_charge(sector).mvp_plan(storage="eager")
-    assert plan.storage_format == "compact"
     basis = np.asarray(dense.blocks_basis([(9, 4)]), dtype=np.int64)
+    assert plan.storage_format == _expected_form(plan, basis) == "slab"
     full = dense.dense_operator(9, rows, vals)[np.ix_(basis, basis)]
     assert np.any(np.abs(full.imag) > 0.1)
     _check(plan, full, 12)
+
+
+def test_sparse_groups_stay_compact():
+    # all-to-all hopping of one particle on 10 sites: 45 x-groups, each row
+    # holds 9 transitions, so the slab would be 45 / 9 = 5 words per entry
+    n = 10
+    terms = []
+    for a in range(n):
+        for b in range(a + 1, n):
+            for code in (1, 2):
+                w = [0] * n
+                w[a] = w[b] = code
+                terms.append((tuple(w), 0.5))
+    terms.append(((3,) + (0,) * (n - 1), 0.2))
+    op = tcp.PauliOperator.from_terms(n, terms)
+    sector = tcp.U1Sector(n, 1)
+    plan = op.restrict_charge(sector).mvp_plan(storage="eager")
+    basis = np.asarray(dense.blocks_basis([(n, 1)]), dtype=np.int64)
+    assert plan.storage_format == _expected_form(plan, basis) == "compact"
+    full = dense.dense_operator(n, *op.code_arrays())[np.ix_(basis, basis)]
+    _check(plan, full, 21)
+
+
+@pytest.mark.parametrize("filling", [(1, 1), (2, 2), (3, 2)])
+def test_block_and_general_sector_forms_match_oracle(filling):
+    # a 2x3 Fermi-Hubbard (Jordan-Wigner): N_up, N_down on spin-major modes
+    # (block sector), then the same operator on interleaved spin orbitals with
+    # N and 2 S_z (general sector); each eager form against the dense oracle
+    codes, coef = lattice.fermi_hubbard_jw(2, 3, 1.0, 4.0, seed=7)
+    op = tcp.PauliOperator.from_code_arrays(codes, coef)
+    blocks = [(6, filling[0]), (6, filling[1])]
+    sector = tcp.ChargeSector(((tcp.AdditiveCharge.number(12, range(6)), filling[0]),
+                               (tcp.AdditiveCharge.number(12, range(6, 12)), filling[1])))
+    plan = op.restrict_charge(sector, storage="eager").mvp_plan(storage="eager")
+    basis = np.asarray(dense.blocks_basis(blocks), dtype=np.int64)
+    assert plan.storage_format == _expected_form(plan, basis)
+    _check(plan, dense.dense_operator(12, codes, coef)[np.ix_(basis, basis)], 5)
+
+    icodes = interleave_spins(codes)
+    iop = tcp.PauliOperator.from_code_arrays(icodes, coef)
+    levels = spin_levels(12)
+    targets = (filling[0] + filling[1], filling[0] - filling[1])
+    charges = [(tcp.AdditiveCharge(12, {a: (int(levels[j, a, 0]), int(levels[j, a, 1])) for a in range(12)}),
+                targets[j]) for j in range(2)]
+    iplan = iop.restrict_charge(tcp.ChargeSector(charges), storage="eager").mvp_plan(storage="eager")
+    ibasis = np.asarray(dense.charge_basis(levels, targets), dtype=np.int64)
+    assert iplan.storage_format == _expected_form(iplan, ibasis)
+    _check(iplan, dense.dense_operator(12, icodes, coef)[np.ix_(ibasis, ibasis)], 6)
