@@ -956,13 +956,21 @@ __global__ __launch_bounds__(256) void csr_compact_spmv_kernel(const uint64_t *_
 
 // ---------------------------------------------------------------- slab form
 // The compact entries regrouped by x-group: slot q's words for consecutive
-// rows are consecutive in memory, and one thread per row visits the slots in
-// turn, so a wave reads 64 packed words of one group in one access and its
+// rows are consecutive in memory (within a chunk of kSlabRows rows), and one
+// thread per row visits the slots in turn, so a wave reads 64 packed words of
+// one group in one access and its
 // gathers, the sources of consecutive rows under one x-mask, fall in few
 // cache lines (the CSR's lanes gather unrelated columns).  Rows without a
 // transition in a group hold kSlabEmpty.
 constexpr uint32_t kSlabEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kSlabUnroll = 8;  // slots whose loads are in flight together
+// Rows per chunk (= the SpMV's workgroup): a chunk's words lie slot after
+// slot in one contiguous run, so a workgroup streams one block of memory
+// (slot-major over all rows put each slot 4 D bytes apart: 5-7 % slower).
+constexpr uint32_t kSlabRows = 256;
+__host__ __device__ __forceinline__ uint64_t slab_word(uint64_t row, uint32_t q, uint32_t nslab) {
+    return (row / kSlabRows) * nslab * kSlabRows + static_cast<uint64_t>(q) * kSlabRows + row % kSlabRows;
+}
 
 __global__ void fill_words_kernel(uint32_t *__restrict__ p, uint64_t count, uint32_t value) {
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < count;
@@ -990,7 +998,7 @@ __global__ __launch_bounds__(256) void slab_scatter_kernel(const uint64_t *__res
                                                            const uint64_t *__restrict__ states, uint64_t dim,
                                                            const uint64_t *__restrict__ xs, uint32_t nx,
                                                            uint32_t *__restrict__ used,
-                                                           const uint32_t *__restrict__ slot,
+                                                           const uint32_t *__restrict__ slot, uint32_t nslab,
                                                            uint32_t *__restrict__ slab,
                                                            unsigned long long *offdiag, uint32_t *bad) {
     unsigned long long local = 0;
@@ -1005,7 +1013,7 @@ __global__ __launch_bounds__(256) void slab_scatter_kernel(const uint64_t *__res
                 atomicExch(bad, 1u);
                 continue;
             }
-            if (slot) slab[static_cast<uint64_t>(slot[g]) * dim + r] = w;
+            if (slot) slab[slab_word(r, slot[g], nslab)] = w;
             else used[g] = 1u;
             ++local;
         }
@@ -1032,7 +1040,7 @@ __global__ __launch_bounds__(256) void slab_spmv_kernel(const uint32_t *__restri
             uint32_t w[kSlabUnroll];
 #pragma unroll
             for (uint32_t u = 0; u < kSlabUnroll; ++u)
-                w[u] = q0 + u < nslab ? slab[static_cast<uint64_t>(q0 + u) * dim + r] : kSlabEmpty;
+                w[u] = q0 + u < nslab ? slab[slab_word(r, q0 + u, nslab)] : kSlabEmpty;
             VT x[kSlabUnroll];
 #pragma unroll
             for (uint32_t u = 0; u < kSlabUnroll; ++u) x[u] = v[w[u] != kSlabEmpty ? (w[u] & kColMask) : r];
@@ -2103,9 +2111,10 @@ struct tcp_csr_plan {
     uint32_t ntable = 0;
     tcp::DeviceBuffer packed, table64, table32, diag64, diag32;
     // slab form (used by the SpMV when `slab`; built on the compact form,
-    // whose packed entries it replaces): word [q * dim + row] holds row's
-    // entry of off-diagonal x-group slot q, packed as in the compact form, or
-    // kSlabEmpty when the row has no transition in that group
+    // whose packed entries it replaces): rows in chunks of kSlabRows, word
+    // slab_word(row, q, nslab) holds row's entry of off-diagonal x-group slot
+    // q, packed as in the compact form, or kSlabEmpty when the row has no
+    // transition in that group
     bool slab = false;
     uint32_t nslab = 0;
     tcp::DeviceBuffer slab_words;
@@ -2242,7 +2251,7 @@ int slab_csr(tcp_csr_plan *plan, States states, std::vector<uint64_t> xs, uint64
     const auto *pk = static_cast<const uint32_t *>(plan->packed.ptr);
     hipLaunchKernelGGL(slab_scatter_kernel, dim3(blocks), dim3(256), 0, nullptr, ip, pk,
                        static_cast<const uint64_t *>(bits.ptr), dim, static_cast<const uint64_t *>(dxs.ptr), nx,
-                       static_cast<uint32_t *>(used.ptr), nullptr, nullptr, offdiag, bad);
+                       static_cast<uint32_t *>(used.ptr), nullptr, 0u, nullptr, offdiag, bad);
     TCP_HIP_CHECK(hipGetLastError());
     uint32_t hflags[4];
     std::vector<uint32_t> hused(nx);
@@ -2256,7 +2265,7 @@ int slab_csr(tcp_csr_plan *plan, States states, std::vector<uint64_t> xs, uint64
         if (hused[g]) hslot[g] = nslab++;
     if (nslab == 0 || static_cast<double>(nslab) * static_cast<double>(dim) > kSlabMaxPad * static_cast<double>(noff))
         return TCP_OK;
-    const uint64_t words = static_cast<uint64_t>(nslab) * dim;
+    const uint64_t words = static_cast<uint64_t>(nslab) * ((dim + kSlabRows - 1) / kSlabRows) * kSlabRows;
     const uint64_t retained = plan->retained_bytes - 4 * plan->nnz + 4 * words;
     if (tcp::exceeds_budget(plan->retained_bytes + 4 * words, max_bytes)) return TCP_OK;
     if (slot.upload(hslot.data(), 4 * nx) || plan->slab_words.allocate(4 * words)) {
@@ -2267,7 +2276,7 @@ int slab_csr(tcp_csr_plan *plan, States states, std::vector<uint64_t> xs, uint64
                        dim3(256), 0, nullptr, static_cast<uint32_t *>(plan->slab_words.ptr), words, kSlabEmpty);
     hipLaunchKernelGGL(slab_scatter_kernel, dim3(blocks), dim3(256), 0, nullptr, ip, pk,
                        static_cast<const uint64_t *>(bits.ptr), dim, static_cast<const uint64_t *>(dxs.ptr), nx,
-                       static_cast<uint32_t *>(used.ptr), static_cast<const uint32_t *>(slot.ptr),
+                       static_cast<uint32_t *>(used.ptr), static_cast<const uint32_t *>(slot.ptr), nslab,
                        static_cast<uint32_t *>(plan->slab_words.ptr), offdiag, bad);
     TCP_HIP_CHECK(hipGetLastError());
     TCP_HIP_CHECK(hipDeviceSynchronize());
@@ -2395,7 +2404,7 @@ int csr_apply_device(const tcp_csr_plan *plan, const void *in, void *out, int dt
     const uint64_t *ip = static_cast<const uint64_t *>(plan->indptr.ptr);
     const uint32_t *cp = static_cast<const uint32_t *>(plan->cols.ptr);
     if (plan->slab) {
-        const uint32_t sblocks = static_cast<uint32_t>((plan->dim + 255) / 256);
+        const uint32_t sblocks = static_cast<uint32_t>((plan->dim + kSlabRows - 1) / kSlabRows);
         const auto *sw = static_cast<const uint32_t *>(plan->slab_words.ptr);
         if (dtype == TCP_C128)
             hipLaunchKernelGGL(slab_spmv_kernel<dvec2>, dim3(sblocks), dim3(256), 0, s, sw, plan->nslab,
