@@ -6,6 +6,7 @@
 // raised when its serial loop would raise it (core/charge/generic.rs:949-1045).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <unordered_map>
 
 #include "charge_sector.hpp"
@@ -38,6 +39,7 @@ struct ChargeArgs {
     const void *tw;          // [T] (re, im)
     const uint64_t *tc1;     // [T] fermion words: modes that must be occupied in the source
     const uint64_t *tc0;     // [T] fermion words: modes that must be empty in the source
+    const int32_t *enc32;    // [n] the encoded weights as int32 (narrow sectors), or null
 };
 
 __device__ __forceinline__ double flip_sign(double x, uint32_t parity) {
@@ -95,6 +97,36 @@ __host__ __device__ __forceinline__ uint64_t gen_unrank(uint32_t n, uint64_t box
     uint64_t bits = 0;
     for (uint32_t a = 0; a < n; ++a) {
         const uint64_t c0 = count[static_cast<uint64_t>(a + 1) * box + static_cast<uint64_t>(idx)];
+        if (r >= c0) {
+            r -= c0;
+            bits |= 1ull << (n - 1 - a);
+            idx -= enc[a];
+        }
+    }
+    return bits;
+}
+
+// gen_rank / gen_unrank in 32-bit arithmetic (every encoded value fits in
+// int32; ranks are below 2^32): the walk of the restricted gather, with the
+// count table and the weights in LDS.
+__device__ __forceinline__ uint32_t gen_rank32(uint32_t n, uint32_t box, int32_t idx, int32_t enc_zero,
+                                               const uint32_t *count, const int32_t *enc, uint64_t s) {
+    uint32_t rank = 0;
+    while (s) {
+        const uint32_t bit = 63u - static_cast<uint32_t>(__builtin_clzll(s));
+        const uint32_t a = n - 1 - bit;
+        rank += count[(a + 1) * box + static_cast<uint32_t>(idx)];
+        idx -= enc[a];
+        s &= ~(1ull << bit);
+    }
+    return idx == enc_zero ? rank : UINT32_MAX;
+}
+
+__device__ __forceinline__ uint64_t gen_unrank32(uint32_t n, uint32_t box, int32_t idx, const uint32_t *count,
+                                                 const int32_t *enc, uint32_t r) {
+    uint64_t bits = 0;
+    for (uint32_t a = 0; a < n; ++a) {
+        const uint32_t c0 = count[(a + 1) * box + static_cast<uint32_t>(idx)];
         if (r >= c0) {
             r -= c0;
             bits |= 1ull << (n - 1 - a);
@@ -165,12 +197,19 @@ struct Amp<fvec2> {
     __device__ static fvec2 make(double r, double i) { return fvec2{static_cast<float>(r), static_cast<float>(i)}; }
 };
 
-// The count table from global memory, or its LDS copy when LDS is set.
+// The count table from global memory, or its LDS copy when LDS is set; with
+// narrow weights, their int32 copy (in LDS after the table when LDS is set).
 template <bool LDS>
-__device__ __forceinline__ const uint32_t *stage_counts(const ChargeArgs &p, uint32_t *lds) {
+__device__ __forceinline__ const uint32_t *stage_counts(const ChargeArgs &p, uint32_t *lds, const int32_t *&enc32) {
+    enc32 = p.enc32;
     if (!LDS) return p.count;
     const uint64_t words = (p.n + 1) * p.box;
     for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) lds[i] = p.count[i];
+    if (p.enc32) {
+        int32_t *le = reinterpret_cast<int32_t *>(lds + words);
+        for (uint32_t i = threadIdx.x; i < p.n; i += blockDim.x) le[i] = p.enc32[i];
+        enc32 = le;
+    }
     __syncthreads();
     return lds;
 }
@@ -178,14 +217,18 @@ __device__ __forceinline__ const uint32_t *stage_counts(const ChargeArgs &p, uin
 // One destination rank per thread: unrank, then for every group the source
 // s = b ^ x, its rank (UINT64_MAX outside the sector), the group's value at s
 // and the gather.  FP64 accumulation; the output is written once.
-template <typename VT, bool COND, bool LDS>
+template <typename VT, bool COND, bool LDS, bool N32>
 __global__ __launch_bounds__(kChargeThreads) void charge_gather_kernel(const VT *__restrict__ v, VT *__restrict__ y,
                                                                        ChargeArgs p) {
     extern __shared__ uint32_t lds_count[];
-    const uint32_t *count = stage_counts<LDS>(p, lds_count);
+    const int32_t *enc32 = nullptr;
+    const uint32_t *count = stage_counts<LDS>(p, lds_count, enc32);
+    const uint32_t box32 = static_cast<uint32_t>(p.box);
+    const int32_t et32 = static_cast<int32_t>(p.enc_target), ez32 = static_cast<int32_t>(p.enc_zero);
     for (uint64_t r = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; r < p.dim;
          r += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        const uint64_t b = gen_unrank(p.n, p.box, p.enc_target, count, p.enc, r);
+        const uint64_t b = N32 ? gen_unrank32(p.n, box32, et32, count, enc32, static_cast<uint32_t>(r))
+                               : gen_unrank(p.n, p.box, p.enc_target, count, p.enc, r);
         double ar = 0.0, ai = 0.0;
         for (uint32_t g = 0; g < p.G; ++g) {
             const uint64_t x = p.gx[g];
@@ -193,7 +236,15 @@ __global__ __launch_bounds__(kChargeThreads) void charge_gather_kernel(const VT 
             const uint64_t s = b ^ x;
             const dvec2 w = group_value<COND>(p, g, s);
             if (w.x == 0.0 && w.y == 0.0) continue;
-            const uint64_t rs = x == 0 ? r : gen_rank(p.n, p.box, p.enc_target, p.enc_zero, count, p.enc, s);
+            uint64_t rs = r;
+            if (x != 0) {
+                if (N32) {
+                    const uint32_t r32 = gen_rank32(p.n, box32, et32, ez32, count, enc32, s);
+                    rs = r32 == UINT32_MAX ? UINT64_MAX : r32;
+                } else {
+                    rs = gen_rank(p.n, p.box, p.enc_target, p.enc_zero, count, p.enc, s);
+                }
+            }
             if (rs >= p.dim) continue;  // unreachable for an in-sector s; keeps the gather in bounds
             const VT sv = v[rs];
             const double sr = Amp<VT>::re(sv), si = Amp<VT>::im(sv);
@@ -332,6 +383,7 @@ ChargeArgs args_of(const tcp::ChargeSector &c, int dtype) {
     a.tw = dtype == TCP_C64 ? c.tw32.ptr : c.tw64.ptr;
     a.tc1 = c.cond ? static_cast<const uint64_t *>(c.tc1.ptr) : nullptr;
     a.tc0 = c.cond ? static_cast<const uint64_t *>(c.tc0.ptr) : nullptr;
+    a.enc32 = c.narrow ? static_cast<const int32_t *>(c.enc32.ptr) : nullptr;
     return a;
 }
 
@@ -562,6 +614,18 @@ int charge_build(const Operator *op, const std::vector<CondTerm> *fermion_terms,
     if (out.cond && ((st = out.tc1.upload(tc1.data(), tc1.size() * 8)) ||
                      (st = out.tc0.upload(tc0.data(), tc0.size() * 8))))
         return st;
+    // 32-bit walk: every encoded weight, the encoded target and zero, and the
+    // table's row offsets fit in 32 bits
+    const int64_t lim = int64_t(1) << 30;
+    bool narrow = std::llabs(geo.enc_target) < lim && std::llabs(geo.enc_zero) < lim && geo.box < (1ull << 31) &&
+                  (static_cast<uint64_t>(n) + 1) * geo.box < (1ull << 32);
+    std::vector<int32_t> e32(std::max<size_t>(1, geo.enc.size()), 0);
+    for (size_t a = 0; a < geo.enc.size(); ++a) {
+        narrow = narrow && std::llabs(geo.enc[a]) < lim;
+        e32[a] = static_cast<int32_t>(geo.enc[a]);
+    }
+    if (narrow && (st = out.enc32.upload(e32.data(), 4 * e32.size()))) return st;
+    out.narrow = narrow;
     // leakage scan (state-independent, once per plan)
     if (out.G == 0 || geo.dim == 0) return TCP_OK;
     unsigned long long *dfirst = nullptr;
@@ -592,19 +656,25 @@ int charge_build(const Operator *op, const std::vector<CondTerm> *fermion_terms,
 int charge_apply(const ChargeSector &c, const void *in, void *out, int dtype, hipStream_t stream) {
     if (c.geo.dim == 0) return TCP_OK;
     const ChargeArgs a = args_of(c, dtype);
-    const uint64_t table = 4ull * (c.geo.n + 1) * c.geo.box;
+    const uint64_t table = 4ull * (c.geo.n + 1) * c.geo.box + (c.narrow ? 4ull * c.geo.n : 0);
     const bool lds = table <= kChargeLdsLimit;
+    const uint32_t lds_bytes = static_cast<uint32_t>(table);
     const uint32_t grid = grid_for(c.geo.dim);
     auto launch = [&](auto vt, auto cond) {
         using VT = typename decltype(vt)::type;
         constexpr bool COND = decltype(cond)::value;
-        if (lds)
-            hipLaunchKernelGGL((charge_gather_kernel<VT, COND, true>), dim3(grid), dim3(kChargeThreads),
-                               static_cast<uint32_t>(table), stream, static_cast<const VT *>(in),
-                               static_cast<VT *>(out), a);
+        if (lds && c.narrow)
+            hipLaunchKernelGGL((charge_gather_kernel<VT, COND, true, true>), dim3(grid), dim3(kChargeThreads),
+                               lds_bytes, stream, static_cast<const VT *>(in), static_cast<VT *>(out), a);
+        else if (lds)
+            hipLaunchKernelGGL((charge_gather_kernel<VT, COND, true, false>), dim3(grid), dim3(kChargeThreads),
+                               lds_bytes, stream, static_cast<const VT *>(in), static_cast<VT *>(out), a);
+        else if (c.narrow)
+            hipLaunchKernelGGL((charge_gather_kernel<VT, COND, false, true>), dim3(grid), dim3(kChargeThreads), 0,
+                               stream, static_cast<const VT *>(in), static_cast<VT *>(out), a);
         else
-            hipLaunchKernelGGL((charge_gather_kernel<VT, COND, false>), dim3(grid), dim3(kChargeThreads), 0, stream,
-                               static_cast<const VT *>(in), static_cast<VT *>(out), a);
+            hipLaunchKernelGGL((charge_gather_kernel<VT, COND, false, false>), dim3(grid), dim3(kChargeThreads), 0,
+                               stream, static_cast<const VT *>(in), static_cast<VT *>(out), a);
     };
     struct D {
         using type = dvec2;

@@ -70,6 +70,10 @@ struct ChargeSector {
     bool leaks = false;
     uint64_t leak_source = 0;
     DeviceBuffer gx, goff, tz, tw64, tw32, tc1, tc0, count, enc, wt;
+    // the encoded weights as int32 when every encoded value fits (the gather
+    // then walks in 32-bit arithmetic with the weights in LDS)
+    bool narrow = false;
+    DeviceBuffer enc32;
     uint64_t retained_bytes = 0;
 };
 
