@@ -427,7 +427,9 @@ struct Solver {
             (st = ybuf.allocate(sizeof(dvec2) * kMaxBasis * kMaxKeep)))
             return st;
         TCP_HIP_CHECK(hipMemsetAsync(sc.ptr, 0, sizeof(dvec2) * kScCount, stream));
-        if ((st = tcp::plan_diagonal(plan, static_cast<double *>(diag.ptr), stream))) return st;
+        // the Jacobi correction's diagonal (a learned preconditioner brings its own multipliers)
+        if (!opt.log_multiplier_fn && (st = tcp::plan_diagonal(plan, static_cast<double *>(diag.ptr), stream)))
+            return st;
         for (uint32_t k = slots; k-- > 0;) {
             vfree.push_back(k);
             wfree.push_back(k);

@@ -2440,6 +2440,13 @@ int csr_apply_device(const tcp_csr_plan *plan, const void *in, void *out, int dt
     return e == hipSuccess ? tcp::ok() : tcp::hip_fail(e, "csr kernel launch");
 }
 
+// out[r] = Re x[r] (the dense diagonal of the compact and slab forms).
+__global__ void real_part_kernel(const dvec2 *__restrict__ x, uint64_t dim, double *__restrict__ out) {
+    for (uint64_t r = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; r < dim;
+         r += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        out[r] = x[r].x;
+}
+
 // Real part of the stored diagonal entry of every row (0 where the row has none).
 __global__ void csr_diagonal_kernel(const uint64_t *__restrict__ indptr, const uint32_t *__restrict__ cols,
                                     const dvec2 *__restrict__ vals, uint64_t dim, double *__restrict__ diag) {
@@ -2478,9 +2485,14 @@ int plan_apply(const tcp_csr_plan *plan, const void *in, void *out, int dtype, h
 int plan_diagonal(const tcp_csr_plan *plan, double *diag, hipStream_t stream) {
     if (plan->dim == 0) return TCP_OK;
     const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((plan->dim + 255) / 256, 4096));
-    hipLaunchKernelGGL(csr_diagonal_kernel, dim3(blocks), dim3(256), 0, stream,
-                       static_cast<const uint64_t *>(plan->indptr.ptr), static_cast<const uint32_t *>(plan->cols.ptr),
-                       static_cast<const dvec2 *>(plan->vals64.ptr), plan->dim, diag);
+    if (plan->compact)  // the compact and slab forms hold the diagonal densely
+        hipLaunchKernelGGL(real_part_kernel, dim3(blocks), dim3(256), 0, stream,
+                           static_cast<const dvec2 *>(plan->diag64.ptr), plan->dim, diag);
+    else
+        hipLaunchKernelGGL(csr_diagonal_kernel, dim3(blocks), dim3(256), 0, stream,
+                           static_cast<const uint64_t *>(plan->indptr.ptr),
+                           static_cast<const uint32_t *>(plan->cols.ptr), static_cast<const dvec2 *>(plan->vals64.ptr),
+                           plan->dim, diag);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TCP_OK : hip_fail(e, "csr diagonal kernel");
 }
