@@ -637,7 +637,7 @@ int tcp_learned_precond_create(const tcp_mlp *mlp, const tcp_sector_plan *plan,
 const void *tcp_learned_precond_log_multiplier(void *ctx, double theta, void *stream);
 /* The callback re-evaluates the network only when theta has moved by more
  * than threshold * scale (scale: the operator's coupling scale) since its last
- * evaluation; 0 re-evaluates on every call.  Default 2. */
+ * evaluation; 0 re-evaluates on every call.  Default 4. */
 int tcp_learned_precond_set_refresh(tcp_learned_precond *precond, double threshold);
 /* Writes feature 0, (H_ii - theta) / scale, of every state (the callback does
  * this itself; training calls it before tcp_mlp_forward / tcp_mlp_gradient on

@@ -17,8 +17,13 @@
 //   O    w <- w - sum_i h_i v_i, with |w'|^2 and <v_i, w'>;
 //   O'   only when |w'|^2 < |w|^2 / 2 (tested on the device): the same
 //        projection again with the new coefficients (DGKS reorthogonalisation);
-//   host one copy of the reductions: column j of T = V^H A V is h (plus the
-//        second-pass correction), beta_j = |w|, v_{j+1} = w / beta_j.
+//   F    on the device: column j of T = V^H A V is h (plus the second-pass
+//        correction) and beta_j = |w|, written to step j's row of a history;
+//        v_{j+1} = w / beta_j (scaled by zero on a breakdown).
+// The steps of one expansion run back to back with no host synchronisation;
+// the host reads the history once, when the basis is full, and assembles T
+// (a breakdown found there truncates the expansion at its step and continues
+// from a fresh vector as before).
 // When the basis holds ncv columns, the Ritz pairs (theta, y) of T give the
 // residual estimates |beta_{ncv-1}| |y_{ncv-1}|; converged when every wanted
 // pair meets tol * max(|theta|, eps^(2/3)) (ARPACK's test).  Otherwise the
@@ -115,6 +120,33 @@ __global__ __launch_bounds__(kThreads) void sum_kernel(const dvec2 *__restrict__
     if (threadIdx.x == 0) out[j] = red[0];
 }
 
+// Row layout of the step history: h_0..h_{n-1}, then {|w|^2 before, after}
+// and {1 / beta (0 on a breakdown or a non-finite norm), 0}.
+constexpr uint32_t kHistRow = kMaxBasis + 2;
+constexpr double kBreakdown = 1e-24;  // |w'|^2 <= kBreakdown |w|^2: an invariant subspace
+
+// Step j's row of the history from the reduced scalars of its projections
+// (the device's own DGKS test decides which pass's values count).
+__global__ void finalize_kernel(const dvec2 *__restrict__ sc, uint32_t n, dvec2 *__restrict__ row) {
+    const bool second = second_pass_needed(sc);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        row[i] = second ? sc[1 + i] + sc[kP1 + 1 + i] : sc[1 + i];
+    if (threadIdx.x == 0) {
+        const double before2 = sc[0].x, after2 = second ? sc[kP2].x : sc[kP1].x;
+        row[kMaxBasis] = dvec2{before2, after2};
+        const bool ok = after2 > kBreakdown * before2 && after2 > 0.0 && after2 < HUGE_VAL && before2 < HUGE_VAL;
+        row[kMaxBasis + 1] = dvec2{ok ? 1.0 / sqrt(after2) : 0.0, 0.0};
+    }
+}
+
+// x *= *alpha (a factor the device computed).
+__global__ void scale_by_kernel(dvec2 *__restrict__ x, const dvec2 *__restrict__ alpha, uint64_t D) {
+    const double a = alpha->x;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        x[i] *= a;
+}
+
 // |A x - theta x|^2 partials for the explicit residual of a returned pair.
 __global__ __launch_bounds__(kThreads) void residual2_kernel(const dvec2 *__restrict__ x, const dvec2 *__restrict__ ax,
                                                              double theta, uint64_t D, dvec2 *__restrict__ partial) {
@@ -146,7 +178,7 @@ struct Lanczos {
     hipStream_t stream;
     uint64_t D = 0;
     uint32_t nb = 0;
-    tcp::DeviceBuffer pool, partial, sc, ybuf;
+    tcp::DeviceBuffer pool, partial, sc, ybuf, hist;
     std::vector<uint32_t> slot;  // column j -> vector slot
     std::vector<dvec2> hsc;
     tcp_lanczos_result *res = nullptr;
@@ -207,6 +239,30 @@ struct Lanczos {
         return TCP_OK;
     }
 
+    // The projections of orthogonalize (n >= 1) and the history row of step
+    // n - 1, with no host synchronisation.
+    int orthogonalize_async(uint32_t n, dvec2 *w) {
+        const Cols<dvec2> V = cols(n);
+        int st = with_maxm(n, [&](auto MM) -> int {
+            constexpr int M = decltype(MM)::value;
+            hipLaunchKernelGGL(dots_kernel<M>, dim3(nb), dim3(kThreads), 0, stream, V, n, w, D, part());
+            sum(n + 1, 0, false);
+            hipLaunchKernelGGL(orth_kernel<M>, dim3(nb), dim3(kThreads), 0, stream, V, n, scp(0), 1u, w, D, part(),
+                               false);
+            sum(n + 1, kP1, false);
+            hipLaunchKernelGGL(orth_kernel<M>, dim3(nb), dim3(kThreads), 0, stream, V, n, scp(0), kP1 + 1, w, D,
+                               part(), true);
+            sum(n + 1, kP2, true);
+            return TCP_OK;
+        });
+        if (st) return st;
+        dvec2 *row = static_cast<dvec2 *>(hist.ptr) + static_cast<size_t>(n - 1) * kHistRow;
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, stream, static_cast<const dvec2 *>(sc.ptr), n,
+                           row);
+        hipLaunchKernelGGL(scale_by_kernel, dim3(nb), dim3(kThreads), 0, stream, w, row + kMaxBasis + 1, D);
+        return TCP_OK;
+    }
+
     int scale(dvec2 *x, double alpha) {
         hipLaunchKernelGGL(scale_kernel<dvec2>, dim3(nb), dim3(kThreads), 0, stream, x, alpha, D);
         return TCP_OK;
@@ -226,7 +282,8 @@ struct Lanczos {
         int st;
         if ((st = pool.allocate(static_cast<size_t>(16) * D * (ncv + 3))) ||
             (st = partial.allocate(sizeof(dvec2) * static_cast<size_t>(nb) * (kMaxBasis + 1))) ||
-            (st = sc.allocate(sizeof(dvec2) * kScalars)) || (st = ybuf.allocate(sizeof(dvec2) * kMaxBasis * kMaxBasis)))
+            (st = sc.allocate(sizeof(dvec2) * kScalars)) || (st = ybuf.allocate(sizeof(dvec2) * kMaxBasis * kMaxBasis)) ||
+            (st = hist.allocate(sizeof(dvec2) * kMaxBasis * kHistRow)))
             return st;
         TCP_HIP_CHECK(hipMemsetAsync(sc.ptr, 0, sizeof(dvec2) * kScalars, stream));
         std::vector<uint32_t> free_slots;
@@ -259,45 +316,61 @@ struct Lanczos {
         std::vector<double> vals, vecs;
         std::vector<int> order;
         for (uint32_t restart = 0;; ++restart) {
-            // expand to ncv columns
+            // expand to ncv columns: the steps back to back on the device,
+            // then one read of their history
             while (m < ncv && !exhausted) {
-                const uint32_t j = m;
-                slot.push_back(take());
-                dvec2 *w = vec(slot[j + 1]);
-                ++result.mvps;
-                if ((st = tcp::plan_apply(plan, vec(slot[j]), w, TCP_C128, stream))) return st;
-                if ((st = orthogonalize(j + 1, w, h, before2, after2))) return st;
-                if (!std::isfinite(after2) || !std::isfinite(before2))
-                    return fail(TCP_ERR_INVALID_ARGUMENT, "non-finite value in the Lanczos recurrence");
-                for (uint32_t i = 0; i <= j; ++i) {
-                    Tat(i, j)[0] = h[i].x;
-                    Tat(i, j)[1] = h[i].y;
-                    Tat(j, i)[0] = h[i].x;
-                    Tat(j, i)[1] = -h[i].y;
+                const uint32_t j0 = m;
+                for (uint32_t j = j0; j < ncv; ++j) {
+                    slot.push_back(take());
+                    ++result.mvps;
+                    if ((st = tcp::plan_apply(plan, vec(slot[j]), vec(slot[j + 1]), TCP_C128, stream))) return st;
+                    if ((st = orthogonalize_async(j + 1, vec(slot[j + 1])))) return st;
                 }
-                Tat(j, j)[1] = 0.0;
-                m = j + 1;
-                if (after2 > 1e-24 * before2 && after2 > 0.0) {
-                    beta = std::sqrt(after2);
-                    scale(w, 1.0 / beta);
-                    continue;
-                }
-                // invariant subspace: continue from a fresh vector orthogonal to
-                // the basis (coupling zero), or stop when the space is spanned
-                beta = 0.0;
-                if (m >= D) {
-                    exhausted = true;
+                std::vector<dvec2> rows(static_cast<size_t>(ncv - j0) * kHistRow);
+                TCP_HIP_CHECK(hipMemcpyAsync(rows.data(), static_cast<dvec2 *>(hist.ptr) + static_cast<size_t>(j0) * kHistRow,
+                                             sizeof(dvec2) * rows.size(), hipMemcpyDeviceToHost, stream));
+                TCP_HIP_CHECK(hipStreamSynchronize(stream));
+                for (uint32_t j = j0; j < ncv; ++j) {
+                    const dvec2 *row = &rows[static_cast<size_t>(j - j0) * kHistRow];
+                    before2 = row[kMaxBasis].x;
+                    after2 = row[kMaxBasis].y;
+                    if (!std::isfinite(after2) || !std::isfinite(before2))
+                        return fail(TCP_ERR_INVALID_ARGUMENT, "non-finite value in the Lanczos recurrence");
+                    for (uint32_t i = 0; i <= j; ++i) {
+                        Tat(i, j)[0] = row[i].x;
+                        Tat(i, j)[1] = row[i].y;
+                        Tat(j, i)[0] = row[i].x;
+                        Tat(j, i)[1] = -row[i].y;
+                    }
+                    Tat(j, j)[1] = 0.0;
+                    m = j + 1;
+                    if (after2 > kBreakdown * before2 && after2 > 0.0) {
+                        beta = std::sqrt(after2);  // the device scaled v_{j+1} by 1 / beta
+                        continue;
+                    }
+                    // invariant subspace at step j: the later steps ran on a zero
+                    // vector and are dropped; continue from a fresh vector
+                    // orthogonal to the basis (coupling zero), or stop when the
+                    // space is spanned
+                    for (uint32_t k = j + 2; k < slot.size(); ++k) free_slots.push_back(slot[k]);
+                    slot.resize(j + 2);
+                    beta = 0.0;
+                    if (m >= D) {
+                        exhausted = true;
+                        break;
+                    }
+                    dvec2 *w = vec(slot[j + 1]);
+                    hipLaunchKernelGGL(init_kernel<dvec2>, dim3(nb), dim3(kThreads), 0, stream, w,
+                                       opt.seed + 0x9e3779b97f4a7c15ull * (++fresh), D);
+                    std::vector<dvec2> discard;
+                    if ((st = orthogonalize(m, w, discard, before2, after2))) return st;
+                    if (!(after2 > 1e-20 * before2) || !(after2 > 0.0)) {
+                        exhausted = true;
+                        break;
+                    }
+                    scale(w, 1.0 / std::sqrt(after2));
                     break;
                 }
-                hipLaunchKernelGGL(init_kernel<dvec2>, dim3(nb), dim3(kThreads), 0, stream, w,
-                                   opt.seed + 0x9e3779b97f4a7c15ull * (++fresh), D);
-                std::vector<dvec2> discard;
-                if ((st = orthogonalize(m, w, discard, before2, after2))) return st;
-                if (!(after2 > 1e-20 * before2) || !(after2 > 0.0)) {
-                    exhausted = true;
-                    break;
-                }
-                scale(w, 1.0 / std::sqrt(after2));
             }
 
             // Ritz pairs of T (ascending), residual estimates of the wanted ones

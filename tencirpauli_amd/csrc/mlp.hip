@@ -41,9 +41,11 @@ constexpr uint32_t kFwdThreads = 512;
 // somewhat different theta is as good a search direction: on the seven
 // sectors the network is trained on (4x4, 3x6 and 4x5 tori) the learned
 // Davidson iteration counts at subspaces 12/4 and 24/4 are the same for
-// every threshold from 0 (evaluate at every iteration) to 2 (the largest
-// tried), while the evaluations fall several-fold.
-constexpr double kRefresh = 2.0;
+// every threshold from 0 (evaluate at every iteration) to 4, the largest
+// such value of those tried (at 8 a few sectors take one iteration more;
+// evaluating once, at the start vector's theta, does not converge), while the
+// evaluations fall several-fold.
+constexpr double kRefresh = 4.0;
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));

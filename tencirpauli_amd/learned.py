@@ -115,7 +115,7 @@ class LearnedPreconditioner:
     (0: on every call).
     """
 
-    def __init__(self, mlp: MLP, restricted: Any, refresh: float = 2.0) -> None:
+    def __init__(self, mlp: MLP, restricted: Any, refresh: float = 4.0) -> None:
         from .symmetry import SectorMvpPlan, lazy_plan_of
 
         plan = lazy_plan_of(restricted)
