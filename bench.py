@@ -222,8 +222,9 @@ def record_traffic(fetch_dir: str, write_dir: str, tag: str, out_path: str) -> d
     s = 16 if tag == "complex128" else 8
     copy_bytes = (1 << cfg["nqubits"]) * s
     fetch_rows, write_rows = _counter_rows(fetch_dir), _counter_rows(write_dir)
-    fetch_kib, nf = _per_dispatch(fetch_rows, "FETCH_SIZE", "mvp_tile_kernel")
-    write_kib, nw = _per_dispatch(write_rows, "WRITE_SIZE", "mvp_tile_kernel")
+    kernel = "mvp_twin_kernel" if cfg["tiling"].endswith("x2") else "mvp_tile_kernel"
+    fetch_kib, nf = _per_dispatch(fetch_rows, "FETCH_SIZE", kernel)
+    write_kib, nw = _per_dispatch(write_rows, "WRITE_SIZE", kernel)
     copy_fetch, _ = _per_dispatch(fetch_rows, "FETCH_SIZE", "copyBuffer", largest=True)
     copy_write, _ = _per_dispatch(write_rows, "WRITE_SIZE", "copyBuffer", largest=True)
     read_scale = copy_bytes / (copy_fetch * 1024.0)
@@ -231,6 +232,7 @@ def record_traffic(fetch_dir: str, write_dir: str, tag: str, out_path: str) -> d
     read_bytes = fetch_kib * 1024.0 * read_scale
     write_bytes = write_kib * 1024.0 * write_scale
     rec = {
+        "kernel": kernel,
         "signature": {"workload": cfg["workload"], "nqubits": cfg["nqubits"], "terms": cfg["terms"],
                       "x_groups": cfg["x_groups"], "classes": cfg["tile_classes"], "tiling": cfg["tiling"],
                       "kernel_sha": cfg["kernel_sha"]},
@@ -249,7 +251,7 @@ def record_traffic(fetch_dir: str, write_dir: str, tag: str, out_path: str) -> d
     old = data.get(tag, {})
     if old.get("signature") == rec["signature"] and "limiter" in old:
         rec["limiter"] = old["limiter"]
-    data.setdefault("kernel", "mvp_tile_kernel")
+    data.pop("kernel", None)  # named per storage
     data.setdefault("method", "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE (KiB per dispatch), each scaled by "
                               "the same counter's reading for a plain device copy of known size; written by "
                               "python bench.py --record-traffic")

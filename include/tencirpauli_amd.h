@@ -172,8 +172,8 @@ int tcp_mvp_plan_create_from_words(uint32_t nqubits, uint64_t nterms, const uint
  * TCP_TILE_XCD_ORDER walks the tiles XCD by XCD (the default of
  * TCP_TILE_AUTO; TCP_TILE_LINEAR_ORDER turns it off).  Spaces smaller than the
  * geometry's tile fall back to a smaller one.  tcp_mvp_plan_create uses
- * TCP_TILE_AUTO: from 24 qubits 1024x8 for complex128 and 512x8 for complex64
- * storage, below that 512x8 and 512x4, all with the XCD order.  Every tiled
+ * TCP_TILE_AUTO: complex128 from 22 qubits 1024x8x2 (below, 512x8), complex64
+ * storage from 24 qubits 512x8 (below, 512x4), all with the XCD order.  Every tiled
  * geometry maps its kernel index onto the amplitude index through a bit
  * permutation the plan builder chooses per operator (fewer source tiles per
  * output tile, more L2 reuse); results differ only in summation order. */
@@ -183,7 +183,10 @@ enum {
     TCP_TILE_1024x8 = 2, /* 2^13-amplitude tiles */
     TCP_TILE_512x16 = 3, /* 2^13-amplitude tiles */
     TCP_TILE_512x4 = 4,  /* 2^11-amplitude tiles, four workgroups per CU */
-    TCP_TILE_1024x4 = 5  /* 2^12-amplitude tiles, 16 waves per workgroup, two per CU */
+    TCP_TILE_1024x4 = 5,  /* 2^12-amplitude tiles, 16 waves per workgroup, two per CU */
+    TCP_TILE_1024x8x2 = 6, /* twin tiles: a workgroup owns two 2^13-amplitude output tiles and
+                              stages each 2^14-amplitude source tile as two halves that serve both */
+    TCP_TILE_512x16x2 = 7  /* twin tiles of 2^13 amplitudes, 512 threads */
 };
 #define TCP_TILE_XCD_ORDER 0x100u
 #define TCP_TILE_LINEAR_ORDER 0x200u /* plain tile order, also for TCP_TILE_AUTO */
@@ -195,7 +198,8 @@ enum {
 int tcp_mvp_plan_create_tiled(const tcp_operator *op, int storage, uint64_t max_bytes, uint32_t tiling,
                               tcp_mvp_plan **out);
 /* Geometry the given dtype's applies use (after the small-space fallback;
- * 0 = small-space kernel) and log2 of its tile. */
+ * 0 = small-space kernel) and log2 of its tile (twin geometries: of each of
+ * the two output tiles, the size of a staged source half). */
 uint32_t tcp_mvp_plan_tiling(const tcp_mvp_plan *plan, int dtype);
 uint32_t tcp_mvp_plan_tile_bits(const tcp_mvp_plan *plan, int dtype);
 

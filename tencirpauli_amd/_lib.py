@@ -43,6 +43,8 @@ TCP_TILE_1024x8 = 2
 TCP_TILE_512x16 = 3
 TCP_TILE_512x4 = 4
 TCP_TILE_1024x4 = 5
+TCP_TILE_1024x8x2 = 6
+TCP_TILE_512x16x2 = 7
 TCP_TILE_XCD_ORDER = 0x100
 TCP_TILE_LINEAR_ORDER = 0x200
 TCP_TILE_PREFETCH_SHIFT = 10

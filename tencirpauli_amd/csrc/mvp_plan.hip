@@ -12,7 +12,8 @@
 // of TPB threads owns one output tile, each thread holding A = 2^L / TPB
 // amplitudes l = t + a*TPB as FP64 accumulators in registers (geometries
 // 512x8 with L = 12, 1024x8 and 512x16 with L = 13; smaller spaces use 512
-// threads and 1-4 amplitudes).  Group g reads source tile (tile ^ (x_g >> L))
+// threads and 1-4 amplitudes), or two output tiles (twin geometries,
+// tile_tables.hpp twin_apply).  Group g reads source tile (tile ^ (x_g >> L))
 // permuted by x_g & (2^L-1).  Groups are sorted by that high part ("class"):
 // the source tile of a class is streamed from HBM into LDS once (widened to
 // FP64) and every group of the class gathers from LDS, so HBM reads per apply
@@ -35,11 +36,14 @@ using tcp::flip_sign;
 using tcp::fvec2;
 using MvpArgs = tcp::TileArgs;
 
-// Tile geometries: threads per workgroup x amplitudes per thread.
+// Tile geometries: threads per workgroup x amplitudes per thread, and
+// whether a workgroup owns twin tiles (two output tiles sharing each staged
+// source half, tile_tables.hpp twin_apply).
 struct Geometry {
-    uint32_t threads, amps;
+    uint32_t threads, amps, twin;
 };
-constexpr Geometry kGeometries[] = {{0, 0}, {512, 8}, {1024, 8}, {512, 16}, {512, 4}, {1024, 4}};
+constexpr Geometry kGeometries[] = {{0, 0, 0},   {512, 8, 0},  {1024, 8, 0}, {512, 16, 0},
+                                    {512, 4, 0}, {1024, 4, 0}, {1024, 8, 1}, {512, 16, 1}};
 constexpr uint32_t kGeometryCount = sizeof(kGeometries) / sizeof(kGeometries[0]);
 constexpr uint32_t kSmallThreads = 512;
 // Host-pointer applies stage through two device buffers; up to this size they
@@ -96,6 +100,22 @@ void mvp_tile_kernel(const VT *__restrict__ v, VT *__restrict__ y, MvpArgs p,
     }
     const uint32_t active = PARTIAL ? static_cast<uint32_t>(dim) : TPB * A;
     tcp::tile_apply<A, VT, TPB, PARTIAL, PF, NT, !PARTIAL>(v + rhs, y + rhs, p, tile, active, lds, lblk);
+}
+
+// Twin-tile kernel: workgroup b owns output tiles 2*pair and 2*pair + 1, with
+// pair = b remapped by the XCD order as in mvp_tile_kernel.
+template <int A, typename VT, uint32_t TPB, int CR = 0>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(waves_per_simd<A, TPB, VT>())))
+void mvp_twin_kernel(const VT *__restrict__ v, VT *__restrict__ y, MvpArgs p, uint64_t dim, uint32_t xcd_shift) {
+    __shared__ dvec2 lds[TPB * A];
+    __shared__ tcp::u32x4 lblk[tcp::block_words(TPB) + 2];
+    const uint64_t rhs = static_cast<uint64_t>(blockIdx.y) * dim;
+    uint64_t pair = blockIdx.x;
+    if (xcd_shift != kNoXcdOrder) {
+        const uint64_t j = pair >> 3;
+        pair = (j & ((1ull << xcd_shift) - 1)) | ((pair & 7u) << xcd_shift) | ((j >> xcd_shift) << (xcd_shift + 3));
+    }
+    tcp::twin_apply<A, VT, TPB, CR>(v + rhs, y + rhs, p, pair, lds, lblk);
 }
 
 // Fill kernel used instead of hipMemsetAsync (safe under graph capture).
@@ -158,6 +178,7 @@ uint32_t tile_xcd_shift(const PlanTiles &pt, uint64_t dim);
 struct PlanTiles {
     uint32_t tiling = 0;  // geometry index (0: small space) | TCP_TILE_XCD_ORDER
     uint32_t tb = 0, ab = 0, L = 0;
+    uint32_t twin = 0;    // 1: twin tiles (workgroups own 2^(L+1) amplitudes)
     uint64_t H = 0, passes = 0;
     uint32_t diag_passes = 0;  // passes with x = 0
     uint32_t n = 0;
@@ -187,8 +208,9 @@ struct PlanTiles {
 // when it chose them and the space has tiles above them, else the top three
 // tile bits; kNoXcdOrder without an XCD order or with fewer than 8 tiles.
 uint32_t tile_xcd_shift(const PlanTiles &pt, uint64_t dim) {
-    if (!(pt.tiling & TCP_TILE_XCD_ORDER) || (dim >> pt.L) < 8) return kNoXcdOrder;
-    const uint32_t top = 63u - static_cast<uint32_t>(__builtin_clzll(dim >> pt.L)) - 3u;
+    const uint64_t wgs = dim >> (pt.L + pt.twin);
+    if (!(pt.tiling & TCP_TILE_XCD_ORDER) || wgs < 8) return kNoXcdOrder;
+    const uint32_t top = 63u - static_cast<uint32_t>(__builtin_clzll(wgs)) - 3u;
     return pt.xcd_bits_at <= top ? pt.xcd_bits_at : top;
 }
 
@@ -217,19 +239,21 @@ using tcp::fail;
 
 // Geometry index for `tiling` on an n-qubit space; 0 means the small-space
 // geometry (512 threads, 1-4 amplitudes each).  Automatic, measured on the
-// 28-qubit lattice workload: spaces of 2^24 amplitudes or more use 1024 x 8
-// for complex128 (its FP64 source tiles stream through L2 at a fixed rate, so
-// the larger tile's fewer classes win) and 512 x 8 for complex64 storage
-// (bound by the pass loop, which the eight amplitudes per thread amortise);
-// smaller spaces use 512 x 8 and 512 x 4, which give more workgroups.
+// 28-qubit lattice workloads and random 22-27-qubit operators: complex128
+// from 2^22 amplitudes uses 1024 x 8 twin tiles (its FP64 source tiles
+// stream into LDS at a fixed rate per CU, so the fewer classes of 2^14
+// amplitudes win: 6-16 % faster than single 1024 x 8 tiles), complex64
+// storage from 2^24 uses 512 x 8 (bound by the pass loop, which the eight
+// amplitudes per thread amortise); smaller spaces use 512 x 8 and 512 x 4,
+// which give more workgroups.
 uint32_t resolve_geometry(uint32_t n, uint32_t tiling, int dtype) {
     uint32_t g = tiling & 0xffu;
     if (g == TCP_TILE_AUTO) {
-        if (n >= 24) g = dtype == TCP_C64 ? TCP_TILE_512x8 : TCP_TILE_1024x8;
-        else g = dtype == TCP_C64 ? TCP_TILE_512x4 : TCP_TILE_512x8;
+        if (dtype == TCP_C64) g = n >= 24 ? TCP_TILE_512x8 : TCP_TILE_512x4;
+        else g = n >= 22 ? TCP_TILE_1024x8x2 : TCP_TILE_512x8;
     }
-    const uint32_t L = log2u(kGeometries[g].threads) + log2u(kGeometries[g].amps);
-    if (n < L) g = n >= 12 ? TCP_TILE_512x8 : (n >= 11 ? TCP_TILE_512x4 : 0);
+    const uint32_t L = log2u(kGeometries[g].threads) + log2u(kGeometries[g].amps) + kGeometries[g].twin;
+    if (n < L) g = n >= 13 ? TCP_TILE_1024x8 : (n >= 12 ? TCP_TILE_512x8 : (n >= 11 ? TCP_TILE_512x4 : 0));
     return g;
 }
 
@@ -243,6 +267,7 @@ int build_tiles(const tcp::Operator &op, uint32_t geo, bool xcd, uint32_t pipeli
     } else {
         pt.tb = log2u(kGeometries[geo].threads);
         pt.ab = log2u(kGeometries[geo].amps);
+        pt.twin = kGeometries[geo].twin;
     }
     pt.L = pt.tb + pt.ab;
     pt.n = n;
@@ -257,7 +282,7 @@ int build_tiles(const tcp::Operator &op, uint32_t geo, bool xcd, uint32_t pipeli
         // 4 MiB L2 exactly); three XCD bits above it
         const uint32_t window = log2u(32u * per_cu) - 1;
         uint8_t pos[64];
-        if (tcp::choose_bit_map(op, pt.tb, pt.ab, window, 3, pos)) {
+        if (tcp::choose_bit_map(op, pt.tb, pt.ab, window, 3, pos, pt.twin)) {
             for (uint32_t c = 0; c < n; ++c) {
                 pt.pos[c] = pos[c];
                 pt.permuted = pt.permuted || pos[c] != c;
@@ -265,7 +290,8 @@ int build_tiles(const tcp::Operator &op, uint32_t geo, bool xcd, uint32_t pipeli
             pt.xcd_bits_at = window;
         }
     }
-    tab.append(op, pt.tb, pt.ab, pt.permuted ? pt.pos : nullptr, geo == 0 ? kSmallThreads : kGeometries[geo].threads);
+    tab.append(op, pt.tb, pt.ab, pt.permuted ? pt.pos : nullptr, geo == 0 ? kSmallThreads : kGeometries[geo].threads,
+               pt.twin != 0, pt.permuted ? pt.xcd_bits_at : 0u);
     pt.H = tab.H;
     pt.passes = tab.passes;
     pt.diag_passes = static_cast<uint32_t>(tab.diag_passes);
@@ -371,6 +397,14 @@ hipError_t launch_tile(const LaunchSpec &ls, const void *in, void *out, uint32_t
     return hipGetLastError();
 }
 
+template <int A, typename VT, uint32_t TPB, int CR = 0>
+hipError_t launch_twin(const LaunchSpec &ls, const void *in, void *out, uint32_t nrhs, hipStream_t stream) {
+    const uint64_t pairs = ls.dim >> (ls.L + 1);
+    hipLaunchKernelGGL((mvp_twin_kernel<A, VT, TPB, CR>), dim3(static_cast<uint32_t>(pairs), nrhs), dim3(TPB), 0, stream,
+                       static_cast<const VT *>(in), static_cast<VT *>(out), ls.args, ls.dim, ls.xcd_shift);
+    return hipGetLastError();
+}
+
 // Geometries with a selectable prefetch depth (1-3 tiles in flight) and load policy.
 template <int A, typename VT, uint32_t TPB>
 hipError_t launch_pipelined(const LaunchSpec &ls, const void *in, void *out, uint32_t nrhs, hipStream_t stream) {
@@ -398,6 +432,8 @@ hipError_t launch_typed(const LaunchSpec &ls, const void *in, void *out, uint32_
         case TCP_TILE_512x16: return launch_tile<16, VT, 512>(ls, in, out, nrhs, stream);
         case TCP_TILE_512x4: return launch_pipelined<4, VT, 512>(ls, in, out, nrhs, stream);
         case TCP_TILE_1024x4: return launch_pipelined<4, VT, 1024>(ls, in, out, nrhs, stream);
+        case TCP_TILE_1024x8x2: return launch_twin<8, VT, 1024, 1>(ls, in, out, nrhs, stream);
+        case TCP_TILE_512x16x2: return launch_twin<16, VT, 512>(ls, in, out, nrhs, stream);
         default: break;
     }
     if (ls.n < kSmallBits) return launch_tile<1, VT, kSmallThreads, true>(ls, in, out, nrhs, stream);
@@ -524,9 +560,12 @@ int tcp_mvp_plan_create_from_words(uint32_t nqubits, uint64_t nterms, const uint
 uint32_t tcp_mvp_plan_nqubits(const tcp_mvp_plan *plan) { return plan ? plan->n : 0; }
 uint64_t tcp_mvp_plan_term_count(const tcp_mvp_plan *plan) { return plan ? plan->T : 0; }
 uint64_t tcp_mvp_plan_group_count(const tcp_mvp_plan *plan) { return plan ? plan->G : 0; }
-uint64_t tcp_mvp_plan_class_count(const tcp_mvp_plan *plan) { return plan ? plan->t128.H : 0; }
+// Twin-tile tables list two stages (halves) per class.
+uint64_t tcp_mvp_plan_class_count(const tcp_mvp_plan *plan) { return plan ? plan->t128.H >> plan->t128.twin : 0; }
 uint64_t tcp_mvp_plan_pass_count(const tcp_mvp_plan *plan) { return plan ? plan->t128.passes : 0; }
-uint64_t tcp_mvp_plan_class_count_for(const tcp_mvp_plan *plan, int dtype) { return plan ? plan->tiles(dtype).H : 0; }
+uint64_t tcp_mvp_plan_class_count_for(const tcp_mvp_plan *plan, int dtype) {
+    return plan ? plan->tiles(dtype).H >> plan->tiles(dtype).twin : 0;
+}
 uint64_t tcp_mvp_plan_dimension(const tcp_mvp_plan *plan) { return plan ? plan->dim() : 0; }
 int tcp_mvp_plan_storage(const tcp_mvp_plan *plan) { return plan ? plan->storage : -1; }
 uint64_t tcp_mvp_plan_transition_count(const tcp_mvp_plan *plan) {

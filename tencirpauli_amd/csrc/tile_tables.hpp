@@ -234,11 +234,12 @@ __device__ __forceinline__ void add_term(const Rec &R, uint32_t t, uint64_t tile
 // compile-time).  The LDS reads of the first four slots of a pass are issued
 // together, then its terms' per-thread sum S is formed, then the FMAs run
 // four slots at a time.
-template <int ZS, int A, uint32_t TPB>
+template <int ZS, int A, uint32_t TPB, int CR = 0>
 __device__ __forceinline__ void passes_uniform(const u32x4 *blk, uint32_t r0, uint32_t r1, uint32_t t, uint32_t t16,
                                                uint64_t tile_id, const char *tile, double *accr, double *acci) {
     constexpr uint32_t TB = __builtin_ctz(TPB);
-    constexpr int C = A >= 8 ? 4 : (A >= 4 ? 2 : A);  // slots read from LDS at a time (bounds live registers)
+    // slots read from LDS at a time (bounds live registers; CR > 0 overrides)
+    constexpr int C = CR > 0 ? CR : (A >= 8 ? 4 : (A >= 4 ? 2 : A));
     uint32_t r = r0;
     while (r < r1) {
         Rec R = read_rec(blk, r);
@@ -285,11 +286,11 @@ __device__ __forceinline__ void passes_uniform(const u32x4 *blk, uint32_t r0, ui
 
 // All passes of one class, slot pattern by slot pattern (zr: the block
 // header, record index where slot pattern z's passes begin, zr[A] the end).
-template <int A, uint32_t TPB, int ZS = 0>
+template <int A, uint32_t TPB, int ZS = 0, int CR = 0>
 __device__ __forceinline__ void passes_all(const u32x4 *blk, const uint32_t *zr, uint32_t t, uint32_t t16,
                                            uint64_t tile_id, const char *tile, double *accr, double *acci) {
-    passes_uniform<ZS, A, TPB>(blk, zr[ZS], zr[ZS + 1], t, t16, tile_id, tile, accr, acci);
-    if constexpr (ZS + 1 < A) passes_all<A, TPB, ZS + 1>(blk, zr, t, t16, tile_id, tile, accr, acci);
+    passes_uniform<ZS, A, TPB, CR>(blk, zr[ZS], zr[ZS + 1], t, t16, tile_id, tile, accr, acci);
+    if constexpr (ZS + 1 < A) passes_all<A, TPB, ZS + 1, CR>(blk, zr, t, t16, tile_id, tile, accr, acci);
 }
 
 // Loads of one source tile into registers; NT marks them non-temporal.
@@ -391,6 +392,91 @@ __device__ __forceinline__ void tile_apply(const VT *__restrict__ vin, VT *__res
         if (live(a)) out[slot_off(a)] = Amp<VT>::make(accr[a], acci[a]);
 }
 
+// Twin tiles: the workgroup owns the two output tiles 2*pair and 2*pair + 1
+// (kernel tile bit 0, coordinate L, tells them apart), 2A FP64 accumulators
+// per thread.  A class is an x-mask part above coordinate L; its source is
+// the 2^(L+1) tile pair ^ (class), staged through LDS as two halves.  A pass
+// whose x has bit L = px, applied while half h is staged, reads that half for
+// output tile h ^ px, so every staged half serves both output tiles: source
+// loads per amplitude follow the class count of 2^(L+1)-amplitude tiles while
+// LDS holds 2^L amplitudes.  Class blocks order passes by (px, slot pattern),
+// header entries [0, A] for px = 0 and [A, 2A] for px = 1.  The class table
+// lists stages (TileTables::append): p.H is their count.  Every thread of
+// the workgroup must call this (it synchronises).
+template <int A, typename VT, uint32_t TPB, int CR = 0>
+__device__ __forceinline__ void twin_apply(const VT *__restrict__ vin, VT *__restrict__ yout, const TileArgs &p,
+                                           uint64_t pair, dvec2 *lds, u32x4 *lblk) {
+    cptr<TileClass> cls = (cptr<TileClass>)p.cls;
+    constexpr uint32_t TB = __builtin_ctz(TPB), AB = __builtin_ctz(static_cast<uint32_t>(A));
+    const uint32_t t = threadIdx.x;
+    const uint64_t base = deposit(p.pos, p.L + 1, p.n - p.L - 1, pair);  // output tile 2*pair
+    const uint64_t half_off = 1ull << p.pos[p.L];                       // its twin, and a class's second half
+    const uint64_t toff = deposit(p.pos, 0, TB, t);
+    auto slot_off = [&](int a) -> uint64_t { return deposit(p.pos, TB, AB, static_cast<uint64_t>(a)); };
+    const VT *__restrict__ src_base = vin + toff;
+    const uint64_t tile0 = pair << 1, tile1 = tile0 | 1u;
+    const uint32_t t16 = t * static_cast<uint32_t>(sizeof(dvec2));
+    const char *tile = reinterpret_cast<const char *>(lds);
+    double accr0[A], acci0[A], accr1[A], acci1[A];
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        accr0[a] = 0.0;
+        acci0[a] = 0.0;
+        accr1[a] = 0.0;
+        acci1[a] = 0.0;
+    }
+    VT pre[A];
+    u32x4 bpre;
+    uint32_t pre_half = 0;
+    // stage s (a class's half, folded into its offset) into the registers
+    auto fetch = [&](uint32_t s) {
+        const uint64_t src = base ^ cls[s].off;
+        const uint32_t boff = cls[s].boff, bwords = cls[s].bwords & 0x7FFFFFFFu;
+        pre_half = cls[s].bwords >> 31;
+#pragma unroll
+        for (int a = 0; a < A; ++a) pre[a] = load_amp<false>(src_base + (src | slot_off(a)));
+        bpre = t < bwords ? p.blk[boff + t] : u32x4{0u, 0u, 0u, 0u};  // bwords <= block_words(TPB)
+    };
+    auto stage = [&]() {
+        __syncthreads();  // every wave is done reading the previous half and block
+#pragma unroll
+        for (int a = 0; a < A; ++a) lds[t + a * TPB] = Amp<VT>::widen(pre[a]);
+        if (t < block_words(TPB)) lblk[t] = bpre;
+        __syncthreads();
+    };
+    auto header = [&](uint32_t *zr) {  // u16 entries, read once into scalar registers
+#pragma unroll
+        for (int z = 0; z <= 2 * A; ++z) {
+            const uint32_t word = reinterpret_cast<const uint32_t *>(lblk)[z >> 1];
+            zr[z] = __builtin_amdgcn_readfirstlane((word >> ((z & 1) * 16)) & 0xFFFFu);
+        }
+    };
+    fetch(0);
+    for (uint32_t s = 0; s < p.H; ++s) {
+        uint32_t zr[2 * A + 1];
+        const uint32_t half = pre_half;
+        stage();
+        if (s + 1 < p.H) fetch(s + 1);
+        header(zr);
+        // passes for output tile 0 have px = half, those for tile 1 px = 1 - half
+        uint32_t z0[A + 1], z1[A + 1];
+#pragma unroll
+        for (int z = 0; z <= A; ++z) {
+            z0[z] = half ? zr[A + z] : zr[z];
+            z1[z] = half ? zr[z] : zr[A + z];
+        }
+        passes_all<A, TPB, 0, CR>(lblk, z0, t, t16, tile0, tile, accr0, acci0);
+        passes_all<A, TPB, 0, CR>(lblk, z1, t, t16, tile1, tile, accr1, acci1);
+    }
+    VT *__restrict__ out0 = yout + base + toff;
+    VT *__restrict__ out1 = yout + (base ^ half_off) + toff;
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        out0[slot_off(a)] = Amp<VT>::make(accr0[a], acci0[a]);
+        out1[slot_off(a)] = Amp<VT>::make(accr1[a], acci1[a]);
+    }
+}
+
 // Diagonal sum of pass g at tile-local position (t, slot) of tile `tile_id`,
 // in FP64 (plan builders: eager transition counts, diagonals).
 __device__ inline dvec2 group_sum(const TileArgs &p, uint32_t g, uint32_t t, uint32_t slot, uint64_t tile_id) {
@@ -417,12 +503,14 @@ __device__ inline dvec2 group_sum(const TileArgs &p, uint32_t g, uint32_t t, uin
 // streams while its workgroups run one class each), then the XCD bits the
 // same way (source tiles the eight XCDs share through the last-level cache),
 // then, among the tile-local bits, the slot bits that split the fewest
-// x-groups into passes.  Ties go to the lowest bit.  Returns false
+// x-groups into passes.  With `twin` one more tile-local bit is chosen and
+// placed at coordinate tb + ab, the lowest tile bit: it tells apart the two
+// output tiles a twin-tile workgroup owns.  Ties go to the lowest bit.  Returns false
 // (identity) for spaces too small to tile, and for operators with more than
 // 16,384 distinct x-masks (where the search would cost more than it saves).
 inline bool choose_bit_map(const Operator &op, uint32_t tb, uint32_t ab, uint32_t window, uint32_t xcd,
-                           uint8_t *pos) {
-    const uint32_t n = op.nqubits, L = tb + ab;
+                           uint8_t *pos, uint32_t twin = 0) {
+    const uint32_t n = op.nqubits, L = tb + ab + twin;
     if (n > 64 || tb < 4 || n < L + window + xcd) return false;
     const std::vector<IndexTerm> terms = index_terms(op);
     std::vector<uint64_t> xs(terms.size());
@@ -477,13 +565,21 @@ inline bool choose_bit_map(const Operator &op, uint32_t tb, uint32_t ab, uint32_
         }
         slot |= 1ull << best_b;
     }
+    // twin tiles: the highest tile-local bit outside the slot and lowest four
+    // bits tells the two output tiles apart (any choice loads the same tiles)
+    uint64_t pair = 0;
+    if (twin) {
+        const uint64_t cand = local & ~slot & ~0xFull;
+        pair = 1ull << (63 - __builtin_clzll(cand));
+    }
     uint32_t c = 0;
     auto place = [&](uint64_t set) {
         for (uint32_t b = 0; b < n; ++b)
             if ((set >> b) & 1ull) pos[c++] = static_cast<uint8_t>(b);
     };
-    place(local & ~slot);  // thread coordinates (bits 0..3 first)
+    place(local & ~slot & ~pair);  // thread coordinates (bits 0..3 first)
     place(slot);
+    place(pair);
     place(win);
     place(xb);
     const uint64_t all = n == 64 ? ~0ull : (1ull << n) - 1;
@@ -507,9 +603,19 @@ struct TileTables {
     // Appends the tables of `op`, with kernel coordinate c at index bit
     // pos[c] (pos = nullptr: the identity), for a kernel of `wg_threads`
     // threads per workgroup (0: 2^tb), which sets the class block size.
+    // With `twin` the classes are the x-mask parts above coordinate L (one
+    // 2^(L+1) source tile, staged as two halves) and a class block orders its
+    // passes by (bit L of x, slot pattern): its header has 2A + 1 entries.
+    // The class table then lists stages, one per (class, half), with the half
+    // folded into the offset and flagged in bit 31 of bwords, ordered run by
+    // run (a run: the classes whose x-mask parts above L differ only in the
+    // `window` lowest bits, the bits that vary across the workgroups one XCD
+    // runs together), the first halves of a run before its second halves: a
+    // source tile that one workgroup stages is staged by another at most a
+    // run's length of stages later, while it is still in L2.
     void append(const Operator &op, uint32_t tb, uint32_t ab, const uint8_t *pos = nullptr,
-                uint32_t wg_threads = 0) {
-        const uint32_t L = tb + ab, A = 1u << ab;
+                uint32_t wg_threads = 0, bool twin = false, uint32_t window = 0) {
+        const uint32_t L = tb + ab, A = 1u << ab, CL = L + (twin ? 1u : 0u);
         const uint64_t lmask = (1ull << L) - 1, tmask = (1ull << tb) - 1;
         std::vector<IndexTerm> terms = index_terms(op);
         const uint64_t T = terms.size();
@@ -526,20 +632,24 @@ struct TileTables {
             }
         }
         auto class_offset = [&](uint64_t hi) {
-            return pos ? deposit(pos, L, n > L ? n - L : 0, hi) : hi << L;
+            return pos ? deposit(pos, CL, n > CL ? n - CL : 0, hi) : hi << CL;
         };
         struct Key {
             uint64_t hi, lo;
+            uint32_t pat;  // block pattern: slot pattern, plus A * (bit L of x) for twin tiles
             uint32_t zs;
             uint64_t index;
         };
         std::vector<Key> keys(T);
-        for (uint64_t k = 0; k < T; ++k)
-            keys[k] = Key{terms[k].x >> L, terms[k].x & lmask, static_cast<uint32_t>((terms[k].z >> tb) & (A - 1)), k};
-        // kernel order: class, slot pattern, x-mask, canonical term order
+        for (uint64_t k = 0; k < T; ++k) {
+            const uint32_t zs = static_cast<uint32_t>((terms[k].z >> tb) & (A - 1));
+            const uint32_t px = twin ? static_cast<uint32_t>((terms[k].x >> L) & 1u) : 0u;
+            keys[k] = Key{terms[k].x >> CL, terms[k].x & lmask, zs + A * px, zs, k};
+        }
+        // kernel order: class, block pattern, x-mask, canonical term order
         std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
             if (a.hi != b.hi) return a.hi < b.hi;
-            if (a.zs != b.zs) return a.zs < b.zs;
+            if (a.pat != b.pat) return a.pat < b.pat;
             if (a.lo != b.lo) return a.lo < b.lo;
             return a.index < b.index;
         });
@@ -560,8 +670,9 @@ struct TileTables {
             uint32_t zs, index;
         };
         std::vector<PassKey> pkeys;
+        std::vector<uint32_t> pats;  // block pattern of each pass
         struct ClassPasses {
-            uint64_t off;
+            uint64_t hi, off;
             size_t gbegin, gend;
         };
         std::vector<ClassPasses> classes;
@@ -569,11 +680,14 @@ struct TileTables {
         while (k < T) {
             if (classes.empty() || keys[k].hi != keys[k - 1].hi) {
                 if (!classes.empty()) classes.back().gend = grp64.size();
-                classes.push_back(ClassPasses{class_offset(keys[k].hi), grp64.size(), 0});
+                classes.push_back(ClassPasses{keys[k].hi, class_offset(keys[k].hi), grp64.size(), 0});
             }
             uint64_t e = k;
-            while (e < T && keys[e].hi == keys[k].hi && keys[e].zs == keys[k].zs && keys[e].lo == keys[k].lo) ++e;
-            pkeys.push_back(PassKey{keys[k].hi, keys[k].lo, keys[k].zs, static_cast<uint32_t>(grp64.size())});
+            while (e < T && keys[e].hi == keys[k].hi && keys[e].pat == keys[k].pat && keys[e].lo == keys[k].lo) ++e;
+            // x-mask order keys use the part above L (twin classes hold two of them)
+            pkeys.push_back(PassKey{terms[keys[k].index].x >> L, keys[k].lo, keys[k].zs,
+                                    static_cast<uint32_t>(grp64.size())});
+            pats.push_back(keys[k].pat);
             for (int fp32 = 0; fp32 < 2; ++fp32) {
                 std::vector<TileGroup> &grp = fp32 ? grp32 : grp64;
                 std::vector<TileTerm> &tv = fp32 ? term32 : term64;
@@ -593,8 +707,28 @@ struct TileTables {
             k = e;
         }
         if (!classes.empty()) classes.back().gend = grp64.size();
-        for (const ClassPasses &c : classes)
-            append_blocks(c.off, c.gbegin, c.gend, ab, wg_threads ? wg_threads : 1u << tb);
+        std::vector<uint64_t> run_of;  // run of each class-table entry (twin stage order)
+        for (size_t i = 0; i < classes.size(); ++i) {
+            const ClassPasses &c = classes[i];
+            append_blocks(c.off, c.gbegin, c.gend, pats.data() + (c.gbegin - grp_base), twin ? 2 * A : A,
+                          wg_threads ? wg_threads : 1u << tb);
+            run_of.resize(cls.size() - cls_base, c.hi >> window);
+        }
+        if (twin) {
+            const uint64_t half_off = pos ? 1ull << pos[L] : 1ull << L;
+            std::vector<TileClass> entries(cls.begin() + cls_base, cls.end()), stages;
+            for (size_t r0 = 0; r0 < entries.size();) {
+                size_t r1 = r0;
+                while (r1 < entries.size() && run_of[r1] == run_of[r0]) ++r1;
+                for (uint32_t half = 0; half < 2; ++half)
+                    for (size_t e = r0; e < r1; ++e)
+                        stages.push_back(TileClass{entries[e].off ^ (half ? half_off : 0ull), entries[e].boff,
+                                                   entries[e].bwords | (half << 31)});
+                r0 = r1;
+            }
+            cls.resize(cls_base);
+            cls.insert(cls.end(), stages.begin(), stages.end());
+        }
         H = cls.size() - cls_base;
         passes = grp64.size() - grp_base;
         // x-mask order for the plan builders
@@ -614,14 +748,14 @@ struct TileTables {
         }
     }
 
-    // The blocks of one class: passes [gb, ge) (ordered by slot pattern) as
-    // term records, packed into blocks of at most block_records(wg_threads, 2^ab)
-    // records; a class that does not fit in one block becomes several
-    // classes with the same source tile, and a pass with more terms than a
-    // block holds is split into passes of consecutive terms (the same sum,
-    // accumulated in two steps).
-    void append_blocks(uint64_t off, size_t gb, size_t ge, uint32_t ab, uint32_t wg_threads) {
-        const uint32_t A = 1u << ab, HS = header_slots(A);
+    // The blocks of one class: passes [gb, ge) (ordered by block pattern,
+    // pat[g - gb] in [0, P)) as term records, packed into blocks of at most
+    // block_records(wg_threads, P) records; a class that does not fit in one
+    // block becomes several classes with the same source tile, and a pass
+    // with more terms than a block holds is split into passes of consecutive
+    // terms (the same sum, accumulated in two steps).
+    void append_blocks(uint64_t off, size_t gb, size_t ge, const uint32_t *pat, uint32_t P, uint32_t wg_threads) {
+        const uint32_t A = P, HS = header_slots(A);
         // 2 * (HS + records) words per block <= block_words(wg_threads); the kernels run
         // 64 or more threads per workgroup, so a block holds 14 or more records
         const uint32_t RMAX = block_records(std::max<uint32_t>(wg_threads, 64), A);
@@ -643,7 +777,7 @@ struct TileTables {
                 all32.push_back(record(P32, term32[q]));
             }
             for (size_t i = 0; i < all64.size(); i += RMAX) {
-                Chunk c{P64.zs, {}, {}};
+                Chunk c{pat[g - gb], {}, {}};
                 const size_t e = std::min(all64.size(), i + RMAX);
                 c.r64.assign(all64.begin() + i, all64.begin() + e);
                 c.r32.assign(all32.begin() + i, all32.begin() + e);

@@ -28,7 +28,8 @@ from ._plan import (
 
 # tile geometries of the full-space kernel (threads x amplitudes per thread)
 TILINGS = {"512x8": _lib.TCP_TILE_512x8, "1024x8": _lib.TCP_TILE_1024x8, "512x16": _lib.TCP_TILE_512x16,
-           "512x4": _lib.TCP_TILE_512x4, "1024x4": _lib.TCP_TILE_1024x4}
+           "512x4": _lib.TCP_TILE_512x4, "1024x4": _lib.TCP_TILE_1024x4,
+           "1024x8x2": _lib.TCP_TILE_1024x8x2, "512x16x2": _lib.TCP_TILE_512x16x2}
 
 _LETTERS = {"I": 0, "X": 1, "Y": 2, "Z": 3}
 PauliInput = Union[str, Sequence[int]]
@@ -397,8 +398,10 @@ class PauliOperator:
         """Compile a reusable matrix-free plan on the current GPU.
 
         ``tiling`` picks the kernel's tile geometry ("512x8", "512x4",
-        "1024x4", "1024x8", "512x16"; None = automatic: 512x8 for complex128
-        and 512x4 for complex64 storage, with the XCD order) and ``xcd_order``
+        "1024x4", "1024x8", "512x16", and the twin-tile "1024x8x2" and
+        "512x16x2", whose workgroups own two output tiles; None = automatic:
+        complex128 1024x8x2 from 22 qubits and 512x8 below, complex64 storage
+        512x8 from 24 qubits and 512x4 below, with the XCD order) and ``xcd_order``
         walks the tiles XCD by XCD; ``prefetch`` (1-3) is the number of source
         tiles each workgroup keeps in flight and ``nontemporal`` marks those
         loads non-temporal (512x8, 512x4 and 1024x4 only).  None of these

@@ -72,7 +72,7 @@ def test_random_operators_every_tile_shape(n):
 
 
 @pytest.mark.parametrize("xcd_order", [False, True])
-@pytest.mark.parametrize("tiling", [None, "512x8", "512x4", "1024x8", "512x16"])
+@pytest.mark.parametrize("tiling", [None, "512x8", "512x4", "1024x8", "512x16", "1024x8x2", "512x16x2"])
 def test_every_tiling_against_oracle(tiling, xcd_order):
     import torch
 
@@ -81,7 +81,7 @@ def test_every_tiling_against_oracle(tiling, xcd_order):
     plan = op.native_mvp_plan(tiling=tiling, xcd_order=xcd_order)
     assert plan.tiling == (tiling or "512x8") and plan.xcd_order == xcd_order
     assert plan.tiling_c64 == (tiling or "512x4")
-    bits = {"512x4": 11, "512x8": 12, "1024x8": 13, "512x16": 13}
+    bits = {"512x4": 11, "512x8": 12, "1024x8": 13, "512x16": 13, "1024x8x2": 13, "512x16x2": 13}
     assert plan.tile_bits == bits[plan.tiling] and plan.tile_bits_c64 == bits[plan.tiling_c64]
     assert plan.pass_count >= plan.group_count
     state = _random_state(17, 92)
@@ -102,13 +102,37 @@ def test_default_tiling_is_xcd_ordered():
 def test_tiling_falls_back_on_small_spaces():
     codes, coef = lattice.random_pauli_sum(12, 80, seed=93)
     op = tcp.PauliOperator.from_code_arrays(codes, coef)
-    for tiling in ("1024x8", "512x16"):
+    for tiling in ("1024x8", "512x16", "1024x8x2", "512x16x2"):
         plan = op.native_mvp_plan(tiling=tiling, xcd_order=True)
         assert plan.tiling == "512x8" and plan.tile_bits == 12
         state = _random_state(12, 94)
         assert rel_err(plan.apply(state), _oracle_apply(op, state)) <= FP64_TOL
     with pytest.raises(ValueError):
         op.native_mvp_plan(tiling="64x64")
+
+
+@pytest.mark.parametrize("tiling", ["1024x8x2", "512x16x2"])
+def test_twin_tiles_with_bit_map_against_oracle(tiling):
+    """Twin tiles on a space large enough for the operator-chosen bit map
+    (window and XCD bits): every row against the oracle, in both storages,
+    and against the single-tile geometry of the same staged tile size."""
+    import torch
+
+    n = 22
+    codes, coef = lattice.random_pauli_sum(n, 300, seed=96, p_identity=0.6)
+    op = tcp.PauliOperator.from_code_arrays(codes, coef * (0.7 + 0.2j))
+    plan = op.native_mvp_plan(tiling=tiling, xcd_order=True)
+    assert plan.tiling == tiling and plan.xcd_order and plan.tile_bits == 13
+    single = op.native_mvp_plan(tiling="1024x8", xcd_order=True)
+    print(f"{tiling}: classes {plan.class_count} (1024x8: {single.class_count})", flush=True)
+    state = _random_state(n, 97)
+    expected = _oracle_apply(op, state)
+    out = plan.apply(state)
+    assert rel_err(out, expected) <= FP64_TOL
+    assert rel_err(out, single.apply(state)) <= FP64_TOL
+    y32 = plan.apply(torch.from_numpy(state).cuda().to(torch.complex64))
+    torch.cuda.synchronize()
+    assert rel_err(y32.cpu().numpy().astype(np.complex128), expected) <= FP32_TOL
 
 
 def test_many_classes_and_large_groups():
