@@ -95,7 +95,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // A term of a pass in a class block (two 16-byte words).
 struct alignas(16) TermRecord {
-    uint32_t lo_last;  // bits 0..15: the pass's x & (2^L - 1); bit 31: the pass's last term
+    uint32_t lo_last;  // bits 0..15: the pass's x & (2^L - 1); bits 29..30: its kind; bit 31: its last term
     uint32_t zlo;      // z & (TPB - 1)
     uint64_t zhi;      // z >> L
     double wr, wi;     // w' (rounded to FP32 in the complex64 blocks)
@@ -222,75 +222,112 @@ __device__ __forceinline__ Rec read_rec(const u32x4 *blk, uint32_t r) {
     return R;
 }
 
-// S += w' (-1)^popc(z & l) for the term of record R at thread t of tile `tile_id`.
-__device__ __forceinline__ void add_term(const Rec &R, uint32_t t, uint64_t tile_id, double &sr, double &si) {
-    const uint32_t par = __popc(R.zlo & t) + __popcll(R.zhi & tile_id);
-    sr += flip_sign(R.wr, par & 1u);
-    si += flip_sign(R.wi, par & 1u);
+// Pass kinds, bits 29-30 of a pass's records' lo_last: the sum S of a pass
+// whose terms all have real w' (imaginary w') is real (imaginary), and its
+// complex multiply-add takes two FMAs per amplitude instead of four.
+constexpr uint32_t kPassComplex = 0, kPassReal = 1, kPassImag = 2;
+constexpr uint32_t kKindShift = 29;
+
+// S += w' (-1)^popc(z & l) for one record, the parts KIND can have.
+template <uint32_t KIND>
+__device__ __forceinline__ void add_term_k(const Rec &R, uint32_t t, uint64_t tile_id, double &sr, double &si) {
+    const uint32_t par = (__popc(R.zlo & t) + __popcll(R.zhi & tile_id)) & 1u;
+    if constexpr (KIND != kPassImag) sr += flip_sign(R.wr, par);
+    if constexpr (KIND != kPassReal) si += flip_sign(R.wi, par);
 }
 
-// acc[a] += (-1)^popc(ZS & a) * S * src[a] for every pass of slot pattern ZS,
-// records [r0, r1) of the staged block (straight-line code: the slot signs are
-// compile-time).  The LDS reads of the first four slots of a pass are issued
-// together, then its terms' per-thread sum S is formed, then the FMAs run
-// four slots at a time.
-template <int ZS, int A, uint32_t TPB, int CR = 0>
+// One pass of kind KIND starting at record r (R = its first record, head =
+// R.lo_last, uniform): acc[a] += (-1)^popc(ZS & a) * S * src[a].  The LDS
+// reads of the first C slots are issued, then the terms' per-thread sum S
+// is formed, then the FMAs run C slots at a time.  Returns the record after
+// the pass.
+template <uint32_t KIND, int ZS, int A, int C, uint32_t TPB>
+__device__ __forceinline__ uint32_t pass_body(const u32x4 *blk, uint32_t r, Rec R, uint32_t head, uint32_t t,
+                                              uint32_t t16, uint64_t tile_id, const char *tile, double *accr,
+                                              double *acci) {
+    constexpr uint32_t TB = __builtin_ctz(TPB);
+    const uint32_t lo = head & 0xFFFFu;
+    const uint32_t tbase = t16 ^ ((lo & (TPB - 1)) * static_cast<uint32_t>(sizeof(dvec2)));
+    const uint32_t ls = lo >> TB;
+    dvec2 s[C];
+#pragma unroll
+    for (int a = 0; a < C; ++a) s[a] = *reinterpret_cast<const dvec2 *>(tile + src_offset<TPB>(tbase, ls, a));
+    double sr = 0.0, si = 0.0;
+    add_term_k<KIND>(R, t, tile_id, sr, si);
+    ++r;
+    for (uint32_t last = head >> 31; !last; ++r) {
+        R = read_rec(blk, r);
+        last = __builtin_amdgcn_readfirstlane(R.lo_last) >> 31;
+        add_term_k<KIND>(R, t, tile_id, sr, si);
+    }
+#pragma unroll
+    for (int a0 = 0; a0 < A; a0 += C) {
+        if (a0 > 0) {
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+                s[c] = *reinterpret_cast<const dvec2 *>(tile + src_offset<TPB>(tbase, ls, a0 + c));
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int a = a0 + c;
+            const double pr = SlotSign<A>::neg(ZS, a) ? -sr : sr;
+            const double pi = SlotSign<A>::neg(ZS, a) ? -si : si;
+            if constexpr (KIND == kPassReal) {
+                accr[a] = fma(pr, s[c].x, accr[a]);
+                acci[a] = fma(pr, s[c].y, acci[a]);
+            } else if constexpr (KIND == kPassImag) {
+                accr[a] = fma(-pi, s[c].y, accr[a]);
+                acci[a] = fma(pi, s[c].x, acci[a]);
+            } else {
+                accr[a] = fma(pr, s[c].x, accr[a]);
+                accr[a] = fma(-pi, s[c].y, accr[a]);
+                acci[a] = fma(pr, s[c].y, acci[a]);
+                acci[a] = fma(pi, s[c].x, acci[a]);
+            }
+        }
+    }
+    return r;
+}
+
+// Every pass of slot pattern ZS, records [r0, r1) of the staged block
+// (straight-line code: the slot signs are compile-time).  The plan builder
+// orders a pattern's passes real, imaginary, complex; with KINDS three loops
+// run them one after the other (no live registers shared between their
+// bodies), the last one on the general code, which is right for a pass of
+// any kind.  Without KINDS only the general loop runs (less code: the twin
+// kernel, at its register limit with one LDS read in flight, measured 4 %
+// faster without the kind loops, the single-tile kernels 2-4 % slower).
+template <int ZS, int A, uint32_t TPB, int CR = 0, bool KINDS = true>
 __device__ __forceinline__ void passes_uniform(const u32x4 *blk, uint32_t r0, uint32_t r1, uint32_t t, uint32_t t16,
                                                uint64_t tile_id, const char *tile, double *accr, double *acci) {
-    constexpr uint32_t TB = __builtin_ctz(TPB);
     // slots read from LDS at a time (bounds live registers; CR > 0 overrides)
     constexpr int C = CR > 0 ? CR : (A >= 8 ? 4 : (A >= 4 ? 2 : A));
     uint32_t r = r0;
+    auto head_of = [&](uint32_t i, Rec &R) {
+        R = read_rec(blk, i);
+        return __builtin_amdgcn_readfirstlane(R.lo_last);
+    };
+    Rec R;
+    uint32_t head = 0;
+    if constexpr (KINDS) {
+        while (r < r1 && (((head = head_of(r, R)) >> kKindShift) & 3u) == kPassReal)
+            r = pass_body<kPassReal, ZS, A, C, TPB>(blk, r, R, head, t, t16, tile_id, tile, accr, acci);
+        while (r < r1 && (((head = head_of(r, R)) >> kKindShift) & 3u) == kPassImag)
+            r = pass_body<kPassImag, ZS, A, C, TPB>(blk, r, R, head, t, t16, tile_id, tile, accr, acci);
+    }
     while (r < r1) {
-        Rec R = read_rec(blk, r);
-        const uint32_t head = __builtin_amdgcn_readfirstlane(R.lo_last);
-        const uint32_t lo = head & 0xFFFFu;
-        const uint32_t tbase = t16 ^ ((lo & (TPB - 1)) * static_cast<uint32_t>(sizeof(dvec2)));
-        const uint32_t ls = lo >> TB;
-        dvec2 s[C];
-#pragma unroll
-        for (int a = 0; a < C; ++a) s[a] = *reinterpret_cast<const dvec2 *>(tile + src_offset<TPB>(tbase, ls, a));
-        double sr = 0.0, si = 0.0;
-        add_term(R, t, tile_id, sr, si);
-        ++r;
-        for (uint32_t last = head >> 31; !last; ++r) {
-            R = read_rec(blk, r);
-            last = __builtin_amdgcn_readfirstlane(R.lo_last) >> 31;
-            add_term(R, t, tile_id, sr, si);
-        }
-#pragma unroll
-        for (int a0 = 0; a0 < A; a0 += C) {
-            if (a0 > 0) {
-#pragma unroll
-                for (int c = 0; c < C; ++c)
-                    s[c] = *reinterpret_cast<const dvec2 *>(tile + src_offset<TPB>(tbase, ls, a0 + c));
-            }
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const int a = a0 + c;
-                if (SlotSign<A>::neg(ZS, a)) {
-                    accr[a] = fma(-sr, s[c].x, accr[a]);
-                    accr[a] = fma(si, s[c].y, accr[a]);
-                    acci[a] = fma(-sr, s[c].y, acci[a]);
-                    acci[a] = fma(-si, s[c].x, acci[a]);
-                } else {
-                    accr[a] = fma(sr, s[c].x, accr[a]);
-                    accr[a] = fma(-si, s[c].y, accr[a]);
-                    acci[a] = fma(sr, s[c].y, acci[a]);
-                    acci[a] = fma(si, s[c].x, acci[a]);
-                }
-            }
-        }
+        head = head_of(r, R);
+        r = pass_body<kPassComplex, ZS, A, C, TPB>(blk, r, R, head, t, t16, tile_id, tile, accr, acci);
     }
 }
 
 // All passes of one class, slot pattern by slot pattern (zr: the block
 // header, record index where slot pattern z's passes begin, zr[A] the end).
-template <int A, uint32_t TPB, int ZS = 0, int CR = 0>
+template <int A, uint32_t TPB, int ZS = 0, int CR = 0, bool KINDS = true>
 __device__ __forceinline__ void passes_all(const u32x4 *blk, const uint32_t *zr, uint32_t t, uint32_t t16,
                                            uint64_t tile_id, const char *tile, double *accr, double *acci) {
-    passes_uniform<ZS, A, TPB, CR>(blk, zr[ZS], zr[ZS + 1], t, t16, tile_id, tile, accr, acci);
-    if constexpr (ZS + 1 < A) passes_all<A, TPB, ZS + 1, CR>(blk, zr, t, t16, tile_id, tile, accr, acci);
+    passes_uniform<ZS, A, TPB, CR, KINDS>(blk, zr[ZS], zr[ZS + 1], t, t16, tile_id, tile, accr, acci);
+    if constexpr (ZS + 1 < A) passes_all<A, TPB, ZS + 1, CR, KINDS>(blk, zr, t, t16, tile_id, tile, accr, acci);
 }
 
 // Loads of one source tile into registers; NT marks them non-temporal.
@@ -465,8 +502,8 @@ __device__ __forceinline__ void twin_apply(const VT *__restrict__ vin, VT *__res
             z0[z] = half ? zr[A + z] : zr[z];
             z1[z] = half ? zr[z] : zr[A + z];
         }
-        passes_all<A, TPB, 0, CR>(lblk, z0, t, t16, tile0, tile, accr0, acci0);
-        passes_all<A, TPB, 0, CR>(lblk, z1, t, t16, tile1, tile, accr1, acci1);
+        passes_all<A, TPB, 0, CR, false>(lblk, z0, t, t16, tile0, tile, accr0, acci0);
+        passes_all<A, TPB, 0, CR, false>(lblk, z1, t, t16, tile1, tile, accr1, acci1);
     }
     VT *__restrict__ out0 = yout + base + toff;
     VT *__restrict__ out1 = yout + (base ^ half_off) + toff;
@@ -639,12 +676,13 @@ struct TileTables {
             uint32_t pat;  // block pattern: slot pattern, plus A * (bit L of x) for twin tiles
             uint32_t zs;
             uint64_t index;
+            uint32_t rank = 0;  // pass kind order: real, imaginary, complex
         };
         std::vector<Key> keys(T);
         for (uint64_t k = 0; k < T; ++k) {
             const uint32_t zs = static_cast<uint32_t>((terms[k].z >> tb) & (A - 1));
             const uint32_t px = twin ? static_cast<uint32_t>((terms[k].x >> L) & 1u) : 0u;
-            keys[k] = Key{terms[k].x >> CL, terms[k].x & lmask, zs + A * px, zs, k};
+            keys[k] = Key{terms[k].x >> CL, terms[k].x & lmask, zs + A * px, zs, k, 0u};
         }
         // kernel order: class, block pattern, x-mask, canonical term order
         std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
@@ -652,6 +690,27 @@ struct TileTables {
             if (a.pat != b.pat) return a.pat < b.pat;
             if (a.lo != b.lo) return a.lo < b.lo;
             return a.index < b.index;
+        });
+        // then, within a class and pattern, passes whose terms' w' are all
+        // real first, all imaginary next, the others last (the kernel's pass
+        // loops run in that order)
+        for (uint64_t k0 = 0; k0 < T;) {
+            uint64_t k1 = k0;
+            bool real = true, imag = true;
+            while (k1 < T && keys[k1].hi == keys[k0].hi && keys[k1].pat == keys[k0].pat && keys[k1].lo == keys[k0].lo) {
+                const IndexTerm &term = terms[keys[k1].index];
+                real = real && term.w.im == 0.0;  // w' = w (-1)^popc(z & x): the same parts are zero
+                imag = imag && term.w.re == 0.0;
+                ++k1;
+            }
+            const uint32_t rank = real ? 0u : (imag ? 1u : 2u);
+            for (uint64_t k = k0; k < k1; ++k) keys[k].rank = rank;
+            k0 = k1;
+        }
+        std::stable_sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
+            if (a.hi != b.hi) return a.hi < b.hi;
+            if (a.pat != b.pat) return a.pat < b.pat;
+            return a.rank < b.rank;
         });
         cls_base = cls.size();
         grp_base = grp64.size();
@@ -776,13 +835,24 @@ struct TileTables {
                 all64.push_back(record(P64, term64[q]));
                 all32.push_back(record(P32, term32[q]));
             }
+            // a chunk is one pass of the kernel: its kind from its records' coefficients
+            auto mark_kind = [](std::vector<TermRecord> &rs) {
+                bool real = true, imag = true;
+                for (const TermRecord &q : rs) {
+                    real = real && q.wi == 0.0;
+                    imag = imag && q.wr == 0.0;
+                }
+                const uint32_t kind = real ? kPassReal : (imag ? kPassImag : kPassComplex);
+                for (TermRecord &q : rs) q.lo_last |= kind << kKindShift;
+                rs.back().lo_last |= 1u << 31;
+            };
             for (size_t i = 0; i < all64.size(); i += RMAX) {
                 Chunk c{pat[g - gb], {}, {}};
                 const size_t e = std::min(all64.size(), i + RMAX);
                 c.r64.assign(all64.begin() + i, all64.begin() + e);
                 c.r32.assign(all32.begin() + i, all32.begin() + e);
-                c.r64.back().lo_last |= 1u << 31;
-                c.r32.back().lo_last |= 1u << 31;
+                mark_kind(c.r64);
+                mark_kind(c.r32);
                 chunks.push_back(std::move(c));
             }
         }
