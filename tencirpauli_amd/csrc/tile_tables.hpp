@@ -236,15 +236,19 @@ __device__ __forceinline__ void add_term_k(const Rec &R, uint32_t t, uint64_t ti
     if constexpr (KIND != kPassReal) si += flip_sign(R.wi, par);
 }
 
-// One pass of kind KIND starting at record r (R = its first record, head =
-// R.lo_last, uniform): acc[a] += (-1)^popc(ZS & a) * S * src[a].  The LDS
-// reads of the first C slots are issued, then the terms' per-thread sum S
-// is formed, then the FMAs run C slots at a time.  Returns the record after
-// the pass.
-template <uint32_t KIND, int ZS, int A, int C, uint32_t TPB>
-__device__ __forceinline__ uint32_t pass_body(const u32x4 *blk, uint32_t r, Rec R, uint32_t head, uint32_t t,
-                                              uint32_t t16, uint64_t tile_id, const char *tile, double *accr,
-                                              double *acci) {
+// One pass of kind KIND starting at record r of a pattern's range [r, r1)
+// (head = the pass's first lo_last, uniform): acc[a] += (-1)^popc(ZS & a) *
+// S * src[a].  The LDS reads of the first C slots are issued, then the
+// terms' per-thread sum S is formed, then the FMAs run C slots at a time.
+// With AHEAD the pass's first record is read here, together with the
+// source reads (head, read ahead, gives the source offset), and the next
+// pass's head is read before the FMAs, which cover its latency; head then
+// becomes the next pass's (if any).  Without AHEAD, R is the pass's first
+// record, read by the caller.  Returns the record after the pass.
+template <uint32_t KIND, int ZS, int A, int C, uint32_t TPB, bool AHEAD>
+__device__ __forceinline__ uint32_t pass_body(const u32x4 *blk, uint32_t r, uint32_t r1, Rec R, uint32_t &head,
+                                              uint32_t t, uint32_t t16, uint64_t tile_id, const char *tile,
+                                              double *accr, double *acci) {
     constexpr uint32_t TB = __builtin_ctz(TPB);
     const uint32_t lo = head & 0xFFFFu;
     const uint32_t tbase = t16 ^ ((lo & (TPB - 1)) * static_cast<uint32_t>(sizeof(dvec2)));
@@ -252,6 +256,7 @@ __device__ __forceinline__ uint32_t pass_body(const u32x4 *blk, uint32_t r, Rec 
     dvec2 s[C];
 #pragma unroll
     for (int a = 0; a < C; ++a) s[a] = *reinterpret_cast<const dvec2 *>(tile + src_offset<TPB>(tbase, ls, a));
+    if constexpr (AHEAD) R = read_rec(blk, r);
     double sr = 0.0, si = 0.0;
     add_term_k<KIND>(R, t, tile_id, sr, si);
     ++r;
@@ -259,6 +264,10 @@ __device__ __forceinline__ uint32_t pass_body(const u32x4 *blk, uint32_t r, Rec 
         R = read_rec(blk, r);
         last = __builtin_amdgcn_readfirstlane(R.lo_last) >> 31;
         add_term_k<KIND>(R, t, tile_id, sr, si);
+    }
+    uint32_t next = 0;
+    if constexpr (AHEAD) {
+        if (r < r1) next = reinterpret_cast<const uint32_t *>(blk)[8 * r];  // lo_last of record r
     }
 #pragma unroll
     for (int a0 = 0; a0 < A; a0 += C) {
@@ -286,6 +295,7 @@ __device__ __forceinline__ uint32_t pass_body(const u32x4 *blk, uint32_t r, Rec 
             }
         }
     }
+    if constexpr (AHEAD) head = __builtin_amdgcn_readfirstlane(next);
     return r;
 }
 
@@ -294,40 +304,50 @@ __device__ __forceinline__ uint32_t pass_body(const u32x4 *blk, uint32_t r, Rec 
 // orders a pattern's passes real, imaginary, complex; with KINDS three loops
 // run them one after the other (no live registers shared between their
 // bodies), the last one on the general code, which is right for a pass of
-// any kind.  Without KINDS only the general loop runs (less code: the twin
-// kernel, at its register limit with one LDS read in flight, measured 4 %
-// faster without the kind loops, the single-tile kernels 2-4 % slower).
-template <int ZS, int A, uint32_t TPB, int CR = 0, bool KINDS = true>
+// any kind.  KINDS and AHEAD (pass_body) serve the single-tile kernels
+// (measured 2-4 % faster with each); the twin kernel, at its register limit
+// with one LDS read in flight, runs without both (each measured 3-5 %
+// slower there).
+template <int ZS, int A, uint32_t TPB, int CR = 0, bool KINDS = true, bool AHEAD = true>
 __device__ __forceinline__ void passes_uniform(const u32x4 *blk, uint32_t r0, uint32_t r1, uint32_t t, uint32_t t16,
                                                uint64_t tile_id, const char *tile, double *accr, double *acci) {
     // slots read from LDS at a time (bounds live registers; CR > 0 overrides)
     constexpr int C = CR > 0 ? CR : (A >= 8 ? 4 : (A >= 4 ? 2 : A));
     uint32_t r = r0;
-    auto head_of = [&](uint32_t i, Rec &R) {
-        R = read_rec(blk, i);
-        return __builtin_amdgcn_readfirstlane(R.lo_last);
-    };
-    Rec R;
+    Rec R{};
     uint32_t head = 0;
+    if constexpr (AHEAD) {
+        if (r0 >= r1) return;
+        head = __builtin_amdgcn_readfirstlane(reinterpret_cast<const uint32_t *>(blk)[8 * r0]);
+    }
+    // the kind of the pass at r (AHEAD: from its head; else its first record is read here)
+    auto kind_at = [&]() {
+        if constexpr (!AHEAD) {
+            R = read_rec(blk, r);
+            head = __builtin_amdgcn_readfirstlane(R.lo_last);
+        }
+        return (head >> kKindShift) & 3u;
+    };
     if constexpr (KINDS) {
-        while (r < r1 && (((head = head_of(r, R)) >> kKindShift) & 3u) == kPassReal)
-            r = pass_body<kPassReal, ZS, A, C, TPB>(blk, r, R, head, t, t16, tile_id, tile, accr, acci);
-        while (r < r1 && (((head = head_of(r, R)) >> kKindShift) & 3u) == kPassImag)
-            r = pass_body<kPassImag, ZS, A, C, TPB>(blk, r, R, head, t, t16, tile_id, tile, accr, acci);
+        while (r < r1 && kind_at() == kPassReal)
+            r = pass_body<kPassReal, ZS, A, C, TPB, AHEAD>(blk, r, r1, R, head, t, t16, tile_id, tile, accr, acci);
+        while (r < r1 && kind_at() == kPassImag)
+            r = pass_body<kPassImag, ZS, A, C, TPB, AHEAD>(blk, r, r1, R, head, t, t16, tile_id, tile, accr, acci);
     }
     while (r < r1) {
-        head = head_of(r, R);
-        r = pass_body<kPassComplex, ZS, A, C, TPB>(blk, r, R, head, t, t16, tile_id, tile, accr, acci);
+        if constexpr (!AHEAD) kind_at();
+        r = pass_body<kPassComplex, ZS, A, C, TPB, AHEAD>(blk, r, r1, R, head, t, t16, tile_id, tile, accr, acci);
     }
 }
 
 // All passes of one class, slot pattern by slot pattern (zr: the block
 // header, record index where slot pattern z's passes begin, zr[A] the end).
-template <int A, uint32_t TPB, int ZS = 0, int CR = 0, bool KINDS = true>
+template <int A, uint32_t TPB, int ZS = 0, int CR = 0, bool KINDS = true, bool AHEAD = true>
 __device__ __forceinline__ void passes_all(const u32x4 *blk, const uint32_t *zr, uint32_t t, uint32_t t16,
                                            uint64_t tile_id, const char *tile, double *accr, double *acci) {
-    passes_uniform<ZS, A, TPB, CR, KINDS>(blk, zr[ZS], zr[ZS + 1], t, t16, tile_id, tile, accr, acci);
-    if constexpr (ZS + 1 < A) passes_all<A, TPB, ZS + 1, CR, KINDS>(blk, zr, t, t16, tile_id, tile, accr, acci);
+    passes_uniform<ZS, A, TPB, CR, KINDS, AHEAD>(blk, zr[ZS], zr[ZS + 1], t, t16, tile_id, tile, accr, acci);
+    if constexpr (ZS + 1 < A)
+        passes_all<A, TPB, ZS + 1, CR, KINDS, AHEAD>(blk, zr, t, t16, tile_id, tile, accr, acci);
 }
 
 // Loads of one source tile into registers; NT marks them non-temporal.
@@ -502,8 +522,8 @@ __device__ __forceinline__ void twin_apply(const VT *__restrict__ vin, VT *__res
             z0[z] = half ? zr[A + z] : zr[z];
             z1[z] = half ? zr[z] : zr[A + z];
         }
-        passes_all<A, TPB, 0, CR, false>(lblk, z0, t, t16, tile0, tile, accr0, acci0);
-        passes_all<A, TPB, 0, CR, false>(lblk, z1, t, t16, tile1, tile, accr1, acci1);
+        passes_all<A, TPB, 0, CR, false, false>(lblk, z0, t, t16, tile0, tile, accr0, acci0);
+        passes_all<A, TPB, 0, CR, false, false>(lblk, z1, t, t16, tile1, tile, accr1, acci1);
     }
     VT *__restrict__ out0 = yout + base + toff;
     VT *__restrict__ out1 = yout + (base ^ half_off) + toff;
