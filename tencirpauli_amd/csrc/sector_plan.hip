@@ -256,18 +256,23 @@ __device__ __forceinline__ GatherGroup load_group(cptr<GatherGroup> groups, uint
     return r;
 }
 
-template <bool COND>
-__device__ __forceinline__ bool gather_term_acts(const GatherArgs &q, uint32_t k, uint64_t s) {
+template <bool COND, typename IT>
+__device__ __forceinline__ bool gather_term_acts(const GatherArgs &q, uint32_t k, IT s) {
     if (!COND) return true;
-    const uint64_t need = ((cptr<uint64_t>)q.tc1)[k];
-    return (s & need) == need && (s & ((cptr<uint64_t>)q.tc0)[k]) == 0;
+    const IT need = static_cast<IT>(((cptr<uint64_t>)q.tc1)[k]);
+    return (s & need) == need && (s & static_cast<IT>(((cptr<uint64_t>)q.tc0)[k])) == 0;
 }
+
+__device__ __forceinline__ uint32_t popcount_of(uint32_t x) { return static_cast<uint32_t>(__popc(x)); }
+__device__ __forceinline__ uint32_t popcount_of(uint64_t x) { return static_cast<uint32_t>(__popcll(x)); }
 
 // NB = number of blocks, fixed at compile time so the per-group loops keep
 // their block and chunk fields in registers.  RPT = consecutive destination
 // ranks per thread (one unrank, then successor steps): 1 for one-block (U(1))
-// sectors, 2 for block sectors.
-template <typename VT, bool LDS, bool COND, uint32_t NB, uint32_t RPT>
+// sectors, 2 for block sectors.  IT holds a state's index bits: uint32_t for
+// spaces of up to 32 qubits (half the integer instructions of the per-group
+// bit work), else uint64_t.
+template <typename VT, bool LDS, bool COND, uint32_t NB, uint32_t RPT, typename IT>
 __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT *__restrict__ v,
                                                                        VT *__restrict__ y, SectorArgs p,
                                                                        GatherArgs q) {
@@ -289,8 +294,9 @@ __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT 
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * RPT;
     for (uint64_t r0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * RPT;
          r0 < p.dim; r0 += stride) {
-        uint64_t b = sector_unrank<NB>(p, r0);
+        uint64_t b64 = sector_unrank<NB>(p, r0);
         for (uint32_t j = 0; j < RPT; ++j) {
+            const IT b = static_cast<IT>(b64);
             const uint64_t r = r0 + j;
             if (r >= p.dim) break;
             // rank-table entry and ones-below count of every chunk of b
@@ -328,27 +334,28 @@ __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT 
                     const uint32_t g = g0 + u;
                     const bool live = g < q.G;
                     const GatherGroup gr = load_group(groups, live ? g : 0);
-                    const uint64_t s = b ^ gr.x;
+                    const IT gx = static_cast<IT>(gr.x);
+                    const IT s = b ^ gx;
                     // s keeps every block's population when, inside each
                     // block, half of the bits x flips are ones in b
                     bool ok = live;
 #pragma unroll
                     for (uint32_t k = 0; k < NB; ++k) {
                         if (!((gr.chunks >> (24 + k)) & 1u)) continue;
-                        const uint64_t xk = NB == 1 ? gr.x : gr.x & q.blockmask[k];
-                        ok &= static_cast<uint32_t>(__popcll(b & xk)) == ((gr.halves >> (8 * k)) & 0xffu);
+                        const IT xk = NB == 1 ? gx : gx & static_cast<IT>(q.blockmask[k]);
+                        ok &= popcount_of(b & xk) == ((gr.halves >> (8 * k)) & 0xffu);
                     }
                     double dr = gr.cre, di = gr.cim;
                     if (ok && gr.nterms != 0) {  // else: a nonzero constant
                         if (gr.nterms & kInlineTerm) {
-                            const uint32_t par = __popcll(gr.z1 & s) & 1u;
+                            const uint32_t par = popcount_of(static_cast<IT>(gr.z1) & s) & 1u;
                             dr += flip_sign(gr.w1re, par);
                             di += flip_sign(gr.w1im, par);
                         }
                         const uint32_t k1 = gr.toff + (gr.nterms & ~kInlineTerm);
                         for (uint32_t k = gr.toff; k < k1; ++k) {
                             if (!gather_term_acts<COND>(q, k, s)) continue;
-                            const uint32_t par = __popcll(tz[k] & s) & 1u;
+                            const uint32_t par = popcount_of(static_cast<IT>(tz[k]) & s) & 1u;
                             const dvec2 w = tw[k];
                             dr += flip_sign(w.x, par);
                             di += flip_sign(w.y, par);
@@ -356,7 +363,7 @@ __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT 
                         ok = dr != 0.0 || di != 0.0;
                     }
                     uint64_t src = r;
-                    if (ok && gr.x != 0) {
+                    if (ok && gx != 0) {
                         // the chunks of each block from the lowest to the highest
                         // one x flips; a run starts from b's ones-below count
                         uint32_t delta = 0, nb = 0;
@@ -373,9 +380,11 @@ __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT 
                         ok = src < p.dim;  // unreachable for in-sector s; keeps the gather in bounds
                         if (!ok) src = r;
                     }
+                    // a group that does not contribute keeps its (finite) weight: its
+                    // gathered value is replaced by zero below
                     rs[u] = src;
-                    wr[u] = ok ? dr : 0.0;
-                    wi[u] = ok ? di : 0.0;
+                    wr[u] = dr;
+                    wi[u] = di;
                     used |= static_cast<uint32_t>(ok) << u;
                 }
                 VT sv[GC];
@@ -393,7 +402,7 @@ __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT 
                 }
             }
             y[r] = Amp<VT>::make(ar, ai);
-            b = sector_next<NB>(p, b);
+            b64 = sector_next<NB>(p, b64);
         }
     }
 }
@@ -1702,19 +1711,31 @@ hipError_t launch_gather(const tcp_sector_plan *plan, const void *in, void *out,
                          hipStream_t stream) {
     SectorArgs a = plan->args(dtype);
     const GatherArgs q = plan->gather_args(dtype);
+    const bool narrow = plan->n <= 32;  // index bits fit 32-bit words
     auto launch = [&](auto nb) {
         constexpr uint32_t NB = decltype(nb)::value;
         constexpr uint32_t RPT = NB == 1 ? 1 : 2;
         const uint64_t per_block = static_cast<uint64_t>(kGatherThreads) * RPT;
         const uint64_t blocks = std::min<uint64_t>((plan->dim + per_block - 1) / per_block, 256 * 16);
-        if (plan->lut_in_lds)
-            hipLaunchKernelGGL((sector_gather_kernel<VT, true, COND, NB, RPT>), dim3(static_cast<uint32_t>(blocks)),
-                               dim3(kGatherThreads), a.lut_words * 4, stream, static_cast<const VT *>(in),
-                               static_cast<VT *>(out), a, q);
-        else
-            hipLaunchKernelGGL((sector_gather_kernel<VT, false, COND, NB, RPT>), dim3(static_cast<uint32_t>(blocks)),
-                               dim3(kGatherThreads), 0, stream, static_cast<const VT *>(in),
-                               static_cast<VT *>(out), a, q);
+        const dim3 grid(static_cast<uint32_t>(blocks)), block(kGatherThreads);
+        const size_t lds = plan->lut_in_lds ? a.lut_words * 4 : 0;
+        const VT *vin = static_cast<const VT *>(in);
+        VT *vout = static_cast<VT *>(out);
+        if (plan->lut_in_lds) {
+            if (narrow)
+                hipLaunchKernelGGL((sector_gather_kernel<VT, true, COND, NB, RPT, uint32_t>), grid, block, lds, stream,
+                                   vin, vout, a, q);
+            else
+                hipLaunchKernelGGL((sector_gather_kernel<VT, true, COND, NB, RPT, uint64_t>), grid, block, lds, stream,
+                                   vin, vout, a, q);
+        } else {
+            if (narrow)
+                hipLaunchKernelGGL((sector_gather_kernel<VT, false, COND, NB, RPT, uint32_t>), grid, block, 0, stream,
+                                   vin, vout, a, q);
+            else
+                hipLaunchKernelGGL((sector_gather_kernel<VT, false, COND, NB, RPT, uint64_t>), grid, block, 0, stream,
+                                   vin, vout, a, q);
+        }
     };
     switch (a.nblocks) {
         case 1: launch(std::integral_constant<uint32_t, 1>()); break;
