@@ -19,9 +19,9 @@
 // FP64) and every group of the class gathers from LDS, so HBM reads per apply
 // are 2^n * (classes) amplitudes instead of 2^n * (groups).  The next class's
 // tile is prefetched into registers while the current one is consumed.
-// Group descriptors and terms are wave-uniform and read through scalar loads;
-// tile_tables.hpp describes the two group kinds (uniform slot pattern, or a
-// Walsh-Hadamard fold over the slot bits).
+// A class's passes (its groups split by slot pattern) and their terms travel
+// as a class block staged in LDS beside the tile; tile_tables.hpp describes
+// the blocks, the passes and their kinds.
 #include <algorithm>
 #include <memory>
 #include <mutex>
