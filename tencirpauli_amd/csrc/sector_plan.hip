@@ -209,7 +209,7 @@ constexpr uint32_t kGatherChunk = 4;
 //    takes the x = 0 group out of the loop.
 constexpr uint32_t kMaxFlatChunks = 12;  // sum of ceil(size / 8) over <= 4 blocks of <= 63 qubits
 // sum of ceil(size / 8) over NB blocks of 32 bits in all: ceil(s / 8) <= (s + 7) / 8
-__host__ __device__ constexpr uint32_t kNarrowChunks(uint32_t nb) { return 3 + nb; }
+__host__ __device__ constexpr uint32_t narrow_chunk_count(uint32_t nb) { return 3 + nb; }
 constexpr uint32_t kInlineTerm = 1u << 31;
 
 struct GatherGroup {
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kGatherThreads) void sector_gather_kernel(const VT 
                                                                        VT *__restrict__ y, SectorArgs p,
                                                                        GatherArgs q) {
     // >= sum of ceil(size / 8) over the blocks: at most 3 + NB chunks fit 32 index bits
-    constexpr uint32_t F = sizeof(IT) == 4 ? kNarrowChunks(NB) : (7 + NB < kMaxFlatChunks ? 7 + NB : kMaxFlatChunks);
+    constexpr uint32_t F = sizeof(IT) == 4 ? narrow_chunk_count(NB) : (7 + NB < kMaxFlatChunks ? 7 + NB : kMaxFlatChunks);
     // groups in flight together: with several blocks, four 64-byte records
     // at once would not fit the scalar registers
     constexpr uint32_t GC = NB == 1 ? kGatherChunk : 2;
@@ -1715,7 +1715,7 @@ hipError_t launch_gather(const tcp_sector_plan *plan, const void *in, void *out,
     SectorArgs a = plan->args(dtype);
     const GatherArgs q = plan->gather_args(dtype);
     // index bits fit 32-bit words (and so do the byte chunks: checked, not assumed)
-    const bool narrow = plan->n <= 32 && q.nflat <= kNarrowChunks(a.nblocks);
+    const bool narrow = plan->n <= 32 && q.nflat <= narrow_chunk_count(a.nblocks);
     auto launch = [&](auto nb) {
         constexpr uint32_t NB = decltype(nb)::value;
         constexpr uint32_t RPT = NB == 1 ? 1 : 2;
