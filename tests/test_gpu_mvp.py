@@ -135,6 +135,29 @@ def test_twin_tiles_with_bit_map_against_oracle(tiling):
     assert rel_err(y32.cpu().numpy().astype(np.complex128), expected) <= FP32_TOL
 
 
+@pytest.mark.parametrize("tiling", ["512x8", "512x4", "1024x8", "512x16", "1024x8x2"])
+def test_real_imaginary_and_complex_passes_against_oracle(tiling):
+    """Passes whose terms' weights are all real, all imaginary, or mixed run on
+    different loops of the single-tile kernels: an operator with real
+    coefficients (real or imaginary weights by the parity of #Y) plus a few
+    complex ones puts all three kinds into the same classes; every row
+    against the oracle, both storages."""
+    import torch
+
+    n = 17
+    codes, coef = lattice.random_pauli_sum(n, 600, seed=98, p_identity=0.6)
+    coef = np.asarray(coef, dtype=np.complex128).copy()
+    coef[::7] *= (0.6 + 0.8j)  # mixed passes where these meet real ones
+    op = tcp.PauliOperator.from_code_arrays(codes, coef)
+    plan = op.native_mvp_plan(tiling=tiling, xcd_order=True)
+    state = _random_state(n, 99)
+    expected = _oracle_apply(op, state)
+    assert rel_err(plan.apply(state), expected) <= FP64_TOL
+    y32 = plan.apply(torch.from_numpy(state).cuda().to(torch.complex64))
+    torch.cuda.synchronize()
+    assert rel_err(y32.cpu().numpy().astype(np.complex128), expected) <= FP32_TOL
+
+
 def test_many_classes_and_large_groups():
     # full-width random strings: nearly every term is its own x-group and class
     codes, coef = lattice.random_pauli_sum(18, 700, seed=5, p_identity=0.3)
